@@ -1,0 +1,174 @@
+"""ctypes binding of libinscy_hip.so (include/inscy.h).
+
+The library is built in-tree by gpu_inscy_amd.build; loading fails with an
+ImportError naming the missing file rather than falling back to anything else.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libinscy_hip.so")
+
+INSCY_OK = 0
+INSCY_ERR_ARGUMENT = 1
+INSCY_ERR_DEVICE = 2
+INSCY_ERR_MEMORY = 3
+INSCY_ERR_UNSUPPORTED = 4
+
+MODE_EXACT = 1
+MODE_BATCHED = 2
+
+EXPORTED = [
+    "inscy_run", "inscy_run_device", "inscy_result_num_subspaces", "inscy_result_subspace",
+    "inscy_result_labels", "inscy_result_num_points", "inscy_result_stats", "inscy_result_free",
+    "inscy_last_error", "inscy_discretise", "inscy_neighbour_counts", "inscy_dense_components",
+]
+
+
+class InscyError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"INSCY error {code}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("neighborhood_size", ctypes.c_float),
+        ("F", ctypes.c_float),
+        ("num_obj", ctypes.c_int32),
+        ("min_size", ctypes.c_int32),
+        ("r", ctypes.c_float),
+        ("number_of_cells", ctypes.c_int32),
+        ("rectangular", ctypes.c_int32),
+        ("entropy_order", ctypes.c_int32),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("calls", ctypes.c_int64),
+        ("trees", ctypes.c_int64),
+        ("queries", ctypes.c_int64),
+        ("candidates", ctypes.c_int64),
+        ("subspaces", ctypes.c_int64),
+        ("clusters", ctypes.c_int64),
+        ("components", ctypes.c_int64),
+        ("seconds", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m gpu_inscy_amd.build` "
+                          "(the engine has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    L.inscy_run.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(Params), ctypes.c_int32,
+                            ctypes.POINTER(vp)]
+    L.inscy_run.restype = ctypes.c_int
+    L.inscy_run_device.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(Params), ctypes.c_int32, vp,
+                                   ctypes.POINTER(vp)]
+    L.inscy_run_device.restype = ctypes.c_int
+    L.inscy_result_num_subspaces.argtypes = [vp]
+    L.inscy_result_num_subspaces.restype = ctypes.c_int32
+    L.inscy_result_subspace.argtypes = [vp, ctypes.c_int32, i32p]
+    L.inscy_result_subspace.restype = ctypes.c_int32
+    L.inscy_result_labels.argtypes = [vp, ctypes.c_int32]
+    L.inscy_result_labels.restype = i32p
+    L.inscy_result_num_points.argtypes = [vp]
+    L.inscy_result_num_points.restype = ctypes.c_int64
+    L.inscy_result_stats.argtypes = [vp, ctypes.POINTER(Stats)]
+    L.inscy_result_stats.restype = ctypes.c_int
+    L.inscy_result_free.argtypes = [vp]
+    L.inscy_result_free.restype = None
+    L.inscy_last_error.argtypes = []
+    L.inscy_last_error.restype = ctypes.c_char_p
+    L.inscy_discretise.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, vp, vp, vp]
+    L.inscy_discretise.restype = ctypes.c_int
+    L.inscy_neighbour_counts.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_float, i32p, ctypes.c_int32,
+                                         vp, ctypes.c_int64, vp]
+    L.inscy_neighbour_counts.restype = ctypes.c_int
+    L.inscy_dense_components.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(Params), i32p,
+                                         ctypes.c_int32, vp]
+    L.inscy_dense_components.restype = ctypes.c_int
+    L.inscy_debug_trace.argtypes = [ctypes.c_int]
+    L.inscy_debug_trace_rows.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]
+    L.inscy_debug_trace_rows.restype = ctypes.c_int64
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != INSCY_OK:
+        raise InscyError(rc, lib().inscy_last_error().decode(errors="replace"))
+
+
+def make_params(neighborhood_size, F, num_obj, min_size, r=1.0, number_of_cells=3, rectangular=False,
+                entropy_order=0) -> Params:
+    return Params(float(neighborhood_size), float(F), int(num_obj), int(min_size), float(r), int(number_of_cells),
+                  int(bool(rectangular)), int(entropy_order))
+
+
+class Result:
+    """Owned copy of an inscy_result: subspaces (ascending dims), canonical
+    labels [#subspaces, n] int32, and the run counters."""
+
+    def __init__(self, handle):
+        L = lib()
+        try:
+            n = int(L.inscy_result_num_points(handle))
+            m = int(L.inscy_result_num_subspaces(handle))
+            dims = (ctypes.c_int32 * 64)()
+            self.subspaces = []
+            self.labels = np.zeros((m, n), dtype=np.int32)
+            for i in range(m):
+                k = L.inscy_result_subspace(handle, i, dims)
+                self.subspaces.append([int(dims[j]) for j in range(k)])
+                ptr = L.inscy_result_labels(handle, i)
+                self.labels[i] = np.ctypeslib.as_array(ptr, shape=(n,))
+            st = Stats()
+            check(L.inscy_result_stats(handle, ctypes.byref(st)))
+            self.stats = {f: getattr(st, f) for f, _ in Stats._fields_}
+        finally:
+            L.inscy_result_free(handle)
+
+
+def run_host(X: np.ndarray, params: Params, mode: int = MODE_EXACT) -> Result:
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    if X.ndim != 2:
+        raise ValueError("X must be a 2-d [n, d] array")
+    h = ctypes.c_void_p()
+    check(lib().inscy_run(X.ctypes.data, X.shape[0], X.shape[1], ctypes.byref(params), mode, ctypes.byref(h)))
+    return Result(h)
+
+
+def run_device(ptr: int, n: int, d: int, params: Params, mode: int = MODE_EXACT, stream: int = 0) -> Result:
+    h = ctypes.c_void_p()
+    check(lib().inscy_run_device(ctypes.c_void_p(ptr), n, d, ctypes.byref(params), mode,
+                                 ctypes.c_void_p(stream or None), ctypes.byref(h)))
+    return Result(h)
+
+
+def debug_trace(on: bool) -> None:
+    lib().inscy_debug_trace(int(on))
+
+
+def debug_trace_rows() -> np.ndarray:
+    L = lib()
+    m = int(L.inscy_debug_trace_rows(None, 0))
+    out = np.zeros((m, 7), dtype=np.int64)
+    if m:
+        L.inscy_debug_trace_rows(out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), m)
+    return out

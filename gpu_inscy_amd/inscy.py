@@ -1,0 +1,159 @@
+"""Drop-in replacement of the reference's Python surface (python/inscy.py).
+
+Same function names, argument order and defaults, same `(subspaces,
+clusterings)` return: lists of ascending dimension lists and, per subspace, a
+list of n labels.  Every entry point runs on the HIP engine
+(libinscy_hip.so); there is no CPU fallback.
+
+Results equal the reference's CPU INSCY (INSCY.CPU -> run_cpu3_weak) with
+labels in canonical form: -1 for noise, clusters numbered 0.. by smallest
+point id (the reference's raw labels are a renaming of these; its -2 border
+marks are noise).
+
+Mapping of the reference's entry points:
+  CPU (python/inscy.py:121-125)          -> exact mode (same results, on the GPU)
+  GPU (:189-192, run_gpu_4)              -> exact mode, one restricted tree at a time
+  GPU5 (:194-197, run_gpu_5)             -> batched mode (same results)
+  GPU3, GPU_star, GPU_memory, GPU_memory2 (:183-214) -> batched mode
+  run_cpu, run_cpu_weak, run_gpu, run_gpu_* (7-argument ops, :110-186) -> exact
+      mode with rectangular=False, entropy_order=0, the defaults those ops had
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .datagen import normalize as _normalize_np
+
+__all__ = [
+    "normalize", "clean_up", "count_number_of_clusters", "run", "CPU", "GPU", "GPU5", "GPU3", "GPU_star",
+    "GPU_memory", "GPU_memory2", "run_cpu", "run_cpu_weak", "run_gpu", "run_gpu_weak", "run_gpu_reduced",
+    "run_gpu_multi", "run_gpu_multi2", "run_gpu_multi2_cl_all", "run_gpu_multi2_cl_re_all",
+    "run_gpu_multi2_cl_multi", "run_gpu_multi2_cl_multi_mem", "run_cpu_gpu_mix", "run_cpu_gpu_mix_cl_steam",
+    "run_gpu_stream", "load_synt",
+]
+
+
+def normalize(x):
+    """Per-column min-max scaling (python/inscy.py:37-42); torch or numpy in, same type out."""
+    try:
+        import torch
+        if isinstance(x, torch.Tensor):
+            mn = x.min(0, keepdim=True)[0]
+            mx = x.max(0, keepdim=True)[0]
+            return (x - mn) / (mx - mn)
+    except ImportError:
+        pass
+    return _normalize_np(x)
+
+
+def run(X, neighborhood_size, F, num_obj, min_size, r=1.0, number_of_cells=3, rectangular=False, entropy_order=0,
+        mode=_lib.MODE_EXACT) -> _lib.Result:
+    """Runs INSCY and returns a Result (numpy labels [#subspaces, n], stats).
+
+    X: float32 [n, d] numpy array, CPU torch tensor, or CUDA torch tensor (then
+    the engine reads it in place on the tensor's device and current stream).
+    """
+    params = _lib.make_params(neighborhood_size, F, num_obj, min_size, r, number_of_cells, rectangular,
+                              entropy_order)
+    try:
+        import torch
+        if isinstance(X, torch.Tensor):
+            if X.dim() != 2:
+                raise ValueError("X must be [n, d]")
+            if X.is_cuda:
+                Xc = X.detach().to(torch.float32).contiguous()
+                with torch.cuda.device(Xc.device):
+                    stream = torch.cuda.current_stream(Xc.device).cuda_stream
+                    return _lib.run_device(Xc.data_ptr(), Xc.shape[0], Xc.shape[1], params, mode, stream)
+            X = X.detach().to(torch.float32).contiguous().numpy()
+    except ImportError:
+        pass
+    return _lib.run_host(np.asarray(X, dtype=np.float32), params, mode)
+
+
+def _lists(res: _lib.Result):
+    return [list(s) for s in res.subspaces], [row.tolist() for row in res.labels]
+
+
+def CPU(X, neighborhood_size, F, num_obj, min_size, r=1., number_of_cells=3, rectangular=False, entropy_order=0):
+    return _lists(run(X, neighborhood_size, F, num_obj, min_size, r, number_of_cells, rectangular, entropy_order,
+                      _lib.MODE_EXACT))
+
+
+def GPU(X, neighborhood_size, F, num_obj, min_size, r=1., number_of_cells=3, rectangular=False, entropy_order=0):
+    return _lists(run(X, neighborhood_size, F, num_obj, min_size, r, number_of_cells, rectangular, entropy_order,
+                      _lib.MODE_EXACT))
+
+
+def GPU5(X, neighborhood_size, F, num_obj, min_size, r=1., number_of_cells=3, rectangular=False, entropy_order=0):
+    return _lists(run(X, neighborhood_size, F, num_obj, min_size, r, number_of_cells, rectangular, entropy_order,
+                      _lib.MODE_BATCHED))
+
+
+GPU3 = GPU5
+GPU_star = GPU5
+GPU_memory = GPU5
+GPU_memory2 = GPU5
+
+
+def _seven(X, neighborhood_size, F, num_obj, min_size, r=1., number_of_cells=3):
+    return CPU(X, neighborhood_size, F, num_obj, min_size, r, number_of_cells, False, 0)
+
+
+run_cpu = _seven
+run_cpu_weak = _seven
+run_gpu = _seven
+run_gpu_weak = _seven
+run_gpu_reduced = _seven
+run_gpu_multi = _seven
+run_gpu_multi2 = _seven
+run_gpu_multi2_cl_all = _seven
+run_gpu_multi2_cl_re_all = _seven
+run_gpu_multi2_cl_multi = _seven
+run_gpu_multi2_cl_multi_mem = _seven
+run_cpu_gpu_mix = _seven
+run_cpu_gpu_mix_cl_steam = _seven
+
+
+def run_gpu_stream(X, neighborhood_size, F, num_obj, min_size, number_of_cells=3):
+    """python/inscy.py:221-225 (no r argument)."""
+    return _seven(X, neighborhood_size, F, num_obj, min_size, 1.0, number_of_cells)
+
+
+def clean_up(subspaces, clusterings, min_size):
+    """python/inscy.py:51-85: drop clusters smaller than min_size and the
+    subspaces left without clusters."""
+    subspaces = np.array(subspaces, dtype=object)
+    clusterings = np.array(clusterings)
+    keep = np.zeros(len(clusterings), dtype=bool)
+    for i in range(len(clusterings)):
+        row = clusterings[i]
+        ids, sizes = np.unique(row[row >= 0], return_counts=True)
+        for c, s in zip(ids, sizes):
+            if s < min_size:
+                row[row == c] = -1
+        keep[i] = bool(np.any(sizes >= min_size))
+    return subspaces[keep], clusterings[keep]
+
+
+def count_number_of_clusters(subspaces, clusterings):
+    """python/inscy.py:88-107: number of distinct non-negative labels summed over subspaces."""
+    total = 0
+    for row in clusterings:
+        row = np.asarray(row)
+        total += int(np.unique(row[row >= 0]).size)
+    return total
+
+
+def load_synt(d, n, cl, re):
+    """Seeded synthetic set of the reference's shape parameters
+    (python/inscy.py:235-250 calls a prebuilt generator; this draws cl clusters
+    in 0.8*d-dimensional subspaces with the project's own generator, seed re)."""
+    import torch
+
+    from .datagen import make_clusters
+    rng = np.random.default_rng(int(re))
+    X, _, _ = make_clusters(rng, int(n), int(d), int(cl), max(1, int(0.8 * d)), 0.05, 0.01)
+    return torch.from_numpy(X)

@@ -1,0 +1,131 @@
+/*
+ * inscy.h — C ABI of the MI355X INSCY engine (libinscy_hip.so).
+ *
+ * The reference exposes INSCY through a pybind/torch extension whose ops take
+ * (X, neighborhood_size, F, num_obj, min_size, r, number_of_cells,
+ * rectangular, entropy_order) and return [[subspaces], [clusterings]]
+ * (reference: inscy_map.cpp:230-270 run_cpu3_weak, :979-1067 run_gpu_4,
+ * :1070-1158 run_gpu_5, :1161-1249 run_gpu_memory, :1344-1432 run_gpu_star,
+ * module table :1844-1872; Python wrappers python/inscy.py:121-125,189-214).
+ * This header is the boundary those ops bind to instead: plain pointers and
+ * sizes, no torch types, errors reported by return code.
+ *
+ * Results are exact: the subspace list and, per subspace, the partition of the
+ * points into clusters equal those of the reference's CPU INSCY
+ * (INSCYCPU3Weak, src/algorithms/inscy/InscyNodeCpu.cu:159-235).  Labels are
+ * canonical: -1 for noise, clusters numbered 0.. in order of their smallest
+ * point id (the reference's raw labels differ only by renaming, and its -2
+ * border marks are noise).
+ *
+ * Threading: calls on different inscy_result objects may run concurrently on
+ * different devices; one device is driven by one call at a time.
+ */
+#ifndef INSCY_H
+#define INSCY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes. */
+#define INSCY_OK 0
+#define INSCY_ERR_ARGUMENT 1    /* invalid argument (sizes, eps <= 0, cells < 1, ...) */
+#define INSCY_ERR_DEVICE 2      /* a HIP runtime call failed */
+#define INSCY_ERR_MEMORY 3      /* device or host allocation failed */
+#define INSCY_ERR_UNSUPPORTED 4 /* a parameter combination this build does not implement */
+
+/* Engine modes. */
+#define INSCY_MODE_EXACT 1   /* one restricted tree at a time (reference GPU / GPU4 role) */
+#define INSCY_MODE_BATCHED 2 /* level-batched exact mode (reference GPU5 role); same results */
+
+/* Parameters of one run; field meaning as in the reference call
+ * INSCY.CPU(X, neighborhood_size, F, num_obj, min_size, r, number_of_cells,
+ * rectangular, entropy_order) (python/inscy.py:121-125). */
+typedef struct inscy_params {
+    float neighborhood_size; /* eps */
+    float F;                 /* density factor */
+    int32_t num_obj;         /* mu: minimum neighbours */
+    int32_t min_size;        /* minimum cluster / region size (absolute count) */
+    float r;                 /* redundancy factor */
+    int32_t number_of_cells; /* c: grid cells per dimension */
+    int32_t rectangular;     /* 1: neighbour-count density, 0: Epanechnikov density */
+    int32_t entropy_order;   /* 0: identity level order (the only order this build runs) */
+} inscy_params;
+
+/* Counters of one run (the reference prints `calls` only, InscyNodeCpu.cu:166). */
+typedef struct inscy_stats {
+    int64_t calls;          /* INSCY recursion invocations */
+    int64_t trees;          /* restricted (subspace, region) trees evaluated by the prune */
+    int64_t queries;        /* epsilon-neighbourhood counts computed on the device */
+    int64_t candidates;     /* candidate pairs examined by the neighbourhood kernels */
+    int64_t subspaces;      /* subspaces in the result */
+    int64_t clusters;       /* clusters in the result */
+    int64_t components;     /* subspaces whose dense components were built */
+    double seconds;         /* wall time of the run inside the library */
+} inscy_stats;
+
+typedef struct inscy_result inscy_result;
+
+/* Runs INSCY on a host array X (row-major float32 [n, d], values in [0, 1]).
+ * Replaces run_cpu3_weak / run_gpu_4 / run_gpu_5 (inscy_map.cpp:230,979,1070):
+ * X is borrowed, the result is owned by the caller (inscy_result_free). */
+int inscy_run(const float *X, int64_t n, int32_t d, const inscy_params *params, int32_t mode,
+              inscy_result **out);
+
+/* Same with X already in device memory (row-major float32 [n, d]) on the
+ * current HIP device; `stream` is a hipStream_t or NULL for the library's own
+ * stream.  This is the form whose time excludes the host-to-device copy. */
+int inscy_run_device(const float *d_X, int64_t n, int32_t d, const inscy_params *params, int32_t mode,
+                     void *stream, inscy_result **out);
+
+/* Number of subspaces in the result, in the reference's output order
+ * (ascending dimension bitmask, util.cu:633-653 vec_cmp). */
+int32_t inscy_result_num_subspaces(const inscy_result *res);
+
+/* Dimensions of subspace i (ascending) into dims_out (capacity d); returns their count. */
+int32_t inscy_result_subspace(const inscy_result *res, int32_t i, int32_t *dims_out);
+
+/* Canonical labels of subspace i: n int32 owned by the result. */
+const int32_t *inscy_result_labels(const inscy_result *res, int32_t i);
+
+int64_t inscy_result_num_points(const inscy_result *res);
+
+int inscy_result_stats(const inscy_result *res, inscy_stats *out);
+
+void inscy_result_free(inscy_result *res);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *inscy_last_error(void);
+
+/* ---- kernel-level entry points (device pointers; used by parity tests) ---- */
+
+/* Discretisation of X into the c-grid and the ceil(1/eps)-grid plus the
+ * s-connection marks (reference: ScyTreeNode.cu:100-102 get_cell_no and
+ * :34-49 construct_s_connection).  Outputs, all dimension-major [d][n]:
+ * cell_c (uint8), cell_e (uint16), near_mask[n] (bit k set when point p lies
+ * within eps of the upper edge of its c-grid cell in dimension k, d <= 64). */
+int inscy_discretise(const float *d_X, int64_t n, int32_t d, int32_t cells, float eps, uint8_t *d_cell_c,
+                     uint16_t *d_cell_e, uint64_t *d_near_mask);
+
+/* epsilon-neighbour counts |N_S(p)| for query points (reference:
+ * ClusteringCpu.cu:52-81 neighborhood with the candidate rule of
+ * ScyTreeNode.cu:896-946): q != p, |cell_e difference| <= 1 in every dimension
+ * of S, and the float distance in S (S order, no fused multiply-add) <= eps.
+ * dims: k ascending dimension ids; queries: nq point ids; out: nq counts. */
+int inscy_neighbour_counts(const float *d_X, int64_t n, int32_t d, float eps, const int32_t *dims, int32_t k,
+                           const int32_t *d_queries, int64_t nq, int32_t *d_out);
+
+/* Dense components of subspace S over all points (reference:
+ * ClusteringCpu.cu:317-389 BFS expansion): comp[p] = smallest point id of p's
+ * component in the dense-dense epsilon graph, or -1 when p is not dense with
+ * the |S|-dimensional threshold. */
+int inscy_dense_components(const float *d_X, int64_t n, int32_t d, const inscy_params *params,
+                           const int32_t *dims, int32_t k, int32_t *d_comp);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* INSCY_H */

@@ -1,0 +1,187 @@
+"""Parity of the HIP engine with the oracle (CPU restatement of the reference's
+CPU INSCY) — exact for every integer output: cells, neighbour counts,
+components, subspace sets, canonical labels, per-tree decisions."""
+
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from gpu_inscy_amd import _lib, datagen
+from gpu_inscy_amd import inscy as INSCY
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    _lib.lib()
+
+
+def host_cells(X, cells, eps):
+    cs = np.float32(1.0) / np.float32(cells)
+    ec = int(np.ceil(1.0 / np.float64(np.float32(eps))))
+    ecs = np.float32(1.0) / np.float32(ec)
+    c = np.minimum((X / cs).astype(np.int32), cells - 1)
+    e = np.minimum((X / ecs).astype(np.int32), ec - 1)
+    upper = (c + 1).astype(np.float32) * cs
+    bound = (upper - np.float32(eps)).astype(np.float32)
+    near = X >= bound
+    return c, e, near
+
+
+@pytest.mark.parametrize("seed,cells,eps", [(0, 4, 0.01), (1, 6, 0.015), (2, 8, 0.02), (3, 3, 0.1), (4, 255, 0.003)])
+def test_discretise_bit_exact(seed, cells, eps):
+    say("discretise", seed, cells, eps)
+    rng = np.random.default_rng(seed)
+    X = rng.random((3000, 7), dtype=np.float32)
+    X[:5] = 0.0
+    X[5:10] = 1.0
+    # values on exact cell edges
+    X[10:20, 0] = np.arange(10, dtype=np.float32) / np.float32(cells)
+    X = np.clip(X, 0, 1).astype(np.float32)
+    n, d = X.shape
+    dX = dev(X)
+    cc = torch.zeros((d, n), dtype=torch.uint8, device="cuda")
+    ce = torch.zeros((d, n), dtype=torch.int16, device="cuda")
+    nm = torch.zeros(n, dtype=torch.int64, device="cuda")
+    _lib.check(_lib.lib().inscy_discretise(dX.data_ptr(), n, d, cells, eps, cc.data_ptr(), ce.data_ptr(),
+                                           nm.data_ptr()))
+    torch.cuda.synchronize()
+    c, e, near = host_cells(X, cells, eps)
+    assert np.array_equal(cc.cpu().numpy().T.astype(np.int32), c)
+    assert np.array_equal(ce.cpu().numpy().T.astype(np.int32) & 0xFFFF, e)
+    bits = nm.cpu().numpy().view(np.uint64)
+    want = np.zeros(n, dtype=np.uint64)
+    for k in range(d):
+        want |= near[:, k].astype(np.uint64) << np.uint64(k)
+    assert np.array_equal(bits, want)
+
+
+def test_discretise_rejects_out_of_range():
+    X = np.full((10, 2), 0.5, dtype=np.float32)
+    X[3, 1] = 1.5
+    dX = dev(X)
+    cc = torch.zeros((2, 10), dtype=torch.uint8, device="cuda")
+    ce = torch.zeros((2, 10), dtype=torch.int16, device="cuda")
+    nm = torch.zeros(10, dtype=torch.int64, device="cuda")
+    rc = _lib.lib().inscy_discretise(dX.data_ptr(), 10, 2, 4, 0.1, cc.data_ptr(), ce.data_ptr(), nm.data_ptr())
+    assert rc == _lib.INSCY_ERR_ARGUMENT
+
+
+@pytest.mark.parametrize("seed,S,eps", [(0, [0], 0.01), (1, [1, 3], 0.015), (2, [0, 2, 4], 0.05),
+                                        (3, [0, 1, 2, 3, 4, 5], 0.1), (4, [5], 0.2)])
+def test_neighbour_counts_match_oracle(seed, S, eps):
+    say("counts", seed, S, eps)
+    w = datagen.small(seed, n=1200, d=6, n_clusters=3, eps=eps, sigma=0.04)
+    X = w.X
+    n, d = X.shape
+    rng = np.random.default_rng(seed)
+    qs = rng.choice(n, size=200, replace=False).astype(np.int32)
+    dims = (ctypes.c_int32 * len(S))(*S)
+    out = torch.zeros(len(qs), dtype=torch.int32, device="cuda")
+    dq = dev(qs)
+    _lib.check(_lib.lib().inscy_neighbour_counts(dev(X).data_ptr(), n, d, eps, dims, len(S), dq.data_ptr(),
+                                                 len(qs), out.data_ptr()))
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    want = np.array([len(O.neighbours(X, eps, S, int(q))) for q in qs])
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("seed,S", [(0, [0, 1]), (1, [2]), (2, [0, 2, 3])])
+def test_dense_components_match_bruteforce(seed, S):
+    say("components", seed, S)
+    w = datagen.small(seed, n=800, d=4, n_clusters=3, eps=0.05, num_obj=4, sigma=0.03)
+    X = w.X
+    n, d = X.shape
+    p = _lib.make_params(w.eps, 1.0, w.num_obj, 0, 1.0, w.cells, True, 0)
+    dims = (ctypes.c_int32 * len(S))(*S)
+    comp = torch.zeros(n, dtype=torch.int32, device="cuda")
+    _lib.check(_lib.lib().inscy_dense_components(dev(X).data_ptr(), n, d, ctypes.byref(p), dims, len(S),
+                                                 comp.data_ptr()))
+    torch.cuda.synchronize()
+    got = comp.cpu().numpy()
+    lists = [O.neighbours(X, w.eps, S, i) for i in range(n)]
+    thr = max(1.0 * O.thresholds(len(S), w.eps, n)["expden"], float(w.num_obj))
+    dense = np.array([np.float32(len(l)) >= np.float32(thr) for l in lists])
+    parent = np.arange(n)
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    for i in range(n):
+        if dense[i]:
+            for j in lists[i]:
+                if dense[j]:
+                    a, b = find(i), find(j)
+                    if a != b:
+                        parent[max(a, b)] = min(a, b)
+    want = np.array([find(i) if dense[i] else -1 for i in range(n)])
+    assert np.array_equal(got, want)
+
+
+def compare_with_oracle(w, mode=_lib.MODE_EXACT):
+    ref = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, 0, trace=True)
+    _lib.debug_trace(True)
+    try:
+        got = INSCY.run(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, 0, mode=mode)
+        trace = _lib.debug_trace_rows()
+    finally:
+        _lib.debug_trace(False)
+    assert got.subspaces == ref.subspaces
+    for s, a, b in zip(ref.subspaces, ref.labels, got.labels):
+        assert np.array_equal(O.canonical(a), b), f"labels differ in subspace {s}"
+    assert got.stats["trees"] == ref.trees
+    assert got.stats["calls"] == ref.calls
+    assert np.array_equal(trace, ref.trace)
+    return got
+
+
+@pytest.mark.parametrize("seed", list(range(12)))
+def test_end_to_end_small_cases(seed):
+    rng = np.random.default_rng(seed + 500)
+    n = int(rng.integers(100, 700))
+    d = int(rng.integers(2, 6))
+    cells = int(rng.integers(2, 7))
+    eps = float(rng.choice([0.02, 0.05, 0.1]))
+    w = datagen.small(seed, n=n, d=d, n_clusters=int(rng.integers(1, 4)), eps=eps, cells=cells,
+                      num_obj=int(rng.integers(2, 6)), min_size=int(rng.integers(2, n // 10 + 3)),
+                      sigma=float(rng.choice([0.02, 0.05])))
+    say("e2e small", seed, n, d, cells, eps)
+    compare_with_oracle(w)
+
+
+def test_end_to_end_config0():
+    w = datagen.config(0)
+    say("e2e config0", w.name)
+    got = compare_with_oracle(w)
+    assert len(got.subspaces) > 0
+
+
+def test_drop_in_lists_and_cuda_input():
+    w = datagen.config(0)
+    say("drop-in", w.name)
+    subs, cls = INSCY.CPU(torch.from_numpy(w.X), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0)
+    subs2, cls2 = INSCY.GPU(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0)
+    assert subs == subs2 and cls == cls2
+    assert len(cls[0]) == w.n
+    assert INSCY.count_number_of_clusters(subs, cls) >= len(subs)
