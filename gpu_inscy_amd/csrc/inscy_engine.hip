@@ -138,26 +138,37 @@ struct Slabs {
 
 struct TreeRef {
     uint64_t smask;
-    const int *P;      // device
+    const int *P;      // device point ids
     const int *nP;     // device count
-    int nP_max;        // host bound
-    const int *spx;
+    int nP_max;        // host bound (exact where the host knows it)
+    const int *spx;    // device path origins
     const uint64_t *spm;
     const int *nsp;
     int nsp_max;
 };
 
+// What the algorithm needs about one subspace S, computed on first use:
+// neighbour counts of every point (the prune reads them for its region), then,
+// once a tree of S survives its prune, the dense components of S over the
+// whole data set and its row in the device result table.
 struct Subspace {
-    int *cnt = nullptr;       // [n] neighbour counts, -1 = not computed yet
-    int *comp = nullptr;      // [n] dense component root or -1
-    int *csize = nullptr;     // [n] component sizes indexed by root
-    uint8_t *inres = nullptr; // [n] component kept in the result
-    bool comps = false;
+    int *cnt = nullptr;        // [n] |N_S(p)| capped, -1 = not counted yet
+    int *comp = nullptr;       // [n] component root (smallest id) or -1
+    int *csize = nullptr;      // [n] component sizes indexed by root
+    uint8_t *inres = nullptr;  // [n] component kept in the result of S
+    int entry = -1;            // row in the device result table
 };
 
 class Engine {
 public:
     Engine(const inscy_params &p, int n, int d, hipStream_t st) : prm(p), n(n), d(d), st(st) {}
+
+    ~Engine() {
+        for (void *p : {(void *) Xt, (void *) cellC, (void *) cellE, (void *) nearm, (void *) small, (void *) stamp,
+                        (void *) qmark, (void *) entries, (void *) scratch})
+            if (p) (void) hipFree(p);
+        if (pin) (void) hipHostFree(pin);
+    }
 
     // ---------------------------------------------------------- set-up
     void setup(const float *d_X) {
@@ -175,39 +186,54 @@ public:
         HIP_OK(hipMalloc(&cellC, nd));
         HIP_OK(hipMalloc(&cellE, nd * sizeof(uint16_t)));
         HIP_OK(hipMalloc(&nearm, (size_t) n * sizeof(uint64_t)));
-        HIP_OK(hipMalloc(&sid1, nd * sizeof(int)));
-        HIP_OK(hipMalloc(&off1, (size_t) d * (ecells + 1) * sizeof(int)));
-        HIP_OK(hipMalloc(&small, 64 * sizeof(unsigned long long)));
+        HIP_OK(hipMalloc(&small, kSmallWords * sizeof(unsigned long long)));
         HIP_OK(hipMalloc(&stamp, (size_t) n * sizeof(int)));
+        HIP_OK(hipMalloc(&qmark, (size_t) n * sizeof(int)));
         HIP_OK(hipHostMalloc(&pin, pin_bytes()));
-        // arena: per recursion depth the child lists (points, paths, survivors)
-        size_t per_level = (size_t) n * (4 * sizeof(int)) + (size_t) n * 2 * (sizeof(int) + sizeof(uint64_t)) * 2;
-        arena.init(per_level * (size_t) (d + 2) + table_bytes_max() + (size_t) n * 16 + (64u << 20));
+        grow_entries(1 << 12);
+        // grid index size: C^j cells with j <= 3 and C^j <= max(8n, 4096)
+        grid_cap = std::max<long long>(8LL * n, 4096);
+        grid_cap = std::max<long long>(std::min<long long>(grid_cap, 1LL << 24), ecells);
+        scan_bytes = scan_temp_bytes((int) grid_cap + 1);
+        // arena: per recursion level the partitions of all children of one tree
+        // (points, weak-dense points, paths) plus one child's pruned paths,
+        // and the scratch of one subspace grid pass
+        size_t per_level = (size_t) 24 * d * n + (size_t) 64 * n;
+        size_t pass = (size_t) n * (32 + (size_t) d * 6) + (size_t) (grid_cap + 1) * 8 + scan_bytes + (1u << 20);
+        size_t want = per_level * (size_t) (d + 2) + table_bytes_max() + pass + (size_t) n * 16 + (64u << 20);
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) want = std::min(want, free_b / 3);
+        arena.init(want);
 
-        int *hist = arena.take<int>((size_t) d * (ecells + 1));
-        HIP_OK(launch_fill_i32(reinterpret_cast<int *>(small), 0, 128, st));
+        int *hist = arena.take<int>((size_t) d * ecells);
+        HIP_OK(launch_fill_i32(reinterpret_cast<int *>(small), 0, 2 * kSmallWords, st));
         HIP_OK(launch_discretise(d_X, n, d, cells, prm.neighborhood_size, Xt, cellC, cellE, nearm,
                                  reinterpret_cast<int *>(small), st));
-        HIP_OK(launch_build_index(cellE, n, d, ecells, hist, off1, sid1, st));
+        HIP_OK(launch_hist1(cellE, n, d, ecells, hist, st));
         HIP_OK(launch_fill_i32(stamp, -1, n, st));
+        HIP_OK(launch_fill_i32(qmark, -1, n, st));
+        std::vector<int> h((size_t) d * ecells);
         HIP_OK(hipMemcpyAsync(pin, small, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(h.data(), hist, h.size() * sizeof(int), hipMemcpyDeviceToHost, st));
         sync();
         if (reinterpret_cast<int *>(pin)[0] != 0)
             throw Failure(INSCY_ERR_ARGUMENT, "X must be finite with every value in [0, 1] (normalize first)");
+        // selectivity of each dimension as an index: expected candidates of a
+        // query in the three-cell slab, sum_c h_c (h_{c-1} + h_c + h_{c+1})
+        score.assign(d, 0.0);
+        for (int k = 0; k < d; ++k) {
+            const int *hk = h.data() + (size_t) k * ecells;
+            for (int c = 0; c < ecells; ++c) {
+                double nb = hk[c] + (c > 0 ? hk[c - 1] : 0) + (c + 1 < ecells ? hk[c + 1] : 0);
+                score[k] += (double) hk[c] * nb;
+            }
+        }
         arena.release(0);
-        D = DevData{Xt, cellC, cellE, sid1, off1, n, d, cells, ecells};
-    }
-
-    ~Engine() {
-        for (void *p : {(void *) Xt, (void *) cellC, (void *) cellE, (void *) nearm, (void *) sid1, (void *) off1,
-                        (void *) small, (void *) stamp, (void *) ptrtab})
-            if (p) (void) hipFree(p);
-        if (pin) (void) hipHostFree(pin);
+        D = DevData{Xt, cellC, cellE, n, d, cells, ecells};
     }
 
     // ---------------------------------------------------------- run
     void run(inscy_result &out) {
-        // root tree: all points, every near-edge chain
         int *P = arena.take<int>(n);
         int *nP = arena.take<int>(1);
         int *spx = arena.take<int>(n);
@@ -215,27 +241,85 @@ public:
         int *nsp = arena.take<int>(1);
         HIP_OK(launch_iota(P, n, st));
         HIP_OK(launch_fill_i32(nP, n, 1, st));
-        HIP_OK(launch_fill_i32(nsp, 0, 1, st));
         HIP_OK(launch_root_paths(nearm, n, spx, spm, nsp, st));
         TreeRef root{0ull, P, nP, n, spx, spm, nsp, n};
         recurse(root, 0);
         emit(out);
     }
 
+    // dense components of one subspace (kernel-level entry point)
+    const Subspace &subspace(uint64_t smask) { return ensure_components(smask); }
+
+    // exact |N_S(p)| of every point, without the threshold cap (kernel-level entry point)
+    const int *exact_counts(uint64_t smask) {
+        Subspace &s = sub_of(smask);
+        SubspaceArg S = arg_of(smask);
+        GridArg G = grid_of(S);
+        size_t mk = arena.mark();
+        GridBuffers B = grid_buffers(S, G);
+        HIP_OK(launch_subspace_counts(D, S, G, s_star, INT_MAX, B, nullptr, 0, s.cnt, nullptr, st));
+        arena.release(mk);
+        return s.cnt;
+    }
+
     inscy_stats stats{};
+
+    struct TraceRow {
+        long long mask;
+        int lo, hi, pre, post, pass, redundant;
+    };
+    bool trace_on = false;
+    std::vector<TraceRow> trace;
 
 private:
     // ---------------------------------------------------------- helpers
-    size_t pin_bytes() const { return std::max<size_t>((size_t) n * 8 + (1u << 16), 1u << 20); }
-    size_t table_slots(int nW, int bits) const {
+    // `small` device words: [0] input check flag, [16..24) path-prune scratch,
+    // [kStatAt..) 64 slot pairs (16 words apart) of neighbour-count statistics
+    static constexpr int kStatAt = 32;
+    static constexpr int kSmallWords = kStatAt + 16 * 64;
+    size_t pin_bytes() const {
+        return std::max<size_t>((size_t) n * 8 + (size_t) 2 * d * 256 * 8 + (1u << 16), 1u << 20);
+    }
+    static size_t table_slots(int nW, int bits) {
         size_t want = (size_t) std::max(1, nW) * (size_t) std::max(1, bits) * 2;
         size_t s = 1024;
         while (s < want) s <<= 1;
         return s;
     }
     size_t table_bytes_max() const { return table_slots(n, d) * sizeof(uint64_t); }
+    int *pin_i(size_t off = 0) { return reinterpret_cast<int *>(pin) + off; }
 
     void sync() { HIP_OK(hipStreamSynchronize(st)); }
+
+    // Device result table and the scratch of its kernels, grown on demand.
+    void grow_entries(int want) {
+        if (want <= ecap) return;
+        int cap = std::max(want, 2 * ecap);
+        Entry *ne = nullptr;
+        int *ns = nullptr;
+        HIP_OK(hipMalloc(&ne, (size_t) cap * sizeof(Entry)));
+        // scratch: nE, nlist, keep, pad | list[cap] | mn[cap] | keepf[n], drop[n], fresh[n] (bytes)
+        HIP_OK(hipMalloc(&ns, (size_t) (8 + 2 * cap) * sizeof(int) + (size_t) 3 * n + 256));
+        if (entries) {
+            sync();
+            HIP_OK(hipMemcpy(ne, entries, (size_t) nentries * sizeof(Entry), hipMemcpyDeviceToDevice));
+            HIP_OK(hipFree(entries));
+            HIP_OK(hipFree(scratch));
+        }
+        entries = ne;
+        scratch = ns;
+        ecap = cap;
+        d_nE = scratch;
+        d_nlist = scratch + 1;
+        d_keep = scratch + 2;
+        d_list = scratch + 8;
+        d_mn = scratch + 8 + cap;
+        keepf = reinterpret_cast<uint8_t *>(scratch + 8 + 2 * cap);
+        dropf = keepf + n;
+        freshf = dropf + n;
+        HIP_OK(launch_fill_i32(d_mn, INT_MAX, cap, st));
+        HIP_OK(launch_fill_i32(d_nE, nentries, 1, st));
+    }
 
     SubspaceArg arg_of(uint64_t smask) const {
         SubspaceArg S{};
@@ -244,6 +328,49 @@ private:
             if ((smask >> k) & 1ull) S.dims[S.k++] = k;
         return S;
     }
+
+    GridArg grid_of(const SubspaceArg &S) const {
+        GridArg G{};
+        std::vector<int> ts(S.k);
+        for (int t = 0; t < S.k; ++t) ts[t] = t;
+        std::stable_sort(ts.begin(), ts.end(), [&](int a, int b) { return score[S.dims[a]] < score[S.dims[b]]; });
+        G.C = ecells;
+        G.j = 1;
+        long long cellsj = ecells;
+        while (G.j < 3 && G.j < S.k && cellsj * ecells <= grid_cap) {
+            cellsj *= ecells;
+            G.j++;
+        }
+        G.checkmask = S.k >= 64 ? ~0ull : ((1ull << S.k) - 1ull);
+        for (int a = 0; a < G.j; ++a) {
+            G.tI[a] = ts[a];
+            G.checkmask &= ~(1ull << ts[a]);
+        }
+        return G;
+    }
+
+    GridBuffers grid_buffers(const SubspaceArg &S, const GridArg &G) {
+        long long ncell = 1;
+        for (int a = 0; a < G.j; ++a) ncell *= G.C;
+        GridBuffers B{};
+        B.keys = arena.take<int>(n);
+        B.rank = arena.take<int>(n);
+        B.hist = arena.take<int>(ncell + 1);
+        B.off = arena.take<int>(ncell + 1);
+        B.order = arena.take<int>(n);
+        B.Ys = arena.take<float>((size_t) S.k * n);
+        B.Es = arena.take<uint16_t>((size_t) S.k * n);
+        B.dsorted = arena.take<uint8_t>(n);
+        B.par = arena.take<int>(n);
+        B.giant = arena.take<int>(1);
+        B.scan_temp = arena.take<char>(scan_bytes);
+        B.scan_bytes = scan_bytes;
+        return B;
+    }
+
+    // counts are compared only with the weak threshold (prune) and the dense
+    // threshold of |S| (clustering), so they are needed up to the larger one
+    int cap_of(int k) const { return std::max(std::max(weak_thr, dense_thr[k]), 1); }
 
     Subspace &sub_of(uint64_t smask) {
         Subspace &s = subs[smask];
@@ -254,266 +381,240 @@ private:
         return s;
     }
 
-    void count(uint64_t smask, Subspace &s, const int *queries, const int *nq, int nq_max) {
+    // |N_S(p)| of the points stamped with markid that are not counted yet.
+    Subspace &ensure_counts(uint64_t smask, int markid) {
+        Subspace &s = sub_of(smask);
+        if (s.entry >= 0) return s;  // all points counted by the components pass
         SubspaceArg S = arg_of(smask);
-        HIP_OK(launch_counts(D, S, queries, nq, nq_max, s_star, s.cnt, small + 8, st));
+        GridArg G = grid_of(S);
+        size_t mk = arena.mark();
+        GridBuffers B = grid_buffers(S, G);
+        HIP_OK(launch_subspace_counts(D, S, G, s_star, cap_of(S.k), B, qmark, markid, s.cnt, small + kStatAt, st));
+        arena.release(mk);
+        return s;
     }
 
-    Subspace &components(uint64_t smask) {
+    // Dense components of S (counting every point not yet counted) and its
+    // result-table row, once per subspace.
+    Subspace &ensure_components(uint64_t smask) {
         Subspace &s = sub_of(smask);
-        if (s.comps) return s;
-        size_t mk = arena.mark();
-        int *q = arena.take<int>(n);
-        int *nq = arena.take<int>(1);
-        HIP_OK(launch_fill_i32(nq, 0, 1, st));
-        HIP_OK(launch_all_queries(s.cnt, n, q, nq, st));
-        count(smask, s, q, nq, n);
+        if (s.entry >= 0) return s;
+        SubspaceArg S = arg_of(smask);
+        GridArg G = grid_of(S);
         s.comp = slabs.take<int>(n);
         s.csize = slabs.take<int>(n);
-        int *parent = arena.take<int>(n);
-        SubspaceArg S = arg_of(smask);
-        HIP_OK(launch_components(D, S, s_star, s.cnt, dense_thr[S.k], parent, s.comp, s.csize, st));
+        s.inres = slabs.take<uint8_t>(n);
+        int *mem = slabs.take<int>(n);
+        // a subspace S' = S minus one dimension whose counts are complete bounds
+        // the counts of S from above (exact where below its own cap)
+        const int *bound = nullptr;
+        int lo = std::min(weak_thr, dense_thr[S.k]);
+        for (int t = 0; t < S.k && !bound; ++t) {
+            uint64_t sub = smask & ~(1ull << S.dims[t]);
+            auto it = subs.find(sub);
+            if (sub && it != subs.end() && it->second.entry >= 0) {
+                bound = it->second.cnt;
+                lo = std::min(lo, cap_of(S.k - 1));
+            }
+        }
+        size_t mk = arena.mark();
+        GridBuffers B = grid_buffers(S, G);
+        HIP_OK(launch_subspace_components(D, S, G, s_star, cap_of(S.k), lo, dense_thr[S.k], B, bound, s.cnt, s.comp,
+                                          s.csize, small + kStatAt, st));
         arena.release(mk);
-        s.comps = true;
+        HIP_OK(launch_fill_u8(s.inres, 0, n, st));
+        grow_entries(nentries + 1);
+        Entry e{smask, s.comp, s.csize, s.inres, mem, 0, 0};
+        s.entry = nentries++;
+        HIP_OK(launch_set_entry(entries, s.entry, e, d_nE, nentries, st));
         stats.components++;
         return s;
     }
 
-    // Result entries H with S ⊆ H (subspace_of, util.cu:619-631), as device
-    // pointer tables; returns their count.
-    int superspaces(uint64_t smask, std::vector<uint64_t> &masks) {
-        masks.clear();
-        for (auto &kv : result)
-            if ((kv.first & smask) == smask) masks.push_back(kv.first);
-        int nH = (int) masks.size();
-        if (!nH) return 0;
-        if (nH > ptrcap) {
-            if (ptrtab) HIP_OK(hipFree(ptrtab));
-            ptrcap = std::max(nH, 2 * ptrcap);
-            HIP_OK(hipMalloc(&ptrtab, (size_t) 3 * ptrcap * sizeof(void *)));
-        }
-        void **h = reinterpret_cast<void **>(pin);
-        for (int i = 0; i < nH; ++i) {
-            Subspace &s = subs[masks[i]];
-            h[i] = s.comp;
-            h[nH + i] = s.csize;
-            h[2 * nH + i] = s.inres;
-        }
-        HIP_OK(hipMemcpyAsync(ptrtab, h, (size_t) 3 * nH * sizeof(void *), hipMemcpyHostToDevice, st));
-        return nH;
-    }
-
     // ---------------------------------------------------------- recursion
+    struct Group {
+        int lo, hi;
+    };
+
     void recurse(const TreeRef &T, int first) {
         stats.calls++;
+        if (first >= d) return;
+        const int nd = d - first;
+        const size_t lvl_mark = arena.mark();
+
+        // restrict + mergeWithNeighbors for every dimension m >= first: the
+        // s-connected cells, then the cell groups (ScyTreeNode.cu:837-851)
+        int *flags = arena.take<int>((size_t) nd * cells);
+        HIP_OK(launch_fill_i32(flags, 0, (size_t) nd * cells, st));
+        HIP_OK(launch_flags_all(T.spx, T.spm, T.nsp, T.nsp_max, cellC, n, first, cells, flags, st));
+        // the subspaces of the children are counted on this tree's points while
+        // the flags travel
+        const int markid = ++mark_id;
+        HIP_OK(launch_mark(T.P, T.nP, T.nP_max, qmark, markid, st));
+        for (int m = first; m < d; ++m) ensure_counts(T.smask | (1ull << m), markid);
+        HIP_OK(hipMemcpyAsync(pin, flags, (size_t) nd * cells * sizeof(int), hipMemcpyDeviceToHost, st));
+        sync();
+        std::vector<std::vector<Group>> groups(nd);
+        std::vector<int> gid((size_t) nd * cells);
         for (int m = first; m < d; ++m) {
-            const uint64_t smask = T.smask | (1ull << m);
-            const uint8_t *cellC_m = cellC + (size_t) m * n;
-            size_t dim_mark = arena.mark();
-
-            // restrict + mergeWithNeighbors: s-connected cells of m
-            int *flags = arena.take<int>(cells);
-            HIP_OK(launch_fill_i32(flags, 0, cells, st));
-            HIP_OK(launch_flags(T.spx, T.spm, T.nsp, T.nsp_max, cellC_m, m, flags, st));
-            HIP_OK(hipMemcpyAsync(pin, flags, cells * sizeof(int), hipMemcpyDeviceToHost, st));
-            sync();
-            std::vector<int> flag(reinterpret_cast<int *>(pin), reinterpret_cast<int *>(pin) + cells);
-
-            uint8_t *lab = arena.take<uint8_t>(n);  // components labelled in this dimension loop
-            bool any_seed = false;
-            HIP_OK(launch_fill_u8(lab, 0, n, st));
-
+            const int *f = pin_i((size_t) (m - first) * cells);
+            auto &gs = groups[m - first];
             for (int c = 0; c < cells; ++c) {
                 int lo = c;
-                while (flag[c] && c < cells - 1) ++c;
-                int hi = c;
-                size_t grp_mark = arena.mark();
-                int *cnt2 = arena.take<int>(4);
-                int *Pr = arena.take<int>(std::max(T.nP_max, 1));
-                int *spx = arena.take<int>(std::max(T.nsp_max, 1));
-                uint64_t *spm = arena.take<uint64_t>(std::max(T.nsp_max, 1));
-                int *W = arena.take<int>(std::max(T.nP_max, 1));
-                HIP_OK(launch_fill_i32(cnt2, 0, 4, st));
-                HIP_OK(launch_filter(T.P, T.nP, T.nP_max, T.spx, T.spm, T.nsp, T.nsp_max, cellC_m, m, lo, hi, Pr,
-                                     spx, spm, cnt2, st));
-                Subspace &sub = sub_of(smask);
-                count(smask, sub, Pr, cnt2, T.nP_max);
-                const int tid = ++tree_id;
-                HIP_OK(launch_weak(Pr, cnt2, T.nP_max, sub.cnt, weak_thr, W, cnt2 + 2, stamp, tid, st));
-                HIP_OK(hipMemcpyAsync(pin, cnt2, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
-                sync();
-                const int nPr = reinterpret_cast<int *>(pin)[0];
-                const int nSPr = reinterpret_cast<int *>(pin)[1];
-                const int nW = reinterpret_cast<int *>(pin)[2];
+                while (f[c] && c < cells - 1) ++c;
+                for (int x = lo; x <= c; ++x) gid[(size_t) (m - first) * cells + x] = (int) gs.size();
+                gs.push_back({lo, c});
+            }
+        }
+        int *d_gid = arena.take<int>(gid.size());
+        std::copy(gid.begin(), gid.end(), pin_i());
+        HIP_OK(hipMemcpyAsync(d_gid, pin, gid.size() * sizeof(int), hipMemcpyHostToDevice, st));
+
+        // partition points, weak-dense points and paths by group, all dimensions
+        struct Part {
+            int *cnt, *P, *W, *x;
+            uint64_t *mk;
+        };
+        std::vector<Part> parts(nd);
+        for (int m = first; m < d; ++m) {
+            const int G = (int) groups[m - first].size();
+            Part &pt = parts[m - first];
+            pt.cnt = arena.take<int>((size_t) 6 * G);
+            pt.P = arena.take<int>(std::max(T.nP_max, 1));
+            pt.W = arena.take<int>(std::max(T.nP_max, 1));
+            pt.x = arena.take<int>(std::max(T.nsp_max, 1));
+            pt.mk = arena.take<uint64_t>(std::max(T.nsp_max, 1));
+            const Subspace &sub = subs[T.smask | (1ull << m)];
+            HIP_OK(launch_partition(T.P, T.nP, T.nP_max, T.spx, T.spm, T.nsp, T.nsp_max, cellC + (size_t) m * n,
+                                    d_gid + (size_t) (m - first) * cells, G, sub.cnt, weak_thr, m, pt.cnt, pt.P, pt.W,
+                                    pt.x, pt.mk, st));
+        }
+        std::vector<size_t> cnt_at(nd);
+        size_t tot = 0;
+        for (int m = first; m < d; ++m) {
+            cnt_at[m - first] = tot;
+            tot += 3 * groups[m - first].size();
+        }
+        int *hostc = pin_i((size_t) nd * cells + 64);
+        for (int m = first; m < d; ++m)
+            HIP_OK(hipMemcpyAsync(hostc + cnt_at[m - first], parts[m - first].cnt,
+                                  3 * groups[m - first].size() * sizeof(int), hipMemcpyDeviceToHost, st));
+        sync();
+        std::vector<int> counts(hostc, hostc + tot);
+
+        for (int m = first; m < d; ++m) {
+            const uint64_t smask = T.smask | (1ull << m);
+            const auto &gs = groups[m - first];
+            const int G = (int) gs.size();
+            const int *c3 = counts.data() + cnt_at[m - first];
+            const Part &pt = parts[m - first];
+            const size_t dim_mark = arena.mark();
+            uint8_t *lab = nullptr;
+            int offW = 0, offS = 0;
+            for (int g = 0; g < G; ++g) {
+                const int nPr = c3[g], nW = c3[G + g], nSP = c3[2 * G + g];
+                const int *W = pt.W + offW;
+                const int *spx = pt.x + offS;
+                const uint64_t *spm = pt.mk + offS;
+                offW += nW;
+                offS += nSP;
                 stats.trees++;
                 const bool pass = nW >= prm.min_size;
-                if (trace_on) trace.push_back({(long long) smask, lo, hi, nPr, nW, pass ? 1 : 0, -1});
-                if (!pass) {
-                    arena.release(grp_mark);
-                    continue;
-                }
+                size_t at = trace.size();
+                if (trace_on) trace.push_back({(long long) smask, gs[g].lo, gs[g].hi, nPr, nW, pass ? 1 : 0, -1});
+                if (!pass) continue;
+                const size_t child_mark = arena.mark();
+                const int tid = ++tree_id;
                 // s-connection paths surviving the prune
-                int *spx2 = arena.take<int>(std::max(nSPr, 1));
-                uint64_t *spm2 = arena.take<uint64_t>(std::max(nSPr, 1));
-                int *nsp2 = cnt2 + 3;
-                if (nSPr > 0) {
+                int *spx2 = arena.take<int>(std::max(nSP, 1));
+                uint64_t *spm2 = arena.take<uint64_t>(std::max(nSP, 1));
+                int *nsp2 = arena.take<int>(1);
+                {
                     size_t tmark = arena.mark();
-                    int bits = 0;
-                    for (int k = m + 1; k < d; ++k) ++bits;
-                    size_t slots = table_slots(nW, bits);
-                    uint64_t *table = arena.take<uint64_t>(slots);
-                    HIP_OK(launch_path_prune(D, smask, W, cnt2 + 2, nW, spx, spm, cnt2 + 1, nSPr, stamp, tid,
-                                             small + 16, table, (uint64_t) (slots - 1), spx2, spm2, nsp2, st));
-                    arena.release(tmark);
+                    size_t slots = table_slots(nW, d - 1 - m);
+                    uint64_t *table = arena.take<uint64_t>(nSP > 0 ? slots : 1);
+                    HIP_OK(launch_path_prune(D, smask, W, pt.cnt + G + g, nW, spx, spm, pt.cnt + 2 * G + g, nSP,
+                                             stamp, tid, small + 16, table, (uint64_t) (slots - 1), spx2, spm2, nsp2,
+                                             st));
+                    arena.release(tmark);  // stream order keeps the table alive for the queued kernels
                 }
-                TreeRef child{smask, W, cnt2 + 2, nW, spx2, spm2, nsp2, nSPr};
-                size_t at = trace.size() - 1;
+                // components of S now: the children's counts are bounded by them
+                Subspace &sub = ensure_components(smask);
+                TreeRef child{smask, W, pt.cnt + G + g, nW, spx2, spm2, nsp2, nSP};
                 recurse(child, m + 1);
-                bool keep = prune_redundancy(smask, W, cnt2 + 2, nW);
-                if (trace_on) trace[at].redundant = keep ? 0 : 1;
-                if (keep) {
-                    Subspace &sc = components(smask);
-                    HIP_OK(launch_seed(W, cnt2 + 2, nW, sc.comp, lab, st));
-                    any_seed = true;
+                // pruneRedundancy then cluster, on the device in stream order
+                const int *nW_dev = pt.cnt + G + g;
+                if (!lab) {
+                    arena.release(child_mark);
+                    lab = arena.take<uint8_t>(n);  // survives until this dimension's join
+                    HIP_OK(launch_fill_u8(lab, 0, n, st));
+                    HIP_OK(launch_redundancy_seed(W, nW_dev, nW, stamp, ++tree_id, entries, d_nE, smask, sub.comp,
+                                                  d_list, d_nlist, d_mn, prm.r, d_keep, lab, st));
+                } else {
+                    HIP_OK(launch_redundancy_seed(W, nW_dev, nW, stamp, ++tree_id, entries, d_nE, smask, sub.comp,
+                                                  d_list, d_nlist, d_mn, prm.r, d_keep, lab, st));
+                    arena.release(child_mark);
                 }
-                arena.release(grp_mark);
+                if (trace_on) {
+                    HIP_OK(hipMemcpyAsync(pin, d_keep, sizeof(int), hipMemcpyDeviceToHost, st));
+                    sync();
+                    trace[at].redundant = pin_i()[0] ? 0 : 1;
+                }
             }
-            if (any_seed) join(smask, lab);
+            if (lab) join(smask, lab);
             arena.release(dim_mark);
         }
+        arena.release(lvl_mark);
     }
 
-    // pruneRedundancy (ScyTreeNode.cu:669-734)
-    bool prune_redundancy(uint64_t smask, const int *W, const int *nW_dev, int nW) {
-        std::vector<uint64_t> masks;
-        int nH = superspaces(smask, masks);
-        if (!nH) return true;
-        size_t mk = arena.mark();
-        int *mn = arena.take<int>(nH);
-        HIP_OK(launch_fill_i32(mn, INT_MAX, nH, st));
-        HIP_OK(launch_redundancy(W, nW_dev, nW, (int const *const *) ptrtab, (int const *const *) (ptrtab + nH),
-                                 (uint8_t const *const *) (ptrtab + 2 * nH), nH, mn, st));
-        HIP_OK(hipMemcpyAsync(pin, mn, nH * sizeof(int), hipMemcpyDeviceToHost, st));
-        sync();
-        arena.release(mk);
-        int max_min = 0;
-        for (int h = 0; h < nH; ++h) {
-            int v = reinterpret_cast<int *>(pin)[h];
-            if (v != INT_MAX && v > max_min) max_min = v;
-        }
-        if (max_min == 0) return true;
-        float lhs = (float) nW * prm.r;
-        return (double) lhs > (double) max_min;
-    }
-
-    // join (util.cu:655-743)
+    // join (util.cu:655-743), on the device
     void join(uint64_t smask, const uint8_t *lab) {
-        Subspace &s = components(smask);
-        size_t mk = arena.mark();
-        int *list = arena.take<int>((size_t) 2 * n);
-        int *nl = arena.take<int>(1);
-        HIP_OK(launch_fill_i32(nl, 0, 1, st));
-        HIP_OK(launch_collect(lab, n, s.csize, list, nl, st));
-        HIP_OK(hipMemcpyAsync(pin, nl, sizeof(int), hipMemcpyDeviceToHost, st));
-        sync();
-        int cnt = reinterpret_cast<int *>(pin)[0];
-        if (cnt == 0) {
-            arena.release(mk);
-            return;
-        }
-        HIP_OK(hipMemcpyAsync(pin, list, (size_t) 2 * cnt * sizeof(int), hipMemcpyDeviceToHost, st));
-        sync();
-        std::vector<int> keep;
-        for (int i = 0; i < cnt; ++i) {
-            int id = reinterpret_cast<int *>(pin)[2 * i], sz = reinterpret_cast<int *>(pin)[2 * i + 1];
-            if (sz >= prm.min_size) keep.push_back(id);
-        }
-        if (keep.empty()) {
-            arena.release(mk);
-            return;
-        }
-        std::vector<uint64_t> masks;
-        int nH = superspaces(smask, masks);
-        if (nH) {
-            uint8_t *kf = arena.take<uint8_t>(n);
-            uint8_t *drop = arena.take<uint8_t>(n);
-            int *ids = arena.take<int>(keep.size());
-            int *hp = reinterpret_cast<int *>(pin) + (size_t) 3 * nH * 2;  // after the pointer table
-            std::copy(keep.begin(), keep.end(), hp);
-            HIP_OK(hipMemcpyAsync(ids, hp, keep.size() * sizeof(int), hipMemcpyHostToDevice, st));
-            HIP_OK(launch_fill_u8(kf, 0, n, st));
-            HIP_OK(launch_fill_u8(drop, 0, n, st));
-            HIP_OK(launch_set_flags(kf, ids, (int) keep.size(), st));
-            HIP_OK(launch_join_check(s.comp, kf, s.csize, n, prm.r, (int const *const *) ptrtab,
-                                     (int const *const *) (ptrtab + nH), (uint8_t const *const *) (ptrtab + 2 * nH),
-                                     nH, drop, st));
-            sync();
-            uint8_t *hd = reinterpret_cast<uint8_t *>(pin);
-            HIP_OK(hipMemcpyAsync(hd, drop, n, hipMemcpyDeviceToHost, st));
-            sync();
-            std::vector<int> kept;
-            for (int id : keep)
-                if (!hd[id]) kept.push_back(id);
-            keep.swap(kept);
-        }
-        if (!keep.empty()) {
-            std::set<int> &entry = result[smask];
-            if (!s.inres) {
-                s.inres = slabs.take<uint8_t>(n);
-                HIP_OK(launch_fill_u8(s.inres, 0, n, st));
-            }
-            int *ids = arena.take<int>(keep.size());
-            int *hp = reinterpret_cast<int *>(pin);
-            std::copy(keep.begin(), keep.end(), hp);
-            HIP_OK(hipMemcpyAsync(ids, hp, keep.size() * sizeof(int), hipMemcpyHostToDevice, st));
-            HIP_OK(launch_set_flags(s.inres, ids, (int) keep.size(), st));
-            sync();
-            entry.insert(keep.begin(), keep.end());
-        }
-        arena.release(mk);
+        Subspace &s = ensure_components(smask);
+        HIP_OK(launch_join(lab, n, s.comp, s.csize, prm.min_size, prm.r, entries, d_nE, smask, s.entry, s.inres,
+                           d_list, d_nlist, keepf, dropf, freshf, st));
     }
 
     void emit(inscy_result &out) {
         out.n = n;
+        sync();
+        std::vector<Entry> tab(nentries);
+        if (nentries) HIP_OK(hipMemcpy(tab.data(), entries, (size_t) nentries * sizeof(Entry), hipMemcpyDeviceToHost));
+        std::vector<int> rows;
+        for (int e = 0; e < nentries; ++e)
+            if (tab[e].nkept > 0) rows.push_back(e);
+        // ascending bitmask == the reference's vec_cmp order (util.cu:633-653)
+        std::sort(rows.begin(), rows.end(), [&](int a, int b) { return tab[a].mask < tab[b].mask; });
         int *lab = arena.take<int>(n);
-        for (auto &kv : result) {
-            Subspace &s = subs[kv.first];
-            HIP_OK(launch_labels(s.comp, s.inres, n, lab, st));
+        for (int e : rows) {
+            HIP_OK(launch_labels(tab[e].comp, tab[e].inres, n, lab, st));
             HIP_OK(hipMemcpyAsync(pin, lab, (size_t) n * sizeof(int), hipMemcpyDeviceToHost, st));
             sync();
-            // canonical labels: clusters numbered in order of their smallest point id,
-            // which is the component root
-            std::vector<int> roots(kv.second.begin(), kv.second.end());
+            // canonical labels: clusters numbered in order of their smallest point
+            // id, which is the component root
+            const int *h = pin_i();
+            std::vector<int> roots;
+            for (int i = 0; i < n; ++i)
+                if (h[i] == i) roots.push_back(i);
             std::unordered_map<int, int> idx;
             for (size_t i = 0; i < roots.size(); ++i) idx[roots[i]] = (int) i;
             std::vector<int32_t> labels(n);
-            const int *h = reinterpret_cast<int *>(pin);
             for (int i = 0; i < n; ++i) labels[i] = h[i] < 0 ? -1 : idx[h[i]];
             std::vector<int32_t> dims;
             for (int k = 0; k < d; ++k)
-                if ((kv.first >> k) & 1ull) dims.push_back(k);
+                if ((tab[e].mask >> k) & 1ull) dims.push_back(k);
             out.subspaces.push_back(std::move(dims));
             out.labels.push_back(std::move(labels));
             stats.clusters += (int64_t) roots.size();
         }
         stats.subspaces = (int64_t) out.subspaces.size();
-        unsigned long long h2[2];
-        HIP_OK(hipMemcpyAsync(pin, small + 8, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-        sync();
-        memcpy(h2, pin, sizeof(h2));
-        stats.candidates = (int64_t) h2[0];
-        stats.queries = (int64_t) h2[1];
+        std::vector<unsigned long long> h2(16 * 64);
+        HIP_OK(hipMemcpy(h2.data(), small + kStatAt, h2.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (int b = 0; b < 64; ++b) {
+            stats.candidates += (int64_t) h2[16 * b];
+            stats.queries += (int64_t) h2[16 * b + 1];
+        }
     }
 
-public:
-    struct TraceRow {
-        long long mask;
-        int lo, hi, pre, post, pass, redundant;
-    };
-    bool trace_on = false;
-    std::vector<TraceRow> trace;
-
-private:
     inscy_params prm;
     int n, d;
     hipStream_t st;
@@ -525,17 +626,24 @@ private:
     uint8_t *cellC = nullptr;
     uint16_t *cellE = nullptr;
     uint64_t *nearm = nullptr;
-    int *sid1 = nullptr, *off1 = nullptr;
     unsigned long long *small = nullptr;
     int *stamp = nullptr;
+    int *qmark = nullptr;  // [n] stamp of the tree whose points are being counted
+    int mark_id = 0;
     void *pin = nullptr;
-    void **ptrtab = nullptr;
-    int ptrcap = 0;
+    std::vector<double> score;  // per-dimension index selectivity (lower is better)
+    long long grid_cap = 0;
+    size_t scan_bytes = 0;
     DevData D{};
     Arena arena;
     Slabs slabs;
     std::unordered_map<uint64_t, Subspace> subs;
-    std::map<uint64_t, std::set<int>> result;  // ascending bitmask == the reference's vec_cmp order
+    // device result table
+    Entry *entries = nullptr;
+    int *scratch = nullptr;
+    int ecap = 0, nentries = 0;
+    int *d_nE = nullptr, *d_nlist = nullptr, *d_keep = nullptr, *d_list = nullptr, *d_mn = nullptr;
+    uint8_t *keepf = nullptr, *dropf = nullptr, *freshf = nullptr;
     int tree_id = 0;
 };
 
@@ -701,51 +809,16 @@ int inscy_discretise(const float *d_X, int64_t n, int32_t d, int32_t cells, floa
     }
 }
 
-// Standalone context for the kernel-level entry points.
+// Kernel-level entry points: the production engine's subspace pass on its own.
 namespace {
-struct Standalone {
-    int n, d, ecells;
-    float *Xt = nullptr;
-    uint8_t *cellC = nullptr;
-    uint16_t *cellE = nullptr;
-    uint64_t *nearm = nullptr;
-    int *sid1 = nullptr, *off1 = nullptr, *hist = nullptr, *bad = nullptr;
-    DevData D{};
-    Standalone(const float *d_X, int n_, int d_, float eps, int cells) : n(n_), d(d_) {
-        ecells = (int) std::ceil(1.0 / (double) eps);
-        size_t nd = (size_t) n * d;
-        HIP_OK(hipMalloc(&Xt, nd * sizeof(float)));
-        HIP_OK(hipMalloc(&cellC, nd));
-        HIP_OK(hipMalloc(&cellE, nd * 2));
-        HIP_OK(hipMalloc(&nearm, (size_t) n * 8));
-        HIP_OK(hipMalloc(&sid1, nd * 4));
-        HIP_OK(hipMalloc(&off1, (size_t) d * (ecells + 1) * 4));
-        HIP_OK(hipMalloc(&hist, (size_t) d * (ecells + 1) * 4));
-        HIP_OK(hipMalloc(&bad, 4));
-        HIP_OK(hipMemset(bad, 0, 4));
-        HIP_OK(launch_discretise(d_X, n, d, cells, eps, Xt, cellC, cellE, nearm, bad, 0));
-        HIP_OK(launch_build_index(cellE, n, d, ecells, hist, off1, sid1, 0));
-        int h = 0;
-        HIP_OK(hipMemcpy(&h, bad, 4, hipMemcpyDeviceToHost));
-        if (h) throw Failure(INSCY_ERR_ARGUMENT, "X must be finite with every value in [0, 1]");
-        D = DevData{Xt, cellC, cellE, sid1, off1, n, d, cells, ecells};
-    }
-    ~Standalone() {
-        for (void *p : {(void *) Xt, (void *) cellC, (void *) cellE, (void *) nearm, (void *) sid1, (void *) off1,
-                        (void *) hist, (void *) bad})
-            if (p) (void) hipFree(p);
-    }
-};
-
-SubspaceArg make_arg(const int32_t *dims, int32_t k, int d) {
-    SubspaceArg S{};
-    S.k = k;
+uint64_t mask_from(const int32_t *dims, int32_t k, int d) {
+    uint64_t m = 0;
     for (int i = 0; i < k; ++i) {
         if (dims[i] < 0 || dims[i] >= d || (i && dims[i] <= dims[i - 1]))
             throw Failure(INSCY_ERR_ARGUMENT, "dims must be ascending ids in [0, d)");
-        S.dims[i] = dims[i];
+        m |= 1ull << dims[i];
     }
-    return S;
+    return m;
 }
 }  // namespace
 
@@ -754,24 +827,12 @@ int inscy_neighbour_counts(const float *d_X, int64_t n, int32_t d, float eps, co
     if (!d_X || !dims || k < 1 || k > d || !d_queries || !d_out || n < 1 || d < 1 || d > kMaxDims || !(eps > 0.0f))
         return fail(INSCY_ERR_ARGUMENT, "invalid argument");
     try {
-        Standalone ctx(d_X, (int) n, d, eps, 1);
-        SubspaceArg S = make_arg(dims, k, d);
-        int *cnt = nullptr, *nqd = nullptr;
-        HIP_OK(hipMalloc(&cnt, (size_t) n * 4));
-        HIP_OK(hipMalloc(&nqd, 4));
-        int nqh = (int) nq;
-        HIP_OK(hipMemcpy(nqd, &nqh, 4, hipMemcpyHostToDevice));
-        HIP_OK(launch_fill_i32(cnt, -1, n, 0));
-        HIP_OK(launch_counts(ctx.D, S, d_queries, nqd, (int) nq, sqrt_threshold(eps), cnt, nullptr, 0));
-        // gather per query (queries may repeat)
-        std::vector<int> hq(nq), hc(n);
-        HIP_OK(hipMemcpy(hq.data(), d_queries, nq * 4, hipMemcpyDeviceToHost));
-        HIP_OK(hipMemcpy(hc.data(), cnt, (size_t) n * 4, hipMemcpyDeviceToHost));
-        std::vector<int> ho(nq);
-        for (int64_t i = 0; i < nq; ++i) ho[i] = hc[hq[i]];
-        HIP_OK(hipMemcpy(d_out, ho.data(), nq * 4, hipMemcpyHostToDevice));
-        HIP_OK(hipFree(cnt));
-        HIP_OK(hipFree(nqd));
+        inscy_params p{eps, 1.0f, 1, 1, 1.0f, 1, 1, 0};
+        Engine eng(p, (int) n, d, nullptr);
+        eng.setup(d_X);
+        const int *cnt = eng.exact_counts(mask_from(dims, k, d));
+        HIP_OK(launch_gather(cnt, d_queries, (int) nq, d_out, nullptr));
+        HIP_OK(hipDeviceSynchronize());
         return INSCY_OK;
     } catch (const Failure &f) {
         return fail(f.code, f.what());
@@ -781,28 +842,15 @@ int inscy_neighbour_counts(const float *d_X, int64_t n, int32_t d, float eps, co
 int inscy_dense_components(const float *d_X, int64_t n, int32_t d, const inscy_params *p, const int32_t *dims,
                            int32_t k, int32_t *d_comp) {
     if (!d_X || !p || !dims || k < 1 || k > d || !d_comp || n < 1 || d < 1 || d > kMaxDims ||
-        !(p->neighborhood_size > 0.0f))
+        !(p->neighborhood_size > 0.0f) || p->number_of_cells < 1 || p->number_of_cells > 255)
         return fail(INSCY_ERR_ARGUMENT, "invalid argument");
     if (!p->rectangular) return fail(INSCY_ERR_UNSUPPORTED, "rectangular=False is not implemented");
     try {
-        Standalone ctx(d_X, (int) n, d, p->neighborhood_size, std::max(1, (int) p->number_of_cells));
-        SubspaceArg S = make_arg(dims, k, d);
-        int *cnt = nullptr, *q = nullptr, *nqd = nullptr, *par = nullptr, *csize = nullptr;
-        HIP_OK(hipMalloc(&cnt, (size_t) n * 4));
-        HIP_OK(hipMalloc(&q, (size_t) n * 4));
-        HIP_OK(hipMalloc(&nqd, 4));
-        HIP_OK(hipMalloc(&par, (size_t) n * 4));
-        HIP_OK(hipMalloc(&csize, (size_t) n * 4));
-        int nn = (int) n;
-        HIP_OK(hipMemcpy(nqd, &nn, 4, hipMemcpyHostToDevice));
-        HIP_OK(launch_iota(q, nn, 0));
-        HIP_OK(launch_fill_i32(cnt, -1, n, 0));
-        float eps = p->neighborhood_size;
-        HIP_OK(launch_counts(ctx.D, S, q, nqd, nn, sqrt_threshold(eps), cnt, nullptr, 0));
-        int thr = count_threshold(std::max(p->F * expden(k, eps, nn), (float) p->num_obj));
-        HIP_OK(launch_components(ctx.D, S, sqrt_threshold(eps), cnt, thr, par, d_comp, csize, 0));
+        Engine eng(*p, (int) n, d, nullptr);
+        eng.setup(d_X);
+        const int *comp = eng.subspace(mask_from(dims, k, d)).comp;
+        HIP_OK(hipMemcpyAsync(d_comp, comp, (size_t) n * sizeof(int), hipMemcpyDeviceToDevice, nullptr));
         HIP_OK(hipDeviceSynchronize());
-        for (void *ptr : {(void *) cnt, (void *) q, (void *) nqd, (void *) par, (void *) csize}) HIP_OK(hipFree(ptr));
         return INSCY_OK;
     } catch (const Failure &f) {
         return fail(f.code, f.what());

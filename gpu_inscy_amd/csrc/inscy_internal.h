@@ -24,49 +24,88 @@ struct DevData {
     const float *Xt;        // [d][n] coordinates
     const uint8_t *cellC;   // [d][n] cell in the c-grid (ScyTreeNode.cu:100-102)
     const uint16_t *cellE;  // [d][n] cell in the ceil(1/eps)-grid (inscy_map.cpp:243-244)
-    const int *sid1;        // [d][n] point ids bucketed by cellE of that dimension
-    const int *off1;        // [d][ecells+1] bucket starts
     int n, d, cells, ecells;
+};
+
+// Grid index of one subspace: the points are sorted by their eps-grid cells in
+// j (1..3) of the subspace's dimensions.
+struct GridArg {
+    int j;               // number of index dimensions
+    int C;               // cells per index dimension (ceil(1/eps))
+    int tI[3];           // positions within the subspace of the index dimensions
+    uint64_t checkmask;  // bit t: dimension S.dims[t] needs the per-candidate cell test
+};
+
+// Scratch of one subspace grid pass (device pointers, sized for n points,
+// k dims and C^j + 1 grid cells).
+struct GridBuffers {
+    int *keys;        // [n]
+    int *rank;        // [n] position of a point within its grid cell
+    int *hist;        // [ncell+1]
+    int *off;         // [ncell+1] cell starts
+    int *order;       // [n] point id at each sorted position
+    float *Ys;        // [k][n] coordinates in sorted order
+    uint16_t *Es;     // [k][n] eps-grid cells in sorted order
+    uint8_t *dsorted; // [n] dense flag per sorted position
+    int *par;         // [n] union-find parents
+    int *giant;       // [1] root of the largest sampled component
+    void *scan_temp;
+    size_t scan_bytes;
+};
+
+// One subspace of the device-resident result table: its dense components and
+// which of them the result holds (nkept > 0 <=> the subspace is in the result).
+struct Entry {
+    uint64_t mask;
+    const int *comp;   // [n] component root of each point, -1 if not dense
+    const int *csize;  // [n] component sizes indexed by root
+    uint8_t *inres;    // [n] component kept in the result
+    int *mem;          // [n] points of the kept components, nmem of them
+    int nkept;
+    int nmem;
 };
 
 // Largest float s with sqrtf(s) <= eps: the distance test fl(sqrt(sum)) <= eps
 // of ClusteringCpu.cu:52-81 is then sum <= s_star without a device sqrt.
 float sqrt_threshold(float eps);
+size_t scan_temp_bytes(int items);
 
-// Launchers (all asynchronous on `st`; return hipError_t of the launch).
+// Launchers (all asynchronous on `st`; return the hipError_t of the launches).
 hipError_t launch_discretise(const float *X, int n, int d, int cells, float eps, float *Xt, uint8_t *cellC,
                              uint16_t *cellE, uint64_t *nearm, int *bad, hipStream_t st);
-hipError_t launch_build_index(const uint16_t *cellE, int n, int d, int ecells, int *hist, int *off1, int *sid1,
-                              hipStream_t st);
-hipError_t launch_flags(const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, const uint8_t *cellC_m,
-                        int m, int *flags, hipStream_t st);
-hipError_t launch_filter(const int *P, const int *nP, int nP_max, const int *spx, const uint64_t *spm,
-                         const int *nsp, int nsp_max, const uint8_t *cellC_m, int m, int lo, int hi, int *outP,
-                         int *outx, uint64_t *outm, int *counters, hipStream_t st);
-hipError_t launch_counts(const DevData &D, const SubspaceArg &S, const int *queries, const int *nq, int nq_max,
-                         float s_star, int *cnt, unsigned long long *cand, hipStream_t st);
-hipError_t launch_weak(const int *P, const int *nP, int nP_max, const int *cnt, int thr, int *W, int *nW,
-                       int *stamp, int tid, hipStream_t st);
+hipError_t launch_hist1(const uint16_t *cellE, int n, int d, int ecells, int *hist, hipStream_t st);
+// |N_S(p)| (capped at `cap`) of the points with cnt == -1 and, when mark is
+// given, mark[p] == markid (stat: candidates examined, points counted)
+hipError_t launch_subspace_counts(const DevData &D, const SubspaceArg &S, const GridArg &G, float s_star, int cap,
+                                  const GridBuffers &B, const int *mark, int markid, int *cnt,
+                                  unsigned long long *stat, hipStream_t st);
+// remaining counts, then the dense components of S (comp = smallest member id or -1)
+// (bound: complete counts of a subspace of S, or null; lo: the smaller threshold of S)
+hipError_t launch_subspace_components(const DevData &D, const SubspaceArg &S, const GridArg &G, float s_star,
+                                      int cap, int lo, int thr_dense, const GridBuffers &B, const int *bound,
+                                      int *cnt, int *comp, int *csize, unsigned long long *stat, hipStream_t st);
+hipError_t launch_mark(const int *P, const int *nP, int nP_max, int *mark, int markid, hipStream_t st);
+hipError_t launch_flags_all(const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, const uint8_t *cellC,
+                            int n, int first, int cells, int *flags, hipStream_t st);
+hipError_t launch_partition(const int *P, const int *nP, int nP_max, const int *spx, const uint64_t *spm,
+                            const int *nsp, int nsp_max, const uint8_t *cellC_m, const int *gid, int G,
+                            const int *weakcnt, int thr, int m, int *cnt, int *outP, int *outW, int *outx,
+                            uint64_t *outm, hipStream_t st);
 hipError_t launch_path_prune(const DevData &D, uint64_t smask, const int *W, const int *nW, int nW_max,
-                             const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, const int *stamp,
-                             int tid, unsigned long long *bits, uint64_t *table, uint64_t tmask, int *outx,
-                             uint64_t *outm, int *nout, hipStream_t st);
-hipError_t launch_all_queries(const int *cnt, int n, int *queries, int *nq, hipStream_t st);
-hipError_t launch_components(const DevData &D, const SubspaceArg &S, float s_star, const int *cnt, int thr,
-                             int *parent, int *comp, int *csize, hipStream_t st);
-hipError_t launch_redundancy(const int *W, const int *nW, int nW_max, int const *const *compH,
-                             int const *const *csizeH, uint8_t const *const *inresH, int nH, int *mn,
-                             hipStream_t st);
-hipError_t launch_seed(const int *W, const int *nW, int nW_max, const int *comp, uint8_t *lab, hipStream_t st);
-hipError_t launch_collect(const uint8_t *lab, int n, const int *csize, int *out, int *nout, hipStream_t st);
-hipError_t launch_join_check(const int *compS, const uint8_t *keep, const int *csizeS, int n, float r,
-                             int const *const *compH, int const *const *csizeH, uint8_t const *const *inresH,
-                             int nH, uint8_t *drop, hipStream_t st);
+                             const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, int *stamp, int tid,
+                             unsigned long long *bits, uint64_t *table, uint64_t tmask, int *outx, uint64_t *outm,
+                             int *nout, hipStream_t st);
+hipError_t launch_redundancy_seed(const int *W, const int *nW_dev, int nW, int *stamp, int tid, const Entry *E,
+                                  const int *nE, uint64_t smask, const int *comp, int *list, int *nlist, int *mn,
+                                  float r, int *keep, uint8_t *lab, hipStream_t st);
+hipError_t launch_join(const uint8_t *lab, int n, const int *compS, const int *csizeS, int min_size, float r,
+                       Entry *E, const int *nE, uint64_t smask, int eS, uint8_t *inresS, int *list, int *nlist,
+                       uint8_t *keepf, uint8_t *drop, uint8_t *fresh, hipStream_t st);
+hipError_t launch_set_entry(Entry *E, int idx, const Entry &e, int *nE, int count, hipStream_t st);
 hipError_t launch_labels(const int *comp, const uint8_t *inres, int n, int *out, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t count, hipStream_t st);
 hipError_t launch_fill_i32(int *p, int v, size_t count, hipStream_t st);
-hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t count, hipStream_t st);
-hipError_t launch_set_flags(uint8_t *flags, const int *ids, int count, hipStream_t st);
+hipError_t launch_gather(const int *src, const int *idx, int count, int *out, hipStream_t st);
 hipError_t launch_iota(int *p, int n, hipStream_t st);
 hipError_t launch_root_paths(const uint64_t *nearm, int n, int *outx, uint64_t *outm, int *nout, hipStream_t st);
 

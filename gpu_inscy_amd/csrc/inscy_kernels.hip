@@ -2,7 +2,7 @@
 //
 // Everything here is integer/byte streaming work plus one FP32 distance loop;
 // nothing is GEMM-shaped (the contraction depth is |S| <= a few dims), so the
-// bound for every kernel is HBM/L2 bandwidth or VALU issue, not MFMA.
+// bound for every kernel is memory bandwidth/latency or VALU issue, not MFMA.
 //
 // Floating point: the reference compares fl(sqrt(sum_k fl(fl(p_k - q_k)^2))) <= eps
 // with separate multiplies and adds in subspace order (ClusteringCpu.cu:40-81).
@@ -14,6 +14,8 @@
 #include <cmath>
 #include <cstring>
 
+#include <hipcub/hipcub.hpp>
+
 #include "inscy_internal.h"
 
 #pragma clang fp contract(off)
@@ -23,18 +25,18 @@ namespace inscy {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kLanesPerQuery = 16;  // lanes cooperating on one query's candidate ranges
 
-__device__ __forceinline__ int ld_acq(const int *p) {
+__device__ __forceinline__ int ld_relaxed(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void st_rel(int *p, int v) {
+__device__ __forceinline__ void st_relaxed(int *p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Wave-aggregated slot allocation for order-free stream compaction: one
-// atomic per wave instead of one per kept element.  Must be reached by every
-// active lane of the wave.
+// atomic per wave.  Must be reached by every active lane of the wave.
 __device__ __forceinline__ int wave_slot(bool keep, int *counter) {
     unsigned long long m = __ballot(keep);
     if (m == 0) return -1;
@@ -45,6 +47,28 @@ __device__ __forceinline__ int wave_slot(bool keep, int *counter) {
     base = __shfl(base, leader);
     return keep ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
 }
+
+// counter[key] += (number of lanes with that key and want), one atomic per
+// distinct key in the wave; returns each wanting lane's slot (old value +
+// rank among the lanes sharing its key).  Must be reached by every active lane.
+__device__ __forceinline__ int wave_add_by_key(int *counter, int key, bool want) {
+    unsigned long long todo = __ballot(want);
+    const int lane = __lane_id();
+    int result = 0;
+    while (todo) {
+        int leader = __ffsll((long long) todo) - 1;
+        int lk = __shfl(key, leader);
+        unsigned long long same = __ballot(want && key == lk) & todo;
+        int b = 0;
+        if (lane == leader) b = atomicAdd(&counter[lk], __popcll(same));
+        b = __shfl(b, leader);
+        if ((same >> lane) & 1ull) result = b + __popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return result;
+}
+
+inline unsigned blocks_for(size_t count) { return (unsigned) ((count + kBlock - 1) / kBlock); }
 
 // ------------------------------------------------------------ discretisation
 // get_cell_no (ScyTreeNode.cu:100-102): min((int)(x / (1/c)), c-1); the
@@ -59,7 +83,7 @@ __global__ void k_discretise(const float *__restrict__ X, int n, int d, int cell
     for (int k = 0; k < d; ++k) {
         float x = X[(size_t) p * d + k];
         if (!(x >= 0.0f && x <= 1.0f)) out_of_range = true;
-        x = fminf(fmaxf(x, 0.0f), 1.0f);  // only reached with valid data when bad stays 0
+        x = fminf(fmaxf(x, 0.0f), 1.0f);  // keeps indices in range; the run is rejected via `bad`
         if (Xt) Xt[(size_t) k * n + p] = x;
         int c = min((int) __fdiv_rn(x, cs), cells - 1);
         int e = min((int) __fdiv_rn(x, ecs), ecells - 1);
@@ -73,213 +97,419 @@ __global__ void k_discretise(const float *__restrict__ X, int n, int d, int cell
     if (out_of_range) atomicOr(bad, 1);
 }
 
-// ------------------------------------------------------------ per-dimension eps-grid buckets
+// Occupancy of the ceil(1/eps)-grid per dimension; the host derives from it
+// which dimensions of a subspace make the most selective grid index.
 __global__ void k_hist1(const uint16_t *__restrict__ cellE, int n, int d, int ecells, int *__restrict__ hist) {
     size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t) n * d) return;
     int k = (int) (i / n);
-    atomicAdd(&hist[(size_t) k * (ecells + 1) + cellE[i]], 1);
+    atomicAdd(&hist[(size_t) k * ecells + cellE[i]], 1);
 }
 
-// one block per dimension: exclusive scan of ecells counts into off1[k][0..ecells]
-__global__ void k_scan1(int *__restrict__ hist, int ecells, int *__restrict__ off1) {
-    int k = blockIdx.x;
-    int *h = hist + (size_t) k * (ecells + 1);
-    int *o = off1 + (size_t) k * (ecells + 1);
-    __shared__ int part[kBlock];
-    int per = (ecells + kBlock - 1) / kBlock;
-    int beg = threadIdx.x * per, end = min(beg + per, ecells);
-    int s = 0;
-    for (int i = beg; i < end; ++i) s += h[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < kBlock; ++t) {
-            int v = part[t];
-            part[t] = acc;
-            acc += v;
-        }
-        o[ecells] = acc;
-    }
-    __syncthreads();
-    int acc = part[threadIdx.x];
-    for (int i = beg; i < end; ++i) {
-        int v = h[i];
-        o[i] = acc;
-        h[i] = acc;  // becomes the scatter cursor
-        acc += v;
-    }
-}
-
-__global__ void k_scatter1(const uint16_t *__restrict__ cellE, int n, int d, int ecells, int *__restrict__ cursor,
-                           int *__restrict__ sid1) {
-    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t) n * d) return;
-    int k = (int) (i / n);
-    int p = (int) (i - (size_t) k * n);
-    int pos = atomicAdd(&cursor[(size_t) k * (ecells + 1) + cellE[i]], 1);
-    sid1[(size_t) k * n + pos] = p;
-}
-
-// ------------------------------------------------------------ restriction
+// ------------------------------------------------------------ expansion of one tree
 // mergeWithNeighbors decision input (ScyTreeNode.cu:837-851 with the flag of
-// restrict_node :262-267): cell c of dimension m is s-connected iff a path
-// branching at m starts from a point in cell c.
-__global__ void k_flags(const int *__restrict__ spx, const uint64_t *__restrict__ spm, const int *__restrict__ nsp,
-                        const uint8_t *__restrict__ cellC_m, int m, int *__restrict__ flags) {
+// restrict_node :262-267): cell c of dimension b is s-connected iff a path
+// branching at b starts from a point in cell c.  One pass sets the flags of
+// every dimension b >= first.
+__global__ void k_flags_all(const int *__restrict__ spx, const uint64_t *__restrict__ spm, const int *__restrict__ nsp,
+                            const uint8_t *__restrict__ cellC, int n, int first, int cells,
+                            int *__restrict__ flags) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *nsp) return;
-    if ((spm[i] >> m) & 1ull) flags[cellC_m[spx[i]]] = 1;
+    int x = spx[i];
+    uint64_t m = first >= 64 ? 0ull : (spm[i] >> first);
+    while (m) {
+        int b = __ffsll((long long) m) - 1;
+        m &= m - 1;
+        flags[b * cells + cellC[(size_t) (b + first) * n + x]] = 1;
+    }
 }
 
-// Points and s-connection paths of the region {cell_m in [lo, hi]}; paths keep
-// only branch dimensions above m (the only ones any descendant can use).
-__global__ void k_filter(const int *__restrict__ P, const int *__restrict__ nP, const int *__restrict__ spx,
-                         const uint64_t *__restrict__ spm, const int *__restrict__ nsp,
-                         const uint8_t *__restrict__ cellC_m, int m, int lo, int hi, int *__restrict__ outP,
-                         int *__restrict__ outx, uint64_t *__restrict__ outm, int *__restrict__ counters,
-                         int nP_blocks) {
-    if ((int) blockIdx.x < nP_blocks) {
+// Counting-sort partition of a tree's points, weak-dense points and paths by
+// the cell group of dimension m (restrict + mergeWithNeighbors for every
+// group at once).  cnt[0..G): points, [G..2G): weak-dense points, [2G..3G):
+// paths that branch above m; [3G..6G): scatter cursors.
+struct PartArg {
+    const int *P;
+    const int *nP;
+    const int *spx;
+    const uint64_t *spm;
+    const int *nsp;
+    const uint8_t *cellC_m;
+    const int *gid;      // [cells] group of each cell of m
+    const int *weakcnt;  // neighbour counts of the subspace S ∪ {m}
+    int thr, m, G;
+    int nP_blocks;
+};
+
+__global__ void k_part_count(PartArg A, int *__restrict__ cnt) {
+    __shared__ int h[3 * 256];
+    for (int t = threadIdx.x; t < 3 * A.G; t += blockDim.x) h[t] = 0;
+    __syncthreads();
+    if ((int) blockIdx.x < A.nP_blocks) {
         int i = blockIdx.x * blockDim.x + threadIdx.x;
-        bool keep = false;
-        int p = 0;
-        if (i < *nP) {
-            p = P[i];
-            int c = cellC_m[p];
-            keep = c >= lo && c <= hi;
+        if (i < *A.nP) {
+            int p = A.P[i];
+            int g = A.gid[A.cellC_m[p]];
+            atomicAdd(&h[g], 1);
+            if (A.weakcnt[p] >= A.thr) atomicAdd(&h[A.G + g], 1);
         }
-        int slot = wave_slot(keep, &counters[0]);
-        if (keep) outP[slot] = p;
     } else {
-        int i = (blockIdx.x - nP_blocks) * blockDim.x + threadIdx.x;
-        bool keep = false;
-        int x = 0;
+        int i = (blockIdx.x - A.nP_blocks) * blockDim.x + threadIdx.x;
+        if (i < *A.nsp) {
+            uint64_t mk = A.m >= 63 ? 0ull : (A.spm[i] & (~0ull << (A.m + 1)));
+            if (mk) atomicAdd(&h[2 * A.G + A.gid[A.cellC_m[A.spx[i]]]], 1);
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 3 * A.G; t += blockDim.x)
+        if (h[t]) atomicAdd(&cnt[t], h[t]);
+}
+
+__global__ void k_part_scatter(PartArg A, int *__restrict__ cnt, int *__restrict__ outP, int *__restrict__ outW,
+                               int *__restrict__ outx, uint64_t *__restrict__ outm) {
+    __shared__ int base[3 * 256];
+    if (threadIdx.x < 3) {
+        int acc = 0;
+        for (int g = 0; g < A.G; ++g) {
+            base[threadIdx.x * A.G + g] = acc;
+            acc += cnt[threadIdx.x * A.G + g];
+        }
+    }
+    __syncthreads();
+    int *cur = cnt + 3 * A.G;
+    if ((int) blockIdx.x < A.nP_blocks) {
+        int i = blockIdx.x * blockDim.x + threadIdx.x;
+        bool in = i < *A.nP;
+        int p = in ? A.P[i] : 0;
+        int g = in ? A.gid[A.cellC_m[p]] : 0;
+        bool weak = in && A.weakcnt[p] >= A.thr;
+        int sp = wave_add_by_key(cur, g, in);
+        int sw = wave_add_by_key(cur + A.G, g, weak);
+        if (in) outP[base[g] + sp] = p;
+        if (weak) outW[base[A.G + g] + sw] = p;
+    } else {
+        int i = (blockIdx.x - A.nP_blocks) * blockDim.x + threadIdx.x;
         uint64_t mk = 0;
-        if (i < *nsp) {
-            x = spx[i];
-            mk = m >= 63 ? 0ull : (spm[i] & (~0ull << (m + 1)));
-            int c = cellC_m[x];
-            keep = mk != 0 && c >= lo && c <= hi;
+        int x = 0, g = 0;
+        if (i < *A.nsp) {
+            mk = A.m >= 63 ? 0ull : (A.spm[i] & (~0ull << (A.m + 1)));
+            x = A.spx[i];
+            g = A.gid[A.cellC_m[x]];
         }
-        int slot = wave_slot(keep, &counters[1]);
-        if (keep) {
-            outx[slot] = x;
-            outm[slot] = mk;
+        int sx = wave_add_by_key(cur + 2 * A.G, g, mk != 0);
+        if (mk) {
+            int pos = base[2 * A.G + g] + sx;
+            outx[pos] = x;
+            outm[pos] = mk;
         }
     }
 }
 
-// ------------------------------------------------------------ epsilon neighbourhood
+// ------------------------------------------------------------ epsilon neighbourhood: subspace grid pass
 // Candidate rule of get_possible_neighbors_from (ScyTreeNode.cu:896-946) on
-// the ceil(1/eps)-grid: |cell_e(q) - cell_e(p)| <= 1 in every dimension of S.
-// The scan runs over the three eps-cells of the dimension of S whose slab
-// around the query holds the fewest points; the other dimensions are checked
-// per candidate.  Query coordinates and cells sit in LDS ([k][block] layout,
-// conflict-free across the wave).
-template <class Visit>
-__device__ __forceinline__ long long visit_neighbours(const DevData &D, const SubspaceArg &S, int q,
-                                                      const float *__restrict__ qx, const int *__restrict__ qe,
-                                                      float s_star, Visit visit) {
+// the ceil(1/eps)-grid: |cell_e(q) - cell_e(p)| <= 1 in every dimension of S;
+// then the float distance in S (ClusteringCpu.cu:52-81).  Per subspace the
+// points are counting-sorted by their cells in the j most selective
+// dimensions of S (j <= 3), so the candidates of a query are 3^(j-1)
+// contiguous ranges of the sorted order; the coordinates and cells of S are
+// gathered into that order ([k][n]).  Queries run in the same sorted order and
+// kLanesPerQuery lanes stride through each query's ranges, so the lanes of a
+// wave read consecutive candidates and neighbouring queries share ranges.
+
+template <class Range>
+__device__ __forceinline__ void for_ranges(const GridArg &G, const int *__restrict__ off, const int *qe, int stride,
+                                           Range f) {
+    const int C = G.C;
+    if (G.j == 1) {
+        int c0 = qe[G.tI[0] * stride];
+        f(off[max(c0 - 1, 0)], off[min(c0 + 1, C - 1) + 1]);
+    } else if (G.j == 2) {
+        int c0 = qe[G.tI[0] * stride], c1 = qe[G.tI[1] * stride];
+        int lo = max(c1 - 1, 0), hi = min(c1 + 1, C - 1);
+        for (int x = max(c0 - 1, 0); x <= min(c0 + 1, C - 1); ++x) f(off[x * C + lo], off[x * C + hi + 1]);
+    } else {
+        int c0 = qe[G.tI[0] * stride], c1 = qe[G.tI[1] * stride], c2 = qe[G.tI[2] * stride];
+        int lo = max(c2 - 1, 0), hi = min(c2 + 1, C - 1);
+        for (int x = max(c0 - 1, 0); x <= min(c0 + 1, C - 1); ++x)
+            for (int y = max(c1 - 1, 0); y <= min(c1 + 1, C - 1); ++y) {
+                int base = (x * C + y) * C;
+                f(off[base + lo], off[base + hi + 1]);
+            }
+    }
+}
+
+// candidate at sorted position jj against the staged query; S order, no FMA
+__device__ __forceinline__ bool is_neighbour(const SubspaceArg &S, const GridArg &G, int n,
+                                             const float *__restrict__ Ys, const uint16_t *__restrict__ Es,
+                                             const float *qy, const int *qe, int stride, int jj, float s_star) {
+    float acc = 0.0f;
+    for (int t = 0; t < S.k; ++t) {
+        if ((G.checkmask >> t) & 1ull) {
+            int de = (int) Es[(size_t) t * n + jj] - qe[t * stride];
+            if (de < -1 || de > 1) return false;
+        }
+        float diff = qy[t * stride] - Ys[(size_t) t * n + jj];
+        float sq = diff * diff;
+        acc = acc + sq;
+        if (acc > s_star) return false;
+    }
+    return true;
+}
+
+__global__ void k_grid_keys(DevData D, SubspaceArg S, GridArg G, int *__restrict__ keys, int *__restrict__ rank,
+                            int *__restrict__ hist) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.n) return;
+    int key = 0;
+    for (int a = 0; a < G.j; ++a) key = key * G.C + D.cellE[(size_t) S.dims[G.tI[a]] * D.n + p];
+    keys[p] = key;
+    rank[p] = atomicAdd(&hist[key], 1);
+}
+
+// Sorted order and the subspace's coordinates/cells gathered into it.
+__global__ void k_grid_scatter(DevData D, SubspaceArg S, const int *__restrict__ keys, const int *__restrict__ rank,
+                               const int *__restrict__ off, int *__restrict__ order, float *__restrict__ Ys,
+                               uint16_t *__restrict__ Es) {
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.n) return;
     const int n = D.n;
-    const int stride = (D.ecells + 1);
-    int best = 0, a = 0, b = 0;
-    int bsize = INT_MAX;
+    int pos = off[keys[p]] + rank[p];
+    order[pos] = p;
     for (int t = 0; t < S.k; ++t) {
         int s = S.dims[t];
-        int e = qe[t * kBlock];
-        int lo = max(e - 1, 0), hi = min(e + 1, D.ecells - 1);
-        int aa = D.off1[(size_t) s * stride + lo], bb = D.off1[(size_t) s * stride + hi + 1];
-        if (bb - aa < bsize) {
-            bsize = bb - aa;
-            best = s;
-            a = aa;
-            b = bb;
-        }
-    }
-    const int *ids = D.sid1 + (size_t) best * n;
-    for (int j = a; j < b; ++j) {
-        int o = ids[j];
-        if (o == q) continue;
-        bool ok = true;
-        float acc = 0.0f;
-        for (int t = 0; t < S.k && ok; ++t) {
-            int s = S.dims[t];
-            int de = (int) D.cellE[(size_t) s * n + o] - qe[t * kBlock];
-            float diff = qx[t * kBlock] - D.Xt[(size_t) s * n + o];
-            float sq = diff * diff;
-            acc = acc + sq;
-            ok = de >= -1 && de <= 1 && acc <= s_star;
-        }
-        if (ok) visit(o);
-    }
-    return (long long) (b - a);
-}
-
-__device__ __forceinline__ void stage_query(const DevData &D, const SubspaceArg &S, int q, float *qx, int *qe) {
-    for (int t = 0; t < S.k; ++t) {
-        int s = S.dims[t];
-        qx[t * kBlock] = D.Xt[(size_t) s * D.n + q];
-        qe[t * kBlock] = D.cellE[(size_t) s * D.n + q];
+        Ys[(size_t) t * n + pos] = D.Xt[(size_t) s * n + p];
+        Es[(size_t) t * n + pos] = D.cellE[(size_t) s * n + p];
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_counts(DevData D, SubspaceArg S, const int *__restrict__ queries,
-                                                   const int *__restrict__ nq, float s_star, int *__restrict__ cnt,
-                                                   unsigned long long *__restrict__ cand) {
+__device__ __forceinline__ void stage_query(const SubspaceArg &S, int n, const float *__restrict__ Ys,
+                                            const uint16_t *__restrict__ Es, int i, float *sy, int *se, int stride) {
+    for (int t = 0; t < S.k; ++t) {
+        sy[t * stride] = Ys[(size_t) t * n + i];
+        se[t * stride] = Es[(size_t) t * n + i];
+    }
+}
+
+// |N_S(p)| of the points still uncounted (cnt == -1) and, when `mark` is
+// given, stamped with `markid`.  Counting stops once `cap` neighbours are
+// found: every comparison the algorithm makes with a count is against a
+// threshold <= cap, so min(|N_S(p)|, cap)-or-more decides them all.
+__global__ __launch_bounds__(kBlock) void k_grid_count(int n, SubspaceArg S, GridArg G, const int *__restrict__ off,
+                                                       const int *__restrict__ order, const float *__restrict__ Ys,
+                                                       const uint16_t *__restrict__ Es, float s_star, int cap,
+                                                       const int *__restrict__ mark, int markid,
+                                                       const int *__restrict__ bound, int lo, int *__restrict__ cnt,
+                                                       unsigned long long *__restrict__ stat) {
+    constexpr int LPQ = kLanesPerQuery, QPB = kBlock / LPQ;
     extern __shared__ char smem[];
-    float *sx = reinterpret_cast<float *>(smem);
-    int *se = reinterpret_cast<int *>(smem + sizeof(float) * kBlock * S.k);
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    long long seen = 0, done = 0;
-    if (i < *nq) {
-        int q = queries[i];
-        if (cnt[q] < 0) {
-            float *qx = sx + threadIdx.x;
-            int *qe = se + threadIdx.x;
-            stage_query(D, S, q, qx, qe);
-            int c = 0;
-            seen = visit_neighbours(D, S, q, qx, qe, s_star, [&](int) { ++c; });
-            cnt[q] = c;
-            done = 1;
+    float *sy = reinterpret_cast<float *>(smem);
+    int *se = reinterpret_cast<int *>(smem + sizeof(float) * QPB * S.k);
+    const int lq = threadIdx.x / LPQ, sub = threadIdx.x % LPQ;
+    const int i = blockIdx.x * QPB + lq;
+    int p = i < n ? order[i] : 0;
+    bool active = i < n && cnt[p] < 0 && (!mark || mark[p] == markid);
+    // N_S(p) is a subset of N_S'(p) for S' a subset of S (the rounded distance
+    // sum only grows with more dimensions), so a complete count of a subspace
+    // of S below both thresholds of S decides p without a scan.
+    if (active && bound) {
+        int b = bound[p];
+        if (b >= 0 && b < lo) {
+            if (sub == 0) cnt[p] = b;
+            active = false;
         }
     }
-    if (cand) {  // cand[0]: candidates examined, cand[1]: queries computed
-        for (int off = 32; off > 0; off >>= 1) {
-            seen += __shfl_down(seen, off);
-            done += __shfl_down(done, off);
+    if (sub == 0 && active) stage_query(S, n, Ys, Es, i, sy + lq, se + lq, QPB);
+    __syncthreads();
+    long long seen = 0;
+    if (active) {
+        const float *qy = sy + lq;
+        const int *qe = se + lq;
+        const int gshift = (__lane_id() / LPQ) * LPQ;
+        int found = 0;
+        for_ranges(G, off, qe, QPB, [&](int a, int b) {
+            for (int base = a; base < b && found < cap; base += LPQ) {
+                int jj = base + sub;
+                bool hit = jj < b && jj != i && is_neighbour(S, G, n, Ys, Es, qy, qe, QPB, jj, s_star);
+                found += __popcll((__ballot(hit) >> gshift) & ((1ull << LPQ) - 1ull));
+                seen += min(LPQ, b - base);
+            }
+        });
+        if (sub == 0) cnt[p] = found;
+    }
+    if (stat) {  // per wave: candidates examined and points counted, into one of 64 slot pairs
+        long long done = (sub == 0 && active) ? 1 : 0;
+        if (sub != 0) seen = 0;
+        for (int o = 32; o > 0; o >>= 1) {
+            seen += __shfl_down(seen, o);
+            done += __shfl_down(done, o);
         }
         if (__lane_id() == 0 && done) {
-            atomicAdd(&cand[0], (unsigned long long) seen);
-            atomicAdd(&cand[1], (unsigned long long) done);
+            unsigned long long *slot = stat + 16 * (blockIdx.x & 63);
+            atomicAdd(&slot[0], (unsigned long long) seen);
+            atomicAdd(&slot[1], (unsigned long long) done);
         }
     }
 }
 
-// weak-density prune (pruneRecursionAndRemove2, ScyTreeNode.cu:582-667): keep
-// points with |N_S(p)| >= thr; stamp them with the tree id for O(1) membership.
-__global__ void k_weak(const int *__restrict__ P, const int *__restrict__ nP, const int *__restrict__ cnt, int thr,
-                       int *__restrict__ W, int *__restrict__ nW, int *__restrict__ stamp, int tid) {
+// dense flag per sorted position, union-find roots, zeroed component sizes
+__global__ void k_dense_init(const int *__restrict__ order, int n, const int *__restrict__ cnt, int thr_dense,
+                             uint8_t *__restrict__ dsorted, int *__restrict__ par, int *__restrict__ csize) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool keep = false;
-    int p = 0;
-    if (i < *nP) {
-        p = P[i];
-        keep = cnt[p] >= thr;
+    if (i >= n) return;
+    int p = order[i];
+    bool dense = cnt[p] >= thr_dense;
+    dsorted[i] = dense ? 1 : 0;
+    par[p] = dense ? p : -1;
+    csize[p] = 0;
+}
+
+// ------------------------------------------------------------ dense components
+// Union-find over the dense-dense epsilon graph; hooking the larger root under
+// the smaller keeps every root the smallest point id of its component, which
+// is also the canonical cluster name.
+__device__ __forceinline__ int uf_find(int *par, int x) {
+    int p = ld_relaxed(&par[x]);
+    while (p != x) {
+        int g = ld_relaxed(&par[p]);
+        if (g != p) st_relaxed(&par[x], g);  // path halving: g is an ancestor of x
+        x = p;
+        p = g;
     }
-    int slot = wave_slot(keep, nW);
-    if (keep) {
-        W[slot] = p;
-        stamp[p] = tid;
+    return x;
+}
+
+__device__ __forceinline__ void uf_union(int *par, int a, int b) {
+    while (true) {
+        a = uf_find(par, a);
+        b = uf_find(par, b);
+        if (a == b) return;
+        if (a > b) {
+            int t = a;
+            a = b;
+            b = t;
+        }
+        int old = atomicCAS(&par[b], b, a);
+        if (old == b) return;
+        b = old;
     }
+}
+
+// Components in the Afforest manner (Sutton, Ben-Nun, Bar-Noy, "Optimizing
+// Parallel Graph Connectivity Computation via Subgraph Sampling", IPDPS 2018):
+// (1) every dense point links to its first two dense neighbours, (2) roots are
+// compressed, (3) the component holding most of a fixed sample of points is
+// taken as the giant one, (4) only points outside it link to all their dense
+// neighbours (both directions, so an edge between the giant component and the
+// rest is still seen from the outside end).
+__global__ __launch_bounds__(kBlock) void k_link_sample(int n, SubspaceArg S, GridArg G, const int *__restrict__ off,
+                                                        const int *__restrict__ order, const float *__restrict__ Ys,
+                                                        const uint16_t *__restrict__ Es,
+                                                        const uint8_t *__restrict__ dsorted, float s_star,
+                                                        int *__restrict__ par) {
+    constexpr int LPQ = kLanesPerQuery, QPB = kBlock / LPQ;
+    extern __shared__ char smem[];
+    float *sy = reinterpret_cast<float *>(smem);
+    int *se = reinterpret_cast<int *>(smem + sizeof(float) * QPB * S.k);
+    const int lq = threadIdx.x / LPQ, sub = threadIdx.x % LPQ;
+    const int i = blockIdx.x * QPB + lq;
+    const bool active = i < n && dsorted[i];
+    if (sub == 0 && active) stage_query(S, n, Ys, Es, i, sy + lq, se + lq, QPB);
+    __syncthreads();
+    if (!active) return;
+    const float *qy = sy + lq;
+    const int *qe = se + lq;
+    const int p = order[i];
+    const int gshift = (__lane_id() / LPQ) * LPQ;  // this query's lanes within the wave
+    const unsigned long long below = (1ull << sub) - 1ull;
+    int found = 0;
+    for_ranges(G, off, qe, QPB, [&](int a, int b) {
+        for (int base = a; base < b && found < 2; base += LPQ) {
+            int jj = base + sub;
+            bool hit = jj < b && jj != i && dsorted[jj] && is_neighbour(S, G, n, Ys, Es, qy, qe, QPB, jj, s_star);
+            unsigned long long mine = (__ballot(hit) >> gshift) & ((1ull << LPQ) - 1ull);
+            if (hit && __popcll(mine & below) < 2 - found) uf_union(par, p, order[jj]);
+            found += min(__popcll(mine), 2 - found);
+        }
+    });
+}
+
+__global__ void k_uf_compress(int *__restrict__ par, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && ld_relaxed(&par[i]) >= 0) st_relaxed(&par[i], uf_find(par, i));
+}
+
+// one block of 256: the most frequent root among 256 evenly spaced points
+__global__ void k_giant(const int *__restrict__ par, int n, int *__restrict__ giant) {
+    __shared__ int roots[256];
+    __shared__ int best_c[256];
+    __shared__ int best_r[256];
+    const int t = threadIdx.x;
+    int p = (int) (((long long) t * n) / 256);
+    roots[t] = p < n ? ld_relaxed(&par[p]) : -1;
+    __syncthreads();
+    int r = roots[t], c = 0;
+    if (r >= 0)
+        for (int u = 0; u < 256; ++u) c += roots[u] == r;
+    best_c[t] = c;
+    best_r[t] = r;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w && (best_c[t + w] > best_c[t] || (best_c[t + w] == best_c[t] && best_r[t + w] < best_r[t] &&
+                                                    best_c[t + w] > 0))) {
+            best_c[t] = best_c[t + w];
+            best_r[t] = best_r[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) *giant = best_c[0] > 0 ? best_r[0] : -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_link_full(int n, SubspaceArg S, GridArg G, const int *__restrict__ off,
+                                                      const int *__restrict__ order, const float *__restrict__ Ys,
+                                                      const uint16_t *__restrict__ Es,
+                                                      const uint8_t *__restrict__ dsorted, float s_star,
+                                                      const int *__restrict__ giant, int *__restrict__ par) {
+    constexpr int LPQ = kLanesPerQuery, QPB = kBlock / LPQ;
+    extern __shared__ char smem[];
+    float *sy = reinterpret_cast<float *>(smem);
+    int *se = reinterpret_cast<int *>(smem + sizeof(float) * QPB * S.k);
+    const int lq = threadIdx.x / LPQ, sub = threadIdx.x % LPQ;
+    const int i = blockIdx.x * QPB + lq;
+    bool active = i < n && dsorted[i];
+    int p = active ? order[i] : 0;
+    if (active && ld_relaxed(&par[p]) == *giant) active = false;
+    if (sub == 0 && active) stage_query(S, n, Ys, Es, i, sy + lq, se + lq, QPB);
+    __syncthreads();
+    if (!active) return;
+    const float *qy = sy + lq;
+    const int *qe = se + lq;
+    int rp = uf_find(par, p);
+    for_ranges(G, off, qe, QPB, [&](int a, int b) {
+        for (int jj = a + sub; jj < b; jj += LPQ) {
+            if (jj == i || !dsorted[jj] || !is_neighbour(S, G, n, Ys, Es, qy, qe, QPB, jj, s_star)) continue;
+            int q = order[jj];
+            if (uf_find(par, q) == rp) continue;
+            uf_union(par, p, q);
+            rp = uf_find(par, p);
+        }
+    });
+}
+
+__global__ void k_uf_flatten(int *__restrict__ par, int n, int *__restrict__ comp, int *__restrict__ csize) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = -1;
+    if (i < n && ld_relaxed(&par[i]) >= 0) c = uf_find(par, i);
+    wave_add_by_key(csize, c, c >= 0);
+    if (i < n) comp[i] = c;
 }
 
 // ------------------------------------------------------------ s-connection paths after the prune
 // A path (b, x) survives the prune of tree R iff some surviving point w shares
 // x's cells on every unrestricted dimension <= b.  (propergate_count2,
 // ScyTreeNode.cu:446-467, erases the s-connection whose same-cell regular
-// sibling lost all its points; the regular node a path hangs from always keeps
-// a point as long as that sibling does.)  Membership is answered by an
+// sibling lost all its points; the regular node a path hangs from keeps a
+// point whenever that sibling does.)  Membership is answered by an
 // open-addressing table of 64-bit entries (32 hash bits | representative point
 // id); every hash match is confirmed by comparing the cells themselves, so the
 // test is exact.
@@ -304,12 +534,16 @@ __device__ __forceinline__ bool same_prefix(const DevData &D, uint64_t smask, in
     return true;
 }
 
+__global__ void k_stamp(const int *__restrict__ W, const int *__restrict__ nW, int *__restrict__ stamp, int tid) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *nW) stamp[W[i]] = tid;
+}
+
 __global__ void k_path_bits(const int *__restrict__ spx, const uint64_t *__restrict__ spm, const int *__restrict__ nsp,
                             const int *__restrict__ stamp, int tid, unsigned long long *__restrict__ bits) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     uint64_t m = 0;
     if (i < *nsp && stamp[spx[i]] != tid) m = spm[i];
-    // wave OR-reduction, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off);
     if (__lane_id() == 0 && m) atomicOr(bits, (unsigned long long) m);
 }
@@ -382,123 +616,130 @@ __global__ void k_path_check(DevData D, uint64_t smask, const int *__restrict__ 
     }
 }
 
-// ------------------------------------------------------------ dense components
-__global__ void k_all_queries(const int *__restrict__ cnt, int n, int *__restrict__ q, int *__restrict__ nq) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool need = i < n && cnt[i] < 0;
-    int slot = wave_slot(need, nq);
-    if (need) q[slot] = i;
+// ------------------------------------------------------------ redundancy and join on the device
+// The result lives on the device as a table of subspace entries; an entry is
+// in the result while nkept > 0.  The decisions run in stream order, which is
+// the reference's depth-first order, without any host round trip.
+
+// entries e whose mask contains smask (subspace_of, util.cu:619-631) and that
+// are in the result now
+__global__ void k_select(const Entry *__restrict__ E, const int *__restrict__ nE, uint64_t smask, int *__restrict__ list,
+                         int *__restrict__ nlist) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const int ne = *nE;
+    for (int e = threadIdx.x; e < ne; e += blockDim.x)
+        if ((E[e].mask & smask) == smask && E[e].nkept > 0) list[atomicAdd(&cnt, 1)] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) *nlist = cnt;
 }
 
-__device__ __forceinline__ int uf_find(int *par, int x) {
-    int p = ld_acq(&par[x]);
-    while (p != x) {
-        int g = ld_acq(&par[p]);
-        if (g != p) st_rel(&par[x], g);  // path halving: g is an ancestor of x
-        x = p;
-        p = g;
-    }
-    return x;
-}
-
-__device__ __forceinline__ void uf_union(int *par, int a, int b) {
-    while (true) {
-        a = uf_find(par, a);
-        b = uf_find(par, b);
-        if (a == b) return;
-        if (a > b) {
-            int t = a;
-            a = b;
-            b = t;
+// pruneRedundancy (ScyTreeNode.cu:669-734): per listed superspace H, the
+// smallest result cluster of H holding a point of the region.  The members of
+// H's result clusters are listed, the region's points are stamped with `tid`.
+__global__ void k_redundancy(const int *__restrict__ stamp, int tid, const Entry *__restrict__ E,
+                             const int *__restrict__ list, const int *__restrict__ nlist, int *__restrict__ mn) {
+    const int nl = *nlist;
+    const int stride = gridDim.x * blockDim.x;
+    for (int h = 0; h < nl; ++h) {
+        const Entry e = E[list[h]];
+        int best = INT_MAX;
+        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < e.nmem; t += stride) {
+            int q = e.mem[t];
+            if (stamp[q] == tid) best = min(best, e.csize[e.comp[q]]);
         }
-        // hook the larger root under the smaller: roots stay component minima
-        int old = atomicCAS(&par[b], b, a);
-        if (old == b) return;
-        b = old;
+        for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_down(best, o));
+        if (__lane_id() == 0 && best != INT_MAX) atomicMin(&mn[h], best);
     }
 }
 
-__global__ void k_uf_init(const int *__restrict__ cnt, int n, int thr, int *__restrict__ par) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) par[i] = cnt[i] >= thr ? i : -1;
-}
-
-__global__ __launch_bounds__(kBlock) void k_uf_link(DevData D, SubspaceArg S, float s_star, int *__restrict__ par) {
-    extern __shared__ char smem[];
-    float *sx = reinterpret_cast<float *>(smem);
-    int *se = reinterpret_cast<int *>(smem + sizeof(float) * kBlock * S.k);
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= D.n || ld_acq(&par[q]) < 0) return;
-    float *qx = sx + threadIdx.x;
-    int *qe = se + threadIdx.x;
-    stage_query(D, S, q, qx, qe);
-    visit_neighbours(D, S, q, qx, qe, s_star, [&](int o) {
-        if (o > q && ld_acq(&par[o]) >= 0) uf_union(par, q, o);
-    });
-}
-
-__global__ void k_uf_flatten(int *__restrict__ par, int n, int *__restrict__ comp, int *__restrict__ csize) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int c = -1;
-    if (ld_acq(&par[i]) >= 0) {
-        c = uf_find(par, i);
-        atomicAdd(&csize[c], 1);
+// keep iff no superspace cluster touches the region or |W| * r > max_h min_h
+__global__ void k_decide(const int *__restrict__ nlist, int *__restrict__ mn, int nW, float r,
+                         int *__restrict__ keep) {
+    if (threadIdx.x != 0) return;
+    const int nl = *nlist;
+    int max_min = 0;
+    for (int h = 0; h < nl; ++h) {
+        if (mn[h] != INT_MAX && mn[h] > max_min) max_min = mn[h];
+        mn[h] = INT_MAX;
     }
-    comp[i] = c;
+    float lhs = (float) nW * r;
+    *keep = (max_min == 0 || (double) lhs > (double) max_min) ? 1 : 0;
 }
 
-// ------------------------------------------------------------ redundancy and join
-// pruneRedundancy (ScyTreeNode.cu:669-734): per superspace H, the smallest
-// result cluster of H that holds a point of the region.
-__global__ void k_redundancy(const int *__restrict__ W, const int *__restrict__ nW, int const *const *compH,
-                             int const *const *csizeH, uint8_t const *const *inresH, int *__restrict__ mn) {
-    int h = blockIdx.y;
+// cluster (ClusteringCpu.cu:317-389): a BFS from region point w labels w's
+// dense component; here the component is marked labelled.
+__global__ void k_seed_if(const int *__restrict__ W, int nW, const int *__restrict__ comp, const int *__restrict__ keep,
+                          uint8_t *__restrict__ lab) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int best = INT_MAX;
-    if (i < *nW) {
-        int w = W[i];
-        int c = compH[h][w];
-        if (c >= 0 && inresH[h][c]) best = csizeH[h][c];
-    }
-    for (int off = 32; off > 0; off >>= 1) best = min(best, __shfl_down(best, off));
-    if (__lane_id() == 0 && best != INT_MAX) atomicMin(&mn[h], best);
-}
-
-// cluster (ClusteringCpu.cu:317-389): the BFS from region point w labels w's
-// dense component; here the component id is marked as labelled.
-__global__ void k_seed(const int *__restrict__ W, const int *__restrict__ nW, const int *__restrict__ comp,
-                       uint8_t *__restrict__ lab) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *nW) return;
+    if (i >= nW || !*keep) return;
     int c = comp[W[i]];
     if (c >= 0) lab[c] = 1;
 }
 
-__global__ void k_collect(const uint8_t *__restrict__ lab, int n, const int *__restrict__ csize, int *__restrict__ out,
-                          int *__restrict__ nout) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool keep = i < n && lab[i];
-    int slot = wave_slot(keep, nout);
-    if (keep) {
-        out[2 * slot] = i;
-        out[2 * slot + 1] = csize[i];
+// join (util.cu:655-743), step 1: labelled components of at least min_size
+__global__ void k_join_keep(const uint8_t *__restrict__ lab, int n, const int *__restrict__ csize, int min_size,
+                            uint8_t *__restrict__ keepf, uint8_t *__restrict__ drop) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    keepf[k] = (lab[k] && csize[k] >= min_size) ? 1 : 0;
+    drop[k] = 0;
+}
+
+// step 2: drop K when one of its points lies in a result cluster of a
+// superspace H larger than |K| * r (walking the members of H's clusters)
+__global__ void k_join_check(const int *__restrict__ compS, const uint8_t *__restrict__ keepf,
+                             const int *__restrict__ csizeS, float r, const Entry *__restrict__ E,
+                             const int *__restrict__ list, const int *__restrict__ nlist, uint8_t *__restrict__ drop) {
+    const int nl = *nlist;
+    const int stride = gridDim.x * blockDim.x;
+    for (int h = 0; h < nl; ++h) {
+        const Entry e = E[list[h]];
+        for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < e.nmem; t += stride) {
+            int q = e.mem[t];
+            int K = compS[q];
+            if (K < 0 || !keepf[K]) continue;
+            if ((float) csizeS[K] * r < (float) e.csize[e.comp[q]]) drop[K] = 1;
+        }
     }
 }
 
-// join (util.cu:655-743): a cluster K of S is dropped when one of its points
-// lies in a result cluster of a superspace H that is larger than |K| * r.
-__global__ void k_join_check(const int *__restrict__ compS, const uint8_t *__restrict__ keep,
-                             const int *__restrict__ csizeS, int n, float r, int const *const *compH,
-                             int const *const *csizeH, uint8_t const *const *inresH, uint8_t *__restrict__ drop) {
-    int h = blockIdx.y;
+// step 3: insert the survivors into the result of S
+__global__ void k_join_apply(const uint8_t *__restrict__ keepf, const uint8_t *__restrict__ drop, int n,
+                             uint8_t *__restrict__ inres, uint8_t *__restrict__ fresh, Entry *__restrict__ E, int eS) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int added = 0;
+    if (k < n) {
+        fresh[k] = 0;
+        if (keepf[k] && !drop[k] && !inres[k]) {
+            inres[k] = 1;
+            fresh[k] = 1;
+            added = 1;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) added += __shfl_down(added, o);
+    if (__lane_id() == 0 && added) atomicAdd(&E[eS].nkept, added);
+}
+
+// step 4: append the members of the newly inserted clusters to S's member list
+__global__ void k_join_members(const int *__restrict__ compS, int n, const uint8_t *__restrict__ fresh,
+                               Entry *__restrict__ E, int eS) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    int k = compS[i];
-    if (k < 0 || !keep[k]) return;
-    int kh = compH[h][i];
-    if (kh < 0 || !inresH[h][kh]) return;
-    if ((float) csizeS[k] * r < (float) csizeH[h][kh]) drop[k] = 1;
+    bool in = false;
+    if (i < n) {
+        int K = compS[i];
+        in = K >= 0 && fresh[K];
+    }
+    int slot = wave_slot(in, &E[eS].nmem);
+    if (in) E[eS].mem[slot] = i;
+}
+
+__global__ void k_set_entry(Entry *__restrict__ E, int idx, Entry e, int *__restrict__ nE, int count) {
+    if (threadIdx.x == 0) {
+        E[idx] = e;
+        *nE = count;
+    }
 }
 
 __global__ void k_labels(const int *__restrict__ comp, const uint8_t *__restrict__ inres, int n, int *__restrict__ out) {
@@ -514,9 +755,9 @@ __global__ void k_fill(T *p, T v, size_t count) {
     if (i < count) p[i] = v;
 }
 
-__global__ void k_set_flags(uint8_t *flags, const int *ids, int count) {
+__global__ void k_gather(const int *__restrict__ src, const int *__restrict__ idx, int count, int *__restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) flags[ids[i]] = 1;
+    if (i < count) out[i] = src[idx[i]];
 }
 
 __global__ void k_iota(int *__restrict__ p, int n) {
@@ -535,8 +776,6 @@ __global__ void k_root_paths(const uint64_t *__restrict__ nearm, int n, int *__r
         outm[slot] = m;
     }
 }
-
-inline unsigned blocks_for(size_t count) { return (unsigned) ((count + kBlock - 1) / kBlock); }
 
 }  // namespace
 
@@ -562,158 +801,183 @@ float sqrt_threshold(float eps) {
     return f;
 }
 
+#define LAUNCH(kernel, grid, block, lds, st, ...) hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__)
+
 hipError_t launch_discretise(const float *X, int n, int d, int cells, float eps, float *Xt, uint8_t *cellC,
                              uint16_t *cellE, uint64_t *nearm, int *bad, hipStream_t st) {
     int ecells = (int) ceil(1.0 / (double) eps);
     float cs = 1.0f / (float) cells;
     float ecs = 1.0f / (float) ecells;
-    hipLaunchKernelGGL(k_discretise, dim3(blocks_for(n)), dim3(kBlock), 0, st, X, n, d, cells, cs, ecells, ecs, eps,
-                       Xt, cellC, cellE, nearm, bad);
+    LAUNCH(k_discretise, dim3(blocks_for(n)), dim3(kBlock), 0, st, X, n, d, cells, cs, ecells, ecs, eps, Xt, cellC,
+           cellE, nearm, bad);
     return hipGetLastError();
 }
 
-hipError_t launch_build_index(const uint16_t *cellE, int n, int d, int ecells, int *hist, int *off1, int *sid1,
-                              hipStream_t st) {
+hipError_t launch_hist1(const uint16_t *cellE, int n, int d, int ecells, int *hist, hipStream_t st) {
     size_t nd = (size_t) n * d;
-    hipLaunchKernelGGL(k_fill<int>, dim3(blocks_for((size_t) d * (ecells + 1))), dim3(kBlock), 0, st, hist, 0,
-                       (size_t) d * (ecells + 1));
-    hipLaunchKernelGGL(k_hist1, dim3(blocks_for(nd)), dim3(kBlock), 0, st, cellE, n, d, ecells, hist);
-    hipLaunchKernelGGL(k_scan1, dim3(d), dim3(kBlock), 0, st, hist, ecells, off1);
-    hipLaunchKernelGGL(k_scatter1, dim3(blocks_for(nd)), dim3(kBlock), 0, st, cellE, n, d, ecells, hist, sid1);
+    LAUNCH(k_fill<int>, dim3(blocks_for((size_t) d * ecells)), dim3(kBlock), 0, st, hist, 0, (size_t) d * ecells);
+    LAUNCH(k_hist1, dim3(blocks_for(nd)), dim3(kBlock), 0, st, cellE, n, d, ecells, hist);
     return hipGetLastError();
 }
 
-hipError_t launch_flags(const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, const uint8_t *cellC_m,
-                        int m, int *flags, hipStream_t st) {
-    if (nsp_max <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_flags, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, spx, spm, nsp, cellC_m, m, flags);
+size_t scan_temp_bytes(int items) {
+    size_t bytes = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int *) nullptr, (int *) nullptr, items) != hipSuccess)
+        return 1u << 20;
+    return bytes;
+}
+
+static hipError_t build_grid(const DevData &D, const SubspaceArg &S, const GridArg &G, const GridBuffers &B,
+                             hipStream_t st) {
+    const int n = D.n;
+    int ncell = 1;
+    for (int a = 0; a < G.j; ++a) ncell *= G.C;
+    LAUNCH(k_fill<int>, dim3(blocks_for(ncell + 1)), dim3(kBlock), 0, st, B.hist, 0, (size_t) ncell + 1);
+    LAUNCH(k_grid_keys, dim3(blocks_for(n)), dim3(kBlock), 0, st, D, S, G, B.keys, B.rank, B.hist);
+    size_t tb = B.scan_bytes;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(B.scan_temp, tb, B.hist, B.off, ncell + 1, st);
+    if (e != hipSuccess) return e;
+    LAUNCH(k_grid_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, D, S, B.keys, B.rank, B.off, B.order, B.Ys, B.Es);
     return hipGetLastError();
 }
 
-hipError_t launch_filter(const int *P, const int *nP, int nP_max, const int *spx, const uint64_t *spm,
-                         const int *nsp, int nsp_max, const uint8_t *cellC_m, int m, int lo, int hi, int *outP,
-                         int *outx, uint64_t *outm, int *counters, hipStream_t st) {
-    int bp = (int) blocks_for(nP_max), bs = (int) blocks_for(nsp_max);
-    if (bp + bs == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_filter, dim3(bp + bs), dim3(kBlock), 0, st, P, nP, spx, spm, nsp, cellC_m, m, lo, hi, outP,
-                       outx, outm, counters, bp);
+static unsigned query_blocks(int n) {
+    return (unsigned) ((n + kBlock / kLanesPerQuery - 1) / (kBlock / kLanesPerQuery));
+}
+
+static size_t query_lds(const SubspaceArg &S) { return (sizeof(float) + sizeof(int)) * (kBlock / kLanesPerQuery) * S.k; }
+
+hipError_t launch_subspace_counts(const DevData &D, const SubspaceArg &S, const GridArg &G, float s_star, int cap,
+                                  const GridBuffers &B, const int *mark, int markid, int *cnt,
+                                  unsigned long long *stat, hipStream_t st) {
+    hipError_t e = build_grid(D, S, G, B, st);
+    if (e != hipSuccess) return e;
+    LAUNCH(k_grid_count, dim3(query_blocks(D.n)), dim3(kBlock), query_lds(S), st, D.n, S, G, B.off, B.order, B.Ys,
+           B.Es, s_star, cap, mark, markid, (const int *) nullptr, 0, cnt, stat);
     return hipGetLastError();
 }
 
-hipError_t launch_counts(const DevData &D, const SubspaceArg &S, const int *queries, const int *nq, int nq_max,
-                         float s_star, int *cnt, unsigned long long *cand, hipStream_t st) {
-    if (nq_max <= 0) return hipSuccess;
-    size_t lds = (sizeof(float) + sizeof(int)) * kBlock * S.k;
-    hipLaunchKernelGGL(k_counts, dim3(blocks_for(nq_max)), dim3(kBlock), lds, st, D, S, queries, nq, s_star, cnt, cand);
+hipError_t launch_subspace_components(const DevData &D, const SubspaceArg &S, const GridArg &G, float s_star,
+                                      int cap, int lo, int thr_dense, const GridBuffers &B, const int *bound,
+                                      int *cnt, int *comp, int *csize, unsigned long long *stat, hipStream_t st) {
+    hipError_t e = build_grid(D, S, G, B, st);
+    if (e != hipSuccess) return e;
+    const int n = D.n;
+    LAUNCH(k_grid_count, dim3(query_blocks(n)), dim3(kBlock), query_lds(S), st, n, S, G, B.off, B.order, B.Ys, B.Es,
+           s_star, cap, (const int *) nullptr, 0, bound, lo, cnt, stat);
+    LAUNCH(k_dense_init, dim3(blocks_for(n)), dim3(kBlock), 0, st, B.order, n, cnt, thr_dense, B.dsorted, B.par, csize);
+    LAUNCH(k_link_sample, dim3(query_blocks(n)), dim3(kBlock), query_lds(S), st, n, S, G, B.off, B.order, B.Ys, B.Es,
+           B.dsorted, s_star, B.par);
+    LAUNCH(k_uf_compress, dim3(blocks_for(n)), dim3(kBlock), 0, st, B.par, n);
+    LAUNCH(k_giant, dim3(1), dim3(256), 0, st, B.par, n, B.giant);
+    LAUNCH(k_link_full, dim3(query_blocks(n)), dim3(kBlock), query_lds(S), st, n, S, G, B.off, B.order, B.Ys, B.Es,
+           B.dsorted, s_star, B.giant, B.par);
+    LAUNCH(k_uf_flatten, dim3(blocks_for(n)), dim3(kBlock), 0, st, B.par, n, comp, csize);
     return hipGetLastError();
 }
 
-hipError_t launch_weak(const int *P, const int *nP, int nP_max, const int *cnt, int thr, int *W, int *nW,
-                       int *stamp, int tid, hipStream_t st) {
+hipError_t launch_mark(const int *P, const int *nP, int nP_max, int *mark, int markid, hipStream_t st) {
     if (nP_max <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_weak, dim3(blocks_for(nP_max)), dim3(kBlock), 0, st, P, nP, cnt, thr, W, nW, stamp, tid);
+    LAUNCH(k_stamp, dim3(blocks_for(nP_max)), dim3(kBlock), 0, st, P, nP, mark, markid);
+    return hipGetLastError();
+}
+
+hipError_t launch_flags_all(const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, const uint8_t *cellC,
+                            int n, int first, int cells, int *flags, hipStream_t st) {
+    if (nsp_max <= 0) return hipSuccess;
+    LAUNCH(k_flags_all, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, spx, spm, nsp, cellC, n, first, cells, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_partition(const int *P, const int *nP, int nP_max, const int *spx, const uint64_t *spm,
+                            const int *nsp, int nsp_max, const uint8_t *cellC_m, const int *gid, int G,
+                            const int *weakcnt, int thr, int m, int *cnt, int *outP, int *outW, int *outx,
+                            uint64_t *outm, hipStream_t st) {
+    int bp = (int) blocks_for(nP_max), bs = (int) blocks_for(nsp_max);
+    LAUNCH(k_fill<int>, dim3(blocks_for((size_t) 6 * G)), dim3(kBlock), 0, st, cnt, 0, (size_t) 6 * G);
+    if (bp + bs == 0) return hipGetLastError();
+    PartArg A{P, nP, spx, spm, nsp, cellC_m, gid, weakcnt, thr, m, G, bp};
+    LAUNCH(k_part_count, dim3(bp + bs), dim3(kBlock), 0, st, A, cnt);
+    LAUNCH(k_part_scatter, dim3(bp + bs), dim3(kBlock), 0, st, A, cnt, outP, outW, outx, outm);
     return hipGetLastError();
 }
 
 hipError_t launch_path_prune(const DevData &D, uint64_t smask, const int *W, const int *nW, int nW_max,
-                             const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, const int *stamp,
-                             int tid, unsigned long long *bits, uint64_t *table, uint64_t tmask, int *outx,
-                             uint64_t *outm, int *nout, hipStream_t st) {
-    if (nsp_max <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill<uint64_t>, dim3(blocks_for(tmask + 1)), dim3(kBlock), 0, st, table, kEmpty,
-                       (size_t) (tmask + 1));
-    hipLaunchKernelGGL(k_fill<unsigned long long>, dim3(1), dim3(kBlock), 0, st, bits, 0ull, (size_t) 1);
-    hipLaunchKernelGGL(k_path_bits, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, spx, spm, nsp, stamp, tid, bits);
+                             const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, int *stamp, int tid,
+                             unsigned long long *bits, uint64_t *table, uint64_t tmask, int *outx, uint64_t *outm,
+                             int *nout, hipStream_t st) {
+    LAUNCH(k_fill<int>, dim3(1), dim3(kBlock), 0, st, nout, 0, (size_t) 1);
+    if (nsp_max <= 0) return hipGetLastError();
+    LAUNCH(k_fill<uint64_t>, dim3(blocks_for(tmask + 1)), dim3(kBlock), 0, st, table, kEmpty, (size_t) (tmask + 1));
+    LAUNCH(k_fill<unsigned long long>, dim3(1), dim3(kBlock), 0, st, bits, 0ull, (size_t) 1);
+    if (nW_max > 0) LAUNCH(k_stamp, dim3(blocks_for(nW_max)), dim3(kBlock), 0, st, W, nW, stamp, tid);
+    LAUNCH(k_path_bits, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, spx, spm, nsp, stamp, tid, bits);
     if (nW_max > 0)
-        hipLaunchKernelGGL(k_path_insert, dim3(blocks_for(nW_max)), dim3(kBlock), 0, st, D, smask, W, nW, bits, table,
-                           tmask);
-    hipLaunchKernelGGL(k_path_check, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, D, smask, spx, spm, nsp, stamp,
-                       tid, table, tmask, outx, outm, nout);
+        LAUNCH(k_path_insert, dim3(blocks_for(nW_max)), dim3(kBlock), 0, st, D, smask, W, nW, bits, table, tmask);
+    LAUNCH(k_path_check, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, D, smask, spx, spm, nsp, stamp, tid, table,
+           tmask, outx, outm, nout);
     return hipGetLastError();
 }
 
-hipError_t launch_all_queries(const int *cnt, int n, int *queries, int *nq, hipStream_t st) {
-    hipLaunchKernelGGL(k_all_queries, dim3(blocks_for(n)), dim3(kBlock), 0, st, cnt, n, queries, nq);
+hipError_t launch_redundancy_seed(const int *W, const int *nW_dev, int nW, int *stamp, int tid, const Entry *E,
+                                  const int *nE, uint64_t smask, const int *comp, int *list, int *nlist, int *mn,
+                                  float r, int *keep, uint8_t *lab, hipStream_t st) {
+    LAUNCH(k_select, dim3(1), dim3(kBlock), 0, st, E, nE, smask, list, nlist);
+    if (nW > 0) LAUNCH(k_stamp, dim3(blocks_for(nW)), dim3(kBlock), 0, st, W, nW_dev, stamp, tid);
+    LAUNCH(k_redundancy, dim3(256), dim3(kBlock), 0, st, stamp, tid, E, list, nlist, mn);
+    LAUNCH(k_decide, dim3(1), dim3(64), 0, st, nlist, mn, nW, r, keep);
+    if (nW > 0) LAUNCH(k_seed_if, dim3(blocks_for(nW)), dim3(kBlock), 0, st, W, nW, comp, keep, lab);
     return hipGetLastError();
 }
 
-hipError_t launch_components(const DevData &D, const SubspaceArg &S, float s_star, const int *cnt, int thr,
-                             int *parent, int *comp, int *csize, hipStream_t st) {
-    const int n = D.n;
-    size_t lds = (sizeof(float) + sizeof(int)) * kBlock * S.k;
-    hipLaunchKernelGGL(k_uf_init, dim3(blocks_for(n)), dim3(kBlock), 0, st, cnt, n, thr, parent);
-    hipLaunchKernelGGL(k_uf_link, dim3(blocks_for(n)), dim3(kBlock), lds, st, D, S, s_star, parent);
-    hipLaunchKernelGGL(k_fill<int>, dim3(blocks_for(n)), dim3(kBlock), 0, st, csize, 0, (size_t) n);
-    hipLaunchKernelGGL(k_uf_flatten, dim3(blocks_for(n)), dim3(kBlock), 0, st, parent, n, comp, csize);
+hipError_t launch_join(const uint8_t *lab, int n, const int *compS, const int *csizeS, int min_size, float r,
+                       Entry *E, const int *nE, uint64_t smask, int eS, uint8_t *inresS, int *list, int *nlist,
+                       uint8_t *keepf, uint8_t *drop, uint8_t *fresh, hipStream_t st) {
+    LAUNCH(k_join_keep, dim3(blocks_for(n)), dim3(kBlock), 0, st, lab, n, csizeS, min_size, keepf, drop);
+    LAUNCH(k_select, dim3(1), dim3(kBlock), 0, st, E, nE, smask, list, nlist);
+    LAUNCH(k_join_check, dim3(256), dim3(kBlock), 0, st, compS, keepf, csizeS, r, E, list, nlist, drop);
+    LAUNCH(k_join_apply, dim3(blocks_for(n)), dim3(kBlock), 0, st, keepf, drop, n, inresS, fresh, E, eS);
+    LAUNCH(k_join_members, dim3(blocks_for(n)), dim3(kBlock), 0, st, compS, n, fresh, E, eS);
     return hipGetLastError();
 }
 
-hipError_t launch_redundancy(const int *W, const int *nW, int nW_max, int const *const *compH,
-                             int const *const *csizeH, uint8_t const *const *inresH, int nH, int *mn,
-                             hipStream_t st) {
-    if (nW_max <= 0 || nH <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_redundancy, dim3(blocks_for(nW_max), nH), dim3(kBlock), 0, st, W, nW, compH, csizeH, inresH,
-                       mn);
-    return hipGetLastError();
-}
-
-hipError_t launch_seed(const int *W, const int *nW, int nW_max, const int *comp, uint8_t *lab, hipStream_t st) {
-    if (nW_max <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_seed, dim3(blocks_for(nW_max)), dim3(kBlock), 0, st, W, nW, comp, lab);
-    return hipGetLastError();
-}
-
-hipError_t launch_collect(const uint8_t *lab, int n, const int *csize, int *out, int *nout, hipStream_t st) {
-    hipLaunchKernelGGL(k_collect, dim3(blocks_for(n)), dim3(kBlock), 0, st, lab, n, csize, out, nout);
-    return hipGetLastError();
-}
-
-hipError_t launch_join_check(const int *compS, const uint8_t *keep, const int *csizeS, int n, float r,
-                             int const *const *compH, int const *const *csizeH, uint8_t const *const *inresH,
-                             int nH, uint8_t *drop, hipStream_t st) {
-    if (nH <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_join_check, dim3(blocks_for(n), nH), dim3(kBlock), 0, st, compS, keep, csizeS, n, r, compH,
-                       csizeH, inresH, drop);
+hipError_t launch_set_entry(Entry *E, int idx, const Entry &e, int *nE, int count, hipStream_t st) {
+    LAUNCH(k_set_entry, dim3(1), dim3(64), 0, st, E, idx, e, nE, count);
     return hipGetLastError();
 }
 
 hipError_t launch_labels(const int *comp, const uint8_t *inres, int n, int *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_labels, dim3(blocks_for(n)), dim3(kBlock), 0, st, comp, inres, n, out);
+    LAUNCH(k_labels, dim3(blocks_for(n)), dim3(kBlock), 0, st, comp, inres, n, out);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t count, hipStream_t st) {
     if (!count) return hipSuccess;
-    hipLaunchKernelGGL(k_fill<uint8_t>, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, v, count);
+    LAUNCH(k_fill<uint8_t>, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, v, count);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_i32(int *p, int v, size_t count, hipStream_t st) {
     if (!count) return hipSuccess;
-    hipLaunchKernelGGL(k_fill<int>, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, v, count);
+    LAUNCH(k_fill<int>, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, v, count);
     return hipGetLastError();
 }
 
-hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t count, hipStream_t st) {
-    if (!count) return hipSuccess;
-    hipLaunchKernelGGL(k_fill<uint64_t>, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, v, count);
-    return hipGetLastError();
-}
-
-hipError_t launch_set_flags(uint8_t *flags, const int *ids, int count, hipStream_t st) {
+hipError_t launch_gather(const int *src, const int *idx, int count, int *out, hipStream_t st) {
     if (count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_set_flags, dim3(blocks_for(count)), dim3(kBlock), 0, st, flags, ids, count);
+    LAUNCH(k_gather, dim3(blocks_for(count)), dim3(kBlock), 0, st, src, idx, count, out);
     return hipGetLastError();
 }
 
 hipError_t launch_iota(int *p, int n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_iota, dim3(blocks_for(n)), dim3(kBlock), 0, st, p, n);
+    LAUNCH(k_iota, dim3(blocks_for(n)), dim3(kBlock), 0, st, p, n);
     return hipGetLastError();
 }
 
 hipError_t launch_root_paths(const uint64_t *nearm, int n, int *outx, uint64_t *outm, int *nout, hipStream_t st) {
-    hipLaunchKernelGGL(k_root_paths, dim3(blocks_for(n)), dim3(kBlock), 0, st, nearm, n, outx, outm, nout);
+    LAUNCH(k_fill<int>, dim3(1), dim3(kBlock), 0, st, nout, 0, (size_t) 1);
+    LAUNCH(k_root_paths, dim3(blocks_for(n)), dim3(kBlock), 0, st, nearm, n, outx, outm, nout);
     return hipGetLastError();
 }
 
