@@ -1,0 +1,179 @@
+"""Benchmark: end-to-end INSCY on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode exact|batched] [--config 1]
+
+One step is one complete INSCY run (all subspaces, all clusters) over the
+synthetic data of the chosen workload, already resident in HBM; the timed
+region spans exactly K steps between device synchronisations.  The reported
+value is restricted (subspace, region) trees evaluated per second, the
+throughput unit of the reference's depth-first search; ms_per_step is the
+end-to-end time of one run.
+
+Also reported:
+  roofline      the neighbour-count kernel (the dominant one): its algorithmic
+                bytes (per counted point 4k+4, per neighbour established 4+4k)
+                over its device time measured with HIP events on the engine's
+                stream during the timed steps, against 8 TB/s HBM; `traffic`
+                comes from the committed rocprofv3 counter summary.
+  cpu_baseline  the oracle (a single-threaded CPU restatement of the
+                reference's CPU INSCY) on the same data for a bounded time
+                budget, as trees per second.
+  parity        the result equals the committed golden digests of the oracle.
+"""
+
+from __future__ import annotations
+
+import argparse
+import hashlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["exact", "batched"], default="exact")
+    ap.add_argument("--config", type=int, default=1, help="workload index (1: n=25000 d=15)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def golden_parity(cfg, res):
+    path = os.path.join(ROOT, "tests", "golden", f"config{cfg}.json")
+    if not os.path.exists(path):
+        return None
+    import numpy as np
+    g = json.load(open(path))
+    if [s for s, _, _, _ in g["subspaces"]] != res.subspaces:
+        return False
+    for (_, _, _, dig), lab in zip(g["subspaces"], res.labels):
+        if hashlib.sha256(np.ascontiguousarray(lab, dtype=np.int32).tobytes()).hexdigest() != dig:
+            return False
+    return res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
+
+
+def traffic_from_profile(cfg):
+    path = os.path.join(ROOT, "profiles", f"k_grid_count_config{cfg}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get("hbm_bytes_per_dispatch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
+              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from gpu_inscy_amd import _lib, datagen
+    from gpu_inscy_amd import inscy as INSCY
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    mode = _lib.MODE_EXACT if args.mode == "exact" else _lib.MODE_BATCHED
+    w = datagen.config(args.config)
+    X = torch.from_numpy(w.X).to("cuda:0")
+    torch.cuda.synchronize()
+
+    def step():
+        return INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, w.entropy_order,
+                         mode=mode)
+
+    res = None
+    for _ in range(max(args.warmup, 0)):
+        res = step()
+    if res is None:
+        res = step()
+    parity = golden_parity(args.config, res)
+
+    _lib.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    runs = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.profile(False)
+
+    trees = sum(r.stats["trees"] for r in runs)
+    kernel_ms = sum(r.stats["count_kernel_ms"] for r in runs)
+    launches = sum(r.stats["count_launches"] for r in runs)
+    kbytes = sum(r.stats["count_bytes"] for r in runs)
+    avg_launch_ms = kernel_ms / max(launches, 1)
+    achieved = (kbytes / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    roofline = {
+        "bound": "hbm",
+        "kernel": "k_grid_count",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 6),
+        "traffic": traffic_from_profile(args.config),
+        "algorithmic_bytes_per_launch": round(kbytes / max(launches, 1), 1),
+        "avg_launch_us": round(avg_launch_ms * 1e3, 3),
+        "launches_per_step": launches // max(args.steps, 1),
+        "kernel_share_of_step": round(kernel_ms / (elapsed * 1e3), 4),
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        ctrees, csecs, finished = O.ref_rate(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular,
+                                             max_seconds=args.cpu_seconds)
+        cpu = {
+            "value": round(ctrees / csecs, 4),
+            "unit": "subspaces/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": (f"first {ctrees} restricted trees of the depth-first run on the same {w.name} data "
+                       f"({csecs:.1f} s budget{'; complete run' if finished else ''}); the full run evaluates "
+                       f"{res.stats['trees']} trees"),
+        }
+
+    out = {
+        "metric": "end-to-end INSCY seconds + subspaces/s; exact vs batched vs learned-prune (F1)",
+        "value": round(trees / elapsed, 2),
+        "unit": "subspaces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{w.name}: float32 n={w.n} d={w.d}, eps={w.eps} c={w.cells} F={w.F} num_obj={w.num_obj} "
+                        f"min_size={w.min_size} r={w.r} rectangular={w.rectangular}",
+            "mode": args.mode,
+            "trees_per_step": trees // max(args.steps, 1),
+            "subspaces": res.stats["subspaces"],
+            "clusters": res.stats["clusters"],
+            "parallelism": "single GPU",
+        },
+        "parity": parity,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -26,7 +26,7 @@ MODE_BATCHED = 2
 EXPORTED = [
     "inscy_run", "inscy_run_device", "inscy_result_num_subspaces", "inscy_result_subspace",
     "inscy_result_labels", "inscy_result_num_points", "inscy_result_stats", "inscy_result_free",
-    "inscy_last_error", "inscy_discretise", "inscy_neighbour_counts", "inscy_dense_components",
+    "inscy_last_error", "inscy_discretise", "inscy_neighbour_counts", "inscy_dense_components", "inscy_profile",
 ]
 
 
@@ -59,6 +59,9 @@ class Stats(ctypes.Structure):
         ("clusters", ctypes.c_int64),
         ("components", ctypes.c_int64),
         ("seconds", ctypes.c_double),
+        ("count_kernel_ms", ctypes.c_double),
+        ("count_launches", ctypes.c_int64),
+        ("count_bytes", ctypes.c_int64),
     ]
 
 
@@ -103,6 +106,8 @@ def lib():
     L.inscy_dense_components.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(Params), i32p,
                                          ctypes.c_int32, vp]
     L.inscy_dense_components.restype = ctypes.c_int
+    L.inscy_profile.argtypes = [ctypes.c_int]
+    L.inscy_profile.restype = ctypes.c_int
     L.inscy_debug_trace.argtypes = [ctypes.c_int]
     L.inscy_debug_trace_rows.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]
     L.inscy_debug_trace_rows.restype = ctypes.c_int64
@@ -159,6 +164,10 @@ def run_device(ptr: int, n: int, d: int, params: Params, mode: int = MODE_EXACT,
     check(lib().inscy_run_device(ctypes.c_void_p(ptr), n, d, ctypes.byref(params), mode,
                                  ctypes.c_void_p(stream or None), ctypes.byref(h)))
     return Result(h)
+
+
+def profile(on: bool) -> None:
+    lib().inscy_profile(int(on))
 
 
 def debug_trace(on: bool) -> None:

@@ -164,6 +164,10 @@ public:
     Engine(const inscy_params &p, int n, int d, hipStream_t st) : prm(p), n(n), d(d), st(st) {}
 
     ~Engine() {
+        for (auto &t : timers) {
+            (void) hipEventDestroy(t.start);
+            (void) hipEventDestroy(t.stop);
+        }
         for (void *p : {(void *) Xt, (void *) cellC, (void *) cellE, (void *) nearm, (void *) small, (void *) stamp,
                         (void *) qmark, (void *) entries, (void *) scratch})
             if (p) (void) hipFree(p);
@@ -257,7 +261,7 @@ public:
         GridArg G = grid_of(S);
         size_t mk = arena.mark();
         GridBuffers B = grid_buffers(S, G);
-        HIP_OK(launch_subspace_counts(D, S, G, s_star, INT_MAX, B, nullptr, 0, s.cnt, nullptr, st));
+        HIP_OK(launch_subspace_counts(D, S, G, s_star, INT_MAX, B, nullptr, 0, s.cnt, nullptr, KernelTimer{}, st));
         arena.release(mk);
         return s.cnt;
     }
@@ -269,6 +273,7 @@ public:
         int lo, hi, pre, post, pass, redundant;
     };
     bool trace_on = false;
+    bool profile_on = false;
     std::vector<TraceRow> trace;
 
 private:
@@ -290,6 +295,16 @@ private:
     int *pin_i(size_t off = 0) { return reinterpret_cast<int *>(pin) + off; }
 
     void sync() { HIP_OK(hipStreamSynchronize(st)); }
+
+    // next event pair when profiling, else an empty timer
+    KernelTimer timer() {
+        if (!profile_on) return KernelTimer{};
+        KernelTimer t;
+        HIP_OK(hipEventCreate(&t.start));
+        HIP_OK(hipEventCreate(&t.stop));
+        timers.push_back(t);
+        return t;
+    }
 
     // Device result table and the scratch of its kernels, grown on demand.
     void grow_entries(int want) {
@@ -389,7 +404,8 @@ private:
         GridArg G = grid_of(S);
         size_t mk = arena.mark();
         GridBuffers B = grid_buffers(S, G);
-        HIP_OK(launch_subspace_counts(D, S, G, s_star, cap_of(S.k), B, qmark, markid, s.cnt, small + kStatAt, st));
+        HIP_OK(launch_subspace_counts(D, S, G, s_star, cap_of(S.k), B, qmark, markid, s.cnt, small + kStatAt, timer(),
+                                      st));
         arena.release(mk);
         return s;
     }
@@ -420,7 +436,7 @@ private:
         size_t mk = arena.mark();
         GridBuffers B = grid_buffers(S, G);
         HIP_OK(launch_subspace_components(D, S, G, s_star, cap_of(S.k), lo, dense_thr[S.k], B, bound, s.cnt, s.comp,
-                                          s.csize, small + kStatAt, st));
+                                          s.csize, small + kStatAt, timer(), st));
         arena.release(mk);
         HIP_OK(launch_fill_u8(s.inres, 0, n, st));
         grow_entries(nentries + 1);
@@ -612,7 +628,14 @@ private:
         for (int b = 0; b < 64; ++b) {
             stats.candidates += (int64_t) h2[16 * b];
             stats.queries += (int64_t) h2[16 * b + 1];
+            stats.count_bytes += (int64_t) h2[16 * b + 2];
         }
+        // device time of the neighbour-count kernel, from the event pairs
+        for (size_t i = 0; i < timers.size(); ++i) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, timers[i].start, timers[i].stop) == hipSuccess) stats.count_kernel_ms += ms;
+        }
+        stats.count_launches = (int64_t) timers.size();
     }
 
     inscy_params prm;
@@ -645,11 +668,13 @@ private:
     int *d_nE = nullptr, *d_nlist = nullptr, *d_keep = nullptr, *d_list = nullptr, *d_mn = nullptr;
     uint8_t *keepf = nullptr, *dropf = nullptr, *freshf = nullptr;
     int tree_id = 0;
+    std::vector<KernelTimer> timers;
 };
 
 static thread_local std::string g_error;
 static thread_local std::vector<Engine::TraceRow> g_trace;
 static bool g_trace_on = false;
+static bool g_profile_on = false;
 
 static int fail(int code, const std::string &msg) {
     g_error = msg;
@@ -684,6 +709,7 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
         {
             Engine eng(*p, (int) n, d, st);
             eng.trace_on = g_trace_on;
+            eng.profile_on = g_profile_on;
             eng.setup(d_X);
             eng.run(*res);
             res->stats = eng.stats;
@@ -764,6 +790,11 @@ int inscy_result_stats(const inscy_result *res, inscy_stats *out) {
 void inscy_result_free(inscy_result *res) { delete res; }
 
 const char *inscy_last_error(void) { return g_error.c_str(); }
+
+int inscy_profile(int on) {
+    g_profile_on = on != 0;
+    return INSCY_OK;
+}
 
 // Test hook (not in the public header): record one trace row per evaluated
 // restricted tree — (mask, lo, hi, points, survivors, pass, redundant).

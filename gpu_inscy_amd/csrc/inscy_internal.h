@@ -65,6 +65,11 @@ struct Entry {
     int nmem;
 };
 
+// Optional event pair bracketing the neighbour-count kernel of a launch.
+struct KernelTimer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
 // Largest float s with sqrtf(s) <= eps: the distance test fl(sqrt(sum)) <= eps
 // of ClusteringCpu.cu:52-81 is then sum <= s_star without a device sqrt.
 float sqrt_threshold(float eps);
@@ -78,12 +83,13 @@ hipError_t launch_hist1(const uint16_t *cellE, int n, int d, int ecells, int *hi
 // given, mark[p] == markid (stat: candidates examined, points counted)
 hipError_t launch_subspace_counts(const DevData &D, const SubspaceArg &S, const GridArg &G, float s_star, int cap,
                                   const GridBuffers &B, const int *mark, int markid, int *cnt,
-                                  unsigned long long *stat, hipStream_t st);
+                                  unsigned long long *stat, const KernelTimer &tm, hipStream_t st);
 // remaining counts, then the dense components of S (comp = smallest member id or -1)
 // (bound: complete counts of a subspace of S, or null; lo: the smaller threshold of S)
 hipError_t launch_subspace_components(const DevData &D, const SubspaceArg &S, const GridArg &G, float s_star,
                                       int cap, int lo, int thr_dense, const GridBuffers &B, const int *bound,
-                                      int *cnt, int *comp, int *csize, unsigned long long *stat, hipStream_t st);
+                                      int *cnt, int *comp, int *csize, unsigned long long *stat,
+                                      const KernelTimer &tm, hipStream_t st);
 hipError_t launch_mark(const int *P, const int *nP, int nP_max, int *mark, int markid, hipStream_t st);
 hipError_t launch_flags_all(const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, const uint8_t *cellC,
                             int n, int first, int cells, int *flags, hipStream_t st);

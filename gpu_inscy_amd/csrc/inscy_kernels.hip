@@ -322,11 +322,11 @@ __global__ __launch_bounds__(kBlock) void k_grid_count(int n, SubspaceArg S, Gri
     if (sub == 0 && active) stage_query(S, n, Ys, Es, i, sy + lq, se + lq, QPB);
     __syncthreads();
     long long seen = 0;
+    int found = 0;
     if (active) {
         const float *qy = sy + lq;
         const int *qe = se + lq;
         const int gshift = (__lane_id() / LPQ) * LPQ;
-        int found = 0;
         for_ranges(G, off, qe, QPB, [&](int a, int b) {
             for (int base = a; base < b && found < cap; base += LPQ) {
                 int jj = base + sub;
@@ -337,17 +337,25 @@ __global__ __launch_bounds__(kBlock) void k_grid_count(int n, SubspaceArg S, Gri
         });
         if (sub == 0) cnt[p] = found;
     }
-    if (stat) {  // per wave: candidates examined and points counted, into one of 64 slot pairs
-        long long done = (sub == 0 && active) ? 1 : 0;
+    if (stat) {
+        // per wave, into one of 64 slot groups (16 words apart): [0] candidates
+        // examined, [1] points counted, [2] algorithmic bytes = per point the
+        // query's k coordinates and its count (4k + 4) plus, per neighbour
+        // established, its id and k coordinates (4 + 4k)
+        const bool mine = sub == 0 && active;
+        long long done = mine ? 1 : 0;
+        long long abytes = mine ? (4ll * S.k + 4) + (long long) found * (4 + 4ll * S.k) : 0;
         if (sub != 0) seen = 0;
         for (int o = 32; o > 0; o >>= 1) {
             seen += __shfl_down(seen, o);
             done += __shfl_down(done, o);
+            abytes += __shfl_down(abytes, o);
         }
         if (__lane_id() == 0 && done) {
             unsigned long long *slot = stat + 16 * (blockIdx.x & 63);
             atomicAdd(&slot[0], (unsigned long long) seen);
             atomicAdd(&slot[1], (unsigned long long) done);
+            atomicAdd(&slot[2], (unsigned long long) abytes);
         }
     }
 }
@@ -849,22 +857,27 @@ static size_t query_lds(const SubspaceArg &S) { return (sizeof(float) + sizeof(i
 
 hipError_t launch_subspace_counts(const DevData &D, const SubspaceArg &S, const GridArg &G, float s_star, int cap,
                                   const GridBuffers &B, const int *mark, int markid, int *cnt,
-                                  unsigned long long *stat, hipStream_t st) {
+                                  unsigned long long *stat, const KernelTimer &tm, hipStream_t st) {
     hipError_t e = build_grid(D, S, G, B, st);
     if (e != hipSuccess) return e;
+    if (tm.start) (void) hipEventRecord(tm.start, st);
     LAUNCH(k_grid_count, dim3(query_blocks(D.n)), dim3(kBlock), query_lds(S), st, D.n, S, G, B.off, B.order, B.Ys,
            B.Es, s_star, cap, mark, markid, (const int *) nullptr, 0, cnt, stat);
+    if (tm.stop) (void) hipEventRecord(tm.stop, st);
     return hipGetLastError();
 }
 
 hipError_t launch_subspace_components(const DevData &D, const SubspaceArg &S, const GridArg &G, float s_star,
                                       int cap, int lo, int thr_dense, const GridBuffers &B, const int *bound,
-                                      int *cnt, int *comp, int *csize, unsigned long long *stat, hipStream_t st) {
+                                      int *cnt, int *comp, int *csize, unsigned long long *stat,
+                                      const KernelTimer &tm, hipStream_t st) {
     hipError_t e = build_grid(D, S, G, B, st);
     if (e != hipSuccess) return e;
     const int n = D.n;
+    if (tm.start) (void) hipEventRecord(tm.start, st);
     LAUNCH(k_grid_count, dim3(query_blocks(n)), dim3(kBlock), query_lds(S), st, n, S, G, B.off, B.order, B.Ys, B.Es,
            s_star, cap, (const int *) nullptr, 0, bound, lo, cnt, stat);
+    if (tm.stop) (void) hipEventRecord(tm.stop, st);
     LAUNCH(k_dense_init, dim3(blocks_for(n)), dim3(kBlock), 0, st, B.order, n, cnt, thr_dense, B.dsorted, B.par, csize);
     LAUNCH(k_link_sample, dim3(query_blocks(n)), dim3(kBlock), query_lds(S), st, n, S, G, B.off, B.order, B.Ys, B.Es,
            B.dsorted, s_star, B.par);

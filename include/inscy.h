@@ -64,6 +64,10 @@ typedef struct inscy_stats {
     int64_t clusters;       /* clusters in the result */
     int64_t components;     /* subspaces whose dense components were built */
     double seconds;         /* wall time of the run inside the library */
+    /* filled when profiling is on (inscy_profile): */
+    double count_kernel_ms; /* device time of the neighbour-count kernel (HIP events on the run's stream) */
+    int64_t count_launches; /* launches of that kernel */
+    int64_t count_bytes;    /* its algorithmic bytes: per counted point 4k+4, per neighbour found 4+4k */
 } inscy_stats;
 
 typedef struct inscy_result inscy_result;
@@ -95,6 +99,11 @@ int64_t inscy_result_num_points(const inscy_result *res);
 int inscy_result_stats(const inscy_result *res, inscy_stats *out);
 
 void inscy_result_free(inscy_result *res);
+
+/* Profiling switch for later runs of this process: brackets every launch of
+ * the neighbour-count kernel with HIP events (no counterpart in the
+ * reference; used by the benchmark's roofline figures). */
+int inscy_profile(int on);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *inscy_last_error(void);

@@ -21,6 +21,7 @@
 // is x86-64 g++ without FMA contraction).
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -531,7 +532,13 @@ struct Ctx {
     std::vector<TraceRow> trace;
     // thresholds
     float weak_rect_thr, weak_phi_thr;
+    // optional wall-clock budget (CPU-baseline sampling): the run stops at the
+    // first prune after the deadline
+    bool timed = false;
+    std::chrono::steady_clock::time_point deadline;
 };
+
+struct Deadline {};
 
 uint64_t mask_of(const std::vector<int> &S) {
     uint64_t m = 0;
@@ -648,6 +655,7 @@ void inscy(const Tree &T, Ctx &C, int first) {
             std::vector<int> pre_pts;
             collect_points(R, 0, pre_pts);
             int pre = (int) pre_pts.size();  // points of the merged region before the prune
+            if (C.timed && std::chrono::steady_clock::now() > C.deadline) throw Deadline{};
             bool pass = prune(R, C);
             C.trees++;
             TraceRow row{mask_of(S), lo, cell, pre, R.npoints, pass ? 1 : 0, -1};
@@ -1082,6 +1090,34 @@ void *oracle_sets_run(const float *X, int n, int d, float eps, float F, int num_
     run->queries = E.queries;
     run->nbr_total = 0;
     return run;
+}
+
+// CPU-baseline sample: runs the oracle for at most max_seconds of wall time
+// and reports the restricted trees evaluated and the seconds taken.
+void oracle_ref_rate(const float *X, int n, int d, float eps, float F, int num_obj, int min_size, float r, int cells,
+                     int rectangular, double max_seconds, long long *trees, double *seconds, int *finished) {
+    auto t0 = std::chrono::steady_clock::now();
+    Data D{X, n, d};
+    std::vector<int> order(d);
+    for (int i = 0; i < d; ++i) order[i] = i;
+    Tree root = build_tree(D, order, cells, eps);
+    int ecells = (int) std::ceil(1.0 / (double) eps);
+    Tree etree = build_tree(D, order, ecells, eps);
+    NeighbourIndex nbr(D, etree, order, false);
+    Ctx C{D, nbr, eps, F, r, num_obj, min_size, rectangular != 0};
+    C.weak_rect_thr = std::max(F * expden_k(d, eps, n), (float) num_obj);
+    C.weak_phi_thr = std::max(F * alpha_k(d, eps, n), (float) num_obj * omega_k(d));
+    C.timed = true;
+    C.deadline = t0 + std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                          std::chrono::duration<double>(max_seconds));
+    *finished = 1;
+    try {
+        inscy(root, C, 0);
+    } catch (const Deadline &) {
+        *finished = 0;
+    }
+    *trees = C.trees;
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 int oracle_ref_count(void *h) { return (int) static_cast<Run *>(h)->subspaces.size(); }

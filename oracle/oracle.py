@@ -105,6 +105,21 @@ def run_ref(X, eps, F, num_obj, min_size, r=1.0, cells=3, rectangular=False, ent
         L.oracle_ref_free(h)
 
 
+def ref_rate(X, eps, F, num_obj, min_size, r=1.0, cells=3, rectangular=False, max_seconds=20.0):
+    """Times the oracle on X for at most max_seconds: (trees evaluated, seconds, finished)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, d = X.shape
+    L = lib()
+    L.oracle_ref_rate.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_double, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_double),
+                                  ctypes.POINTER(ctypes.c_int)]
+    trees, secs, fin = ctypes.c_longlong(), ctypes.c_double(), ctypes.c_int()
+    L.oracle_ref_rate(_fp(X), n, d, eps, F, num_obj, min_size, r, cells, int(rectangular), float(max_seconds),
+                      ctypes.byref(trees), ctypes.byref(secs), ctypes.byref(fin))
+    return int(trees.value), float(secs.value), bool(fin.value)
+
+
 def neighbours(X, eps, S, p, literal=False):
     X = np.ascontiguousarray(X, dtype=np.float32)
     n, d = X.shape
