@@ -185,3 +185,67 @@ def test_drop_in_lists_and_cuda_input():
     assert subs == subs2 and cls == cls2
     assert len(cls[0]) == w.n
     assert INSCY.count_number_of_clusters(subs, cls) >= len(subs)
+
+
+def digests(res):
+    import hashlib
+    return [hashlib.sha256(np.ascontiguousarray(l, dtype=np.int32).tobytes()).hexdigest() for l in res.labels]
+
+
+def test_config1_matches_golden():
+    import json
+    import os
+    say("config1 golden")
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "config1.json")))
+    w = datagen.config(1)
+    res = INSCY.run(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0)
+    assert res.subspaces == [s for s, _, _, _ in g["subspaces"]]
+    assert digests(res) == [x[3] for x in g["subspaces"]]
+    assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
+
+
+def test_out_of_range_input_is_an_error():
+    say("out of range")
+    X = np.full((50, 3), 0.5, dtype=np.float32)
+    X[7, 2] = 1.25
+    with pytest.raises(_lib.InscyError) as e:
+        INSCY.run(torch.from_numpy(X).cuda(), 0.1, 1.0, 2, 5, 1.0, 3, True, 0)
+    assert e.value.code == _lib.INSCY_ERR_ARGUMENT
+
+
+def test_caller_stream_and_host_input_agree():
+    say("streams")
+    w = datagen.config(0)
+    a = INSCY.run(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        X = torch.from_numpy(w.X).cuda()
+        b = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0)
+    torch.cuda.synchronize()
+    assert a.subspaces == b.subspaces and np.array_equal(a.labels, b.labels)
+
+
+def check_invariants(res, w):
+    masks = [sum(1 << k for k in s) for s in res.subspaces]
+    assert masks == sorted(masks) and len(set(masks)) == len(masks)
+    for lab in res.labels:
+        assert lab.min() >= -1
+        k = int(lab.max()) + 1
+        assert k >= 1
+        # canonical: cluster c's smallest member precedes cluster c+1's
+        firsts = [int(np.flatnonzero(lab == c)[0]) for c in range(k)]
+        assert firsts == sorted(firsts)
+        sizes = np.bincount(lab[lab >= 0])
+        assert sizes.min() >= w.min_size
+
+
+def test_config2_full_size_deterministic_and_consistent():
+    say("config2 full size")
+    w = datagen.config(2)
+    X = torch.from_numpy(w.X).cuda()
+    a = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0)
+    b = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0)
+    assert a.subspaces == b.subspaces and np.array_equal(a.labels, b.labels)
+    assert a.stats["trees"] == b.stats["trees"] and a.stats["calls"] == b.stats["calls"]
+    check_invariants(a, w)
+    say("config2", a.stats)

@@ -1,0 +1,107 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every symbol of include/inscy.h, argument errors come back as codes without
+touching the device, the generator is deterministic, and the drop-in module
+mirrors the reference's Python surface (python/inscy.py)."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from gpu_inscy_amd import _lib, datagen
+from gpu_inscy_amd import inscy as INSCY
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "inscy.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(inscy_[a-z_]+)\s*\(", src)))
+
+
+def test_library_exports_every_header_function():
+    lib = _lib.lib()
+    names = header_functions()
+    assert len(names) >= 13
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(_lib.EXPORTED) <= set(names)
+
+
+def test_argument_errors_are_codes_not_crashes():
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    X = np.zeros((4, 2), dtype=np.float32)
+    p = _lib.make_params(0.1, 1.0, 2, 1, 1.0, 3, True, 0)
+    assert L.inscy_run(X.ctypes.data, 4, 2, None, 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
+    assert b"params" in L.inscy_last_error()
+    assert L.inscy_run(X.ctypes.data, 0, 2, ctypes.byref(p), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
+    assert L.inscy_run(X.ctypes.data, 4, 65, ctypes.byref(p), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
+    bad = _lib.make_params(0.0, 1.0, 2, 1, 1.0, 3, True, 0)
+    assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(bad), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
+    cells = _lib.make_params(0.1, 1.0, 2, 1, 1.0, 0, True, 0)
+    assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(cells), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
+    ent = _lib.make_params(0.1, 1.0, 2, 1, 1.0, 3, True, 1)
+    assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(ent), 1, ctypes.byref(h)) == _lib.INSCY_ERR_UNSUPPORTED
+    assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(p), 7, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
+    assert not h.value
+    with pytest.raises(_lib.InscyError):
+        _lib.check(_lib.INSCY_ERR_ARGUMENT)
+
+
+def test_result_accessors_tolerate_null():
+    L = _lib.lib()
+    assert L.inscy_result_num_subspaces(None) == 0
+    assert L.inscy_result_subspace(None, 0, None) == -1
+    assert not L.inscy_result_labels(None, 0)
+    assert L.inscy_result_num_points(None) == 0
+    L.inscy_result_free(None)
+
+
+def test_generator_is_deterministic_and_normalised():
+    a = datagen.config(0)
+    b = datagen.config(0)
+    assert np.array_equal(a.X, b.X) and a.X.dtype == np.float32
+    assert a.X.shape == (1500, 5)
+    assert np.allclose(a.X.min(axis=0), 0) and np.allclose(a.X.max(axis=0), 1)
+    assert (a.truth >= 0).sum() == 1350 and len(a.subspaces) == 4
+    c = datagen.config(0, repeat=1)
+    assert not np.array_equal(a.X, c.X)
+
+
+def test_normalize_matches_reference_formula():
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(50, 3)).astype(np.float32)
+    want = (x - x.min(0, keepdims=True)) / (x.max(0, keepdims=True) - x.min(0, keepdims=True))
+    assert np.array_equal(datagen.normalize(x), want.astype(np.float32))
+    torch = pytest.importorskip("torch")
+    t = INSCY.normalize(torch.from_numpy(x))
+    assert np.array_equal(t.numpy(), want.astype(np.float32))
+
+
+def test_drop_in_surface_mirrors_reference_names():
+    for name in ["CPU", "GPU", "GPU5", "GPU3", "GPU_star", "GPU_memory", "GPU_memory2", "run_cpu", "run_cpu_weak",
+                 "run_gpu", "run_gpu_weak", "run_gpu_reduced", "run_gpu_multi", "run_gpu_multi2",
+                 "run_gpu_multi2_cl_all", "run_gpu_multi2_cl_re_all", "run_gpu_multi2_cl_multi",
+                 "run_gpu_multi2_cl_multi_mem", "run_cpu_gpu_mix", "run_cpu_gpu_mix_cl_steam", "run_gpu_stream",
+                 "normalize", "clean_up", "count_number_of_clusters", "load_synt"]:
+        assert callable(getattr(INSCY, name)), name
+    import inspect
+    sig = inspect.signature(INSCY.CPU)
+    assert list(sig.parameters) == ["X", "neighborhood_size", "F", "num_obj", "min_size", "r", "number_of_cells",
+                                    "rectangular", "entropy_order"]
+    assert sig.parameters["r"].default == 1.0 and sig.parameters["number_of_cells"].default == 3
+    assert sig.parameters["rectangular"].default is False and sig.parameters["entropy_order"].default == 0
+
+
+def test_count_number_of_clusters_and_clean_up():
+    subs = [[0], [1], [0, 1]]
+    cl = [[0, 0, 1, -1, -2], [-1, -1, -1, -1, -1], [3, 3, 3, 3, 5]]
+    assert INSCY.count_number_of_clusters(subs, cl) == 4
+    s2, c2 = INSCY.clean_up(subs, cl, 2)
+    assert [list(x) for x in s2] == [[0], [0, 1]]
+    assert c2[0].tolist() == [0, 0, -1, -1, -2]
+    assert c2[1].tolist() == [3, 3, 3, 3, -1]

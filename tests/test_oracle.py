@@ -1,0 +1,249 @@
+"""Pins the oracle (the CPU restatement of the reference's CPU INSCY) against
+what fixes it independently of its own code: closed forms, brute force, a
+literal walk of the neighbourhood tree, a second restatement (set-based) of
+the whole algorithm, planted mistakes that the cross-check must catch, and the
+committed golden fixture of the first workload."""
+
+import math
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+from scipy.sparse import csr_matrix
+from scipy.sparse.csgraph import connected_components
+
+from gpu_inscy_amd import datagen
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# ------------------------------------------------------------------ thresholds
+def ball_closed_form(k):
+    # c(k) = PI^(k/2) / Gamma(k/2 + 1) with the reference's PI = 3.14 used both in
+    # the power and in Gamma(1/2) = sqrt(PI) (ClusteringCpu.cu:25,136-150):
+    # even k: PI^(k/2) / (k/2)!;  odd k: PI^((k-1)/2) 2^((k+1)/2) / k!!
+    if k % 2 == 0:
+        return 3.14 ** (k // 2) / math.factorial(k // 2)
+    double_fact = math.prod(range(k, 0, -2))
+    return 3.14 ** ((k - 1) // 2) * 2 ** ((k + 1) // 2) / double_fact
+
+
+@pytest.mark.parametrize("k", list(range(1, 33)))
+def test_unit_ball_volume_closed_form(k):
+    got = O.thresholds(k, 0.01, 1000)["c"]
+    assert got == pytest.approx(ball_closed_form(k), rel=2e-6)
+
+
+def test_unit_ball_uses_pi_314_not_math_pi():
+    got = O.thresholds(2, 0.01, 1000)["c"]
+    assert got == pytest.approx(3.14, rel=1e-7)
+    assert abs(got - math.pi) > 1e-3
+    assert O.thresholds(1, 0.01, 1000)["c"] == 2.0  # PI cancels in one dimension
+    assert O.thresholds(3, 0.01, 1000)["c"] == pytest.approx(4 * 3.14 / 3, rel=1e-7)
+
+
+@pytest.mark.parametrize("k,eps,n", [(1, 0.01, 1500), (2, 0.015, 25000), (5, 0.02, 100000), (15, 0.015, 25000),
+                                     (20, 0.02, 100000)])
+def test_density_thresholds_closed_form(k, eps, n):
+    t = O.thresholds(k, eps, n)
+    c = ball_closed_form(k)
+    e32 = float(np.float32(eps))
+    assert t["expden"] == pytest.approx(n * c * e32 ** k, rel=1e-5)
+    assert t["alpha"] == pytest.approx(2 * n * e32 ** k * c / (k + 2), rel=1e-5)
+    assert t["omega"] == pytest.approx(2.0 / (k + 2), rel=1e-7)
+
+
+# ------------------------------------------------------------------ neighbourhood
+def brute_neighbours(X, eps, S, p):
+    """O(n) brute force of the reference's rule: eps-grid cells differ by <= 1 in
+    every dim of S, float32 sum of squares in S order, sqrt <= eps, q != p."""
+    X = np.asarray(X, dtype=np.float32)
+    ec = int(math.ceil(1.0 / float(np.float32(eps))))
+    ecs = np.float32(1.0) / np.float32(ec)
+    cells = np.minimum((X / ecs).astype(np.int64), ec - 1)
+    ok = np.ones(len(X), dtype=bool)
+    for k in S:
+        ok &= np.abs(cells[:, k] - cells[p, k]) <= 1
+    acc = np.zeros(len(X), dtype=np.float32)
+    for k in S:
+        diff = (X[p, k] - X[:, k]).astype(np.float32)
+        acc = (acc + (diff * diff).astype(np.float32)).astype(np.float32)
+    dist = np.sqrt(acc).astype(np.float32)
+    ok &= dist <= np.float32(eps)
+    ok[p] = False
+    return set(np.flatnonzero(ok).tolist())
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_neighbours_index_equals_literal_walk_and_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    w = datagen.small(seed, n=400, d=5, n_clusters=2, eps=0.05, sigma=0.03)
+    for _ in range(8):
+        k = int(rng.integers(1, 6))
+        S = sorted(rng.choice(5, size=k, replace=False).tolist())
+        p = int(rng.integers(0, w.n))
+        eps = float(rng.choice([0.01, 0.05, 0.1]))
+        a = O.neighbours(w.X, eps, S, p, literal=False)
+        b = O.neighbours(w.X, eps, S, p, literal=True)
+        assert a.tolist() == b.tolist()  # same list, same order (tree traversal order)
+        assert set(a.tolist()) == brute_neighbours(w.X, eps, S, p)
+
+
+def test_grid_rule_misses_pairs_the_plain_distance_would_keep():
+    # eps = 0.01 gives ceil(1/eps) = 101 cells of width < eps: a pair 0.0099.. apart
+    # across two cell boundaries is within eps but not a candidate
+    X = np.array([[0.02966], [0.03961], [0.5]], dtype=np.float32)
+    w = np.float32(1.0) / np.float32(101)
+    assert int(X[0, 0] / w) == 2 and int(X[1, 0] / w) == 4
+    assert abs(np.float32(X[1, 0] - X[0, 0])) <= np.float32(0.01)
+    assert O.neighbours(X, 0.01, [0], 0).tolist() == []
+
+
+def test_dimension_order_entropy():
+    rng = np.random.default_rng(3)
+    X = rng.random((500, 4)).astype(np.float32)
+    X[:, 2] = np.where(rng.random(500) < 0.9, X[:, 2] * 0.3, X[:, 2]).astype(np.float32)  # skewed, no empty cell
+    assert O.dimension_order(X, 3, 0).tolist() == [0, 1, 2, 3]
+    order = O.dimension_order(X, 3, 1).tolist()
+    assert order[0] == 2 and sorted(order) == [0, 1, 2, 3]
+    assert O.dimension_order(X, 3, -1).tolist() == order[::-1]
+    # an empty cell makes the reference's entropy 0*log(0) = NaN
+    # (inscy_map.cpp:94-95); the order is still a permutation
+    X[:, 1] = np.float32(0.1)
+    assert sorted(O.dimension_order(X, 3, 1).tolist()) == [0, 1, 2, 3]
+
+
+# ------------------------------------------------------------------ two restatements agree
+def fuzz_case(seed):
+    rng = np.random.default_rng(seed + 99991)
+    n = int(rng.integers(3, 16))
+    d = int(rng.integers(2, 6))
+    cells = int(rng.integers(2, 5))
+    eps = float(rng.choice([0.05, 0.1, 0.2, 0.3, 0.5]))
+    mu = int(rng.integers(1, 4))
+    ms = int(rng.integers(1, 4))
+    rect = bool(rng.integers(0, 2))
+    X = (rng.integers(0, 2 * cells + 1, size=(n, d)) / (2.0 * cells)).astype(np.float32)
+    X += (rng.random((n, d)) * 0.02).astype(np.float32)
+    X = np.clip(X, 0, 1).astype(np.float32)
+    X[0] = 0.0
+    X[1] = 1.0
+    return X, eps, mu, ms, cells, rect
+
+
+def small_case(seed):
+    rng = np.random.default_rng(seed + 500)
+    n = int(rng.integers(60, 300))
+    d = int(rng.integers(2, 6))
+    cells = int(rng.integers(2, 6))
+    eps = float(rng.choice([0.02, 0.05, 0.1, 0.15]))
+    mu = int(rng.integers(1, 6))
+    ms = int(rng.integers(1, max(2, n // 8)))
+    rect = bool(rng.integers(0, 2))
+    r = float(rng.choice([1.0, 0.5, 1.5]))
+    w = datagen.small(seed, n=n, d=d, n_clusters=int(rng.integers(1, 4)), eps=eps, cells=cells, num_obj=mu,
+                      min_size=ms, rectangular=rect, sigma=float(rng.choice([0.02, 0.05, 0.1])))
+    return w.X, eps, mu, ms, cells, rect, r
+
+
+def agree(X, eps, mu, ms, cells, rect, r=1.0):
+    a = O.run_ref(X, eps, 1.0, mu, ms, r, cells, rect, 0, trace=True)
+    b = O.run_sets(X, eps, 1.0, mu, ms, r, cells, rect, threads=1, trace=True)
+    ok, msg = O.same_result(a, b)
+    return ok and np.array_equal(a.trace, b.trace) and a.calls == b.calls, msg
+
+
+def test_pointer_tree_and_set_restatements_agree_tiny():
+    bad = [s for s in range(400) if not agree(*fuzz_case(s))[0]]
+    assert not bad, f"seeds {bad[:10]}"
+
+
+def test_pointer_tree_and_set_restatements_agree_small():
+    bad = [s for s in range(60) if not agree(*small_case(s))[0]]
+    assert not bad, f"seeds {bad[:10]}"
+
+
+@pytest.mark.parametrize("mutation", [1, 3])
+def test_cross_check_catches_planted_mistakes(mutation):
+    # the set restatement run with a planted mistake in its s-connection prune
+    # (1: paths never pruned, 3: the sibling rule ignored) must disagree with the
+    # pointer-tree restatement on some of the same cases
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from tests.test_oracle import agree, fuzz_case\n"
+            "print(sum(1 for s in range(300) if not agree(*fuzz_case(s))[0]))\n") % ROOT
+    env = dict(os.environ, ORACLE_SETS_MUTATE=str(mutation))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert int(out.stdout.strip().splitlines()[-1]) > 0
+
+
+def test_simplified_path_rule_is_exact():
+    # the engine's rule (a path survives iff a surviving point shares its cells
+    # on every unrestricted dimension up to its branch) equals the reference's
+    # three-part rule on every case
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from tests.test_oracle import agree, fuzz_case, small_case\n"
+            "bad = sum(1 for s in range(300) if not agree(*fuzz_case(s))[0])\n"
+            "bad += sum(1 for s in range(40) if not agree(*small_case(s))[0])\n"
+            "print(bad)\n") % ROOT
+    env = dict(os.environ, ORACLE_SETS_MUTATE="5")
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert int(out.stdout.strip().splitlines()[-1]) == 0
+
+
+# ------------------------------------------------------------------ clusters are dense components
+def test_clusters_are_components_of_the_dense_graph():
+    w = datagen.config(0)
+    ref = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0)
+    n = w.n
+    for S, lab in zip(ref.subspaces, ref.labels):
+        lists = [brute_neighbours(w.X, w.eps, S, p) for p in range(n)]
+        thr = max(w.F * O.thresholds(len(S), w.eps, n)["expden"], float(w.num_obj))
+        dense = np.array([np.float32(len(l)) >= np.float32(thr) for l in lists])
+        rows, cols = [], []
+        for p in range(n):
+            if dense[p]:
+                for q in lists[p]:
+                    if dense[q]:
+                        rows.append(p)
+                        cols.append(q)
+        g = csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n))
+        _, comp = connected_components(g, directed=False)
+        canon = O.canonical(lab)
+        for c in range(canon.max() + 1):
+            members = np.flatnonzero(canon == c)
+            assert dense[members].all()
+            comp_ids = set(comp[members].tolist())
+            assert len(comp_ids) == 1
+            whole = np.flatnonzero((comp == comp[members[0]]) & dense)
+            assert np.array_equal(np.sort(members), whole)
+            assert len(members) >= w.min_size
+
+
+# ------------------------------------------------------------------ golden fixture
+def test_oracle_matches_config0_golden():
+    g = np.load(os.path.join(ROOT, "tests", "golden", "config0.npz"))
+    w = datagen.config(0)
+    ref = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0, trace=True)
+    masks = [sum(1 << k for k in s) for s in ref.subspaces]
+    assert masks == g["subspace_masks"].tolist()
+    assert np.array_equal(np.stack([O.canonical(l) for l in ref.labels]), g["labels"])
+    assert np.array_equal(ref.trace, g["trace"])
+
+
+def test_literal_walk_oracle_matches_index_oracle_on_config0():
+    w = datagen.config(0)
+    a = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0, literal=False, trace=True)
+    b = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0, literal=True, trace=True)
+    assert O.same_result(a, b)[0] and np.array_equal(a.trace, b.trace)
+
+
+def test_non_rectangular_mode_oracles_agree_on_config0_shape():
+    w = datagen.config(0)
+    a = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, False, 0, trace=True)
+    b = O.run_sets(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, False, threads=2, trace=True)
+    assert O.same_result(a, b)[0]
