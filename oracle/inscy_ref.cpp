@@ -1159,6 +1159,56 @@ void oracle_ref_trace(void *h, long long *out) {
 
 void oracle_ref_free(void *h) { delete static_cast<Run *>(h); }
 
+// join / pruneRedundancy on hand-built inputs.  The result map is given as
+// nres entries: dims (ascending, lengths in res_k), labels [nres][n].
+// oracle_join_test writes the updated map (same entry order as the map's
+// vec_cmp order, at most nres+1 entries) and returns the new entry count.
+static Result make_result(int n, int nres, const int *res_k, const int *res_dims, const int *res_labels) {
+    Result r;
+    int at = 0;
+    for (int e = 0; e < nres; ++e) {
+        std::vector<int> S(res_dims + at, res_dims + at + res_k[e]);
+        at += res_k[e];
+        r.emplace(S, std::vector<int>(res_labels + (size_t) e * n, res_labels + (size_t) (e + 1) * n));
+    }
+    return r;
+}
+
+int oracle_join_test(int n, int nres, const int *res_k, const int *res_dims, const int *res_labels, const int *cl_in,
+                     int k, const int *dims, int min_size, float r, int *out_k, int *out_dims, int *out_labels) {
+    Result res = make_result(n, nres, res_k, res_dims, res_labels);
+    std::vector<int> cl(cl_in, cl_in + n);
+    join(res, cl, std::vector<int>(dims, dims + k), min_size, r);
+    int e = 0, at = 0;
+    for (const auto &kv : res) {
+        out_k[e] = (int) kv.first.size();
+        for (int x : kv.first) out_dims[at++] = x;
+        std::memcpy(out_labels + (size_t) e * n, kv.second.data(), n * sizeof(int));
+        ++e;
+    }
+    return e;
+}
+
+// pruneRedundancy of a region holding `pts` (its point count npoints) in subspace dims.
+int oracle_redundancy_test(int n, int nres, const int *res_k, const int *res_dims, const int *res_labels,
+                           const int *pts, int npts, int k, const int *dims, float r) {
+    Result res = make_result(n, nres, res_k, res_dims, res_labels);
+    Tree t;
+    t.rdims.assign(dims, dims + k);
+    t.add(-1, npts, true);
+    t.nodes[0].pts.assign(pts, pts + npts);
+    t.npoints = npts;
+    // prune_redundancy reads only the region's points and count, C.result and C.r
+    Data D{nullptr, n, 0};
+    Tree et;
+    et.cells = 1;
+    et.cell_size = 1.0f;
+    NeighbourIndex idx(D, et, {}, true);
+    Ctx C{D, idx, 1.0f, 1.0f, r, 1, 1, true};
+    C.result = res;
+    return prune_redundancy(t, C) ? 1 : 0;
+}
+
 // Pieces exposed for unit checks against closed forms and brute force.
 float oracle_unit_ball(int k) { return unit_ball(k); }
 float oracle_alpha(int k, float eps, int n) { return alpha_k(k, eps, n); }

@@ -201,3 +201,43 @@ def same_result(a: OracleRun, b: OracleRun) -> tuple:
         if not np.array_equal(ca, cb):
             return False, f"labels differ in {s}: {int((ca != cb).sum())} points"
     return True, "ok"
+
+
+def _flat_result(result, n):
+    ks = np.array([len(s) for s, _ in result], dtype=np.int32)
+    dims = np.array([x for s, _ in result for x in s] or [0], dtype=np.int32)
+    labs = np.ascontiguousarray(np.array([l for _, l in result], dtype=np.int32).reshape(len(result), n))
+    return ks, dims, labs
+
+
+def join_test(result, cl, S, min_size, r=1.0):
+    """join (util.cu:655-743) on a hand-built result [(dims, labels)] and
+    clustering cl of subspace S; returns the updated result in map order."""
+    n = len(cl)
+    L = lib()
+    ks, dims, labs = _flat_result(result, n)
+    m = len(result) + 1
+    ok = np.zeros(m, dtype=np.int32)
+    od = np.zeros(64 * m, dtype=np.int32)
+    ol = np.zeros((m, n), dtype=np.int32)
+    S = np.asarray(S, dtype=np.int32)
+    cl = np.ascontiguousarray(cl, dtype=np.int32)
+    L.oracle_join_test.restype = ctypes.c_int
+    e = L.oracle_join_test(n, len(result), _ip(ks), _ip(dims), _ip(labs) if len(result) else None, _ip(cl),
+                           len(S), _ip(S), min_size, ctypes.c_float(r), _ip(ok), _ip(od), _ip(ol))
+    out, at = [], 0
+    for i in range(e):
+        out.append((od[at:at + ok[i]].tolist(), ol[i].tolist()))
+        at += ok[i]
+    return out
+
+
+def redundancy_test(result, n, pts, S, r=1.0):
+    """pruneRedundancy (ScyTreeNode.cu:669-734) of a region with points pts in S."""
+    L = lib()
+    ks, dims, labs = _flat_result(result, n)
+    pts = np.ascontiguousarray(pts, dtype=np.int32)
+    S = np.asarray(S, dtype=np.int32)
+    L.oracle_redundancy_test.restype = ctypes.c_int
+    return bool(L.oracle_redundancy_test(n, len(result), _ip(ks), _ip(dims), _ip(labs) if len(result) else None,
+                                         _ip(pts), len(pts), len(S), _ip(S), ctypes.c_float(r)))

@@ -247,3 +247,50 @@ def test_non_rectangular_mode_oracles_agree_on_config0_shape():
     a = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, False, 0, trace=True)
     b = O.run_sets(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, False, threads=2, trace=True)
     assert O.same_result(a, b)[0]
+
+
+# ------------------------------------------------------------------ join / pruneRedundancy, hand-built
+def test_join_nothing_labelled_leaves_result():
+    res = [([0, 1], [0, 0, -1, -1])]
+    assert O.join_test(res, [-1, -2, -1, -1], [0], 1) == res
+
+
+def test_join_drops_small_clusters_then_inserts():
+    out = O.join_test([], [0, 0, 0, 1, -1], [2], 2)
+    assert out == [([2], [0, 0, 0, -1, -1])]
+
+
+def test_join_redundancy_against_superspace_depends_on_r():
+    res = [([0, 1], [5, 5, 5, 5, -1])]  # superspace cluster of 4 holding the new cluster's points
+    cl = [0, 0, 0, -1, -1]               # cluster of 3 in S = {0}
+    assert O.join_test(res, cl, [0], 1, r=1.0) == res            # 3 * 1.0 < 4: removed, nothing inserted
+    assert O.join_test(res, cl, [0], 1, r=1.5) == [([0], cl), ([0, 1], [5, 5, 5, 5, -1])]  # 4.5 >= 4: kept
+    # {1, 2} is not a superspace of {0}: no removal
+    res2 = [([1, 2], [5, 5, 5, 5, -1])]
+    assert O.join_test(res2, cl, [0], 1) == [([0], cl), ([1, 2], [5, 5, 5, 5, -1])]
+
+
+def test_join_merges_into_existing_entry_with_offset_labels():
+    res = [([1], [0, 0, -1, -1, -1])]    # max label m = 0
+    cl = [-1, -2, -1, 3, 3]
+    out = O.join_test(res, cl, [1], 1)
+    # cl >= 0 -> m + 1 + cl = 4; cl == -2 copied; others keep the old label
+    assert out == [([1], [0, -2, -1, 4, 4])]
+
+
+def test_join_result_order_is_ascending_bitmask():
+    res = [([0, 1], [0, -1, -1]), ([2], [0, -1, -1]), ([1], [0, -1, -1])]
+    out = O.join_test(res, [0, 0, -1], [0, 2], 1, r=10.0)
+    assert [s for s, _ in out] == [[1], [0, 1], [2], [0, 2]]
+
+
+def test_redundancy_hand_built():
+    res = [([0, 1], [7, 7, 7, 9, 9, -1])]   # cluster 7 of 3, cluster 9 of 2
+    assert not O.redundancy_test(res, 6, [3, 4], [0], r=1.0)   # 2 * 1.0 > 2 is false
+    assert O.redundancy_test(res, 6, [3, 4], [0], r=1.5)       # 3 > 2
+    assert not O.redundancy_test(res, 6, [0, 3], [0], r=1.0)   # min over touched clusters = 2
+    assert O.redundancy_test(res, 6, [5], [0], r=1.0)           # touches no cluster: keep
+    assert O.redundancy_test(res, 6, [3, 4], [2], r=1.0)       # {0, 1} is not a superspace of {2}
+    res2 = res + [([0, 2], [1, 1, 1, 1, 1, -1])]              # second superspace, min 5
+    assert not O.redundancy_test(res2, 6, [0, 1, 2, 3], [0], r=1.0)   # max of mins = 5 >= 4
+    assert O.redundancy_test(res2, 6, [0, 1, 2, 3, 4, 5], [0], r=1.0)  # 6 > 5
