@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libinscy_hip.so")
-SOURCES = ["inscy_kernels.hip", "inscy_engine.hip"]
+SOURCES = ["inscy_kernels.hip", "inscy_level_kernels.hip", "inscy_level.hip", "inscy_engine.hip"]
 ARCH = os.environ.get("INSCY_OFFLOAD_ARCH", "gfx950")
 
 

@@ -1,0 +1,156 @@
+// Device helpers shared by the kernel files of the MI355X INSCY engine:
+// wave-level compaction, the grid-index candidate walk and distance test,
+// union-find, and the hash of the path prune.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "inscy_internal.h"
+
+namespace inscy {
+
+
+constexpr int kBlock = 256;
+constexpr int kLanesPerQuery = 16;  // lanes cooperating on one query's candidate ranges
+
+__device__ __forceinline__ int ld_relaxed(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_relaxed(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave-aggregated slot allocation for order-free stream compaction: one
+// atomic per wave.  Must be reached by every active lane of the wave.
+__device__ __forceinline__ int wave_slot(bool keep, int *counter) {
+    unsigned long long m = __ballot(keep);
+    if (m == 0) return -1;
+    int lane = __lane_id();
+    int leader = __ffsll((long long) m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader);
+    return keep ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
+}
+
+// counter[key] += (number of lanes with that key and want), one atomic per
+// distinct key in the wave; returns each wanting lane's slot (old value +
+// rank among the lanes sharing its key).  Must be reached by every active lane.
+__device__ __forceinline__ int wave_add_by_key(int *counter, int key, bool want) {
+    unsigned long long todo = __ballot(want);
+    const int lane = __lane_id();
+    int result = 0;
+    while (todo) {
+        int leader = __ffsll((long long) todo) - 1;
+        int lk = __shfl(key, leader);
+        unsigned long long same = __ballot(want && key == lk) & todo;
+        int b = 0;
+        if (lane == leader) b = atomicAdd(&counter[lk], __popcll(same));
+        b = __shfl(b, leader);
+        if ((same >> lane) & 1ull) result = b + __popcll(same & ((1ull << lane) - 1ull));
+        todo &= ~same;
+    }
+    return result;
+}
+
+inline unsigned blocks_for(size_t count) { return (unsigned) ((count + kBlock - 1) / kBlock); }
+
+template <class Range>
+__device__ __forceinline__ void for_ranges(const GridArg &G, const int *__restrict__ off, const int *qe, int stride,
+                                           Range f) {
+    const int C = G.C;
+    if (G.j == 1) {
+        int c0 = qe[G.tI[0] * stride];
+        f(off[max(c0 - 1, 0)], off[min(c0 + 1, C - 1) + 1]);
+    } else if (G.j == 2) {
+        int c0 = qe[G.tI[0] * stride], c1 = qe[G.tI[1] * stride];
+        int lo = max(c1 - 1, 0), hi = min(c1 + 1, C - 1);
+        for (int x = max(c0 - 1, 0); x <= min(c0 + 1, C - 1); ++x) f(off[x * C + lo], off[x * C + hi + 1]);
+    } else {
+        int c0 = qe[G.tI[0] * stride], c1 = qe[G.tI[1] * stride], c2 = qe[G.tI[2] * stride];
+        int lo = max(c2 - 1, 0), hi = min(c2 + 1, C - 1);
+        for (int x = max(c0 - 1, 0); x <= min(c0 + 1, C - 1); ++x)
+            for (int y = max(c1 - 1, 0); y <= min(c1 + 1, C - 1); ++y) {
+                int base = (x * C + y) * C;
+                f(off[base + lo], off[base + hi + 1]);
+            }
+    }
+}
+
+// candidate at sorted position jj against the staged query; S order, no FMA
+__device__ __forceinline__ bool is_neighbour(const SubspaceArg &S, const GridArg &G, int n,
+                                             const float *__restrict__ Ys, const uint16_t *__restrict__ Es,
+                                             const float *qy, const int *qe, int stride, int jj, float s_star) {
+    float acc = 0.0f;
+    for (int t = 0; t < S.k; ++t) {
+        if ((G.checkmask >> t) & 1ull) {
+            int de = (int) Es[(size_t) t * n + jj] - qe[t * stride];
+            if (de < -1 || de > 1) return false;
+        }
+        float diff = qy[t * stride] - Ys[(size_t) t * n + jj];
+        float sq = diff * diff;
+        acc = acc + sq;
+        if (acc > s_star) return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void stage_query(const SubspaceArg &S, int n, const float *__restrict__ Ys,
+                                            const uint16_t *__restrict__ Es, int i, float *sy, int *se, int stride) {
+    for (int t = 0; t < S.k; ++t) {
+        sy[t * stride] = Ys[(size_t) t * n + i];
+        se[t * stride] = Es[(size_t) t * n + i];
+    }
+}
+
+__device__ __forceinline__ int uf_find(int *par, int x) {
+    int p = ld_relaxed(&par[x]);
+    while (p != x) {
+        int g = ld_relaxed(&par[p]);
+        if (g != p) st_relaxed(&par[x], g);  // path halving: g is an ancestor of x
+        x = p;
+        p = g;
+    }
+    return x;
+}
+
+__device__ __forceinline__ void uf_union(int *par, int a, int b) {
+    while (true) {
+        a = uf_find(par, a);
+        b = uf_find(par, b);
+        if (a == b) return;
+        if (a > b) {
+            int t = a;
+            a = b;
+            b = t;
+        }
+        int old = atomicCAS(&par[b], b, a);
+        if (old == b) return;
+        b = old;
+    }
+}
+
+constexpr uint64_t kEmpty = ~0ull;
+
+__device__ __forceinline__ uint64_t mix64(uint64_t h) {
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    h *= 0xc4ceb9fe1a85ec53ull;
+    h ^= h >> 33;
+    return h;
+}
+
+__device__ __forceinline__ bool same_prefix(const DevData &D, uint64_t smask, int a, int b, int upto) {
+    if (b < 0 || b >= D.n) return false;
+    for (int k = 0; k <= upto; ++k) {
+        if ((smask >> k) & 1ull) continue;
+        if (D.cellC[(size_t) k * D.n + a] != D.cellC[(size_t) k * D.n + b]) return false;
+    }
+    return true;
+}
+
+}  // namespace inscy

@@ -1,675 +1,7 @@
-// Host driver and C ABI of the MI355X INSCY engine.
-//
-// The reference drives INSCY from the host with one pointer tree per
-// recursion node (INSCYCPU3Weak, src/algorithms/inscy/InscyNodeCpu.cu:159-235)
-// or with device SCY-tree arrays plus many synchronous scalar reads per
-// restricted tree (InscyArrayGpu4/5, InscyArrayGpuMulti2.cu:569-836).  Here a
-// tree lives on the device as three flat lists:
-//   P   — its points (int32),
-//   SPx/SPm — its s-connection paths: origin point and a bitmask of the
-//         dimensions at which that point's chain branches (only dimensions
-//         above the last restricted one are kept, the only ones a descendant
-//         can still restrict),
-// plus the host-side bitmask S of its restricted dimensions.  restrict +
-// mergeWithNeighbors become one flag pass and one filter pass, the weak-density
-// prune is a neighbour-count pass plus compaction, and clustering works on the
-// dense components of S over the whole data set, computed once per subspace.
-// The host keeps the depth-first order of the reference because
-// pruneRedundancy and join read the partial result.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <chrono>
-#include <climits>
-#include <cmath>
-#include <cstring>
-#include <map>
-#include <memory>
-#include <set>
-#include <stdexcept>
-#include <string>
-#include <unordered_map>
-#include <vector>
-
-#include "../../include/inscy.h"
-#include "inscy_internal.h"
-
-struct inscy_result {
-    int64_t n = 0;
-    std::vector<std::vector<int32_t>> subspaces;
-    std::vector<std::vector<int32_t>> labels;
-    inscy_stats stats{};
-};
+// C ABI of the MI355X INSCY engine (include/inscy.h).
+#include "inscy_engine.h"
 
 namespace inscy {
-
-struct Failure : std::runtime_error {
-    int code;
-    Failure(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIP_OK(expr)                                                                                       \
-    do {                                                                                                   \
-        hipError_t e_ = (expr);                                                                            \
-        if (e_ != hipSuccess)                                                                              \
-            throw Failure(e_ == hipErrorOutOfMemory ? INSCY_ERR_MEMORY : INSCY_ERR_DEVICE,                 \
-                          std::string(#expr) + ": " + hipGetErrorString(e_));                              \
-    } while (0)
-
-// ------------------------------------------------------------ thresholds
-// ClusteringCpu.cu:27-38,136-183 with PI = 3.14; float results of
-// double-promoted pow() as in the reference.
-static float gamma_half(int m) {
-    if (m == 2) return 1.0f;
-    if (m == 1) return (float) std::sqrt(3.14);
-    return (float) ((m / 2.0 - 1.0) * (double) gamma_half(m - 2));
-}
-static float ball(int k) {
-    float v = (float) std::pow(3.14, k / 2.0);
-    return v / gamma_half(k + 2);
-}
-static float expden(int k, float eps, int n) {
-    float nc = (float) n * ball(k);
-    float v = (float) ((double) nc * std::pow((double) eps, (double) k));
-    return (float) ((double) v / std::pow(1.0, (double) k));
-}
-// Smallest integer count c with (float)c >= t (counts are compared as floats
-// in the reference, ScyTreeNode.cu:616 and ClusteringCpu.cu:176-183).
-static int count_threshold(float t) {
-    if (!(t > 0.0f)) return 0;
-    double c = std::ceil((double) t);
-    if (c > 2147483647.0) return INT_MAX;
-    int v = (int) c;
-    while (v > 0 && (float) (v - 1) >= t) --v;
-    while ((float) v < t) ++v;
-    return v;
-}
-
-// ------------------------------------------------------------ memory
-// Stack-disciplined bump allocator for the per-tree lists of the depth-first
-// recursion (a child's lists are released when its subtree is done).
-struct Arena {
-    char *base = nullptr;
-    size_t cap = 0, top = 0;
-    ~Arena() {
-        if (base) (void) hipFree(base);
-    }
-    void init(size_t bytes) {
-        cap = bytes;
-        HIP_OK(hipMalloc(&base, bytes));
-    }
-    template <class T>
-    T *take(size_t count) {
-        size_t at = (top + 255) & ~(size_t) 255;
-        size_t bytes = std::max<size_t>(count * sizeof(T), 1);
-        if (at + bytes > cap) throw Failure(INSCY_ERR_MEMORY, "device arena exhausted");
-        top = at + bytes;
-        return reinterpret_cast<T *>(base + at);
-    }
-    size_t mark() const { return top; }
-    void release(size_t m) { top = m; }
-};
-
-// Grow-only slabs for per-subspace arrays kept for the whole run.
-struct Slabs {
-    std::vector<void *> chunks;
-    char *cur = nullptr;
-    size_t left = 0, chunk = 256u << 20;
-    ~Slabs() {
-        for (void *p : chunks) (void) hipFree(p);
-    }
-    template <class T>
-    T *take(size_t count) {
-        size_t bytes = ((count * sizeof(T)) + 255) & ~(size_t) 255;
-        if (bytes > left) {
-            size_t sz = std::max(chunk, bytes);
-            void *p = nullptr;
-            HIP_OK(hipMalloc(&p, sz));
-            chunks.push_back(p);
-            cur = static_cast<char *>(p);
-            left = sz;
-        }
-        T *r = reinterpret_cast<T *>(cur);
-        cur += bytes;
-        left -= bytes;
-        return r;
-    }
-};
-
-struct TreeRef {
-    uint64_t smask;
-    const int *P;      // device point ids
-    const int *nP;     // device count
-    int nP_max;        // host bound (exact where the host knows it)
-    const int *spx;    // device path origins
-    const uint64_t *spm;
-    const int *nsp;
-    int nsp_max;
-};
-
-// What the algorithm needs about one subspace S, computed on first use:
-// neighbour counts of every point (the prune reads them for its region), then,
-// once a tree of S survives its prune, the dense components of S over the
-// whole data set and its row in the device result table.
-struct Subspace {
-    int *cnt = nullptr;        // [n] |N_S(p)| capped, -1 = not counted yet
-    int *comp = nullptr;       // [n] component root (smallest id) or -1
-    int *csize = nullptr;      // [n] component sizes indexed by root
-    uint8_t *inres = nullptr;  // [n] component kept in the result of S
-    int entry = -1;            // row in the device result table
-};
-
-class Engine {
-public:
-    Engine(const inscy_params &p, int n, int d, hipStream_t st) : prm(p), n(n), d(d), st(st) {}
-
-    ~Engine() {
-        for (auto &t : timers) {
-            (void) hipEventDestroy(t.start);
-            (void) hipEventDestroy(t.stop);
-        }
-        for (void *p : {(void *) Xt, (void *) cellC, (void *) cellE, (void *) nearm, (void *) small, (void *) stamp,
-                        (void *) qmark, (void *) entries, (void *) scratch})
-            if (p) (void) hipFree(p);
-        if (pin) (void) hipHostFree(pin);
-    }
-
-    // ---------------------------------------------------------- set-up
-    void setup(const float *d_X) {
-        cells = prm.number_of_cells;
-        ecells = (int) std::ceil(1.0 / (double) prm.neighborhood_size);
-        if (ecells > 65535) throw Failure(INSCY_ERR_UNSUPPORTED, "neighborhood_size below 1/65535 is not supported");
-        s_star = sqrt_threshold(prm.neighborhood_size);
-        weak_thr = count_threshold(std::max(prm.F * expden(d, prm.neighborhood_size, n), (float) prm.num_obj));
-        for (int k = 0; k <= d; ++k)
-            dense_thr.push_back(
-                count_threshold(std::max(prm.F * expden(k, prm.neighborhood_size, n), (float) prm.num_obj)));
-
-        size_t nd = (size_t) n * d;
-        HIP_OK(hipMalloc(&Xt, nd * sizeof(float)));
-        HIP_OK(hipMalloc(&cellC, nd));
-        HIP_OK(hipMalloc(&cellE, nd * sizeof(uint16_t)));
-        HIP_OK(hipMalloc(&nearm, (size_t) n * sizeof(uint64_t)));
-        HIP_OK(hipMalloc(&small, kSmallWords * sizeof(unsigned long long)));
-        HIP_OK(hipMalloc(&stamp, (size_t) n * sizeof(int)));
-        HIP_OK(hipMalloc(&qmark, (size_t) n * sizeof(int)));
-        HIP_OK(hipHostMalloc(&pin, pin_bytes()));
-        grow_entries(1 << 12);
-        // grid index size: C^j cells with j <= 3 and C^j <= max(8n, 4096)
-        grid_cap = std::max<long long>(8LL * n, 4096);
-        grid_cap = std::max<long long>(std::min<long long>(grid_cap, 1LL << 24), ecells);
-        scan_bytes = scan_temp_bytes((int) grid_cap + 1);
-        // arena: per recursion level the partitions of all children of one tree
-        // (points, weak-dense points, paths) plus one child's pruned paths,
-        // and the scratch of one subspace grid pass
-        size_t per_level = (size_t) 24 * d * n + (size_t) 64 * n;
-        size_t pass = (size_t) n * (32 + (size_t) d * 6) + (size_t) (grid_cap + 1) * 8 + scan_bytes + (1u << 20);
-        size_t want = per_level * (size_t) (d + 2) + table_bytes_max() + pass + (size_t) n * 16 + (64u << 20);
-        size_t free_b = 0, total_b = 0;
-        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) want = std::min(want, free_b / 3);
-        arena.init(want);
-
-        int *hist = arena.take<int>((size_t) d * ecells);
-        HIP_OK(launch_fill_i32(reinterpret_cast<int *>(small), 0, 2 * kSmallWords, st));
-        HIP_OK(launch_discretise(d_X, n, d, cells, prm.neighborhood_size, Xt, cellC, cellE, nearm,
-                                 reinterpret_cast<int *>(small), st));
-        HIP_OK(launch_hist1(cellE, n, d, ecells, hist, st));
-        HIP_OK(launch_fill_i32(stamp, -1, n, st));
-        HIP_OK(launch_fill_i32(qmark, -1, n, st));
-        std::vector<int> h((size_t) d * ecells);
-        HIP_OK(hipMemcpyAsync(pin, small, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpyAsync(h.data(), hist, h.size() * sizeof(int), hipMemcpyDeviceToHost, st));
-        sync();
-        if (reinterpret_cast<int *>(pin)[0] != 0)
-            throw Failure(INSCY_ERR_ARGUMENT, "X must be finite with every value in [0, 1] (normalize first)");
-        // selectivity of each dimension as an index: expected candidates of a
-        // query in the three-cell slab, sum_c h_c (h_{c-1} + h_c + h_{c+1})
-        score.assign(d, 0.0);
-        for (int k = 0; k < d; ++k) {
-            const int *hk = h.data() + (size_t) k * ecells;
-            for (int c = 0; c < ecells; ++c) {
-                double nb = hk[c] + (c > 0 ? hk[c - 1] : 0) + (c + 1 < ecells ? hk[c + 1] : 0);
-                score[k] += (double) hk[c] * nb;
-            }
-        }
-        arena.release(0);
-        D = DevData{Xt, cellC, cellE, n, d, cells, ecells};
-    }
-
-    // ---------------------------------------------------------- run
-    void run(inscy_result &out) {
-        int *P = arena.take<int>(n);
-        int *nP = arena.take<int>(1);
-        int *spx = arena.take<int>(n);
-        uint64_t *spm = arena.take<uint64_t>(n);
-        int *nsp = arena.take<int>(1);
-        HIP_OK(launch_iota(P, n, st));
-        HIP_OK(launch_fill_i32(nP, n, 1, st));
-        HIP_OK(launch_root_paths(nearm, n, spx, spm, nsp, st));
-        TreeRef root{0ull, P, nP, n, spx, spm, nsp, n};
-        recurse(root, 0);
-        emit(out);
-    }
-
-    // dense components of one subspace (kernel-level entry point)
-    const Subspace &subspace(uint64_t smask) { return ensure_components(smask); }
-
-    // exact |N_S(p)| of every point, without the threshold cap (kernel-level entry point)
-    const int *exact_counts(uint64_t smask) {
-        Subspace &s = sub_of(smask);
-        SubspaceArg S = arg_of(smask);
-        GridArg G = grid_of(S);
-        size_t mk = arena.mark();
-        GridBuffers B = grid_buffers(S, G);
-        HIP_OK(launch_subspace_counts(D, S, G, s_star, INT_MAX, B, nullptr, 0, s.cnt, nullptr, KernelTimer{}, st));
-        arena.release(mk);
-        return s.cnt;
-    }
-
-    inscy_stats stats{};
-
-    struct TraceRow {
-        long long mask;
-        int lo, hi, pre, post, pass, redundant;
-    };
-    bool trace_on = false;
-    bool profile_on = false;
-    std::vector<TraceRow> trace;
-
-private:
-    // ---------------------------------------------------------- helpers
-    // `small` device words: [0] input check flag, [16..24) path-prune scratch,
-    // [kStatAt..) 64 slot pairs (16 words apart) of neighbour-count statistics
-    static constexpr int kStatAt = 32;
-    static constexpr int kSmallWords = kStatAt + 16 * 64;
-    size_t pin_bytes() const {
-        return std::max<size_t>((size_t) n * 8 + (size_t) 2 * d * 256 * 8 + (1u << 16), 1u << 20);
-    }
-    static size_t table_slots(int nW, int bits) {
-        size_t want = (size_t) std::max(1, nW) * (size_t) std::max(1, bits) * 2;
-        size_t s = 1024;
-        while (s < want) s <<= 1;
-        return s;
-    }
-    size_t table_bytes_max() const { return table_slots(n, d) * sizeof(uint64_t); }
-    int *pin_i(size_t off = 0) { return reinterpret_cast<int *>(pin) + off; }
-
-    void sync() { HIP_OK(hipStreamSynchronize(st)); }
-
-    // next event pair when profiling, else an empty timer
-    KernelTimer timer() {
-        if (!profile_on) return KernelTimer{};
-        KernelTimer t;
-        HIP_OK(hipEventCreate(&t.start));
-        HIP_OK(hipEventCreate(&t.stop));
-        timers.push_back(t);
-        return t;
-    }
-
-    // Device result table and the scratch of its kernels, grown on demand.
-    void grow_entries(int want) {
-        if (want <= ecap) return;
-        int cap = std::max(want, 2 * ecap);
-        Entry *ne = nullptr;
-        int *ns = nullptr;
-        HIP_OK(hipMalloc(&ne, (size_t) cap * sizeof(Entry)));
-        // scratch: nE, nlist, keep, pad | list[cap] | mn[cap] | keepf[n], drop[n], fresh[n] (bytes)
-        HIP_OK(hipMalloc(&ns, (size_t) (8 + 2 * cap) * sizeof(int) + (size_t) 3 * n + 256));
-        if (entries) {
-            sync();
-            HIP_OK(hipMemcpy(ne, entries, (size_t) nentries * sizeof(Entry), hipMemcpyDeviceToDevice));
-            HIP_OK(hipFree(entries));
-            HIP_OK(hipFree(scratch));
-        }
-        entries = ne;
-        scratch = ns;
-        ecap = cap;
-        d_nE = scratch;
-        d_nlist = scratch + 1;
-        d_keep = scratch + 2;
-        d_list = scratch + 8;
-        d_mn = scratch + 8 + cap;
-        keepf = reinterpret_cast<uint8_t *>(scratch + 8 + 2 * cap);
-        dropf = keepf + n;
-        freshf = dropf + n;
-        HIP_OK(launch_fill_i32(d_mn, INT_MAX, cap, st));
-        HIP_OK(launch_fill_i32(d_nE, nentries, 1, st));
-    }
-
-    SubspaceArg arg_of(uint64_t smask) const {
-        SubspaceArg S{};
-        S.k = 0;
-        for (int k = 0; k < d; ++k)
-            if ((smask >> k) & 1ull) S.dims[S.k++] = k;
-        return S;
-    }
-
-    GridArg grid_of(const SubspaceArg &S) const {
-        GridArg G{};
-        std::vector<int> ts(S.k);
-        for (int t = 0; t < S.k; ++t) ts[t] = t;
-        std::stable_sort(ts.begin(), ts.end(), [&](int a, int b) { return score[S.dims[a]] < score[S.dims[b]]; });
-        G.C = ecells;
-        G.j = 1;
-        long long cellsj = ecells;
-        while (G.j < 3 && G.j < S.k && cellsj * ecells <= grid_cap) {
-            cellsj *= ecells;
-            G.j++;
-        }
-        G.checkmask = S.k >= 64 ? ~0ull : ((1ull << S.k) - 1ull);
-        for (int a = 0; a < G.j; ++a) {
-            G.tI[a] = ts[a];
-            G.checkmask &= ~(1ull << ts[a]);
-        }
-        return G;
-    }
-
-    GridBuffers grid_buffers(const SubspaceArg &S, const GridArg &G) {
-        long long ncell = 1;
-        for (int a = 0; a < G.j; ++a) ncell *= G.C;
-        GridBuffers B{};
-        B.keys = arena.take<int>(n);
-        B.rank = arena.take<int>(n);
-        B.hist = arena.take<int>(ncell + 1);
-        B.off = arena.take<int>(ncell + 1);
-        B.order = arena.take<int>(n);
-        B.Ys = arena.take<float>((size_t) S.k * n);
-        B.Es = arena.take<uint16_t>((size_t) S.k * n);
-        B.dsorted = arena.take<uint8_t>(n);
-        B.par = arena.take<int>(n);
-        B.giant = arena.take<int>(1);
-        B.scan_temp = arena.take<char>(scan_bytes);
-        B.scan_bytes = scan_bytes;
-        return B;
-    }
-
-    // counts are compared only with the weak threshold (prune) and the dense
-    // threshold of |S| (clustering), so they are needed up to the larger one
-    int cap_of(int k) const { return std::max(std::max(weak_thr, dense_thr[k]), 1); }
-
-    Subspace &sub_of(uint64_t smask) {
-        Subspace &s = subs[smask];
-        if (!s.cnt) {
-            s.cnt = slabs.take<int>(n);
-            HIP_OK(launch_fill_i32(s.cnt, -1, n, st));
-        }
-        return s;
-    }
-
-    // |N_S(p)| of the points stamped with markid that are not counted yet.
-    Subspace &ensure_counts(uint64_t smask, int markid) {
-        Subspace &s = sub_of(smask);
-        if (s.entry >= 0) return s;  // all points counted by the components pass
-        SubspaceArg S = arg_of(smask);
-        GridArg G = grid_of(S);
-        size_t mk = arena.mark();
-        GridBuffers B = grid_buffers(S, G);
-        HIP_OK(launch_subspace_counts(D, S, G, s_star, cap_of(S.k), B, qmark, markid, s.cnt, small + kStatAt, timer(),
-                                      st));
-        arena.release(mk);
-        return s;
-    }
-
-    // Dense components of S (counting every point not yet counted) and its
-    // result-table row, once per subspace.
-    Subspace &ensure_components(uint64_t smask) {
-        Subspace &s = sub_of(smask);
-        if (s.entry >= 0) return s;
-        SubspaceArg S = arg_of(smask);
-        GridArg G = grid_of(S);
-        s.comp = slabs.take<int>(n);
-        s.csize = slabs.take<int>(n);
-        s.inres = slabs.take<uint8_t>(n);
-        int *mem = slabs.take<int>(n);
-        // a subspace S' = S minus one dimension whose counts are complete bounds
-        // the counts of S from above (exact where below its own cap)
-        const int *bound = nullptr;
-        int lo = std::min(weak_thr, dense_thr[S.k]);
-        for (int t = 0; t < S.k && !bound; ++t) {
-            uint64_t sub = smask & ~(1ull << S.dims[t]);
-            auto it = subs.find(sub);
-            if (sub && it != subs.end() && it->second.entry >= 0) {
-                bound = it->second.cnt;
-                lo = std::min(lo, cap_of(S.k - 1));
-            }
-        }
-        size_t mk = arena.mark();
-        GridBuffers B = grid_buffers(S, G);
-        HIP_OK(launch_subspace_components(D, S, G, s_star, cap_of(S.k), lo, dense_thr[S.k], B, bound, s.cnt, s.comp,
-                                          s.csize, small + kStatAt, timer(), st));
-        arena.release(mk);
-        HIP_OK(launch_fill_u8(s.inres, 0, n, st));
-        grow_entries(nentries + 1);
-        Entry e{smask, s.comp, s.csize, s.inres, mem, 0, 0};
-        s.entry = nentries++;
-        HIP_OK(launch_set_entry(entries, s.entry, e, d_nE, nentries, st));
-        stats.components++;
-        return s;
-    }
-
-    // ---------------------------------------------------------- recursion
-    struct Group {
-        int lo, hi;
-    };
-
-    void recurse(const TreeRef &T, int first) {
-        stats.calls++;
-        if (first >= d) return;
-        const int nd = d - first;
-        const size_t lvl_mark = arena.mark();
-
-        // restrict + mergeWithNeighbors for every dimension m >= first: the
-        // s-connected cells, then the cell groups (ScyTreeNode.cu:837-851)
-        int *flags = arena.take<int>((size_t) nd * cells);
-        HIP_OK(launch_fill_i32(flags, 0, (size_t) nd * cells, st));
-        HIP_OK(launch_flags_all(T.spx, T.spm, T.nsp, T.nsp_max, cellC, n, first, cells, flags, st));
-        // the subspaces of the children are counted on this tree's points while
-        // the flags travel
-        const int markid = ++mark_id;
-        HIP_OK(launch_mark(T.P, T.nP, T.nP_max, qmark, markid, st));
-        for (int m = first; m < d; ++m) ensure_counts(T.smask | (1ull << m), markid);
-        HIP_OK(hipMemcpyAsync(pin, flags, (size_t) nd * cells * sizeof(int), hipMemcpyDeviceToHost, st));
-        sync();
-        std::vector<std::vector<Group>> groups(nd);
-        std::vector<int> gid((size_t) nd * cells);
-        for (int m = first; m < d; ++m) {
-            const int *f = pin_i((size_t) (m - first) * cells);
-            auto &gs = groups[m - first];
-            for (int c = 0; c < cells; ++c) {
-                int lo = c;
-                while (f[c] && c < cells - 1) ++c;
-                for (int x = lo; x <= c; ++x) gid[(size_t) (m - first) * cells + x] = (int) gs.size();
-                gs.push_back({lo, c});
-            }
-        }
-        int *d_gid = arena.take<int>(gid.size());
-        std::copy(gid.begin(), gid.end(), pin_i());
-        HIP_OK(hipMemcpyAsync(d_gid, pin, gid.size() * sizeof(int), hipMemcpyHostToDevice, st));
-
-        // partition points, weak-dense points and paths by group, all dimensions
-        struct Part {
-            int *cnt, *P, *W, *x;
-            uint64_t *mk;
-        };
-        std::vector<Part> parts(nd);
-        for (int m = first; m < d; ++m) {
-            const int G = (int) groups[m - first].size();
-            Part &pt = parts[m - first];
-            pt.cnt = arena.take<int>((size_t) 6 * G);
-            pt.P = arena.take<int>(std::max(T.nP_max, 1));
-            pt.W = arena.take<int>(std::max(T.nP_max, 1));
-            pt.x = arena.take<int>(std::max(T.nsp_max, 1));
-            pt.mk = arena.take<uint64_t>(std::max(T.nsp_max, 1));
-            const Subspace &sub = subs[T.smask | (1ull << m)];
-            HIP_OK(launch_partition(T.P, T.nP, T.nP_max, T.spx, T.spm, T.nsp, T.nsp_max, cellC + (size_t) m * n,
-                                    d_gid + (size_t) (m - first) * cells, G, sub.cnt, weak_thr, m, pt.cnt, pt.P, pt.W,
-                                    pt.x, pt.mk, st));
-        }
-        std::vector<size_t> cnt_at(nd);
-        size_t tot = 0;
-        for (int m = first; m < d; ++m) {
-            cnt_at[m - first] = tot;
-            tot += 3 * groups[m - first].size();
-        }
-        int *hostc = pin_i((size_t) nd * cells + 64);
-        for (int m = first; m < d; ++m)
-            HIP_OK(hipMemcpyAsync(hostc + cnt_at[m - first], parts[m - first].cnt,
-                                  3 * groups[m - first].size() * sizeof(int), hipMemcpyDeviceToHost, st));
-        sync();
-        std::vector<int> counts(hostc, hostc + tot);
-
-        for (int m = first; m < d; ++m) {
-            const uint64_t smask = T.smask | (1ull << m);
-            const auto &gs = groups[m - first];
-            const int G = (int) gs.size();
-            const int *c3 = counts.data() + cnt_at[m - first];
-            const Part &pt = parts[m - first];
-            const size_t dim_mark = arena.mark();
-            uint8_t *lab = nullptr;
-            int offW = 0, offS = 0;
-            for (int g = 0; g < G; ++g) {
-                const int nPr = c3[g], nW = c3[G + g], nSP = c3[2 * G + g];
-                const int *W = pt.W + offW;
-                const int *spx = pt.x + offS;
-                const uint64_t *spm = pt.mk + offS;
-                offW += nW;
-                offS += nSP;
-                stats.trees++;
-                const bool pass = nW >= prm.min_size;
-                size_t at = trace.size();
-                if (trace_on) trace.push_back({(long long) smask, gs[g].lo, gs[g].hi, nPr, nW, pass ? 1 : 0, -1});
-                if (!pass) continue;
-                const size_t child_mark = arena.mark();
-                const int tid = ++tree_id;
-                // s-connection paths surviving the prune
-                int *spx2 = arena.take<int>(std::max(nSP, 1));
-                uint64_t *spm2 = arena.take<uint64_t>(std::max(nSP, 1));
-                int *nsp2 = arena.take<int>(1);
-                {
-                    size_t tmark = arena.mark();
-                    size_t slots = table_slots(nW, d - 1 - m);
-                    uint64_t *table = arena.take<uint64_t>(nSP > 0 ? slots : 1);
-                    HIP_OK(launch_path_prune(D, smask, W, pt.cnt + G + g, nW, spx, spm, pt.cnt + 2 * G + g, nSP,
-                                             stamp, tid, small + 16, table, (uint64_t) (slots - 1), spx2, spm2, nsp2,
-                                             st));
-                    arena.release(tmark);  // stream order keeps the table alive for the queued kernels
-                }
-                // components of S now: the children's counts are bounded by them
-                Subspace &sub = ensure_components(smask);
-                TreeRef child{smask, W, pt.cnt + G + g, nW, spx2, spm2, nsp2, nSP};
-                recurse(child, m + 1);
-                // pruneRedundancy then cluster, on the device in stream order
-                const int *nW_dev = pt.cnt + G + g;
-                if (!lab) {
-                    arena.release(child_mark);
-                    lab = arena.take<uint8_t>(n);  // survives until this dimension's join
-                    HIP_OK(launch_fill_u8(lab, 0, n, st));
-                    HIP_OK(launch_redundancy_seed(W, nW_dev, nW, stamp, ++tree_id, entries, d_nE, smask, sub.comp,
-                                                  d_list, d_nlist, d_mn, prm.r, d_keep, lab, st));
-                } else {
-                    HIP_OK(launch_redundancy_seed(W, nW_dev, nW, stamp, ++tree_id, entries, d_nE, smask, sub.comp,
-                                                  d_list, d_nlist, d_mn, prm.r, d_keep, lab, st));
-                    arena.release(child_mark);
-                }
-                if (trace_on) {
-                    HIP_OK(hipMemcpyAsync(pin, d_keep, sizeof(int), hipMemcpyDeviceToHost, st));
-                    sync();
-                    trace[at].redundant = pin_i()[0] ? 0 : 1;
-                }
-            }
-            if (lab) join(smask, lab);
-            arena.release(dim_mark);
-        }
-        arena.release(lvl_mark);
-    }
-
-    // join (util.cu:655-743), on the device
-    void join(uint64_t smask, const uint8_t *lab) {
-        Subspace &s = ensure_components(smask);
-        HIP_OK(launch_join(lab, n, s.comp, s.csize, prm.min_size, prm.r, entries, d_nE, smask, s.entry, s.inres,
-                           d_list, d_nlist, keepf, dropf, freshf, st));
-    }
-
-    void emit(inscy_result &out) {
-        out.n = n;
-        sync();
-        std::vector<Entry> tab(nentries);
-        if (nentries) HIP_OK(hipMemcpy(tab.data(), entries, (size_t) nentries * sizeof(Entry), hipMemcpyDeviceToHost));
-        std::vector<int> rows;
-        for (int e = 0; e < nentries; ++e)
-            if (tab[e].nkept > 0) rows.push_back(e);
-        // ascending bitmask == the reference's vec_cmp order (util.cu:633-653)
-        std::sort(rows.begin(), rows.end(), [&](int a, int b) { return tab[a].mask < tab[b].mask; });
-        int *lab = arena.take<int>(n);
-        for (int e : rows) {
-            HIP_OK(launch_labels(tab[e].comp, tab[e].inres, n, lab, st));
-            HIP_OK(hipMemcpyAsync(pin, lab, (size_t) n * sizeof(int), hipMemcpyDeviceToHost, st));
-            sync();
-            // canonical labels: clusters numbered in order of their smallest point
-            // id, which is the component root
-            const int *h = pin_i();
-            std::vector<int> roots;
-            for (int i = 0; i < n; ++i)
-                if (h[i] == i) roots.push_back(i);
-            std::unordered_map<int, int> idx;
-            for (size_t i = 0; i < roots.size(); ++i) idx[roots[i]] = (int) i;
-            std::vector<int32_t> labels(n);
-            for (int i = 0; i < n; ++i) labels[i] = h[i] < 0 ? -1 : idx[h[i]];
-            std::vector<int32_t> dims;
-            for (int k = 0; k < d; ++k)
-                if ((tab[e].mask >> k) & 1ull) dims.push_back(k);
-            out.subspaces.push_back(std::move(dims));
-            out.labels.push_back(std::move(labels));
-            stats.clusters += (int64_t) roots.size();
-        }
-        stats.subspaces = (int64_t) out.subspaces.size();
-        std::vector<unsigned long long> h2(16 * 64);
-        HIP_OK(hipMemcpy(h2.data(), small + kStatAt, h2.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-        for (int b = 0; b < 64; ++b) {
-            stats.candidates += (int64_t) h2[16 * b];
-            stats.queries += (int64_t) h2[16 * b + 1];
-            stats.count_bytes += (int64_t) h2[16 * b + 2];
-        }
-        // device time of the neighbour-count kernel, from the event pairs
-        for (size_t i = 0; i < timers.size(); ++i) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, timers[i].start, timers[i].stop) == hipSuccess) stats.count_kernel_ms += ms;
-        }
-        stats.count_launches = (int64_t) timers.size();
-    }
-
-    inscy_params prm;
-    int n, d;
-    hipStream_t st;
-    int cells = 0, ecells = 0;
-    float s_star = 0.f;
-    int weak_thr = 0;
-    std::vector<int> dense_thr;
-    float *Xt = nullptr;
-    uint8_t *cellC = nullptr;
-    uint16_t *cellE = nullptr;
-    uint64_t *nearm = nullptr;
-    unsigned long long *small = nullptr;
-    int *stamp = nullptr;
-    int *qmark = nullptr;  // [n] stamp of the tree whose points are being counted
-    int mark_id = 0;
-    void *pin = nullptr;
-    std::vector<double> score;  // per-dimension index selectivity (lower is better)
-    long long grid_cap = 0;
-    size_t scan_bytes = 0;
-    DevData D{};
-    Arena arena;
-    Slabs slabs;
-    std::unordered_map<uint64_t, Subspace> subs;
-    // device result table
-    Entry *entries = nullptr;
-    int *scratch = nullptr;
-    int ecap = 0, nentries = 0;
-    int *d_nE = nullptr, *d_nlist = nullptr, *d_keep = nullptr, *d_list = nullptr, *d_mn = nullptr;
-    uint8_t *keepf = nullptr, *dropf = nullptr, *freshf = nullptr;
-    int tree_id = 0;
-    std::vector<KernelTimer> timers;
-};
 
 static thread_local std::string g_error;
 static thread_local std::vector<Engine::TraceRow> g_trace;
@@ -710,8 +42,12 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
             Engine eng(*p, (int) n, d, st);
             eng.trace_on = g_trace_on;
             eng.profile_on = g_profile_on;
+            eng.levels_mode = mode == INSCY_MODE_BATCHED;
             eng.setup(d_X);
-            eng.run(*res);
+            if (eng.levels_mode)
+                eng.run_levels(*res);
+            else
+                eng.run(*res);
             res->stats = eng.stats;
             if (g_trace_on) g_trace = eng.trace;
         }

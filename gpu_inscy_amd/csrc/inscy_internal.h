@@ -70,6 +70,96 @@ struct KernelTimer {
     hipEvent_t start = nullptr, stop = nullptr;
 };
 
+// ------------------------------------------------------------ level-synchronous passes
+// Every pass of the level-synchronous driver runs over many segments (trees,
+// subspaces, (tree, dimension) pairs) in one launch.  Segment s owns blocks
+// [blk_off[s], blk_off[s+1]) of the grid; the descriptors live in device memory.
+//
+// Path origins carry one extra bit: x >= 0 when the origin is a point of the
+// tree that holds the path, ~x when it is not (the s-connection of a point
+// pruned by the weak-density test).
+
+// One subspace of a batch: its grid index lives in the batch buffers at sorted
+// positions [base, base + n), coordinates at [coord, coord + k*n), grid cells
+// at [hoff, hoff + ncell).
+struct BatchSub {
+    SubspaceArg S;
+    GridArg G;
+    long long base, coord, hoff, qoff;
+    int *cnt;           // [n] neighbour counts (capped), -1 = not counted
+    const int *bound;   // [n] complete counts of a subspace of S, or null
+    int *comp;          // [n] component roots (components pass)
+    int *csize;         // [n] component sizes (components pass)
+    int ncell, cap, lo, thr_dense, qmax, pad;
+};
+
+// Device buffers of one batch's grid indexes.
+struct BatchIndex {
+    int *keys, *rank;     // [B*n]
+    int *hist, *off;      // [nhist]
+    int *order;           // [B*n] point id at each sorted position
+    float *Ys;            // coordinates in sorted order
+    uint16_t *Es;         // eps-grid cells in sorted order
+    uint8_t *qflag;       // [B*n] query flag per sorted position
+    int *qlist;           // local sorted positions of the queries
+    int *qcount;          // [B]
+    uint8_t *dsorted;     // [B*n] dense flag per sorted position
+    int *par;             // [B*n] union-find parents (point ids within each subspace)
+    int *giant;           // [B]
+    void *scan_temp;
+    size_t scan_bytes;
+    int nhist;            // sum of ncell + 1
+};
+
+// Tree whose s-connection paths set the merge flags of its dimensions >= first.
+struct FlagTree {
+    const int *spx;
+    const uint64_t *spm;
+    const int *nsp;  // device count
+    int first, flag_off;
+};
+
+// Points of a tree that are queries of subspace b.
+struct QuerySeg {
+    const int *P;
+    int len, b;
+};
+
+// Partition of one tree by the cell groups of dimension m (k_part_* per pair).
+struct PartPair {
+    const int *P;
+    const int *spx;
+    const uint64_t *spm;
+    const int *gid;      // [cells]
+    const int *weakcnt;  // counts of S ∪ {m}
+    int *cnt;            // [6G]: points, weak-dense points, paths per group; then cursors
+    int *outW;
+    int *outx;
+    uint64_t *outm;
+    int nP, nsp, m, G, bp, pad;
+};
+
+// Path prune of one surviving tree.
+struct PruneChild {
+    uint64_t smask;
+    uint64_t tmask;
+    const int *W;
+    const int *spx;
+    const uint64_t *spm;
+    uint64_t *table;
+    unsigned long long *bits;
+    int *outx;
+    uint64_t *outm;
+    int *nout;
+    int nW, nsp, bw, pad;
+};
+
+struct CopySeg {
+    const int *src;
+    int *dst;
+    int len, pad;
+};
+
 // Largest float s with sqrtf(s) <= eps: the distance test fl(sqrt(sum)) <= eps
 // of ClusteringCpu.cu:52-81 is then sum <= s_star without a device sqrt.
 float sqrt_threshold(float eps);
@@ -114,5 +204,37 @@ hipError_t launch_fill_i32(int *p, int v, size_t count, hipStream_t st);
 hipError_t launch_gather(const int *src, const int *idx, int count, int *out, hipStream_t st);
 hipError_t launch_iota(int *p, int n, hipStream_t st);
 hipError_t launch_root_paths(const uint64_t *nearm, int n, int *outx, uint64_t *outm, int *nout, hipStream_t st);
+
+// level-synchronous passes (blk_off: per-segment block offsets, nblk = blk_off[nseg])
+hipError_t launch_b_flags(const FlagTree *T, const int *blk_off, int nt, int nblk, const uint8_t *cellC, int n,
+                          int cells, int *flags, hipStream_t st);
+// grid indexes of nsub subspaces: keys, one scan over all grid cells, scatter
+hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, hipStream_t st);
+// queries in sorted order: flag the points of the segments, compact per subspace
+hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySeg *segs, const int *blk_off,
+                            int nseg, int nblk, const BatchIndex &X, hipStream_t st);
+// neighbour counts of the listed queries (blk_off over subspaces), or of every
+// uncounted point when blk_off is null (grid.y = subspace)
+hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, int nblk, int n, int kmax,
+                          const BatchIndex &X, float s_star, unsigned long long *stat, const KernelTimer &tm,
+                          hipStream_t st);
+// dense components of nsub subspaces whose counts are complete
+hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, const BatchIndex &X, float s_star,
+                               hipStream_t st);
+hipError_t launch_b_partition(const PartPair *pairs, const int *blk_off, int npairs, int nblk, const uint8_t *cellC,
+                              int n, int thr, hipStream_t st);
+// path prune of many trees (blk_w over their points, blk_p over their paths)
+hipError_t launch_b_prune(const DevData &D, const PruneChild *C, int nc, const int *blk_w, int nblk_w,
+                          const int *blk_p, int nblk_p, hipStream_t st);
+hipError_t launch_b_copy(const CopySeg *segs, const int *blk_off, int nseg, int nblk, hipStream_t st);
+hipError_t launch_b_entries(Entry *E, int at, const Entry *src, int count, int *nE, hipStream_t st);
+hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t count, hipStream_t st);
+// pruneRedundancy + cluster seeding of one tree whose point count is known on the host;
+// keep_log[log_idx] receives the decision when keep_log is given
+hipError_t launch_redundancy_seed_n(const int *W, int nW, int *stamp, int tid, const Entry *E, const int *nE,
+                                    uint64_t smask, const int *comp, int *list, int *nlist, int *mn, float r,
+                                    int *keep, uint8_t *lab, int *keep_log, int log_idx, hipStream_t st);
+constexpr int kBlockThreads = 256;   // threads per block of every pass
+constexpr int kQueriesPerBlock = 16;  // queries per block of the neighbour passes
 
 }  // namespace inscy

@@ -16,6 +16,7 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include "inscy_device.h"
 #include "inscy_internal.h"
 
 #pragma clang fp contract(off)
@@ -24,51 +25,6 @@ namespace inscy {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kLanesPerQuery = 16;  // lanes cooperating on one query's candidate ranges
-
-__device__ __forceinline__ int ld_relaxed(const int *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void st_relaxed(int *p, int v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Wave-aggregated slot allocation for order-free stream compaction: one
-// atomic per wave.  Must be reached by every active lane of the wave.
-__device__ __forceinline__ int wave_slot(bool keep, int *counter) {
-    unsigned long long m = __ballot(keep);
-    if (m == 0) return -1;
-    int lane = __lane_id();
-    int leader = __ffsll((long long) m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, __popcll(m));
-    base = __shfl(base, leader);
-    return keep ? base + __popcll(m & ((1ull << lane) - 1ull)) : -1;
-}
-
-// counter[key] += (number of lanes with that key and want), one atomic per
-// distinct key in the wave; returns each wanting lane's slot (old value +
-// rank among the lanes sharing its key).  Must be reached by every active lane.
-__device__ __forceinline__ int wave_add_by_key(int *counter, int key, bool want) {
-    unsigned long long todo = __ballot(want);
-    const int lane = __lane_id();
-    int result = 0;
-    while (todo) {
-        int leader = __ffsll((long long) todo) - 1;
-        int lk = __shfl(key, leader);
-        unsigned long long same = __ballot(want && key == lk) & todo;
-        int b = 0;
-        if (lane == leader) b = atomicAdd(&counter[lk], __popcll(same));
-        b = __shfl(b, leader);
-        if ((same >> lane) & 1ull) result = b + __popcll(same & ((1ull << lane) - 1ull));
-        todo &= ~same;
-    }
-    return result;
-}
-
-inline unsigned blocks_for(size_t count) { return (unsigned) ((count + kBlock - 1) / kBlock); }
 
 // ------------------------------------------------------------ discretisation
 // get_cell_no (ScyTreeNode.cu:100-102): min((int)(x / (1/c)), c-1); the
@@ -217,45 +173,6 @@ __global__ void k_part_scatter(PartArg A, int *__restrict__ cnt, int *__restrict
 // kLanesPerQuery lanes stride through each query's ranges, so the lanes of a
 // wave read consecutive candidates and neighbouring queries share ranges.
 
-template <class Range>
-__device__ __forceinline__ void for_ranges(const GridArg &G, const int *__restrict__ off, const int *qe, int stride,
-                                           Range f) {
-    const int C = G.C;
-    if (G.j == 1) {
-        int c0 = qe[G.tI[0] * stride];
-        f(off[max(c0 - 1, 0)], off[min(c0 + 1, C - 1) + 1]);
-    } else if (G.j == 2) {
-        int c0 = qe[G.tI[0] * stride], c1 = qe[G.tI[1] * stride];
-        int lo = max(c1 - 1, 0), hi = min(c1 + 1, C - 1);
-        for (int x = max(c0 - 1, 0); x <= min(c0 + 1, C - 1); ++x) f(off[x * C + lo], off[x * C + hi + 1]);
-    } else {
-        int c0 = qe[G.tI[0] * stride], c1 = qe[G.tI[1] * stride], c2 = qe[G.tI[2] * stride];
-        int lo = max(c2 - 1, 0), hi = min(c2 + 1, C - 1);
-        for (int x = max(c0 - 1, 0); x <= min(c0 + 1, C - 1); ++x)
-            for (int y = max(c1 - 1, 0); y <= min(c1 + 1, C - 1); ++y) {
-                int base = (x * C + y) * C;
-                f(off[base + lo], off[base + hi + 1]);
-            }
-    }
-}
-
-// candidate at sorted position jj against the staged query; S order, no FMA
-__device__ __forceinline__ bool is_neighbour(const SubspaceArg &S, const GridArg &G, int n,
-                                             const float *__restrict__ Ys, const uint16_t *__restrict__ Es,
-                                             const float *qy, const int *qe, int stride, int jj, float s_star) {
-    float acc = 0.0f;
-    for (int t = 0; t < S.k; ++t) {
-        if ((G.checkmask >> t) & 1ull) {
-            int de = (int) Es[(size_t) t * n + jj] - qe[t * stride];
-            if (de < -1 || de > 1) return false;
-        }
-        float diff = qy[t * stride] - Ys[(size_t) t * n + jj];
-        float sq = diff * diff;
-        acc = acc + sq;
-        if (acc > s_star) return false;
-    }
-    return true;
-}
 
 __global__ void k_grid_keys(DevData D, SubspaceArg S, GridArg G, int *__restrict__ keys, int *__restrict__ rank,
                             int *__restrict__ hist) {
@@ -283,13 +200,6 @@ __global__ void k_grid_scatter(DevData D, SubspaceArg S, const int *__restrict__
     }
 }
 
-__device__ __forceinline__ void stage_query(const SubspaceArg &S, int n, const float *__restrict__ Ys,
-                                            const uint16_t *__restrict__ Es, int i, float *sy, int *se, int stride) {
-    for (int t = 0; t < S.k; ++t) {
-        sy[t * stride] = Ys[(size_t) t * n + i];
-        se[t * stride] = Es[(size_t) t * n + i];
-    }
-}
 
 // |N_S(p)| of the points still uncounted (cnt == -1) and, when `mark` is
 // given, stamped with `markid`.  Counting stops once `cap` neighbours are
@@ -376,32 +286,6 @@ __global__ void k_dense_init(const int *__restrict__ order, int n, const int *__
 // Union-find over the dense-dense epsilon graph; hooking the larger root under
 // the smaller keeps every root the smallest point id of its component, which
 // is also the canonical cluster name.
-__device__ __forceinline__ int uf_find(int *par, int x) {
-    int p = ld_relaxed(&par[x]);
-    while (p != x) {
-        int g = ld_relaxed(&par[p]);
-        if (g != p) st_relaxed(&par[x], g);  // path halving: g is an ancestor of x
-        x = p;
-        p = g;
-    }
-    return x;
-}
-
-__device__ __forceinline__ void uf_union(int *par, int a, int b) {
-    while (true) {
-        a = uf_find(par, a);
-        b = uf_find(par, b);
-        if (a == b) return;
-        if (a > b) {
-            int t = a;
-            a = b;
-            b = t;
-        }
-        int old = atomicCAS(&par[b], b, a);
-        if (old == b) return;
-        b = old;
-    }
-}
 
 // Components in the Afforest manner (Sutton, Ben-Nun, Bar-Noy, "Optimizing
 // Parallel Graph Connectivity Computation via Subgraph Sampling", IPDPS 2018):
@@ -522,29 +406,15 @@ __global__ void k_uf_flatten(int *__restrict__ par, int n, int *__restrict__ com
 // id); every hash match is confirmed by comparing the cells themselves, so the
 // test is exact.
 
-constexpr uint64_t kEmpty = ~0ull;
-
-__device__ __forceinline__ uint64_t mix64(uint64_t h) {
-    h ^= h >> 33;
-    h *= 0xff51afd7ed558ccdull;
-    h ^= h >> 33;
-    h *= 0xc4ceb9fe1a85ec53ull;
-    h ^= h >> 33;
-    return h;
-}
-
-__device__ __forceinline__ bool same_prefix(const DevData &D, uint64_t smask, int a, int b, int upto) {
-    if (b < 0 || b >= D.n) return false;
-    for (int k = 0; k <= upto; ++k) {
-        if ((smask >> k) & 1ull) continue;
-        if (D.cellC[(size_t) k * D.n + a] != D.cellC[(size_t) k * D.n + b]) return false;
-    }
-    return true;
-}
 
 __global__ void k_stamp(const int *__restrict__ W, const int *__restrict__ nW, int *__restrict__ stamp, int tid) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < *nW) stamp[W[i]] = tid;
+}
+
+__global__ void k_stamp_n(const int *__restrict__ W, int nW, int *__restrict__ stamp, int tid) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nW) stamp[W[i]] = tid;
 }
 
 __global__ void k_path_bits(const int *__restrict__ spx, const uint64_t *__restrict__ spm, const int *__restrict__ nsp,
@@ -664,7 +534,7 @@ __global__ void k_redundancy(const int *__restrict__ stamp, int tid, const Entry
 
 // keep iff no superspace cluster touches the region or |W| * r > max_h min_h
 __global__ void k_decide(const int *__restrict__ nlist, int *__restrict__ mn, int nW, float r,
-                         int *__restrict__ keep) {
+                         int *__restrict__ keep, int *__restrict__ log, int log_idx) {
     if (threadIdx.x != 0) return;
     const int nl = *nlist;
     int max_min = 0;
@@ -673,7 +543,9 @@ __global__ void k_decide(const int *__restrict__ nlist, int *__restrict__ mn, in
         mn[h] = INT_MAX;
     }
     float lhs = (float) nW * r;
-    *keep = (max_min == 0 || (double) lhs > (double) max_min) ? 1 : 0;
+    const int k = (max_min == 0 || (double) lhs > (double) max_min) ? 1 : 0;
+    *keep = k;
+    if (log) log[log_idx] = k;
 }
 
 // cluster (ClusteringCpu.cu:317-389): a BFS from region point w labels w's
@@ -938,7 +810,18 @@ hipError_t launch_redundancy_seed(const int *W, const int *nW_dev, int nW, int *
     LAUNCH(k_select, dim3(1), dim3(kBlock), 0, st, E, nE, smask, list, nlist);
     if (nW > 0) LAUNCH(k_stamp, dim3(blocks_for(nW)), dim3(kBlock), 0, st, W, nW_dev, stamp, tid);
     LAUNCH(k_redundancy, dim3(256), dim3(kBlock), 0, st, stamp, tid, E, list, nlist, mn);
-    LAUNCH(k_decide, dim3(1), dim3(64), 0, st, nlist, mn, nW, r, keep);
+    LAUNCH(k_decide, dim3(1), dim3(64), 0, st, nlist, mn, nW, r, keep, (int *) nullptr, 0);
+    if (nW > 0) LAUNCH(k_seed_if, dim3(blocks_for(nW)), dim3(kBlock), 0, st, W, nW, comp, keep, lab);
+    return hipGetLastError();
+}
+
+hipError_t launch_redundancy_seed_n(const int *W, int nW, int *stamp, int tid, const Entry *E, const int *nE,
+                                    uint64_t smask, const int *comp, int *list, int *nlist, int *mn, float r,
+                                    int *keep, uint8_t *lab, int *keep_log, int log_idx, hipStream_t st) {
+    LAUNCH(k_select, dim3(1), dim3(kBlock), 0, st, E, nE, smask, list, nlist);
+    if (nW > 0) LAUNCH(k_stamp_n, dim3(blocks_for(nW)), dim3(kBlock), 0, st, W, nW, stamp, tid);
+    LAUNCH(k_redundancy, dim3(256), dim3(kBlock), 0, st, stamp, tid, E, list, nlist, mn);
+    LAUNCH(k_decide, dim3(1), dim3(64), 0, st, nlist, mn, nW, r, keep, keep_log, log_idx);
     if (nW > 0) LAUNCH(k_seed_if, dim3(blocks_for(nW)), dim3(kBlock), 0, st, W, nW, comp, keep, lab);
     return hipGetLastError();
 }
@@ -967,6 +850,12 @@ hipError_t launch_labels(const int *comp, const uint8_t *inres, int n, int *out,
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t count, hipStream_t st) {
     if (!count) return hipSuccess;
     LAUNCH(k_fill<uint8_t>, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, v, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t count, hipStream_t st) {
+    if (!count) return hipSuccess;
+    LAUNCH(k_fill<uint64_t>, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, v, count);
     return hipGetLastError();
 }
 

@@ -1,0 +1,531 @@
+// Level-synchronous driver of the MI355X INSCY engine (INSCY_MODE_BATCHED).
+//
+// In the reference's depth-first recursion (INSCYCPU3Weak,
+// src/algorithms/inscy/InscyNodeCpu.cu:159-235) only pruneRedundancy and join
+// read the result built so far.  restrict, mergeWithNeighbors, the
+// weak-density prune and the s-connection prune read nothing but the parent
+// tree, and the dense components of a subspace read nothing but the data.  So
+// the tree of restricted trees is grown here one level at a time:
+//   - all trees of a level get their merge flags in one pass and one read-back;
+//   - the level's child subspaces go through in batches: one grid-index build,
+//     one neighbour-count pass over the trees' points, one partition pass of
+//     every (tree, dimension) pair and one read-back of the group sizes;
+//   - the surviving children of a batch get their path prune, and their new
+//     subspaces their dense components, a handful of launches for all of them.
+// pruneRedundancy and join are then replayed on the device in the reference's
+// depth-first order, which gives the depth-first driver's result exactly.
+
+#include "inscy_engine.h"
+
+namespace inscy {
+
+namespace {
+constexpr size_t kMaxBatch = 512;  // child subspaces per batch
+constexpr uint64_t kEmptySlot = ~0ull;  // free slot of a path-prune table
+inline int nblocks(long long count) { return (int) ((count + kBlockThreads - 1) / kBlockThreads); }
+}  // namespace
+
+void Engine::put(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return;
+    const size_t need = (bytes + 255) & ~(size_t) 255;
+    if (up_top + need > up.cap) {
+        if (up_top > 0) sync();  // staged copies still read the buffer
+        up_top = 0;
+        if (need > up.cap) up.get(std::max(need, 2 * up.cap));
+    }
+    std::memcpy(up.p + up_top, src, bytes);
+    HIP_OK(hipMemcpyAsync(dst, up.p + up_top, bytes, hipMemcpyHostToDevice, st));
+    up_top += need;
+}
+
+void Engine::run_levels(inscy_result &out) {
+    lt.clear();
+    npass = 0;
+    store.chunk = (size_t) 256 << 20;
+    batch.chunk = (size_t) 256 << 20;
+    up.get((size_t) 4 << 20);
+    size_t free_b = 0, total_b = 0;
+    batch_budget = (size_t) 2 << 30;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+        batch_budget = std::min<size_t>(std::max<size_t>(free_b / 8, (size_t) 256 << 20), (size_t) 8 << 30);
+
+    // the root tree: every point, the root s-connection paths
+    Pool &L0 = lvl[cur_lvl];
+    lvl[cur_lvl ^ 1].reset();
+    int *P = store.take<int>(n);
+    HIP_OK(launch_iota(P, n, st));
+    int *cnt0 = L0.take<int>(1);
+    LTree root;
+    root.first = 0;
+    root.pass = true;
+    root.W = P;
+    root.nW = n;
+    root.pre = n;
+    root.spx = L0.take<int>(n);
+    root.spm = L0.take<uint64_t>(n);
+    root.nsp_max = n;
+    root.slot = 0;
+    HIP_OK(launch_root_paths(nearm, n, root.spx, root.spm, cnt0, st));
+    lt.push_back(std::move(root));
+    stats.calls = 1;
+
+    std::vector<int> frontier{0};
+    const int *counts = cnt0;
+    while (!frontier.empty()) {
+        expand_level(frontier, counts);
+        frontier.swap(lv_next);
+        counts = lv_next_counts;
+        lvl[cur_lvl].reset();
+        cur_lvl ^= 1;
+    }
+
+    // children in the reference's visiting order: dimension, then cell group
+    for (auto &t : lt)
+        std::stable_sort(t.kids.begin(), t.kids.end(), [&](int a, int b) {
+            return lt[a].m != lt[b].m ? lt[a].m < lt[b].m : lt[a].lo < lt[b].lo;
+        });
+    int *keep_log = store.take<int>(std::max(npass, 1));
+    replay_tree(0, keep_log);
+    if (trace_on) {
+        std::vector<int> keep(std::max(npass, 1));
+        HIP_OK(hipMemcpyAsync(keep.data(), keep_log, keep.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+        sync();
+        trace.clear();
+        trace_tree(0, keep);
+    }
+    emit(out);
+}
+
+// One level: merge flags and exact path counts of the frontier, then the
+// child subspaces in batches.  Fills lv_next (surviving children that can
+// still restrict) and lv_next_counts (their path counters).
+void Engine::expand_level(const std::vector<int> &frontier, const int *counts) {
+    Pool &cur = lvl[cur_lvl];
+    Pool &nxt = lvl[cur_lvl ^ 1];
+    lv_next.clear();
+    lv_nslots = 0;
+    lv_next_counts = nullptr;
+    lv_act.clear();
+    for (int t : frontier)
+        if (lt[t].first < d) lv_act.push_back(t);
+    if (lv_act.empty()) return;
+    const int na = (int) lv_act.size();
+
+    // merge flags of every dimension >= first of every tree
+    std::vector<FlagTree> ft(na);
+    std::vector<int> fblk(na + 1, 0);
+    lv_gidoff.assign(na, 0);
+    int ftot = 0;
+    for (int a = 0; a < na; ++a) {
+        const LTree &t = lt[lv_act[a]];
+        ft[a] = FlagTree{t.spx, t.spm, counts + t.slot, t.first, ftot};
+        lv_gidoff[a] = ftot;
+        ftot += (d - t.first) * cells;
+        fblk[a + 1] = fblk[a] + nblocks(t.nsp_max);
+    }
+    int *flags = cur.take<int>(ftot);
+    HIP_OK(launch_fill_i32(flags, 0, ftot, st));
+    FlagTree *dft = upload(cur, ft);
+    int *dfb = upload(cur, fblk);
+    HIP_OK(launch_b_flags(dft, dfb, na, fblk[na], cellC, n, cells, flags, st));
+    const int nf = (int) frontier.size();
+    char *h = down.get((size_t) (ftot + nf) * sizeof(int));
+    HIP_OK(hipMemcpyAsync(h, flags, (size_t) ftot * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(h + (size_t) ftot * sizeof(int), counts, (size_t) nf * sizeof(int), hipMemcpyDeviceToHost,
+                          st));
+    sync();
+    const int *hf = reinterpret_cast<const int *>(h);
+    const int *hc = hf + ftot;
+    int max_slot = -1;
+    for (int t : frontier) max_slot = std::max(max_slot, lt[t].slot);
+    if (max_slot >= nf) throw Failure(INSCY_ERR_DEVICE, "internal: path counter slot out of range");
+    for (int t : frontier) lt[t].nsp = std::min(hc[lt[t].slot], lt[t].nsp_max);
+
+    // cell groups (restrict + mergeWithNeighbors, ScyTreeNode.cu:837-851)
+    lv_groups.assign(na, {});
+    std::vector<int> gid(ftot);
+    long long max_children = 0;
+    for (int a = 0; a < na; ++a) {
+        const LTree &t = lt[lv_act[a]];
+        lv_groups[a].resize(d - t.first);
+        for (int m = t.first; m < d; ++m) {
+            const int *f = hf + lv_gidoff[a] + (m - t.first) * cells;
+            int *gm = gid.data() + lv_gidoff[a] + (m - t.first) * cells;
+            auto &gs = lv_groups[a][m - t.first];
+            for (int c = 0; c < cells; ++c) {
+                int lo = c;
+                while (f[c] && c < cells - 1) ++c;
+                for (int x = lo; x <= c; ++x) gm[x] = (int) gs.size();
+                gs.push_back({lo, c});
+            }
+            max_children += (long long) gs.size();
+        }
+    }
+    lv_gid = upload(cur, gid);
+    lv_next_counts = nxt.take<int>(std::max<long long>(max_children, 1));
+    HIP_OK(launch_fill_i32(lv_next_counts, 0, std::max<long long>(max_children, 1), st));
+
+    // child subspaces: per run of frontier trees with one mask, S | {m} for m >= first
+    std::vector<ChildSub> cs;
+    std::set<uint64_t> seen;
+    for (int a = 0; a < na;) {
+        int b = a;
+        while (b < na && lt[lv_act[b]].smask == lt[lv_act[a]].smask) ++b;
+        const LTree &t = lt[lv_act[a]];
+        // the trees of one mask are created together, so each child subspace
+        // is reached from exactly one run
+        if (!seen.insert(t.smask).second) throw Failure(INSCY_ERR_DEVICE, "internal: frontier masks not grouped");
+        for (int m = t.first; m < d; ++m) cs.push_back({t.smask | (1ull << m), m, a, b});
+        a = b;
+    }
+
+    // batches bounded by memory, count and 32-bit positions
+    size_t i = 0;
+    while (i < cs.size()) {
+        size_t j = i;
+        size_t bytes = 0;
+        long long hsum = 1;
+        while (j < cs.size() && j - i < kMaxBatch) {
+            const ChildSub &c = cs[j];
+            SubspaceArg S = arg_of(c.s);
+            GridArg G = grid_of(S);
+            long long ncell = 1;
+            for (int q = 0; q < G.j; ++q) ncell *= G.C;
+            size_t est = (size_t) n * (26 + 6 * (size_t) S.k) + (size_t) ncell * 8;
+            for (int a = c.t0; a < c.t1; ++a) {
+                const LTree &t = lt[lv_act[a]];
+                est += (size_t) t.nW * (8 + 32 * (size_t) (d - 1 - c.m)) + (size_t) t.nsp * 12 + 6 * 256 * 4;
+            }
+            const bool fits = bytes + est <= batch_budget && (long long) (j - i + 1) * n < INT_MAX &&
+                              hsum + ncell < INT_MAX;
+            if (j > i && !fits) break;
+            bytes += est;
+            hsum += ncell;
+            ++j;
+        }
+        process_batch(cs, i, j);
+        i = j;
+    }
+}
+
+void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1) {
+    Pool &nxt = lvl[cur_lvl ^ 1];
+    batch.reset();
+    const int B = (int) (i1 - i0);
+    const int QPB = kQueriesPerBlock;
+
+    // ---- grid indexes of the batch's subspaces
+    std::vector<BatchSub> hs(B);
+    long long coord = 0, hoff = 0, qoff = 0;
+    int kmax = 1;
+    int *cntB = batch.take<int>((size_t) B * n);
+    for (int b = 0; b < B; ++b) {
+        const ChildSub &c = cs[i0 + b];
+        BatchSub &x = hs[b];
+        std::memset(&x, 0, sizeof(x));
+        x.S = arg_of(c.s);
+        x.G = grid_of(x.S);
+        long long ncell = 1;
+        for (int q = 0; q < x.G.j; ++q) ncell *= x.G.C;
+        long long qmax = 0;
+        for (int a = c.t0; a < c.t1; ++a) qmax += lt[lv_act[a]].nW;
+        x.base = (long long) b * n;
+        x.coord = coord;
+        x.hoff = hoff;
+        x.qoff = qoff;
+        x.cnt = cntB + (size_t) b * n;
+        x.ncell = (int) ncell;
+        x.cap = cap_of(x.S.k);
+        x.thr_dense = dense_thr[x.S.k];
+        x.lo = 0;
+        x.qmax = (int) qmax;
+        coord += (long long) x.S.k * n;
+        hoff += ncell;
+        qoff += qmax;
+        kmax = std::max(kmax, x.S.k);
+    }
+    BatchIndex X{};
+    X.nhist = (int) (hoff + 1);
+    X.keys = batch.take<int>((size_t) B * n);
+    X.rank = batch.take<int>((size_t) B * n);
+    X.order = batch.take<int>((size_t) B * n);
+    X.hist = batch.take<int>(X.nhist);
+    X.off = batch.take<int>(X.nhist);
+    X.Ys = batch.take<float>(coord);
+    X.Es = batch.take<uint16_t>(coord);
+    X.qflag = batch.take<uint8_t>((size_t) B * n);
+    X.qlist = batch.take<int>(std::max<long long>(qoff, 1));
+    X.qcount = batch.take<int>(B);
+    X.dsorted = batch.take<uint8_t>((size_t) B * n);
+    X.par = batch.take<int>((size_t) B * n);
+    X.giant = batch.take<int>(B);
+    X.scan_bytes = scan_temp_bytes(X.nhist);
+    X.scan_temp = batch.take<char>(X.scan_bytes);
+    HIP_OK(launch_fill_i32(cntB, -1, (size_t) B * n, st));
+    BatchSub *dsubs = upload(batch, hs);
+    HIP_OK(launch_b_index(D, dsubs, B, X, st));
+
+    // ---- queries: the points of the trees each subspace refines, in sorted order
+    std::vector<QuerySeg> qs;
+    std::vector<int> qblk{0};
+    for (int b = 0; b < B; ++b) {
+        const ChildSub &c = cs[i0 + b];
+        for (int a = c.t0; a < c.t1; ++a) {
+            const LTree &t = lt[lv_act[a]];
+            if (t.nW <= 0) continue;
+            qs.push_back({t.W, t.nW, b});
+            qblk.push_back(qblk.back() + nblocks(t.nW));
+        }
+    }
+    QuerySeg *dqs = upload(batch, qs);
+    int *dqb = upload(batch, qblk);
+    HIP_OK(launch_b_queries(dsubs, B, n, dqs, dqb, (int) qs.size(), qblk.back(), X, st));
+    std::vector<int> cblk(B + 1, 0);
+    for (int b = 0; b < B; ++b) cblk[b + 1] = cblk[b] + (hs[b].qmax + QPB - 1) / QPB;
+    int *dcb = upload(batch, cblk);
+    HIP_OK(launch_b_count(dsubs, B, dcb, cblk[B], n, kmax, X, s_star, small + kStatAt, timer(), st));
+
+    // ---- partition of every (tree, dimension) pair by the cell groups
+    struct PairInfo {
+        int a, b, G, cnt_off;
+    };
+    std::vector<PairInfo> pinfo;
+    int pc_total = 0;
+    for (int b = 0; b < B; ++b) {
+        const ChildSub &c = cs[i0 + b];
+        for (int a = c.t0; a < c.t1; ++a) {
+            const LTree &t = lt[lv_act[a]];
+            const int G = (int) lv_groups[a][c.m - t.first].size();
+            pinfo.push_back({a, b, G, pc_total});
+            pc_total += 6 * G;
+        }
+    }
+    const int np = (int) pinfo.size();
+    int *pcnt = batch.take<int>(std::max(pc_total, 1));
+    HIP_OK(launch_fill_i32(pcnt, 0, std::max(pc_total, 1), st));
+    std::vector<PartPair> pairs(np);
+    std::vector<int> pblk(np + 1, 0);
+    for (int j = 0; j < np; ++j) {
+        const PairInfo &pi = pinfo[j];
+        const ChildSub &c = cs[i0 + pi.b];
+        const LTree &t = lt[lv_act[pi.a]];
+        PartPair &pp = pairs[j];
+        std::memset(&pp, 0, sizeof(pp));
+        pp.P = t.W;
+        pp.nP = t.nW;
+        pp.spx = t.spx;
+        pp.spm = t.spm;
+        pp.nsp = t.nsp;
+        pp.gid = lv_gid + lv_gidoff[pi.a] + (c.m - t.first) * cells;
+        pp.weakcnt = hs[pi.b].cnt;
+        pp.cnt = pcnt + pi.cnt_off;
+        pp.outW = batch.take<int>(std::max(t.nW, 1));
+        pp.outx = batch.take<int>(std::max(t.nsp, 1));
+        pp.outm = batch.take<uint64_t>(std::max(t.nsp, 1));
+        pp.m = c.m;
+        pp.G = pi.G;
+        pp.bp = nblocks(t.nW);
+        pblk[j + 1] = pblk[j] + pp.bp + nblocks(t.nsp);
+    }
+    PartPair *dpairs = upload(batch, pairs);
+    int *dpb = upload(batch, pblk);
+    HIP_OK(launch_b_partition(dpairs, dpb, np, pblk[np], cellC, n, weak_thr, st));
+    char *h = down.get((size_t) std::max(pc_total, 1) * sizeof(int));
+    HIP_OK(hipMemcpyAsync(h, pcnt, (size_t) std::max(pc_total, 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+    const std::vector<int> hcnt(reinterpret_cast<const int *>(h), reinterpret_cast<const int *>(h) + pc_total);
+
+    // ---- children: trace rows, surviving trees, their point lists and path prunes
+    std::vector<CopySeg> copies;
+    std::vector<PruneChild> prune;
+    std::vector<size_t> prune_slots;
+    std::vector<char> passing(B, 0);
+    size_t table_total = 0;
+    for (int j = 0; j < np; ++j) {
+        const PairInfo &pi = pinfo[j];
+        const ChildSub &c = cs[i0 + pi.b];
+        const int tparent = lv_act[pi.a];
+        const int first = lt[tparent].first;
+        const int G = pi.G;
+        const int *c3 = hcnt.data() + pi.cnt_off;
+        int offW = 0, offS = 0;
+        for (int g = 0; g < G; ++g) {
+            const int nPr = c3[g], nW = c3[G + g], nSP = c3[2 * G + g];
+            LTree ch;
+            ch.smask = c.s;
+            ch.first = c.m + 1;
+            ch.m = c.m;
+            ch.lo = lv_groups[pi.a][c.m - first][g].lo;
+            ch.hi = lv_groups[pi.a][c.m - first][g].hi;
+            ch.pre = nPr;
+            ch.nW = nW;
+            ch.pass = nW >= prm.min_size;
+            stats.trees++;
+            if (ch.pass) {
+                stats.calls++;
+                ch.pass_id = npass++;
+                passing[pi.b] = 1;
+                int *Wst = store.take<int>(std::max(nW, 1));
+                if (nW > 0) copies.push_back({pairs[j].outW + offW, Wst, nW, 0});
+                ch.W = Wst;
+                if (ch.first < d) {
+                    ch.slot = lv_nslots++;
+                    ch.nsp_max = nSP;
+                    ch.spx = nxt.take<int>(std::max(nSP, 1));
+                    ch.spm = nxt.take<uint64_t>(std::max(nSP, 1));
+                    if (nSP > 0) {
+                        PruneChild pc;
+                        std::memset(&pc, 0, sizeof(pc));
+                        pc.smask = c.s;
+                        const size_t slots = table_slots(nW, d - 1 - c.m);
+                        pc.tmask = (uint64_t) (slots - 1);
+                        pc.W = pairs[j].outW + offW;
+                        pc.spx = pairs[j].outx + offS;
+                        pc.spm = pairs[j].outm + offS;
+                        pc.outx = ch.spx;
+                        pc.outm = ch.spm;
+                        pc.nout = lv_next_counts + ch.slot;
+                        pc.nW = nW;
+                        pc.nsp = nSP;
+                        prune_slots.push_back(table_total);
+                        table_total += slots;
+                        prune.push_back(pc);
+                    }
+                }
+            }
+            const int idx = (int) lt.size();
+            const bool expand = ch.pass && ch.first < d;
+            lt.push_back(std::move(ch));
+            lt[tparent].kids.push_back(idx);
+            if (expand) lv_next.push_back(idx);
+            offW += nW;
+            offS += nSP;
+        }
+    }
+
+    // ---- path prune of the surviving children
+    if (!prune.empty()) {
+        const int nc = (int) prune.size();
+        uint64_t *tables = batch.take<uint64_t>(table_total);
+        unsigned long long *bits = batch.take<unsigned long long>(nc);
+        HIP_OK(launch_fill_u64(tables, kEmptySlot, table_total, st));
+        HIP_OK(launch_fill_u64(reinterpret_cast<uint64_t *>(bits), 0ull, nc, st));
+        std::vector<int> bw(nc + 1, 0), bp(nc + 1, 0);
+        for (int c = 0; c < nc; ++c) {
+            prune[c].table = tables + prune_slots[c];
+            prune[c].bits = bits + c;
+            bw[c + 1] = bw[c] + nblocks(prune[c].nW);
+            bp[c + 1] = bp[c] + nblocks(prune[c].nsp);
+        }
+        PruneChild *dpc = upload(batch, prune);
+        int *dbw = upload(batch, bw);
+        int *dbp = upload(batch, bp);
+        HIP_OK(launch_b_prune(D, dpc, nc, dbw, bw[nc], dbp, bp[nc], st));
+    }
+
+    // ---- dense components of the subspaces that gained a surviving tree
+    std::vector<BatchSub> hc;
+    std::vector<Entry> ents;
+    int ncomp = 0;
+    for (int b = 0; b < B; ++b) ncomp += passing[b];
+    if (ncomp > 0) {
+        int *cntP = slabs.take<int>((size_t) ncomp * n);
+        int *compP = slabs.take<int>((size_t) ncomp * n);
+        int *csizeP = slabs.take<int>((size_t) ncomp * n);
+        int *memP = slabs.take<int>((size_t) ncomp * n);
+        uint8_t *inresP = slabs.take<uint8_t>((size_t) ncomp * n);
+        HIP_OK(launch_fill_u8(inresP, 0, (size_t) ncomp * n, st));
+        grow_entries(nentries + ncomp);
+        int ci = 0;
+        for (int b = 0; b < B; ++b) {
+            if (!passing[b]) continue;
+            const uint64_t s = cs[i0 + b].s;
+            BatchSub x = hs[b];
+            x.cnt = cntP + (size_t) ci * n;
+            x.comp = compP + (size_t) ci * n;
+            x.csize = csizeP + (size_t) ci * n;
+            copies.push_back({hs[b].cnt, x.cnt, n, 0});
+            // a subspace S' = S minus one dimension with complete counts bounds
+            // the counts of S from above
+            const int k = x.S.k;
+            int lo = std::min(weak_thr, dense_thr[k]);
+            x.bound = nullptr;
+            for (int t = 0; t < k && !x.bound; ++t) {
+                const uint64_t sub = s & ~(1ull << x.S.dims[t]);
+                auto it = subs.find(sub);
+                if (sub && it != subs.end() && it->second.entry >= 0) {
+                    x.bound = it->second.cnt;
+                    lo = std::min(lo, cap_of(k - 1));
+                }
+            }
+            x.lo = lo;
+            hc.push_back(x);
+            Subspace &sb = subs[s];
+            sb.cnt = x.cnt;
+            sb.comp = x.comp;
+            sb.csize = x.csize;
+            sb.inres = inresP + (size_t) ci * n;
+            sb.entry = nentries + ci;
+            ents.push_back(Entry{s, sb.comp, sb.csize, sb.inres, memP + (size_t) ci * n, 0, 0});
+            ++ci;
+        }
+    }
+    if (!copies.empty()) {
+        std::vector<int> cb(copies.size() + 1, 0);
+        for (size_t c = 0; c < copies.size(); ++c) cb[c + 1] = cb[c] + nblocks(copies[c].len);
+        CopySeg *dcs = upload(batch, copies);
+        int *dcb2 = upload(batch, cb);
+        HIP_OK(launch_b_copy(dcs, dcb2, (int) copies.size(), cb.back(), st));
+    }
+    if (ncomp > 0) {
+        BatchSub *dhc = upload(batch, hc);
+        HIP_OK(launch_b_count(dhc, ncomp, nullptr, 0, n, kmax, X, s_star, small + kStatAt, timer(), st));
+        HIP_OK(launch_b_components(dhc, ncomp, n, kmax, X, s_star, st));
+        Entry *de = upload(batch, ents);
+        HIP_OK(launch_b_entries(entries, nentries, de, ncomp, d_nE, st));
+        nentries += ncomp;
+        stats.components += ncomp;
+    }
+}
+
+// pruneRedundancy and join of the subtree of f, in the reference's order:
+// per dimension, every surviving child's subtree first, then its redundancy
+// test and cluster seeds, then the join of the dimension's clusters.
+void Engine::replay_tree(int f, int *keep_log) {
+    const std::vector<int> kids = lt[f].kids;
+    size_t i = 0;
+    while (i < kids.size()) {
+        const int m = lt[kids[i]].m;
+        size_t j = i;
+        while (j < kids.size() && lt[kids[j]].m == m) ++j;
+        const uint64_t s = lt[kids[i]].smask;
+        const size_t mk = arena.mark();
+        uint8_t *lab = nullptr;
+        for (size_t u = i; u < j; ++u) {
+            const int c = kids[u];
+            if (!lt[c].pass) continue;
+            replay_tree(c, keep_log);
+            if (!lab) {
+                lab = arena.take<uint8_t>(n);
+                HIP_OK(launch_fill_u8(lab, 0, n, st));
+            }
+            const Subspace &sub = subs[s];
+            HIP_OK(launch_redundancy_seed_n(lt[c].W, lt[c].nW, stamp, ++tree_id, entries, d_nE, s, sub.comp, d_list,
+                                            d_nlist, d_mn, prm.r, d_keep, lab, keep_log, lt[c].pass_id, st));
+        }
+        if (lab) join(s, lab);
+        arena.release(mk);
+        i = j;
+    }
+}
+
+void Engine::trace_tree(int f, const std::vector<int> &keep) {
+    for (int c : lt[f].kids) {
+        const LTree &t = lt[c];
+        trace.push_back({(long long) t.smask, t.lo, t.hi, t.pre, t.nW, t.pass ? 1 : 0,
+                         t.pass ? (keep[t.pass_id] ? 0 : 1) : -1});
+        if (t.pass) trace_tree(c, keep);
+    }
+}
+
+}  // namespace inscy

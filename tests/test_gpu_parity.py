@@ -177,6 +177,30 @@ def test_end_to_end_config0():
     assert len(got.subspaces) > 0
 
 
+@pytest.mark.parametrize("seed", list(range(12)))
+def test_batched_small_cases_match_oracle(seed):
+    # the same cases as the depth-first test, through the level-synchronous
+    # driver: identical subspaces, labels and per-tree decisions (trace rows
+    # in the reference's depth-first order)
+    rng = np.random.default_rng(seed + 500)
+    n = int(rng.integers(100, 700))
+    d = int(rng.integers(2, 6))
+    cells = int(rng.integers(2, 7))
+    eps = float(rng.choice([0.02, 0.05, 0.1]))
+    w = datagen.small(seed, n=n, d=d, n_clusters=int(rng.integers(1, 4)), eps=eps, cells=cells,
+                      num_obj=int(rng.integers(2, 6)), min_size=int(rng.integers(2, n // 10 + 3)),
+                      sigma=float(rng.choice([0.02, 0.05])))
+    say("batched small", seed, n, d, cells, eps)
+    compare_with_oracle(w, mode=_lib.MODE_BATCHED)
+
+
+def test_batched_config0_matches_oracle():
+    w = datagen.config(0)
+    say("batched config0", w.name)
+    got = compare_with_oracle(w, mode=_lib.MODE_BATCHED)
+    assert len(got.subspaces) > 0
+
+
 def test_drop_in_lists_and_cuda_input():
     w = datagen.config(0)
     say("drop-in", w.name)
@@ -199,6 +223,19 @@ def test_config1_matches_golden():
     g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "config1.json")))
     w = datagen.config(1)
     res = INSCY.run(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0)
+    assert res.subspaces == [s for s, _, _, _ in g["subspaces"]]
+    assert digests(res) == [x[3] for x in g["subspaces"]]
+    assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
+
+
+def test_batched_config1_matches_golden():
+    import json
+    import os
+    say("batched config1 golden")
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "config1.json")))
+    w = datagen.config(1)
+    res = INSCY.run(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0,
+                    mode=_lib.MODE_BATCHED)
     assert res.subspaces == [s for s, _, _, _ in g["subspaces"]]
     assert digests(res) == [x[3] for x in g["subspaces"]]
     assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
@@ -249,3 +286,8 @@ def test_config2_full_size_deterministic_and_consistent():
     assert a.stats["trees"] == b.stats["trees"] and a.stats["calls"] == b.stats["calls"]
     check_invariants(a, w)
     say("config2", a.stats)
+    # the level-synchronous driver gives the same result, trees and calls
+    c = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0, mode=_lib.MODE_BATCHED)
+    assert a.subspaces == c.subspaces and np.array_equal(a.labels, c.labels)
+    assert a.stats["trees"] == c.stats["trees"] and a.stats["calls"] == c.stats["calls"]
+    say("config2 batched", c.stats)
