@@ -24,6 +24,7 @@ MODE_EXACT = 1
 MODE_BATCHED = 2
 
 EXPORTED = [
+    "inscy_result_labels_all",
     "inscy_run", "inscy_run_device", "inscy_result_num_subspaces", "inscy_result_subspace",
     "inscy_result_labels", "inscy_result_num_points", "inscy_result_stats", "inscy_result_free",
     "inscy_last_error", "inscy_discretise", "inscy_neighbour_counts", "inscy_dense_components", "inscy_profile",
@@ -62,6 +63,9 @@ class Stats(ctypes.Structure):
         ("count_kernel_ms", ctypes.c_double),
         ("count_launches", ctypes.c_int64),
         ("count_bytes", ctypes.c_int64),
+        ("expand_seconds", ctypes.c_double),
+        ("replay_seconds", ctypes.c_double),
+        ("emit_seconds", ctypes.c_double),
     ]
 
 
@@ -90,6 +94,8 @@ def lib():
     L.inscy_result_subspace.restype = ctypes.c_int32
     L.inscy_result_labels.argtypes = [vp, ctypes.c_int32]
     L.inscy_result_labels.restype = i32p
+    L.inscy_result_labels_all.argtypes = [vp]
+    L.inscy_result_labels_all.restype = i32p
     L.inscy_result_num_points.argtypes = [vp]
     L.inscy_result_num_points.restype = ctypes.c_int64
     L.inscy_result_stats.argtypes = [vp, ctypes.POINTER(Stats)]
@@ -126,28 +132,44 @@ def make_params(neighborhood_size, F, num_obj, min_size, r=1.0, number_of_cells=
                   int(bool(rectangular)), int(entropy_order))
 
 
+class _Handle:
+    """Owns an inscy_result; freed when the last array viewing it goes away."""
+
+    def __init__(self, handle):
+        self.handle = handle
+        self._free = lib().inscy_result_free  # kept: module globals may be gone at exit
+
+    def __del__(self):
+        if self.handle:
+            self._free(self.handle)
+            self.handle = None
+
+
 class Result:
-    """Owned copy of an inscy_result: subspaces (ascending dims), canonical
-    labels [#subspaces, n] int32, and the run counters."""
+    """An inscy_result: subspaces (ascending dims), canonical labels
+    [#subspaces, n] int32 (a view of the library's contiguous label array, kept
+    alive by the array itself), and the run counters."""
 
     def __init__(self, handle):
         L = lib()
-        try:
-            n = int(L.inscy_result_num_points(handle))
-            m = int(L.inscy_result_num_subspaces(handle))
-            dims = (ctypes.c_int32 * 64)()
-            self.subspaces = []
+        owner = _Handle(handle)
+        n = int(L.inscy_result_num_points(handle))
+        m = int(L.inscy_result_num_subspaces(handle))
+        dims = (ctypes.c_int32 * 64)()
+        self.subspaces = []
+        for i in range(m):
+            k = L.inscy_result_subspace(handle, i, dims)
+            self.subspaces.append([int(dims[j]) for j in range(k)])
+        if m > 0 and n > 0:
+            ptr = L.inscy_result_labels_all(handle)
+            buf = (ctypes.c_int32 * (m * n)).from_address(ctypes.addressof(ptr.contents))
+            buf._owner = owner  # the buffer keeps the result alive
+            self.labels = np.frombuffer(buf, dtype=np.int32).reshape(m, n)
+        else:
             self.labels = np.zeros((m, n), dtype=np.int32)
-            for i in range(m):
-                k = L.inscy_result_subspace(handle, i, dims)
-                self.subspaces.append([int(dims[j]) for j in range(k)])
-                ptr = L.inscy_result_labels(handle, i)
-                self.labels[i] = np.ctypeslib.as_array(ptr, shape=(n,))
-            st = Stats()
-            check(L.inscy_result_stats(handle, ctypes.byref(st)))
-            self.stats = {f: getattr(st, f) for f, _ in Stats._fields_}
-        finally:
-            L.inscy_result_free(handle)
+        st = Stats()
+        check(L.inscy_result_stats(handle, ctypes.byref(st)))
+        self.stats = {f: getattr(st, f) for f, _ in Stats._fields_}
 
 
 def run_host(X: np.ndarray, params: Params, mode: int = MODE_EXACT) -> Result:

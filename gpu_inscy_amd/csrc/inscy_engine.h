@@ -24,22 +24,41 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
+
+#include <sys/mman.h>
 
 #include "../../include/inscy.h"
 #include "inscy_internal.h"
 
+// Host array for the result labels: 2 MiB aligned and advised for huge pages,
+// so that first touch of a multi-GB result costs few page faults.
+struct HostFree {
+    void operator()(int32_t *p) const { std::free(p); }
+};
+inline int32_t *alloc_labels(size_t count) {
+    const size_t align = (size_t) 2 << 20;
+    const size_t bytes = ((std::max<size_t>(count, 1) * sizeof(int32_t)) + align - 1) & ~(align - 1);
+    void *p = nullptr;
+    if (posix_memalign(&p, align, bytes) != 0) throw std::bad_alloc();
+    (void) madvise(p, bytes, MADV_HUGEPAGE);
+    return static_cast<int32_t *>(p);
+}
+
 struct inscy_result {
     int64_t n = 0;
     std::vector<std::vector<int32_t>> subspaces;
-    std::vector<std::vector<int32_t>> labels;
+    std::unique_ptr<int32_t[], HostFree> labels;  // [subspaces][n], canonical
     inscy_stats stats{};
 };
 
@@ -214,6 +233,7 @@ struct Subspace {
     int *comp = nullptr;       // [n] component root (smallest id) or -1
     int *csize = nullptr;      // [n] component sizes indexed by root
     uint8_t *inres = nullptr;  // [n] component kept in the result of S
+    int *tstamp = nullptr;     // [n] surviving tree of S holding the point (level-synchronous driver)
     int entry = -1;            // row in the device result table
 };
 
@@ -312,8 +332,13 @@ public:
         HIP_OK(launch_fill_i32(nP, n, 1, st));
         HIP_OK(launch_root_paths(nearm, n, spx, spm, nsp, st));
         TreeRef root{0ull, P, nP, n, spx, spm, nsp, n};
+        auto t0 = std::chrono::steady_clock::now();
         recurse(root, 0);
+        sync();
+        auto t1 = std::chrono::steady_clock::now();
         emit(out);
+        stats.expand_seconds = std::chrono::duration<double>(t1 - t0).count();
+        stats.emit_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
     }
 
     // dense components of one subspace (kernel-level entry point)
@@ -658,6 +683,9 @@ private:
                            d_list, d_nlist, keepf, dropf, freshf, st));
     }
 
+    // Result in the reference's order (ascending bitmask == vec_cmp,
+    // util.cu:633-653) with canonical labels computed on the device, read back
+    // in chunks of rows straight into the result's contiguous label array.
     void emit(inscy_result &out) {
         out.n = n;
         sync();
@@ -666,31 +694,81 @@ private:
         std::vector<int> rows;
         for (int e = 0; e < nentries; ++e)
             if (tab[e].nkept > 0) rows.push_back(e);
-        // ascending bitmask == the reference's vec_cmp order (util.cu:633-653)
         std::sort(rows.begin(), rows.end(), [&](int a, int b) { return tab[a].mask < tab[b].mask; });
-        int *lab = arena.take<int>(n);
+        const int m = (int) rows.size();
         for (int e : rows) {
-            HIP_OK(launch_labels(tab[e].comp, tab[e].inres, n, lab, st));
-            HIP_OK(hipMemcpyAsync(pin, lab, (size_t) n * sizeof(int), hipMemcpyDeviceToHost, st));
-            sync();
-            // canonical labels: clusters numbered in order of their smallest point
-            // id, which is the component root
-            const int *h = pin_i();
-            std::vector<int> roots;
-            for (int i = 0; i < n; ++i)
-                if (h[i] == i) roots.push_back(i);
-            std::unordered_map<int, int> idx;
-            for (size_t i = 0; i < roots.size(); ++i) idx[roots[i]] = (int) i;
-            std::vector<int32_t> labels(n);
-            for (int i = 0; i < n; ++i) labels[i] = h[i] < 0 ? -1 : idx[h[i]];
             std::vector<int32_t> dims;
             for (int k = 0; k < d; ++k)
                 if ((tab[e].mask >> k) & 1ull) dims.push_back(k);
             out.subspaces.push_back(std::move(dims));
-            out.labels.push_back(std::move(labels));
-            stats.clusters += (int64_t) roots.size();
         }
-        stats.subspaces = (int64_t) out.subspaces.size();
+        out.labels.reset(alloc_labels((size_t) std::max(m, 1) * n));
+        // chunks of rows: labels on the device, read back into one of two
+        // pinned buffers while the host copies the previous chunk out with
+        // several threads
+        const size_t chunk_bytes = (size_t) 128 << 20;
+        const int R = (int) std::max<size_t>(1, std::min<size_t>((size_t) m, chunk_bytes / ((size_t) n * 4)));
+        if (m > 0) {
+            batch.reset();
+            const size_t items = (size_t) R * n + 1;
+            int *flags = batch.take<int>(items);
+            int *idx = batch.take<int>(items);
+            int *lab = batch.take<int>((size_t) R * n);
+            int *cnts = batch.take<int>(R);
+            const size_t sb = scan_temp_bytes((int) items);
+            void *tmp = batch.take<char>(sb);
+            const size_t stage_bytes = (size_t) R * n * sizeof(int32_t) + (size_t) R * sizeof(int) + 256;
+            char *stage[2] = {emit_stage[0].get(stage_bytes), emit_stage[1].get(stage_bytes)};
+            hipEvent_t done[2];
+            HIP_OK(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
+            HIP_OK(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+            const unsigned nthr = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+            auto copy_out = [&](int c0, int r, const char *src) {
+                int32_t *dst = out.labels.get() + (size_t) c0 * n;
+                const size_t total = (size_t) r * n;
+                const size_t per = (total + nthr - 1) / nthr;
+                std::vector<std::thread> th;
+                for (unsigned t = 0; t < nthr; ++t) {
+                    const size_t lo = std::min(total, t * per), hi = std::min(total, lo + per);
+                    if (lo < hi)
+                        th.emplace_back([=] {
+                            std::memcpy(dst + lo, reinterpret_cast<const int32_t *>(src) + lo,
+                                        (hi - lo) * sizeof(int32_t));
+                        });
+                }
+                for (auto &t : th) t.join();
+                const int *hc = reinterpret_cast<const int *>(src + (size_t) r * n * sizeof(int32_t));
+                for (int t = 0; t < r; ++t) stats.clusters += hc[t];
+            };
+            int prev_c0 = -1, prev_r = 0, prev_buf = 0;
+            int buf = 0;
+            for (int c0 = 0; c0 < m; c0 += R) {
+                const int r = std::min(R, m - c0);
+                std::vector<EmitRow> er(r);
+                for (int t = 0; t < r; ++t) er[t] = EmitRow{tab[rows[c0 + t]].comp, tab[rows[c0 + t]].inres};
+                EmitRow *der = upload(batch, er);
+                HIP_OK(launch_emit(der, r, n, flags, idx, lab, cnts, tmp, sb, st));
+                HIP_OK(hipMemcpyAsync(stage[buf], lab, (size_t) r * n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+                HIP_OK(hipMemcpyAsync(stage[buf] + (size_t) r * n * sizeof(int32_t), cnts, (size_t) r * sizeof(int),
+                                      hipMemcpyDeviceToHost, st));
+                HIP_OK(hipEventRecord(done[buf], st));
+                if (prev_c0 >= 0) {
+                    HIP_OK(hipEventSynchronize(done[prev_buf]));
+                    copy_out(prev_c0, prev_r, stage[prev_buf]);
+                }
+                // the device reuses flags/idx/lab for the next chunk only after
+                // this chunk's read-back, in stream order
+                prev_c0 = c0;
+                prev_r = r;
+                prev_buf = buf;
+                buf ^= 1;
+            }
+            HIP_OK(hipEventSynchronize(done[prev_buf]));
+            copy_out(prev_c0, prev_r, stage[prev_buf]);
+            (void) hipEventDestroy(done[0]);
+            (void) hipEventDestroy(done[1]);
+        }
+        stats.subspaces = (int64_t) m;
         std::vector<unsigned long long> h2(16 * 64);
         HIP_OK(hipMemcpy(h2.data(), small + kStatAt, h2.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
         for (int b = 0; b < 64; ++b) {
@@ -747,7 +825,6 @@ private:
         uint64_t *spm = nullptr;
         int nsp_max = 0, nsp = 0;
         int slot = -1;               // its path counter in the level's counter array
-        int pass_id = -1;            // index among the surviving trees
         std::vector<int> kids;       // children in (dimension, cell group) order
     };
     // one child subspace S | {m} of the frontier trees act[t0..t1) (all of mask S)
@@ -760,10 +837,11 @@ private:
     Pool lvl[2];    // frontier paths and counters of the current / next level
     Pool batch;     // one batch of child subspaces
     int cur_lvl = 0;
-    int npass = 0;
     HostBuf up, down;
+    HostBuf emit_stage[2];
     size_t up_top = 0;
     size_t batch_budget = 0;
+    bool batch_trace = std::getenv("INSCY_TRACE_BATCHES") != nullptr;  // one stderr line per batch
     std::vector<int> lv_act, lv_next, lv_gidoff;
     std::vector<std::vector<std::vector<Group>>> lv_groups;
     int *lv_gid = nullptr;
@@ -779,7 +857,14 @@ private:
     }
     void expand_level(const std::vector<int> &frontier, const int *counts);
     void process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1);
-    void replay_tree(int f, int *keep_log);
+    void replay_tree(int f);
+    // replay state: superset rows per subspace, decisions per tree
+    std::vector<uint64_t> emask;                 // mask of every result-table row
+    std::unordered_map<uint64_t, std::pair<int, int>> sup_at;  // subspace -> (offset, count) in d_sup
+    int *d_sup = nullptr;
+    int *keepv = nullptr;
+    int *rep_words = nullptr;  // [0] running maximum, [1] finished blocks
+    void set_bounds(uint64_t s, BatchSub &x);
     void trace_tree(int f, const std::vector<int> &keep);
     std::vector<KernelTimer> timers;
 };

@@ -111,8 +111,13 @@ int32_t inscy_result_subspace(const inscy_result *res, int32_t i, int32_t *dims_
 }
 
 const int32_t *inscy_result_labels(const inscy_result *res, int32_t i) {
-    if (!res || i < 0 || i >= (int32_t) res->labels.size()) return nullptr;
-    return res->labels[i].data();
+    if (!res || i < 0 || i >= (int32_t) res->subspaces.size() || !res->labels) return nullptr;
+    return res->labels.get() + (size_t) i * res->n;
+}
+
+const int32_t *inscy_result_labels_all(const inscy_result *res) {
+    if (!res || res->subspaces.empty()) return nullptr;
+    return res->labels.get();
 }
 
 int64_t inscy_result_num_points(const inscy_result *res) { return res ? res->n : 0; }

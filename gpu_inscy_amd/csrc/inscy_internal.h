@@ -82,15 +82,16 @@ struct KernelTimer {
 // One subspace of a batch: its grid index lives in the batch buffers at sorted
 // positions [base, base + n), coordinates at [coord, coord + k*n), grid cells
 // at [hoff, hoff + ncell).
+constexpr int kMaxBounds = 8;
 struct BatchSub {
     SubspaceArg S;
     GridArg G;
     long long base, coord, hoff, qoff;
     int *cnt;           // [n] neighbour counts (capped), -1 = not counted
-    const int *bound;   // [n] complete counts of a subspace of S, or null
+    const int *bound[kMaxBounds];  // [n] complete counts of subspaces S minus one dimension
     int *comp;          // [n] component roots (components pass)
     int *csize;         // [n] component sizes (components pass)
-    int ncell, cap, lo, thr_dense, qmax, pad;
+    int nbound, ncell, cap, lo, thr_dense, qmax;
 };
 
 // Device buffers of one batch's grid indexes.
@@ -100,9 +101,13 @@ struct BatchIndex {
     int *order;           // [B*n] point id at each sorted position
     float *Ys;            // coordinates in sorted order
     uint16_t *Es;         // eps-grid cells in sorted order
-    uint8_t *qflag;       // [B*n] query flag per sorted position
-    int *qlist;           // local sorted positions of the queries
-    int *qcount;          // [B]
+    uint8_t *qflag;       // [B*n] query flag per (global) sorted position
+    int *qlist;           // global sorted positions of the queries, ascending
+    int *nsel;            // [1] number of queries
+    int *qstart;          // [B+1] first query of each subspace in qlist
+    void *select_temp;
+    size_t select_bytes;
+    int *nb;              // [B*n*2] first two neighbours found by the count passes
     uint8_t *dsorted;     // [B*n] dense flag per sorted position
     int *par;             // [B*n] union-find parents (point ids within each subspace)
     int *giant;           // [B]
@@ -164,6 +169,7 @@ struct CopySeg {
 // of ClusteringCpu.cu:52-81 is then sum <= s_star without a device sqrt.
 float sqrt_threshold(float eps);
 size_t scan_temp_bytes(int items);
+size_t select_temp_bytes(int items);
 
 // Launchers (all asynchronous on `st`; return the hipError_t of the launches).
 hipError_t launch_discretise(const float *X, int n, int d, int cells, float eps, float *Xt, uint8_t *cellC,
@@ -198,7 +204,14 @@ hipError_t launch_join(const uint8_t *lab, int n, const int *compS, const int *c
                        Entry *E, const int *nE, uint64_t smask, int eS, uint8_t *inresS, int *list, int *nlist,
                        uint8_t *keepf, uint8_t *drop, uint8_t *fresh, hipStream_t st);
 hipError_t launch_set_entry(Entry *E, int idx, const Entry &e, int *nE, int count, hipStream_t st);
-hipError_t launch_labels(const int *comp, const uint8_t *inres, int n, int *out, hipStream_t st);
+// canonical labels of r result subspaces: flags and idx hold r*n + 1 ints,
+// labels r*n, counts r (clusters per row)
+struct EmitRow {
+    const int *comp;
+    const uint8_t *inres;
+};
+hipError_t launch_emit(const EmitRow *rows, int r, int n, int *flags, int *idx, int *labels, int *counts,
+                       void *scan_temp, size_t scan_bytes, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t count, hipStream_t st);
 hipError_t launch_fill_i32(int *p, int v, size_t count, hipStream_t st);
 hipError_t launch_gather(const int *src, const int *idx, int count, int *out, hipStream_t st);
@@ -228,12 +241,16 @@ hipError_t launch_b_prune(const DevData &D, const PruneChild *C, int nc, const i
                           const int *blk_p, int nblk_p, hipStream_t st);
 hipError_t launch_b_copy(const CopySeg *segs, const int *blk_off, int nseg, int nblk, hipStream_t st);
 hipError_t launch_b_entries(Entry *E, int at, const Entry *src, int count, int *nE, hipStream_t st);
+// dst[src[i]] = pad for every segment (the tree stamps of the replay)
+hipError_t launch_b_stamp(const CopySeg *segs, const int *blk_off, int nseg, int nblk, hipStream_t st);
+// replay: pruneRedundancy of one tree (keepv[tree] = decision; mx, done: two
+// zeroed words reset by the launch), join of one (parent, dimension) pair
+hipError_t launch_rep_redundancy(const Entry *E, const int *list, int nlist, const int *tstamp, int tree, int nW,
+                                 float r, int *mx, unsigned *done, int *keepv, hipStream_t st);
+hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *keepv, const int *compS,
+                           const int *csizeS, int min_size, float r, int n, Entry *E, const int *list, int nlist,
+                           int eS, uint8_t *inresS, uint8_t *keepf, uint8_t *drop, uint8_t *fresh, hipStream_t st);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t count, hipStream_t st);
-// pruneRedundancy + cluster seeding of one tree whose point count is known on the host;
-// keep_log[log_idx] receives the decision when keep_log is given
-hipError_t launch_redundancy_seed_n(const int *W, int nW, int *stamp, int tid, const Entry *E, const int *nE,
-                                    uint64_t smask, const int *comp, int *list, int *nlist, int *mn, float r,
-                                    int *keep, uint8_t *lab, int *keep_log, int log_idx, hipStream_t st);
 constexpr int kBlockThreads = 256;   // threads per block of every pass
 constexpr int kQueriesPerBlock = 16;  // queries per block of the neighbour passes
 

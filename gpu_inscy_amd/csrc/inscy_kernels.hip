@@ -412,11 +412,6 @@ __global__ void k_stamp(const int *__restrict__ W, const int *__restrict__ nW, i
     if (i < *nW) stamp[W[i]] = tid;
 }
 
-__global__ void k_stamp_n(const int *__restrict__ W, int nW, int *__restrict__ stamp, int tid) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nW) stamp[W[i]] = tid;
-}
-
 __global__ void k_path_bits(const int *__restrict__ spx, const uint64_t *__restrict__ spm, const int *__restrict__ nsp,
                             const int *__restrict__ stamp, int tid, unsigned long long *__restrict__ bits) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -534,7 +529,7 @@ __global__ void k_redundancy(const int *__restrict__ stamp, int tid, const Entry
 
 // keep iff no superspace cluster touches the region or |W| * r > max_h min_h
 __global__ void k_decide(const int *__restrict__ nlist, int *__restrict__ mn, int nW, float r,
-                         int *__restrict__ keep, int *__restrict__ log, int log_idx) {
+                         int *__restrict__ keep) {
     if (threadIdx.x != 0) return;
     const int nl = *nlist;
     int max_min = 0;
@@ -543,9 +538,7 @@ __global__ void k_decide(const int *__restrict__ nlist, int *__restrict__ mn, in
         mn[h] = INT_MAX;
     }
     float lhs = (float) nW * r;
-    const int k = (max_min == 0 || (double) lhs > (double) max_min) ? 1 : 0;
-    *keep = k;
-    if (log) log[log_idx] = k;
+    *keep = (max_min == 0 || (double) lhs > (double) max_min) ? 1 : 0;
 }
 
 // cluster (ClusteringCpu.cu:317-389): a BFS from region point w labels w's
@@ -622,11 +615,30 @@ __global__ void k_set_entry(Entry *__restrict__ E, int idx, Entry e, int *__rest
     }
 }
 
-__global__ void k_labels(const int *__restrict__ comp, const uint8_t *__restrict__ inres, int n, int *__restrict__ out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Canonical labels of a chunk of result subspaces (grid.y = row): clusters
+// are numbered by their smallest point id, which is the component root.
+// Pass 1 flags the kept roots, an exclusive scan over the whole chunk numbers
+// them, pass 2 writes every point's number relative to its row's first root.
+__global__ void k_emit_roots(const EmitRow *__restrict__ rows, int n, int *__restrict__ flags) {
+    const EmitRow R = rows[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int c = comp[i];
-    out[i] = (c >= 0 && inres[c]) ? c : -1;
+    flags[(size_t) blockIdx.y * n + i] = (R.comp[i] == i && R.inres[i]) ? 1 : 0;
+}
+
+__global__ void k_emit_labels(const EmitRow *__restrict__ rows, int n, const int *__restrict__ idx,
+                              int *__restrict__ labels) {
+    const EmitRow R = rows[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t row = (size_t) blockIdx.y * n;
+    const int c = R.comp[i];
+    labels[row + i] = (c >= 0 && R.inres[c]) ? idx[row + c] - idx[row] : -1;
+}
+
+__global__ void k_emit_counts(const int *__restrict__ idx, int n, int r, int *__restrict__ counts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < r) counts[t] = idx[(size_t) (t + 1) * n] - idx[(size_t) t * n];
 }
 
 template <class T>
@@ -810,18 +822,7 @@ hipError_t launch_redundancy_seed(const int *W, const int *nW_dev, int nW, int *
     LAUNCH(k_select, dim3(1), dim3(kBlock), 0, st, E, nE, smask, list, nlist);
     if (nW > 0) LAUNCH(k_stamp, dim3(blocks_for(nW)), dim3(kBlock), 0, st, W, nW_dev, stamp, tid);
     LAUNCH(k_redundancy, dim3(256), dim3(kBlock), 0, st, stamp, tid, E, list, nlist, mn);
-    LAUNCH(k_decide, dim3(1), dim3(64), 0, st, nlist, mn, nW, r, keep, (int *) nullptr, 0);
-    if (nW > 0) LAUNCH(k_seed_if, dim3(blocks_for(nW)), dim3(kBlock), 0, st, W, nW, comp, keep, lab);
-    return hipGetLastError();
-}
-
-hipError_t launch_redundancy_seed_n(const int *W, int nW, int *stamp, int tid, const Entry *E, const int *nE,
-                                    uint64_t smask, const int *comp, int *list, int *nlist, int *mn, float r,
-                                    int *keep, uint8_t *lab, int *keep_log, int log_idx, hipStream_t st) {
-    LAUNCH(k_select, dim3(1), dim3(kBlock), 0, st, E, nE, smask, list, nlist);
-    if (nW > 0) LAUNCH(k_stamp_n, dim3(blocks_for(nW)), dim3(kBlock), 0, st, W, nW, stamp, tid);
-    LAUNCH(k_redundancy, dim3(256), dim3(kBlock), 0, st, stamp, tid, E, list, nlist, mn);
-    LAUNCH(k_decide, dim3(1), dim3(64), 0, st, nlist, mn, nW, r, keep, keep_log, log_idx);
+    LAUNCH(k_decide, dim3(1), dim3(64), 0, st, nlist, mn, nW, r, keep);
     if (nW > 0) LAUNCH(k_seed_if, dim3(blocks_for(nW)), dim3(kBlock), 0, st, W, nW, comp, keep, lab);
     return hipGetLastError();
 }
@@ -842,8 +843,17 @@ hipError_t launch_set_entry(Entry *E, int idx, const Entry &e, int *nE, int coun
     return hipGetLastError();
 }
 
-hipError_t launch_labels(const int *comp, const uint8_t *inres, int n, int *out, hipStream_t st) {
-    LAUNCH(k_labels, dim3(blocks_for(n)), dim3(kBlock), 0, st, comp, inres, n, out);
+hipError_t launch_emit(const EmitRow *rows, int r, int n, int *flags, int *idx, int *labels, int *counts,
+                       void *scan_temp, size_t scan_bytes, hipStream_t st) {
+    if (r <= 0) return hipSuccess;
+    const size_t items = (size_t) r * n + 1;
+    LAUNCH(k_fill<int>, dim3(1), dim3(kBlock), 0, st, flags + (size_t) r * n, 0, (size_t) 1);
+    LAUNCH(k_emit_roots, dim3(blocks_for(n), r), dim3(kBlock), 0, st, rows, n, flags);
+    size_t tb = scan_bytes;
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(scan_temp, tb, flags, idx, (int) items, st);
+    if (e != hipSuccess) return e;
+    LAUNCH(k_emit_labels, dim3(blocks_for(n), r), dim3(kBlock), 0, st, rows, n, idx, labels);
+    LAUNCH(k_emit_counts, dim3(blocks_for(r)), dim3(kBlock), 0, st, idx, n, r, counts);
     return hipGetLastError();
 }
 

@@ -40,7 +40,7 @@ void Engine::put(void *dst, const void *src, size_t bytes) {
 
 void Engine::run_levels(inscy_result &out) {
     lt.clear();
-    npass = 0;
+    emask.clear();
     store.chunk = (size_t) 256 << 20;
     batch.chunk = (size_t) 256 << 20;
     up.get((size_t) 4 << 20);
@@ -69,6 +69,7 @@ void Engine::run_levels(inscy_result &out) {
     lt.push_back(std::move(root));
     stats.calls = 1;
 
+    auto t0 = std::chrono::steady_clock::now();
     std::vector<int> frontier{0};
     const int *counts = cnt0;
     while (!frontier.empty()) {
@@ -84,16 +85,51 @@ void Engine::run_levels(inscy_result &out) {
         std::stable_sort(t.kids.begin(), t.kids.end(), [&](int a, int b) {
             return lt[a].m != lt[b].m ? lt[a].m < lt[b].m : lt[a].lo < lt[b].lo;
         });
-    int *keep_log = store.take<int>(std::max(npass, 1));
-    replay_tree(0, keep_log);
+    sync();
+    auto t1 = std::chrono::steady_clock::now();
+    // superset rows of every subspace with a row: AND of per-dimension bitsets
+    {
+        const int ne = (int) emask.size();
+        const int words = (ne + 63) / 64;
+        std::vector<std::vector<uint64_t>> has(d, std::vector<uint64_t>(words, 0));
+        for (int e = 0; e < ne; ++e)
+            for (int k = 0; k < d; ++k)
+                if ((emask[e] >> k) & 1ull) has[k][e >> 6] |= 1ull << (e & 63);
+        std::vector<int> flat;
+        std::vector<uint64_t> acc(words);
+        sup_at.clear();
+        for (int e = 0; e < ne; ++e) {
+            const uint64_t sm = emask[e];
+            std::fill(acc.begin(), acc.end(), ~0ull);
+            for (int k = 0; k < d; ++k)
+                if ((sm >> k) & 1ull)
+                    for (int w = 0; w < words; ++w) acc[w] &= has[k][w];
+            const int at = (int) flat.size();
+            for (int w = 0; w < words; ++w)
+                for (uint64_t b = acc[w]; b; b &= b - 1) flat.push_back(w * 64 + __builtin_ctzll(b));
+            sup_at[sm] = {at, (int) flat.size() - at};
+        }
+        d_sup = upload(store, flat);
+    }
+    keepv = store.take<int>(lt.size());
+    rep_words = store.take<int>(2);
+    HIP_OK(launch_fill_i32(rep_words, 0, 2, st));
+    HIP_OK(launch_fill_u8(keepf, 0, (size_t) 2 * n, st));  // keepf and dropf are adjacent
+    replay_tree(0);
+    sync();
+    auto t2 = std::chrono::steady_clock::now();
     if (trace_on) {
-        std::vector<int> keep(std::max(npass, 1));
-        HIP_OK(hipMemcpyAsync(keep.data(), keep_log, keep.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+        std::vector<int> keep(lt.size());
+        HIP_OK(hipMemcpyAsync(keep.data(), keepv, keep.size() * sizeof(int), hipMemcpyDeviceToHost, st));
         sync();
         trace.clear();
         trace_tree(0, keep);
     }
+    auto t3 = std::chrono::steady_clock::now();
     emit(out);
+    stats.expand_seconds = std::chrono::duration<double>(t1 - t0).count();
+    stats.replay_seconds = std::chrono::duration<double>(t2 - t1).count();
+    stats.emit_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t3).count();
 }
 
 // One level: merge flags and exact path counts of the frontier, then the
@@ -237,8 +273,8 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         x.ncell = (int) ncell;
         x.cap = cap_of(x.S.k);
         x.thr_dense = dense_thr[x.S.k];
-        x.lo = 0;
         x.qmax = (int) qmax;
+        set_bounds(c.s, x);
         coord += (long long) x.S.k * n;
         hoff += ncell;
         qoff += qmax;
@@ -255,7 +291,11 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.Es = batch.take<uint16_t>(coord);
     X.qflag = batch.take<uint8_t>((size_t) B * n);
     X.qlist = batch.take<int>(std::max<long long>(qoff, 1));
-    X.qcount = batch.take<int>(B);
+    X.nsel = batch.take<int>(1);
+    X.qstart = batch.take<int>(B + 1);
+    X.select_bytes = select_temp_bytes(B * n);
+    X.select_temp = batch.take<char>(X.select_bytes);
+    X.nb = batch.take<int>((size_t) 2 * B * n);
     X.dsorted = batch.take<uint8_t>((size_t) B * n);
     X.par = batch.take<int>((size_t) B * n);
     X.giant = batch.take<int>(B);
@@ -283,7 +323,17 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     std::vector<int> cblk(B + 1, 0);
     for (int b = 0; b < B; ++b) cblk[b + 1] = cblk[b] + (hs[b].qmax + QPB - 1) / QPB;
     int *dcb = upload(batch, cblk);
+    const bool dbg = batch_trace;
+    std::chrono::steady_clock::time_point tq0, tq1;
+    if (dbg) {
+        sync();
+        tq0 = std::chrono::steady_clock::now();
+    }
     HIP_OK(launch_b_count(dsubs, B, dcb, cblk[B], n, kmax, X, s_star, small + kStatAt, timer(), st));
+    if (dbg) {
+        sync();
+        tq1 = std::chrono::steady_clock::now();
+    }
 
     // ---- partition of every (tree, dimension) pair by the cell groups
     struct PairInfo {
@@ -337,6 +387,8 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
 
     // ---- children: trace rows, surviving trees, their point lists and path prunes
     std::vector<CopySeg> copies;
+    std::vector<CopySeg> stamps;  // W points -> tree index, into the subspace's stamp array
+    std::vector<int> stamp_b;
     std::vector<PruneChild> prune;
     std::vector<size_t> prune_slots;
     std::vector<char> passing(B, 0);
@@ -363,10 +415,13 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
             stats.trees++;
             if (ch.pass) {
                 stats.calls++;
-                ch.pass_id = npass++;
                 passing[pi.b] = 1;
                 int *Wst = store.take<int>(std::max(nW, 1));
-                if (nW > 0) copies.push_back({pairs[j].outW + offW, Wst, nW, 0});
+                if (nW > 0) {
+                    copies.push_back({pairs[j].outW + offW, Wst, nW, 0});
+                    stamps.push_back({pairs[j].outW + offW, nullptr, nW, (int) lt.size()});
+                    stamp_b.push_back(pi.b);
+                }
                 ch.W = Wst;
                 if (ch.first < d) {
                     ch.slot = lv_nslots++;
@@ -434,7 +489,10 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         int *csizeP = slabs.take<int>((size_t) ncomp * n);
         int *memP = slabs.take<int>((size_t) ncomp * n);
         uint8_t *inresP = slabs.take<uint8_t>((size_t) ncomp * n);
+        int *stampP = slabs.take<int>((size_t) ncomp * n);
         HIP_OK(launch_fill_u8(inresP, 0, (size_t) ncomp * n, st));
+        HIP_OK(launch_fill_i32(stampP, -1, (size_t) ncomp * n, st));
+        std::vector<int *> stamp_of(B, nullptr);
         grow_entries(nentries + ncomp);
         int ci = 0;
         for (int b = 0; b < B; ++b) {
@@ -445,30 +503,20 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
             x.comp = compP + (size_t) ci * n;
             x.csize = csizeP + (size_t) ci * n;
             copies.push_back({hs[b].cnt, x.cnt, n, 0});
-            // a subspace S' = S minus one dimension with complete counts bounds
-            // the counts of S from above
-            const int k = x.S.k;
-            int lo = std::min(weak_thr, dense_thr[k]);
-            x.bound = nullptr;
-            for (int t = 0; t < k && !x.bound; ++t) {
-                const uint64_t sub = s & ~(1ull << x.S.dims[t]);
-                auto it = subs.find(sub);
-                if (sub && it != subs.end() && it->second.entry >= 0) {
-                    x.bound = it->second.cnt;
-                    lo = std::min(lo, cap_of(k - 1));
-                }
-            }
-            x.lo = lo;
             hc.push_back(x);
             Subspace &sb = subs[s];
             sb.cnt = x.cnt;
             sb.comp = x.comp;
             sb.csize = x.csize;
             sb.inres = inresP + (size_t) ci * n;
+            sb.tstamp = stampP + (size_t) ci * n;
+            stamp_of[b] = sb.tstamp;
             sb.entry = nentries + ci;
+            emask.push_back(s);
             ents.push_back(Entry{s, sb.comp, sb.csize, sb.inres, memP + (size_t) ci * n, 0, 0});
             ++ci;
         }
+        for (size_t u = 0; u < stamps.size(); ++u) stamps[u].dst = stamp_of[stamp_b[u]];
     }
     if (!copies.empty()) {
         std::vector<int> cb(copies.size() + 1, 0);
@@ -477,10 +525,45 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         int *dcb2 = upload(batch, cb);
         HIP_OK(launch_b_copy(dcs, dcb2, (int) copies.size(), cb.back(), st));
     }
+    if (!stamps.empty()) {
+        std::vector<int> sbk(stamps.size() + 1, 0);
+        for (size_t c = 0; c < stamps.size(); ++c) sbk[c + 1] = sbk[c] + nblocks(stamps[c].len);
+        CopySeg *dss = upload(batch, stamps);
+        int *dsb = upload(batch, sbk);
+        HIP_OK(launch_b_stamp(dss, dsb, (int) stamps.size(), sbk.back(), st));
+    }
     if (ncomp > 0) {
         BatchSub *dhc = upload(batch, hc);
+        std::chrono::steady_clock::time_point tc0, tc1, tc2;
+        if (dbg) {
+            sync();
+            tc0 = std::chrono::steady_clock::now();
+        }
         HIP_OK(launch_b_count(dhc, ncomp, nullptr, 0, n, kmax, X, s_star, small + kStatAt, timer(), st));
+        if (dbg) {
+            sync();
+            tc1 = std::chrono::steady_clock::now();
+        }
         HIP_OK(launch_b_components(dhc, ncomp, n, kmax, X, s_star, st));
+        if (dbg) {
+            sync();
+            tc2 = std::chrono::steady_clock::now();
+            int kmin = 64, kmx = 0;
+            long long qs = 0;
+            for (int b = 0; b < B; ++b) {
+                kmin = std::min(kmin, hs[b].S.k);
+                kmx = std::max(kmx, hs[b].S.k);
+                qs += hs[b].qmax;
+            }
+            auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+                return std::chrono::duration<double, std::milli>(b - a).count();
+            };
+            std::fprintf(stderr,
+                         "[inscy batch] B=%d k=%d..%d cap=%d..%d queries<=%lld list_count=%.3fms "
+                         "comp=%d all_count=%.3fms components=%.3fms\n",
+                         B, kmin, kmx, cap_of(kmx), cap_of(kmin), qs, ms(tq0, tq1), ncomp, ms(tc0, tc1),
+                         ms(tc1, tc2));
+        }
         Entry *de = upload(batch, ents);
         HIP_OK(launch_b_entries(entries, nentries, de, ncomp, d_nE, st));
         nentries += ncomp;
@@ -488,33 +571,55 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     }
 }
 
+// The subspaces S minus one dimension whose counts are complete (their
+// dense components were built) bound the counts of S from above; a bound
+// decides a point only below both thresholds of S and below the cap of the
+// smaller subspaces, under which their counts are exact.
+void Engine::set_bounds(uint64_t s, BatchSub &x) {
+    const int k = x.S.k;
+    int lo = std::min(weak_thr, dense_thr[k]);
+    x.nbound = 0;
+    for (int t = 0; t < k && x.nbound < kMaxBounds; ++t) {
+        const uint64_t sub = s & ~(1ull << x.S.dims[t]);
+        auto it = subs.find(sub);
+        if (sub && it != subs.end() && it->second.entry >= 0) x.bound[x.nbound++] = it->second.cnt;
+    }
+    if (x.nbound > 0) lo = std::min(lo, cap_of(k - 1));
+    x.lo = lo;
+}
+
 // pruneRedundancy and join of the subtree of f, in the reference's order:
 // per dimension, every surviving child's subtree first, then its redundancy
-// test and cluster seeds, then the join of the dimension's clusters.
-void Engine::replay_tree(int f, int *keep_log) {
-    const std::vector<int> kids = lt[f].kids;
+// decision, then the join of the clusters of the kept children.
+void Engine::replay_tree(int f) {
+    const std::vector<int> &kids = lt[f].kids;
     size_t i = 0;
     while (i < kids.size()) {
         const int m = lt[kids[i]].m;
         size_t j = i;
-        while (j < kids.size() && lt[kids[j]].m == m) ++j;
-        const uint64_t s = lt[kids[i]].smask;
-        const size_t mk = arena.mark();
-        uint8_t *lab = nullptr;
-        for (size_t u = i; u < j; ++u) {
-            const int c = kids[u];
-            if (!lt[c].pass) continue;
-            replay_tree(c, keep_log);
-            if (!lab) {
-                lab = arena.take<uint8_t>(n);
-                HIP_OK(launch_fill_u8(lab, 0, n, st));
-            }
-            const Subspace &sub = subs[s];
-            HIP_OK(launch_redundancy_seed_n(lt[c].W, lt[c].nW, stamp, ++tree_id, entries, d_nE, s, sub.comp, d_list,
-                                            d_nlist, d_mn, prm.r, d_keep, lab, keep_log, lt[c].pass_id, st));
+        int t_lo = INT_MAX, t_hi = -1;
+        bool any = false;
+        while (j < kids.size() && lt[kids[j]].m == m) {
+            t_lo = std::min(t_lo, kids[j]);
+            t_hi = std::max(t_hi, kids[j] + 1);
+            any |= lt[kids[j]].pass;
+            ++j;
         }
-        if (lab) join(s, lab);
-        arena.release(mk);
+        if (any) {
+            const uint64_t s = lt[kids[i]].smask;
+            const Subspace &sub = subs[s];
+            const auto sl = sup_at.at(s);
+            const int *list = d_sup + sl.first;
+            for (size_t u = i; u < j; ++u) {
+                const int c = kids[u];
+                if (!lt[c].pass) continue;
+                replay_tree(c);
+                HIP_OK(launch_rep_redundancy(entries, list, sl.second, sub.tstamp, c, lt[c].nW, prm.r, rep_words,
+                                             reinterpret_cast<unsigned *>(rep_words + 1), keepv, st));
+            }
+            HIP_OK(launch_rep_join(sub.tstamp, t_lo, t_hi, keepv, sub.comp, sub.csize, prm.min_size, prm.r, n, entries,
+                                   list, sl.second, sub.entry, sub.inres, keepf, dropf, freshf, st));
+        }
         i = j;
     }
 }
@@ -523,7 +628,7 @@ void Engine::trace_tree(int f, const std::vector<int> &keep) {
     for (int c : lt[f].kids) {
         const LTree &t = lt[c];
         trace.push_back({(long long) t.smask, t.lo, t.hi, t.pre, t.nW, t.pass ? 1 : 0,
-                         t.pass ? (keep[t.pass_id] ? 0 : 1) : -1});
+                         t.pass ? (keep[c] ? 0 : 1) : -1});
         if (t.pass) trace_tree(c, keep);
     }
 }

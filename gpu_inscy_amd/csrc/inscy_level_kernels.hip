@@ -10,6 +10,7 @@
 // Floating point follows inscy_kernels.hip: no contraction, and the distance
 // test is sum <= s_star.
 
+#include <algorithm>
 #include <climits>
 
 #include <hipcub/hipcub.hpp>
@@ -72,21 +73,28 @@ __global__ void k_b_keys(DevData D, const BatchSub *__restrict__ subs, int *__re
 }
 
 // `off` is one exclusive scan over the grid cells of every subspace of the
-// batch, so positions are global: subspace b holds [b*n, (b+1)*n).
+// batch, so positions are global: subspace b holds [b*n, (b+1)*n).  The
+// scatter writes one int per point; the coordinates and cells are then
+// gathered in sorted order, so their writes are coalesced.
 __global__ void k_b_scatter(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ keys,
-                            const int *__restrict__ rank, const int *__restrict__ off, int *__restrict__ order,
-                            float *__restrict__ Ys, uint16_t *__restrict__ Es) {
+                            const int *__restrict__ rank, const int *__restrict__ off, int *__restrict__ order) {
+    const BatchSub &B = subs[blockIdx.y];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= D.n) return;
+    order[off[B.hoff + keys[B.base + p]] + rank[B.base + p]] = p;
+}
+
+__global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ order,
+                           float *__restrict__ Ys, uint16_t *__restrict__ Es) {
     const BatchSub &B = subs[blockIdx.y];
     const int n = D.n;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const int pos = off[B.hoff + keys[B.base + p]] + rank[B.base + p];
-    order[pos] = p;
-    const long long local = pos - B.base;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int p = order[B.base + i];
     for (int t = 0; t < B.S.k; ++t) {
         const int s = B.S.dims[t];
-        Ys[B.coord + (long long) t * n + local] = D.Xt[(size_t) s * n + p];
-        Es[B.coord + (long long) t * n + local] = D.cellE[(size_t) s * n + p];
+        Ys[B.coord + (long long) t * n + i] = D.Xt[(size_t) s * n + p];
+        Es[B.coord + (long long) t * n + i] = D.cellE[(size_t) s * n + p];
     }
 }
 
@@ -103,14 +111,27 @@ __global__ void k_b_qmark(const BatchSub *__restrict__ subs, const QuerySeg *__r
     qflag[off[B.hoff + keys[B.base + p]] + rank[B.base + p]] = 1;
 }
 
-__global__ void k_b_qcompact(const BatchSub *__restrict__ subs, int n, const uint8_t *__restrict__ qflag,
-                             int *__restrict__ qlist, int *__restrict__ qcount) {
-    const int b = blockIdx.y;
-    const BatchSub &B = subs[b];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool q = i < n && qflag[B.base + i];
-    const int slot = wave_slot(q, &qcount[b]);
-    if (q && slot < B.qmax) qlist[B.qoff + slot] = i;
+// qstart[b] = first index of the selected (ascending global) positions that is
+// >= b*n; qstart[nsub] = the number selected
+__global__ void k_b_qstart(const int *__restrict__ qlist, const int *__restrict__ nsel, int nsub, int n,
+                           int *__restrict__ qstart) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b > nsub) return;
+    const int total = *nsel;
+    if (b == nsub) {
+        qstart[b] = total;
+        return;
+    }
+    const long long key = (long long) b * n;
+    int lo = 0, hi = total;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (qlist[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    qstart[b] = lo;
 }
 
 // ------------------------------------------------------------ neighbour counts
@@ -122,8 +143,8 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
                                                     int nsub, int n, const int *__restrict__ off,
                                                     const int *__restrict__ order, const float *__restrict__ Ys,
                                                     const uint16_t *__restrict__ Es, const int *__restrict__ qlist,
-                                                    const int *__restrict__ qcount, float s_star,
-                                                    unsigned long long *__restrict__ stat) {
+                                                    const int *__restrict__ qstart, float s_star,
+                                                    int *__restrict__ nb, unsigned long long *__restrict__ stat) {
     constexpr int LPQ = kLanesPerQuery;
     extern __shared__ char smem[];
     int b, blk;
@@ -142,9 +163,9 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
     int i = 0;
     bool active;
     if (LIST) {
-        const int q = blk * QPB + lq;
-        active = q < min(qcount[b], B.qmax);
-        if (active) i = qlist[B.qoff + q];
+        const int q = qstart[b] + blk * QPB + lq;
+        active = q < qstart[b + 1];
+        if (active) i = (int) (qlist[q] - B.base);
     } else {
         i = blk * QPB + lq;
         active = i < n;
@@ -154,11 +175,16 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
     int *cnt = B.cnt;
     const int p = active ? order[B.base + i] : 0;
     active = active && cnt[p] < 0;
-    // N_S(p) is a subset of N_S'(p) for S' a subset of S: a complete count of
-    // a subspace of S below both thresholds of S decides p without a scan
-    if (active && B.bound) {
-        const int bb = B.bound[p];
-        if (bb >= 0 && bb < B.lo) {
+    // N_S(p) is a subset of N_S'(p) for S' a subset of S: the smallest
+    // complete count of the subspaces S minus one dimension, when below both
+    // thresholds of S, decides p without a scan (and stays an upper bound)
+    if (active && B.nbound > 0) {
+        int bb = INT_MAX;
+        for (int t = 0; t < B.nbound; ++t) {
+            const int v = B.bound[t][p];
+            if (v >= 0) bb = min(bb, v);
+        }
+        if (bb < B.lo) {
             if (sub == 0) cnt[p] = bb;
             active = false;
         }
@@ -171,15 +197,23 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
         const float *qy = sy + lq;
         const int *qe = se + lq;
         const int gshift = (__lane_id() / LPQ) * LPQ;
+        const unsigned long long below = (1ull << sub) - 1ull;
         const int cap = B.cap;
         const int base0 = (int) B.base;
+        int *nbp = nb + 2 * (B.base + p);
         for_ranges(B.G, off + B.hoff, qe, QPB, [&](int a, int e) {
             a -= base0;
             e -= base0;
             for (int base = a; base < e && found < cap; base += LPQ) {
                 const int jj = base + sub;
                 const bool hit = jj < e && jj != i && is_neighbour(B.S, B.G, n, Yb, Eb, qy, qe, QPB, jj, s_star);
-                found += __popcll((__ballot(hit) >> gshift) & ((1ull << LPQ) - 1ull));
+                const unsigned long long mine = (__ballot(hit) >> gshift) & ((1ull << LPQ) - 1ull);
+                // the first two neighbours found: the sample edges of the components pass
+                if (hit && found < 2) {
+                    const int at = found + __popcll(mine & below);
+                    if (at < 2) nbp[at] = order[B.base + jj];
+                }
+                found += __popcll(mine);
                 seen += min(LPQ, e - base);
             }
         });
@@ -221,46 +255,25 @@ __global__ void k_b_dense_init(const BatchSub *__restrict__ subs, int n, const i
     B.csize[p] = 0;
 }
 
-__global__ __launch_bounds__(kBlock) void k_b_link_sample(const BatchSub *__restrict__ subs, int n,
-                                                          const int *__restrict__ off, const int *__restrict__ order,
-                                                          const float *__restrict__ Ys, const uint16_t *__restrict__ Es,
-                                                          const uint8_t *__restrict__ dsorted, float s_star,
-                                                          int *__restrict__ par) {
-    constexpr int LPQ = kLanesPerQuery;
-    extern __shared__ char smem[];
+// Sample phase of the components: every dense point links to the dense ones
+// among the first two neighbours its count pass recorded (any edges would do:
+// the full phase below links every point outside the giant component to all
+// its dense neighbours, so the result does not depend on the sample).
+__global__ void k_b_link_sample(const BatchSub *__restrict__ subs, int n, const int *__restrict__ nb,
+                                int *__restrict__ par) {
     const BatchSub &B = subs[blockIdx.y];
-    const int k = B.S.k;
-    float *sy = reinterpret_cast<float *>(smem);
-    int *se = reinterpret_cast<int *>(smem + sizeof(float) * QPB * k);
-    const int lq = threadIdx.x / LPQ, sub = threadIdx.x % LPQ;
-    const int i = blockIdx.x * QPB + lq;
-    const uint8_t *ds = dsorted + B.base;
-    const int *ord = order + B.base;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int *cnt = B.cnt;
+    const int c = cnt[p];
+    if (c < B.thr_dense) return;
     int *pr = par + B.base;
-    const float *Yb = Ys + B.coord;
-    const uint16_t *Eb = Es + B.coord;
-    const bool active = i < n && ds[i];
-    if (sub == 0 && active) stage_query(B.S, n, Yb, Eb, i, sy + lq, se + lq, QPB);
-    __syncthreads();
-    if (!active) return;
-    const float *qy = sy + lq;
-    const int *qe = se + lq;
-    const int p = ord[i];
-    const int gshift = (__lane_id() / LPQ) * LPQ;
-    const unsigned long long below = (1ull << sub) - 1ull;
-    const int base0 = (int) B.base;
-    int found = 0;
-    for_ranges(B.G, off + B.hoff, qe, QPB, [&](int a, int e) {
-        a -= base0;
-        e -= base0;
-        for (int base = a; base < e && found < 2; base += LPQ) {
-            const int jj = base + sub;
-            const bool hit = jj < e && jj != i && ds[jj] && is_neighbour(B.S, B.G, n, Yb, Eb, qy, qe, QPB, jj, s_star);
-            const unsigned long long mine = (__ballot(hit) >> gshift) & ((1ull << LPQ) - 1ull);
-            if (hit && __popcll(mine & below) < 2 - found) uf_union(pr, p, ord[jj]);
-            found += min(__popcll(mine), 2 - found);
-        }
-    });
+    const int *nbp = nb + 2 * (B.base + p);
+    const int k = min(c, 2);
+    for (int t = 0; t < k; ++t) {
+        const int q = nbp[t];
+        if (q >= 0 && q < n && cnt[q] >= B.thr_dense) uf_union(pr, p, q);
+    }
 }
 
 __global__ void k_b_uf_compress(const BatchSub *__restrict__ subs, int n, int *__restrict__ par) {
@@ -512,6 +525,125 @@ __global__ void k_b_path_check(DevData D, const PruneChild *__restrict__ C, cons
     }
 }
 
+// ------------------------------------------------------------ replay of pruneRedundancy and join
+// The depth-first order of the reference is replayed with one launch per
+// surviving tree and four per join.  `list` holds the result-table rows whose
+// subspace contains S (fixed once the expansion is done); a row counts while
+// its nkept > 0, which the joins earlier in stream order have set.  A tree is
+// known by its index; tstamp[p] is the tree of S whose weak-dense points hold p.
+
+// pruneRedundancy (ScyTreeNode.cu:669-734): per listed superspace H in the
+// result (one block per H), the smallest result cluster of H holding a point of
+// the tree; the largest of these minima decides.  The last block to finish
+// writes the decision.
+__global__ void k_rep_redundancy(const Entry *__restrict__ E, const int *__restrict__ list, int nlist,
+                                 const int *__restrict__ tstamp, int tree, int nW, float r, int *__restrict__ mx,
+                                 unsigned *__restrict__ done, int *__restrict__ keepv) {
+    __shared__ int wmin[kBlock / 64];
+    __shared__ bool last;
+    for (int h = blockIdx.x; h < nlist; h += gridDim.x) {
+        const Entry e = E[list[h]];
+        if (e.nkept <= 0) continue;
+        int best = INT_MAX;
+        for (int t = threadIdx.x; t < e.nmem; t += blockDim.x) {
+            const int q = e.mem[t];
+            if (tstamp[q] == tree) best = min(best, e.csize[e.comp[q]]);
+        }
+        for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_down(best, o));
+        if (__lane_id() == 0) wmin[threadIdx.x / 64] = best;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int b = INT_MAX;
+            for (int w = 0; w < kBlock / 64; ++w) b = min(b, wmin[w]);
+            if (b != INT_MAX) atomicMax(mx, b);
+        }
+        __syncthreads();
+    }
+    __threadfence();
+    if (threadIdx.x == 0) last = atomicAdd(done, 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        const int m = __hip_atomic_load(mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const float lhs = (float) nW * r;
+        keepv[tree] = (m == 0 || (double) lhs > (double) m) ? 1 : 0;
+        __hip_atomic_store(mx, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// join (util.cu:655-743) of the clusters of the kept trees [t_lo, t_hi) of one
+// (parent, dimension) pair: cluster (ClusteringCpu.cu:317-389) labels the dense
+// component of every point of a kept tree; components of at least min_size
+// are candidates
+__global__ void k_rep_join_mark(const int *__restrict__ tstamp, int t_lo, int t_hi, const int *__restrict__ keepv,
+                                const int *__restrict__ comp, const int *__restrict__ csize, int min_size, int n,
+                                uint8_t *__restrict__ keepf) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int t = tstamp[p];
+    if (t < t_lo || t >= t_hi || !keepv[t]) return;
+    const int c = comp[p];
+    if (c >= 0 && csize[c] >= min_size) keepf[c] = 1;
+}
+
+// a candidate K is dropped when one of its points lies in a result cluster of
+// a superspace H larger than |K| * r (one block per H)
+__global__ void k_rep_join_check(const Entry *__restrict__ E, const int *__restrict__ list, int nlist,
+                                 const int *__restrict__ compS, const uint8_t *__restrict__ keepf,
+                                 const int *__restrict__ csizeS, float r, uint8_t *__restrict__ drop) {
+    for (int h = blockIdx.x; h < nlist; h += gridDim.x) {
+        const Entry e = E[list[h]];
+        if (e.nkept <= 0) continue;
+        for (int t = threadIdx.x; t < e.nmem; t += blockDim.x) {
+            const int q = e.mem[t];
+            const int K = compS[q];
+            if (K < 0 || !keepf[K]) continue;
+            if ((float) csizeS[K] * r < (float) e.csize[e.comp[q]]) drop[K] = 1;
+        }
+    }
+}
+
+// survivors join the result of S; the candidate flags are cleared for the next join
+__global__ void k_rep_join_apply(uint8_t *__restrict__ keepf, uint8_t *__restrict__ drop, int n,
+                                 uint8_t *__restrict__ inres, uint8_t *__restrict__ fresh, Entry *__restrict__ E,
+                                 int eS) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int added = 0;
+    if (k < n) {
+        uint8_t f = 0;
+        if (keepf[k] && !drop[k] && !inres[k]) {
+            inres[k] = 1;
+            f = 1;
+            added = 1;
+        }
+        fresh[k] = f;
+        keepf[k] = 0;
+        drop[k] = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) added += __shfl_down(added, o);
+    if (__lane_id() == 0 && added) atomicAdd(&E[eS].nkept, added);
+}
+
+// members of the newly kept clusters join S's member list
+__global__ void k_rep_join_members(const int *__restrict__ compS, int n, const uint8_t *__restrict__ fresh,
+                                   Entry *__restrict__ E, int eS) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool in = false;
+    if (i < n) {
+        const int K = compS[i];
+        in = K >= 0 && fresh[K];
+    }
+    const int slot = wave_slot(in, &E[eS].nmem);
+    if (in) E[eS].mem[slot] = i;
+}
+
+__global__ void k_b_stamp(const CopySeg *__restrict__ segs, const int *__restrict__ blk_off, int ns) {
+    const int s = seg_of(blk_off, ns, blockIdx.x);
+    const CopySeg S = segs[s];
+    const int i = (blockIdx.x - blk_off[s]) * blockDim.x + threadIdx.x;
+    if (i < S.len) S.dst[S.src[i]] = S.pad;
+}
+
 // ------------------------------------------------------------ small helpers
 __global__ void k_b_copy(const CopySeg *__restrict__ segs, const int *__restrict__ blk_off, int ns) {
     const int s = seg_of(blk_off, ns, blockIdx.x);
@@ -555,9 +687,17 @@ hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, cons
     size_t tb = X.scan_bytes;
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(X.scan_temp, tb, X.hist, X.off, X.nhist, st);
     if (e != hipSuccess) return e;
-    LAUNCH(k_b_scatter, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.keys, X.rank, X.off, X.order,
-           X.Ys, X.Es);
+    LAUNCH(k_b_scatter, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.keys, X.rank, X.off, X.order);
+    LAUNCH(k_b_gather, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.order, X.Ys, X.Es);
     return hipGetLastError();
+}
+
+size_t select_temp_bytes(int items) {
+    size_t bytes = 0;
+    if (hipcub::DeviceSelect::Flagged(nullptr, bytes, hipcub::CountingInputIterator<int>(0), (const uint8_t *) nullptr,
+                                      (int *) nullptr, (int *) nullptr, items) != hipSuccess)
+        return 1u << 20;
+    return bytes;
 }
 
 hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySeg *segs, const int *blk_off,
@@ -565,10 +705,13 @@ hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySe
     if (nsub <= 0) return hipSuccess;
     const size_t nb = (size_t) nsub * n;
     LAUNCH(k_b_fill<uint8_t>, dim3(blocks_for(nb)), dim3(kBlock), 0, st, X.qflag, (uint8_t) 0, nb);
-    LAUNCH(k_b_fill<int>, dim3(blocks_for(nsub)), dim3(kBlock), 0, st, X.qcount, 0, (size_t) nsub);
     if (nblk > 0)
         LAUNCH(k_b_qmark, dim3(nblk), dim3(kBlock), 0, st, subs, segs, blk_off, nseg, X.keys, X.rank, X.off, X.qflag);
-    LAUNCH(k_b_qcompact, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.qflag, X.qlist, X.qcount);
+    size_t tb = X.select_bytes;
+    hipError_t e = hipcub::DeviceSelect::Flagged(X.select_temp, tb, hipcub::CountingInputIterator<int>(0), X.qflag,
+                                                 X.qlist, X.nsel, (int) nb, st);
+    if (e != hipSuccess) return e;
+    LAUNCH(k_b_qstart, dim3(blocks_for(nsub + 1)), dim3(kBlock), 0, st, X.qlist, X.nsel, nsub, n, X.qstart);
     return hipGetLastError();
 }
 
@@ -580,10 +723,10 @@ hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, in
     if (tm.start) (void) hipEventRecord(tm.start, st);
     if (blk_off)
         LAUNCH(k_b_count<true>, dim3(nblk), dim3(kBlock), count_lds(kmax), st, subs, blk_off, nsub, n, X.off, X.order,
-               X.Ys, X.Es, X.qlist, X.qcount, s_star, stat);
+               X.Ys, X.Es, X.qlist, X.qstart, s_star, X.nb, stat);
     else
         LAUNCH(k_b_count<false>, dim3(query_blocks_n(n), nsub), dim3(kBlock), count_lds(kmax), st, subs,
-               (const int *) nullptr, nsub, n, X.off, X.order, X.Ys, X.Es, X.qlist, X.qcount, s_star, stat);
+               (const int *) nullptr, nsub, n, X.off, X.order, X.Ys, X.Es, X.qlist, X.qstart, s_star, X.nb, stat);
     if (tm.stop) (void) hipEventRecord(tm.stop, st);
     return hipGetLastError();
 }
@@ -592,8 +735,7 @@ hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, 
                                hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
     LAUNCH(k_b_dense_init, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.order, X.dsorted, X.par);
-    LAUNCH(k_b_link_sample, dim3(query_blocks_n(n), nsub), dim3(kBlock), count_lds(kmax), st, subs, n, X.off,
-           X.order, X.Ys, X.Es, X.dsorted, s_star, X.par);
+    LAUNCH(k_b_link_sample, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.nb, X.par);
     LAUNCH(k_b_uf_compress, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.par);
     LAUNCH(k_b_giant, dim3(nsub), dim3(256), 0, st, subs, n, X.par, X.giant);
     LAUNCH(k_b_link_full, dim3(query_blocks_n(n), nsub), dim3(kBlock), count_lds(kmax), st, subs, n, X.off, X.order,
@@ -622,6 +764,32 @@ hipError_t launch_b_prune(const DevData &D, const PruneChild *C, int nc, const i
 hipError_t launch_b_copy(const CopySeg *segs, const int *blk_off, int nseg, int nblk, hipStream_t st) {
     if (nblk <= 0) return hipSuccess;
     LAUNCH(k_b_copy, dim3(nblk), dim3(kBlock), 0, st, segs, blk_off, nseg);
+    return hipGetLastError();
+}
+
+hipError_t launch_b_stamp(const CopySeg *segs, const int *blk_off, int nseg, int nblk, hipStream_t st) {
+    if (nblk <= 0) return hipSuccess;
+    LAUNCH(k_b_stamp, dim3(nblk), dim3(kBlock), 0, st, segs, blk_off, nseg);
+    return hipGetLastError();
+}
+
+hipError_t launch_rep_redundancy(const Entry *E, const int *list, int nlist, const int *tstamp, int tree, int nW,
+                                 float r, int *mx, unsigned *done, int *keepv, hipStream_t st) {
+    const int grid = std::max(1, std::min(nlist, 256));
+    LAUNCH(k_rep_redundancy, dim3(grid), dim3(kBlock), 0, st, E, list, nlist, tstamp, tree, nW, r, mx, done, keepv);
+    return hipGetLastError();
+}
+
+hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *keepv, const int *compS,
+                           const int *csizeS, int min_size, float r, int n, Entry *E, const int *list, int nlist,
+                           int eS, uint8_t *inresS, uint8_t *keepf, uint8_t *drop, uint8_t *fresh, hipStream_t st) {
+    LAUNCH(k_rep_join_mark, dim3(blocks_for(n)), dim3(kBlock), 0, st, tstamp, t_lo, t_hi, keepv, compS, csizeS,
+           min_size, n, keepf);
+    if (nlist > 0)
+        LAUNCH(k_rep_join_check, dim3(std::min(nlist, 256)), dim3(kBlock), 0, st, E, list, nlist, compS, keepf, csizeS,
+               r, drop);
+    LAUNCH(k_rep_join_apply, dim3(blocks_for(n)), dim3(kBlock), 0, st, keepf, drop, n, inresS, fresh, E, eS);
+    LAUNCH(k_rep_join_members, dim3(blocks_for(n)), dim3(kBlock), 0, st, compS, n, fresh, E, eS);
     return hipGetLastError();
 }
 

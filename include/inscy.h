@@ -68,6 +68,13 @@ typedef struct inscy_stats {
     double count_kernel_ms; /* device time of the neighbour-count kernel (HIP events on the run's stream) */
     int64_t count_launches; /* launches of that kernel */
     int64_t count_bytes;    /* its algorithmic bytes: per counted point 4k+4, per neighbour found 4+4k */
+    /* wall time of the phases of the run (seconds): growing the restricted
+     * trees (restrict, prunes, counts, dense components), the replay of
+     * pruneRedundancy and join (level-synchronous mode; inside the first
+     * phase in exact mode), and reading the result back */
+    double expand_seconds;
+    double replay_seconds;
+    double emit_seconds;
 } inscy_stats;
 
 typedef struct inscy_result inscy_result;
@@ -93,6 +100,12 @@ int32_t inscy_result_subspace(const inscy_result *res, int32_t i, int32_t *dims_
 
 /* Canonical labels of subspace i: n int32 owned by the result. */
 const int32_t *inscy_result_labels(const inscy_result *res, int32_t i);
+
+/* Labels of every subspace as one contiguous row-major [num_subspaces][n]
+ * array owned by the result (row i is inscy_result_labels(res, i)).  No
+ * counterpart in the reference, which returns one tensor per subspace; it lets
+ * a caller take the whole result without copying. */
+const int32_t *inscy_result_labels_all(const inscy_result *res);
 
 int64_t inscy_result_num_points(const inscy_result *res);
 
