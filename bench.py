@@ -7,14 +7,18 @@ synthetic data of the chosen workload, already resident in HBM; the timed
 region spans exactly K steps between device synchronisations.  The reported
 value is restricted (subspace, region) trees evaluated per second, the
 throughput unit of the reference's depth-first search; ms_per_step is the
-end-to-end time of one run.
+end-to-end time of one run (labels of every result subspace back in host
+memory).  The default path is the level-synchronous driver (`batched`, exact
+results); the depth-first driver (`exact`, one restricted tree at a time) is
+timed beside it outside the timed region and reported as depth_first_ms.
 
 Also reported:
-  roofline      the neighbour-count kernel (the dominant one): its algorithmic
-                bytes (per counted point 4k+4, per neighbour established 4+4k)
-                over its device time measured with HIP events on the engine's
-                stream during the timed steps, against 8 TB/s HBM; `traffic`
-                comes from the committed rocprofv3 counter summary.
+  roofline      the neighbour-count kernel (the dominant one; k_b_count in the
+                level-synchronous driver, k_grid_count in the depth-first one):
+                its algorithmic bytes (per counted point 4k+4, per neighbour
+                established 4+4k) over its device time measured with HIP events
+                on the engine's stream during the timed steps, against 8 TB/s
+                HBM; `traffic` comes from the committed rocprofv3 counter summary.
   cpu_baseline  the oracle (a single-threaded CPU restatement of the
                 reference's CPU INSCY) on the same data for a bounded time
                 budget, as trees per second.
@@ -41,7 +45,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["exact", "batched"], default="exact")
+    ap.add_argument("--mode", choices=["exact", "batched"], default="batched")
+    ap.add_argument("--depth-first-steps", type=int, default=2,
+                    help="untimed runs of the depth-first driver for the comparison (0: skip)")
     ap.add_argument("--config", type=int, default=1, help="workload index (1: n=25000 d=15)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -62,8 +68,8 @@ def golden_parity(cfg, res):
     return res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
 
 
-def traffic_from_profile(cfg):
-    path = os.path.join(ROOT, "profiles", f"k_grid_count_config{cfg}_traffic.json")
+def traffic_from_profile(kernel, cfg):
+    path = os.path.join(ROOT, "profiles", f"{kernel}_config{cfg}_traffic.json")
     if not os.path.exists(path):
         return None
     try:
@@ -117,19 +123,34 @@ def main():
     kbytes = sum(r.stats["count_bytes"] for r in runs)
     avg_launch_ms = kernel_ms / max(launches, 1)
     achieved = (kbytes / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    kernel = "k_b_count" if args.mode == "batched" else "k_grid_count"
     roofline = {
         "bound": "hbm",
-        "kernel": "k_grid_count",
+        "kernel": kernel,
         "achieved": round(achieved, 2),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 6),
-        "traffic": traffic_from_profile(args.config),
+        "traffic": traffic_from_profile(kernel, args.config),
         "algorithmic_bytes_per_launch": round(kbytes / max(launches, 1), 1),
         "avg_launch_us": round(avg_launch_ms * 1e3, 3),
         "launches_per_step": launches // max(args.steps, 1),
         "kernel_share_of_step": round(kernel_ms / (elapsed * 1e3), 4),
     }
+
+    other = None
+    if args.depth_first_steps > 0:
+        omode = _lib.MODE_EXACT if mode == _lib.MODE_BATCHED else _lib.MODE_BATCHED
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.depth_first_steps):
+            ores = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, w.entropy_order,
+                             mode=omode)
+        torch.cuda.synchronize()
+        other = (time.perf_counter() - t1) / args.depth_first_steps * 1e3
+        import numpy as np
+        same = ores.subspaces == res.subspaces and np.array_equal(ores.labels, res.labels)
+    ms_step = elapsed / args.steps * 1e3
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -153,7 +174,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "ms_per_step": round(ms_step, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -163,6 +184,11 @@ def main():
             "workload": f"{w.name}: float32 n={w.n} d={w.d}, eps={w.eps} c={w.cells} F={w.F} num_obj={w.num_obj} "
                         f"min_size={w.min_size} r={w.r} rectangular={w.rectangular}",
             "mode": args.mode,
+            "depth_first_ms" if args.mode == "batched" else "batched_ms": round(other, 3) if other else None,
+            "speedup_vs_depth_first": (round(other / ms_step, 3) if args.mode == "batched" else
+                                       round(ms_step / other, 3)) if other else None,
+            "modes_agree": same if other else None,
+            "phase_seconds": {k: round(runs[-1].stats[k + "_seconds"], 4) for k in ("expand", "replay", "emit")},
             "trees_per_step": trees // max(args.steps, 1),
             "subspaces": res.stats["subspaces"],
             "clusters": res.stats["clusters"],

@@ -94,6 +94,14 @@ inline float expden(int k, float eps, int n) {
     float v = (float) ((double) nc * std::pow((double) eps, (double) k));
     return (float) ((double) v / std::pow(1.0, (double) k));
 }
+// alpha(k), omega(k) (ClusteringCpu.cu:36-38,152-157): the phi thresholds of
+// the non-rectangular density, float results of double-promoted pow()
+inline float alpha_thr(int k, float eps, int n) {
+    float v = (float) ((double) (2 * n) * std::pow((double) eps, (double) k) * (double) ball(k));
+    return (float) ((double) v / (std::pow(1.0, (double) k) * (double) (k + 2)));
+}
+inline float omega_thr(int k) { return (float) (2.0 / (k + 2.0)); }
+
 // Smallest integer count c with (float)c >= t (counts are compared as floats
 // in the reference, ScyTreeNode.cu:616 and ClusteringCpu.cu:176-183).
 inline int count_threshold(float t) {
@@ -247,7 +255,7 @@ public:
             (void) hipEventDestroy(t.stop);
         }
         for (void *p : {(void *) Xt, (void *) cellC, (void *) cellE, (void *) nearm, (void *) small, (void *) stamp,
-                        (void *) qmark, (void *) entries, (void *) scratch})
+                        (void *) qmark, (void *) entries, (void *) scratch, (void *) rank_perm})
             if (p) (void) hipFree(p);
         if (pin) (void) hipHostFree(pin);
     }
@@ -262,6 +270,13 @@ public:
         for (int k = 0; k <= d; ++k)
             dense_thr.push_back(
                 count_threshold(std::max(prm.F * expden(k, prm.neighborhood_size, n), (float) prm.num_obj)));
+        // rectangular = false: phi against max(F alpha, num_obj omega), weak
+        // with all d dimensions, dense with |S| (ClusteringCpu.cu:166-176)
+        phi_mode = prm.rectangular == 0;
+        phi_weak = std::max(prm.F * alpha_thr(d, prm.neighborhood_size, n), (float) prm.num_obj * omega_thr(d));
+        for (int k = 0; k <= d; ++k)
+            phi_dense.push_back(
+                std::max(prm.F * alpha_thr(k, prm.neighborhood_size, n), (float) prm.num_obj * omega_thr(k)));
 
         size_t nd = (size_t) n * d;
         HIP_OK(hipMalloc(&Xt, nd * sizeof(float)));
@@ -295,6 +310,18 @@ public:
         HIP_OK(launch_discretise(d_X, n, d, cells, prm.neighborhood_size, Xt, cellC, cellE, nearm,
                                  reinterpret_cast<int *>(small), st));
         HIP_OK(launch_hist1(cellE, n, d, ecells, hist, st));
+        if (phi_mode) {
+            // leaf order of the neighbourhood tree (levels in dimension order)
+            HIP_OK(hipMalloc(&rank_perm, (size_t) n * sizeof(int)));
+            std::vector<int> level(d);
+            for (int k = 0; k < d; ++k) level[k] = k;
+            const size_t tb = rank_order_temp_bytes(n);
+            int *tmp = arena.take<int>(n);
+            uint16_t *k1 = arena.take<uint16_t>(n);
+            uint16_t *k2 = arena.take<uint16_t>(n);
+            void *temp = arena.take<char>(tb);
+            HIP_OK(launch_rank_order(cellE, n, d, level.data(), rank_perm, tmp, k1, k2, temp, tb, st));
+        }
         HIP_OK(launch_fill_i32(stamp, -1, n, st));
         HIP_OK(launch_fill_i32(qmark, -1, n, st));
         std::vector<int> h((size_t) d * ecells);
@@ -841,6 +868,10 @@ private:
     HostBuf emit_stage[2];
     size_t up_top = 0;
     size_t batch_budget = 0;
+    bool phi_mode = false;          // rectangular = false
+    float phi_weak = 0.f;
+    std::vector<float> phi_dense;   // [k]
+    int *rank_perm = nullptr;       // [n] points in the neighbourhood tree's leaf order
     bool batch_trace = std::getenv("INSCY_TRACE_BATCHES") != nullptr;  // one stderr line per batch
     std::vector<int> lv_act, lv_next, lv_gidoff;
     std::vector<std::vector<std::vector<Group>>> lv_groups;

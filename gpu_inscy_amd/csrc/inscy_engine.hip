@@ -22,7 +22,6 @@ static int check_params(int64_t n, int32_t d, const inscy_params *p, int32_t mod
     if (p->number_of_cells < 1 || p->number_of_cells > 255)
         return fail(INSCY_ERR_ARGUMENT, "number_of_cells must be in [1, 255]");
     if (p->entropy_order != 0) return fail(INSCY_ERR_UNSUPPORTED, "entropy_order != 0 is not implemented");
-    if (p->rectangular == 0) return fail(INSCY_ERR_UNSUPPORTED, "rectangular=False (phi density) is not implemented");
     if (mode != INSCY_MODE_EXACT && mode != INSCY_MODE_BATCHED) return fail(INSCY_ERR_ARGUMENT, "unknown mode");
     return INSCY_OK;
 }
@@ -42,7 +41,9 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
             Engine eng(*p, (int) n, d, st);
             eng.trace_on = g_trace_on;
             eng.profile_on = g_profile_on;
-            eng.levels_mode = mode == INSCY_MODE_BATCHED;
+            // phi density (rectangular = false) is implemented in the
+            // level-synchronous driver only; it serves both modes
+            eng.levels_mode = mode == INSCY_MODE_BATCHED || p->rectangular == 0;
             eng.setup(d_X);
             if (eng.levels_mode)
                 eng.run_levels(*res);

@@ -92,6 +92,16 @@ struct BatchSub {
     int *comp;          // [n] component roots (components pass)
     int *csize;         // [n] component sizes (components pass)
     int nbound, ncell, cap, lo, thr_dense, qmax;
+    // cell test of the candidates: the first (up to 4) checked dimensions of
+    // S, packed 16 bits each into one word per sorted position; the others
+    // (restmask, positions in S) are tested one by one
+    int nchk4;
+    int chk[4];
+    uint64_t restmask;
+    // phi mode (rectangular = false): dbit = 2 (dense = bit 1 of the stored
+    // decision bits), thresholds of the weak and the dense test; count mode: dbit = 0
+    int dbit;
+    float phi_w, phi_d;
 };
 
 // Device buffers of one batch's grid indexes.
@@ -108,6 +118,10 @@ struct BatchIndex {
     void *select_temp;
     size_t select_bytes;
     int *nb;              // [B*n*2] first two neighbours found by the count passes
+    uint64_t *sig;        // [B*n] packed cells of the checked dimensions per sorted position
+    int2 *fb;             // phi mode: (subspace, point) pairs to recompute in leaf order
+    int *nfb;             // [1] their number
+    int fbcap;
     uint8_t *dsorted;     // [B*n] dense flag per sorted position
     int *par;             // [B*n] union-find parents (point ids within each subspace)
     int *giant;           // [B]
@@ -228,14 +242,22 @@ hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySe
                             int nseg, int nblk, const BatchIndex &X, hipStream_t st);
 // neighbour counts of the listed queries (blk_off over subspaces), or of every
 // uncounted point when blk_off is null (grid.y = subspace)
+// phi: rectangular = false, the stored values are decision bits (1 weak-dense,
+// 2 dense); points near a threshold are recomputed in leaf order (rank_order)
 hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, int nblk, int n, int kmax,
-                          const BatchIndex &X, float s_star, unsigned long long *stat, const KernelTimer &tm,
-                          hipStream_t st);
+                          const BatchIndex &X, float s_star, float eps, bool phi, const DevData &D,
+                          const int *rank_order, unsigned long long *stat, const KernelTimer &tm, hipStream_t st);
 // dense components of nsub subspaces whose counts are complete
 hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, const BatchIndex &X, float s_star,
                                hipStream_t st);
+// weak test: count >= thr, or bit wbit of the phi decision bits when wbit != 0
 hipError_t launch_b_partition(const PartPair *pairs, const int *blk_off, int npairs, int nblk, const uint8_t *cellC,
-                              int n, int thr, hipStream_t st);
+                              int n, int thr, int wbit, hipStream_t st);
+// points sorted by their eps cells over all dimensions in `level` order, ties
+// by id: the leaf order of the reference's neighbourhood tree (d stable passes)
+hipError_t launch_rank_order(const uint16_t *cellE, int n, int d, const int *level, int *perm, int *tmp,
+                             uint16_t *keys, uint16_t *keys2, void *temp, size_t temp_bytes, hipStream_t st);
+size_t rank_order_temp_bytes(int n);
 // path prune of many trees (blk_w over their points, blk_p over their paths)
 hipError_t launch_b_prune(const DevData &D, const PruneChild *C, int nc, const int *blk_w, int nblk_w,
                           const int *blk_p, int nblk_p, hipStream_t st);
@@ -245,12 +267,13 @@ hipError_t launch_b_entries(Entry *E, int at, const Entry *src, int count, int *
 hipError_t launch_b_stamp(const CopySeg *segs, const int *blk_off, int nseg, int nblk, hipStream_t st);
 // replay: pruneRedundancy of one tree (keepv[tree] = decision; mx, done: two
 // zeroed words reset by the launch), join of one (parent, dimension) pair
-hipError_t launch_rep_redundancy(const Entry *E, const int *list, int nlist, const int *tstamp, int tree, int nW,
-                                 float r, int *mx, unsigned *done, int *keepv, hipStream_t st);
+hipError_t launch_rep_redundancy(const Entry *E, const int *list, int nlist, const int *tstamp, const int *W,
+                                 int tree, int nW, float r, int *mx, unsigned *done, int *keepv, hipStream_t st);
 hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *keepv, const int *compS,
                            const int *csizeS, int min_size, float r, int n, Entry *E, const int *list, int nlist,
                            int eS, uint8_t *inresS, uint8_t *keepf, uint8_t *drop, uint8_t *fresh, hipStream_t st);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t count, hipStream_t st);
+constexpr int kRangeWords = 20;      // per query in LDS: 9 range starts, 10 prefix sums, count
 constexpr int kBlockThreads = 256;   // threads per block of every pass
 constexpr int kQueriesPerBlock = 16;  // queries per block of the neighbour passes
 

@@ -657,6 +657,12 @@ __global__ void k_iota(int *__restrict__ p, int n) {
     if (i < n) p[i] = i;
 }
 
+__global__ void k_rank_keys(const uint16_t *__restrict__ cellE, int n, int dim, const int *__restrict__ perm,
+                            uint16_t *__restrict__ keys) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = cellE[(size_t) dim * n + perm[i]];
+}
+
 // Root s-connection paths: every point with at least one near-edge dimension.
 __global__ void k_root_paths(const uint64_t *__restrict__ nearm, int n, int *__restrict__ outx,
                              uint64_t *__restrict__ outm, int *__restrict__ nout) {
@@ -884,6 +890,37 @@ hipError_t launch_gather(const int *src, const int *idx, int count, int *out, hi
 hipError_t launch_iota(int *p, int n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     LAUNCH(k_iota, dim3(blocks_for(n)), dim3(kBlock), 0, st, p, n);
+    return hipGetLastError();
+}
+
+size_t rank_order_temp_bytes(int n) {
+    size_t bytes = 0;
+    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint16_t *) nullptr, (uint16_t *) nullptr,
+                                           (const int *) nullptr, (int *) nullptr, n, 0, 16) != hipSuccess)
+        return 1u << 20;
+    return bytes;
+}
+
+// Least-significant dimension first, one stable radix pass per dimension,
+// starting from ascending ids: the result is sorted by the cells in `level`
+// order with ties by id.
+hipError_t launch_rank_order(const uint16_t *cellE, int n, int d, const int *level, int *perm, int *tmp,
+                             uint16_t *keys, uint16_t *keys2, void *temp, size_t temp_bytes, hipStream_t st) {
+    LAUNCH(k_iota, dim3(blocks_for(n)), dim3(kBlock), 0, st, perm, n);
+    int *a = perm, *b = tmp;
+    for (int t = d - 1; t >= 0; --t) {
+        LAUNCH(k_rank_keys, dim3(blocks_for(n)), dim3(kBlock), 0, st, cellE, n, level[t], a, keys);
+        size_t tb = temp_bytes;
+        hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, tb, keys, keys2, a, b, n, 0, 16, st);
+        if (e != hipSuccess) return e;
+        int *x = a;
+        a = b;
+        b = x;
+    }
+    if (a != perm) {
+        hipError_t e = hipMemcpyAsync(perm, a, (size_t) n * sizeof(int), hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return e;
+    }
     return hipGetLastError();
 }
 

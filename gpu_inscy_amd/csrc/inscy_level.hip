@@ -275,6 +275,18 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         x.thr_dense = dense_thr[x.S.k];
         x.qmax = (int) qmax;
         set_bounds(c.s, x);
+        x.dbit = phi_mode ? 2 : 0;
+        x.phi_w = phi_weak;
+        x.phi_d = phi_dense[x.S.k];
+        x.nchk4 = 0;
+        x.restmask = 0;
+        for (int t = 0; t < x.S.k; ++t) {
+            if (!((x.G.checkmask >> t) & 1ull)) continue;
+            if (x.nchk4 < 4)
+                x.chk[x.nchk4++] = t;
+            else
+                x.restmask |= 1ull << t;
+        }
         coord += (long long) x.S.k * n;
         hoff += ncell;
         qoff += qmax;
@@ -296,6 +308,11 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.select_bytes = select_temp_bytes(B * n);
     X.select_temp = batch.take<char>(X.select_bytes);
     X.nb = batch.take<int>((size_t) 2 * B * n);
+    X.sig = batch.take<uint64_t>((size_t) B * n);
+    X.nfb = batch.take<int>(1);
+    X.fbcap = phi_mode ? B * n : 1;  // each point of each subspace is sent at most once per pass
+    X.fb = batch.take<int2>(X.fbcap);
+    HIP_OK(launch_fill_i32(X.nb, -1, (size_t) 2 * B * n, st));
     X.dsorted = batch.take<uint8_t>((size_t) B * n);
     X.par = batch.take<int>((size_t) B * n);
     X.giant = batch.take<int>(B);
@@ -329,7 +346,8 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         sync();
         tq0 = std::chrono::steady_clock::now();
     }
-    HIP_OK(launch_b_count(dsubs, B, dcb, cblk[B], n, kmax, X, s_star, small + kStatAt, timer(), st));
+    HIP_OK(launch_b_count(dsubs, B, dcb, cblk[B], n, kmax, X, s_star, prm.neighborhood_size, phi_mode, D, rank_perm,
+                          small + kStatAt, timer(), st));
     if (dbg) {
         sync();
         tq1 = std::chrono::steady_clock::now();
@@ -379,7 +397,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     }
     PartPair *dpairs = upload(batch, pairs);
     int *dpb = upload(batch, pblk);
-    HIP_OK(launch_b_partition(dpairs, dpb, np, pblk[np], cellC, n, weak_thr, st));
+    HIP_OK(launch_b_partition(dpairs, dpb, np, pblk[np], cellC, n, weak_thr, phi_mode ? 1 : 0, st));
     char *h = down.get((size_t) std::max(pc_total, 1) * sizeof(int));
     HIP_OK(hipMemcpyAsync(h, pcnt, (size_t) std::max(pc_total, 1) * sizeof(int), hipMemcpyDeviceToHost, st));
     sync();
@@ -539,7 +557,8 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
             sync();
             tc0 = std::chrono::steady_clock::now();
         }
-        HIP_OK(launch_b_count(dhc, ncomp, nullptr, 0, n, kmax, X, s_star, small + kStatAt, timer(), st));
+        HIP_OK(launch_b_count(dhc, ncomp, nullptr, 0, n, kmax, X, s_star, prm.neighborhood_size, phi_mode, D,
+                              rank_perm, small + kStatAt, timer(), st));
         if (dbg) {
             sync();
             tc1 = std::chrono::steady_clock::now();
@@ -579,6 +598,8 @@ void Engine::set_bounds(uint64_t s, BatchSub &x) {
     const int k = x.S.k;
     int lo = std::min(weak_thr, dense_thr[k]);
     x.nbound = 0;
+    x.lo = lo;
+    if (phi_mode) return;  // decision bits are no counts
     for (int t = 0; t < k && x.nbound < kMaxBounds; ++t) {
         const uint64_t sub = s & ~(1ull << x.S.dims[t]);
         auto it = subs.find(sub);
@@ -614,7 +635,7 @@ void Engine::replay_tree(int f) {
                 const int c = kids[u];
                 if (!lt[c].pass) continue;
                 replay_tree(c);
-                HIP_OK(launch_rep_redundancy(entries, list, sl.second, sub.tstamp, c, lt[c].nW, prm.r, rep_words,
+                HIP_OK(launch_rep_redundancy(entries, list, sl.second, sub.tstamp, lt[c].W, c, lt[c].nW, prm.r, rep_words,
                                              reinterpret_cast<unsigned *>(rep_words + 1), keepv, st));
             }
             HIP_OK(launch_rep_join(sub.tstamp, t_lo, t_hi, keepv, sub.comp, sub.csize, prm.min_size, prm.r, n, entries,

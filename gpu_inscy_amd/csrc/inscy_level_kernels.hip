@@ -85,7 +85,7 @@ __global__ void k_b_scatter(DevData D, const BatchSub *__restrict__ subs, const 
 }
 
 __global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ order,
-                           float *__restrict__ Ys, uint16_t *__restrict__ Es) {
+                           float *__restrict__ Ys, uint16_t *__restrict__ Es, uint64_t *__restrict__ sig) {
     const BatchSub &B = subs[blockIdx.y];
     const int n = D.n;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -96,6 +96,9 @@ __global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const i
         Ys[B.coord + (long long) t * n + i] = D.Xt[(size_t) s * n + p];
         Es[B.coord + (long long) t * n + i] = D.cellE[(size_t) s * n + p];
     }
+    uint64_t w = 0;
+    for (int f = 0; f < B.nchk4; ++f) w |= (uint64_t) D.cellE[(size_t) B.S.dims[B.chk[f]] * n + p] << (16 * f);
+    sig[B.base + i] = w;
 }
 
 // ------------------------------------------------------------ query lists in sorted order
@@ -134,18 +137,113 @@ __global__ void k_b_qstart(const int *__restrict__ qlist, const int *__restrict_
     qstart[b] = lo;
 }
 
+// ------------------------------------------------------------ candidate walk
+// The candidates of a query are up to 9 runs of sorted positions (grid cells
+// of the index dimensions).  They are concatenated into one index space
+// [0, total): q[0..8] run starts (local positions), q[9..18] prefix sums of
+// the run lengths, q[19] the number of runs.  A lane maps its candidate
+// number to a position with a run pointer that only moves forward.
+__device__ __forceinline__ void build_runs(const GridArg &G, const int *__restrict__ off, const int *qe, int stride,
+                                           int base0, int *q) {
+    int nr = 0, acc = 0;
+    q[9] = 0;
+    for_ranges(G, off, qe, stride, [&](int a, int e) {
+        if (e > a) {
+            q[nr] = a - base0;
+            acc += e - a;
+            q[10 + nr] = acc;
+            ++nr;
+        }
+    });
+    q[19] = nr;
+}
+
+__device__ __forceinline__ int run_pos(const int *q, int c, int &r) {
+    while (c >= q[10 + r]) ++r;
+    return q[r] + (c - q[9 + r]);
+}
+
+constexpr int kFastDims = 8;
+
+// the packed cells of the checked dimensions lie within one cell of the query's
+__device__ __forceinline__ bool cells_near(uint64_t w, uint64_t qsig, int nc) {
+    bool ok = true;
+    for (int f = 0; f < nc; ++f) {
+        const int dc = (int) ((w >> (16 * f)) & 0xFFFFull) - (int) ((qsig >> (16 * f)) & 0xFFFFull);
+        ok = ok && dc >= -1 && dc <= 1;
+    }
+    return ok;
+}
+
+// Candidate at local sorted position jj against the staged query: cells of the
+// checked dimensions within one (packed word first), then the distance summed
+// in subspace order with separate multiplies and adds, compared with s_star.
+__device__ __forceinline__ bool cand_test(const BatchSub &B, int n, const float *__restrict__ Yb,
+                                          const uint16_t *__restrict__ Eb, const uint64_t *__restrict__ sigb,
+                                          uint64_t qsig, const float *qy, const int *qe, int stride, int jj,
+                                          float s_star, float &acc_out) {
+    const int nc = B.nchk4;
+    if (nc > 0 && !cells_near(sigb[jj], qsig, nc)) return false;
+    const uint64_t rest = B.restmask;
+    const int k = B.S.k;
+    float acc = 0.0f;
+    for (int t = 0; t < k; ++t) {
+        if ((rest >> t) & 1ull) {
+            const int de = (int) Eb[(size_t) t * n + jj] - qe[t * stride];
+            if (de < -1 || de > 1) return false;
+        }
+        const float diff = qy[t * stride] - Yb[(size_t) t * n + jj];
+        const float sq = diff * diff;
+        acc = acc + sq;
+        if (acc > s_star) return false;
+    }
+    acc_out = acc;
+    return true;
+}
+
+// term of phi (ClusteringCpu.cu:116-124) for a neighbour at squared distance
+// acc: d = sqrtf(acc) / eps in float, then 1 - d*d in double
+__device__ __forceinline__ double phi_term(float acc, float eps) {
+    const float dist = __fsqrt_rn(acc);
+    const float r = __fdiv_rn(dist, eps);
+    const float sq = __fmul_rn(r, r);
+    return 1.0 - (double) sq;
+}
+
+// The reference sums phi in float in the order of the neighbourhood tree's
+// leaves.  The passes here sum the same terms in double in scan order: with N
+// terms the float sum in leaf order lies within a relative N * 2^-22 of it
+// (each float rounding moves the running sum by at most 2^-24 of the final
+// value), so a threshold outside that band is decided exactly; points inside
+// it are recomputed in leaf order (k_b_phi_exact).
+constexpr double kPhiRel = 1.0 / 4194304.0;
+
+// density tests on a stored value: counts (bit == 0) or phi decision bits
+__device__ __forceinline__ bool meets(int v, int thr, int bit) { return bit ? (v >= 0 && (v & bit) != 0) : v >= thr; }
+
+// LDS of the neighbour passes: per query k coordinates, k cells, its packed
+// cells (2 words) and its runs
+__device__ __forceinline__ void lds_layout(char *smem, int k, float *&sy, int *&se, uint64_t *&qs, int *&qr) {
+    sy = reinterpret_cast<float *>(smem);
+    se = reinterpret_cast<int *>(smem + sizeof(float) * QPB * k);
+    qs = reinterpret_cast<uint64_t *>(smem + 8 * (size_t) QPB * k);
+    qr = reinterpret_cast<int *>(smem + 8 * (size_t) QPB * k + 8 * QPB);
+}
+
 // ------------------------------------------------------------ neighbour counts
 // k_grid_count over a batch: LIST = the compacted queries of each subspace
 // (blocks by segment), else every sorted position of subspace blockIdx.y whose
 // count is still missing (components pass, with the subset bound).
-template <bool LIST>
+template <bool LIST, bool PHI>
 __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__ subs, const int *__restrict__ blk_off,
                                                     int nsub, int n, const int *__restrict__ off,
                                                     const int *__restrict__ order, const float *__restrict__ Ys,
-                                                    const uint16_t *__restrict__ Es, const int *__restrict__ qlist,
-                                                    const int *__restrict__ qstart, float s_star,
-                                                    int *__restrict__ nb, unsigned long long *__restrict__ stat) {
-    constexpr int LPQ = kLanesPerQuery;
+                                                    const uint16_t *__restrict__ Es, const uint64_t *__restrict__ sig,
+                                                    const int *__restrict__ qlist, const int *__restrict__ qstart,
+                                                    float s_star, float eps, int *__restrict__ nb,
+                                                    int2 *__restrict__ fb, int *__restrict__ nfb, int fbcap,
+                                                    unsigned long long *__restrict__ stat) {
+    constexpr int LPQ = kLanesPerQuery, U = 4, STEP = LPQ * U;
     extern __shared__ char smem[];
     int b, blk;
     if (LIST) {
@@ -157,8 +255,10 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
     }
     const BatchSub &B = subs[b];
     const int k = B.S.k;
-    float *sy = reinterpret_cast<float *>(smem);
-    int *se = reinterpret_cast<int *>(smem + sizeof(float) * QPB * k);
+    float *sy;
+    int *se, *qr;
+    uint64_t *qs;
+    lds_layout(smem, k, sy, se, qs, qr);
     const int lq = threadIdx.x / LPQ, sub = threadIdx.x % LPQ;
     int i = 0;
     bool active;
@@ -172,6 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
     }
     const float *Yb = Ys + B.coord;
     const uint16_t *Eb = Es + B.coord;
+    const uint64_t *sigb = sig + B.base;
     int *cnt = B.cnt;
     const int p = active ? order[B.base + i] : 0;
     active = active && cnt[p] < 0;
@@ -189,35 +290,150 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
             active = false;
         }
     }
-    if (sub == 0 && active) stage_query(B.S, n, Yb, Eb, i, sy + lq, se + lq, QPB);
+    const int base0 = (int) B.base;
+    int *q = qr + lq * kRangeWords;
+    if (sub == 0 && active) {
+        stage_query(B.S, n, Yb, Eb, i, sy + lq, se + lq, QPB);
+        qs[lq] = sigb[i];
+        build_runs(B.G, off + B.hoff, se + lq, QPB, base0, q);
+    }
     __syncthreads();
     long long seen = 0;
     int found = 0;
     if (active) {
         const float *qy = sy + lq;
         const int *qe = se + lq;
+        const uint64_t qsig = qs[lq];
         const int gshift = (__lane_id() / LPQ) * LPQ;
         const unsigned long long below = (1ull << sub) - 1ull;
         const int cap = B.cap;
-        const int base0 = (int) B.base;
+        const int total = q[9 + q[19]];
+        // the walk starts at a pseudo-random candidate of the query and wraps:
+        // the first two neighbours found are then spread over the candidates
+        // (random sample edges for the components pass, as Afforest draws
+        // them) instead of the same few points at the start of a shared run
+        const int rot = total > 0 ? (int) (((uint32_t) p * 2654435761u) % (uint32_t) total) : 0;
+        bool wrapped = false;
         int *nbp = nb + 2 * (B.base + p);
-        for_ranges(B.G, off + B.hoff, qe, QPB, [&](int a, int e) {
-            a -= base0;
-            e -= base0;
-            for (int base = a; base < e && found < cap; base += LPQ) {
-                const int jj = base + sub;
-                const bool hit = jj < e && jj != i && is_neighbour(B.S, B.G, n, Yb, Eb, qy, qe, QPB, jj, s_star);
-                const unsigned long long mine = (__ballot(hit) >> gshift) & ((1ull << LPQ) - 1ull);
+        // k <= 8 with every checked dimension packed: all loads of a step are
+        // issued before any is used; the full sum compared once equals the
+        // early-exit test (the partial sums never decrease)
+        const bool fast = k <= kFastDims && B.restmask == 0;
+        const int nc = B.nchk4;
+        float qv[kFastDims];
+#pragma unroll
+        for (int t = 0; t < kFastDims; ++t) qv[t] = t < k ? qy[t * QPB] : 0.0f;
+        int r = 0;
+        // phi: running double sum of the terms (the same on all lanes of the
+        // query); it may stop once it is certainly above both thresholds
+        double dsum = 0.0;
+        const double thr_max = PHI ? (double) fmaxf(B.phi_w, B.phi_d) : 0.0;
+        bool more = true;
+        for (int c0 = 0; c0 < total && more; c0 += STEP) {
+            int jj[U];
+            bool hit[U];
+            float accs[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int c = c0 + u * LPQ + sub;
+                jj[u] = -1;
+                if (c < total) {
+                    int cr = c + rot;
+                    if (cr >= total) {
+                        cr -= total;
+                        if (!wrapped) {
+                            wrapped = true;
+                            r = 0;
+                        }
+                    }
+                    jj[u] = run_pos(q, cr, r);
+                }
+            }
+            if (fast) {
+                uint64_t w[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) w[u] = (jj[u] >= 0 && nc > 0) ? sigb[jj[u]] : 0ull;
+                bool ok[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) ok[u] = jj[u] >= 0 && jj[u] != i && cells_near(w[u], qsig, nc);
+                float y[U][kFastDims];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+#pragma unroll
+                    for (int t = 0; t < kFastDims; ++t)
+                        y[u][t] = (ok[u] && t < k) ? Yb[(size_t) t * n + jj[u]] : 0.0f;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    float acc = 0.0f;
+#pragma unroll
+                    for (int t = 0; t < kFastDims; ++t) {
+                        if (t < k) {
+                            const float diff = qv[t] - y[u][t];
+                            const float sq = diff * diff;
+                            acc = acc + sq;
+                        }
+                    }
+                    hit[u] = ok[u] && acc <= s_star;
+                    accs[u] = acc;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    accs[u] = 0.0f;
+                    hit[u] = jj[u] >= 0 && jj[u] != i &&
+                             cand_test(B, n, Yb, Eb, sigb, qsig, qy, qe, QPB, jj[u], s_star, accs[u]);
+                }
+            }
+            if (PHI) {
+                double part = 0.0;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (hit[u]) part += phi_term(accs[u], eps);
+                for (int o = LPQ / 2; o > 0; o >>= 1) part += __shfl_xor(part, o);
+                dsum += part;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned long long mine = (__ballot(hit[u]) >> gshift) & ((1ull << LPQ) - 1ull);
                 // the first two neighbours found: the sample edges of the components pass
-                if (hit && found < 2) {
+                if (hit[u] && found < 2) {
                     const int at = found + __popcll(mine & below);
-                    if (at < 2) nbp[at] = order[B.base + jj];
+                    if (at < 2) nbp[at] = order[B.base + jj[u]];
                 }
                 found += __popcll(mine);
-                seen += min(LPQ, e - base);
             }
-        });
-        if (sub == 0) cnt[p] = found;
+            seen += min(STEP, total - c0);
+            if (PHI)
+                more = dsum * (1.0 - (double) total * kPhiRel) < thr_max;
+            else
+                more = found < cap;
+        }
+        if (PHI) {
+            // decision bits: 1 weak-dense, 2 dense; a threshold inside the
+            // rounding band sends the point to the leaf-order recomputation
+            const double lo = dsum * (1.0 - (double) found * kPhiRel);
+            const double hi = dsum * (1.0 + (double) found * kPhiRel);
+            int code = 0;
+            bool unsure = false;
+            if (!more) {
+                code = 3;
+            } else {
+                const double tw = (double) B.phi_w, td = (double) B.phi_d;
+                if (lo >= tw) code |= 1;
+                else if (!(hi < tw)) unsure = true;
+                if (lo >= td) code |= 2;
+                else if (!(hi < td)) unsure = true;
+            }
+            if (sub == 0) {
+                cnt[p] = code;
+                if (unsure) {
+                    const int at = atomicAdd(nfb, 1);
+                    if (at < fbcap) fb[at] = make_int2(b, p);
+                }
+            }
+        } else if (sub == 0) {
+            cnt[p] = found;
+        }
     }
     if (stat) {
         // per wave into one of 64 slot groups: [0] candidates examined, [1]
@@ -249,7 +465,7 @@ __global__ void k_b_dense_init(const BatchSub *__restrict__ subs, int n, const i
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int p = order[B.base + i];
-    const bool dense = B.cnt[p] >= B.thr_dense;
+    const bool dense = meets(B.cnt[p], B.thr_dense, B.dbit);
     dsorted[B.base + i] = dense ? 1 : 0;
     par[B.base + p] = dense ? p : -1;
     B.csize[p] = 0;
@@ -259,21 +475,20 @@ __global__ void k_b_dense_init(const BatchSub *__restrict__ subs, int n, const i
 // among the first two neighbours its count pass recorded (any edges would do:
 // the full phase below links every point outside the giant component to all
 // its dense neighbours, so the result does not depend on the sample).
-__global__ void k_b_link_sample(const BatchSub *__restrict__ subs, int n, const int *__restrict__ nb,
+// One round per recorded neighbour, with the roots compressed in between, so
+// that the finds of the second round walk short paths (Afforest's rounds).
+__global__ void k_b_link_sample(const BatchSub *__restrict__ subs, int n, const int *__restrict__ nb, int round,
                                 int *__restrict__ par) {
     const BatchSub &B = subs[blockIdx.y];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     const int *cnt = B.cnt;
     const int c = cnt[p];
-    if (c < B.thr_dense) return;
-    int *pr = par + B.base;
-    const int *nbp = nb + 2 * (B.base + p);
-    const int k = min(c, 2);
-    for (int t = 0; t < k; ++t) {
-        const int q = nbp[t];
-        if (q >= 0 && q < n && cnt[q] >= B.thr_dense) uf_union(pr, p, q);
-    }
+    if (!meets(c, B.thr_dense, B.dbit)) return;
+    // the recorded neighbours: min(count, 2) of them (phi mode records them too)
+    const int q = nb[2 * (B.base + p) + round];
+    if (B.dbit == 0 && c <= round) return;
+    if (q >= 0 && q < n && meets(cnt[q], B.thr_dense, B.dbit)) uf_union(par + B.base, p, q);
 }
 
 __global__ void k_b_uf_compress(const BatchSub *__restrict__ subs, int n, int *__restrict__ par) {
@@ -314,42 +529,51 @@ __global__ void k_b_giant(const BatchSub *__restrict__ subs, int n, const int *_
 __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restrict__ subs, int n,
                                                         const int *__restrict__ off, const int *__restrict__ order,
                                                         const float *__restrict__ Ys, const uint16_t *__restrict__ Es,
+                                                        const uint64_t *__restrict__ sig,
                                                         const uint8_t *__restrict__ dsorted, float s_star,
                                                         const int *__restrict__ giant, int *__restrict__ par) {
     constexpr int LPQ = kLanesPerQuery;
     extern __shared__ char smem[];
     const BatchSub &B = subs[blockIdx.y];
     const int k = B.S.k;
-    float *sy = reinterpret_cast<float *>(smem);
-    int *se = reinterpret_cast<int *>(smem + sizeof(float) * QPB * k);
+    float *sy;
+    int *se, *qr;
+    uint64_t *qs;
+    lds_layout(smem, k, sy, se, qs, qr);
     const int lq = threadIdx.x / LPQ, sub = threadIdx.x % LPQ;
     const int i = blockIdx.x * QPB + lq;
     const uint8_t *ds = dsorted + B.base;
     const int *ord = order + B.base;
+    const uint64_t *sigb = sig + B.base;
     int *pr = par + B.base;
     const float *Yb = Ys + B.coord;
     const uint16_t *Eb = Es + B.coord;
     bool active = i < n && ds[i];
     const int p = active ? ord[i] : 0;
     if (active && ld_relaxed(&pr[p]) == giant[blockIdx.y]) active = false;
-    if (sub == 0 && active) stage_query(B.S, n, Yb, Eb, i, sy + lq, se + lq, QPB);
+    int *q = qr + lq * kRangeWords;
+    if (sub == 0 && active) {
+        stage_query(B.S, n, Yb, Eb, i, sy + lq, se + lq, QPB);
+        qs[lq] = sigb[i];
+        build_runs(B.G, off + B.hoff, se + lq, QPB, (int) B.base, q);
+    }
     __syncthreads();
     if (!active) return;
     const float *qy = sy + lq;
     const int *qe = se + lq;
-    const int base0 = (int) B.base;
+    const uint64_t qsig = qs[lq];
+    const int total = q[9 + q[19]];
     int rp = uf_find(pr, p);
-    for_ranges(B.G, off + B.hoff, qe, QPB, [&](int a, int e) {
-        a -= base0;
-        e -= base0;
-        for (int jj = a + sub; jj < e; jj += LPQ) {
-            if (jj == i || !ds[jj] || !is_neighbour(B.S, B.G, n, Yb, Eb, qy, qe, QPB, jj, s_star)) continue;
-            const int q = ord[jj];
-            if (uf_find(pr, q) == rp) continue;
-            uf_union(pr, p, q);
-            rp = uf_find(pr, p);
-        }
-    });
+    int r = 0;
+    for (int c = sub; c < total; c += LPQ) {
+        const int jj = run_pos(q, c, r);
+        float acc;
+        if (jj == i || !ds[jj] || !cand_test(B, n, Yb, Eb, sigb, qsig, qy, qe, QPB, jj, s_star, acc)) continue;
+        const int qq = ord[jj];
+        if (uf_find(pr, qq) == rp) continue;
+        uf_union(pr, p, qq);
+        rp = uf_find(pr, p);
+    }
 }
 
 __global__ void k_b_uf_flatten(const BatchSub *__restrict__ subs, int n, int *__restrict__ par) {
@@ -367,7 +591,7 @@ __global__ void k_b_uf_flatten(const BatchSub *__restrict__ subs, int n, int *__
 // paths are written; a path's origin keeps its "point of this tree" bit when
 // the origin survives the weak-density prune of the group.
 __global__ void k_b_part_count(const PartPair *__restrict__ pairs, const int *__restrict__ blk_off, int np,
-                               const uint8_t *__restrict__ cellC, int n, int thr) {
+                               const uint8_t *__restrict__ cellC, int n, int thr, int wbit) {
     __shared__ int h[3 * 256];
     const int j = seg_of(blk_off, np, blockIdx.x);
     const PartPair &A = pairs[j];
@@ -382,7 +606,7 @@ __global__ void k_b_part_count(const PartPair *__restrict__ pairs, const int *__
             const int p = A.P[i];
             const int g = A.gid[cm[p]];
             atomicAdd(&h[g], 1);
-            if (A.weakcnt[p] >= thr) atomicAdd(&h[G + g], 1);
+            if (meets(A.weakcnt[p], thr, wbit)) atomicAdd(&h[G + g], 1);
         }
     } else {
         const int i = (blk - A.bp) * blockDim.x + threadIdx.x;
@@ -397,7 +621,7 @@ __global__ void k_b_part_count(const PartPair *__restrict__ pairs, const int *__
 }
 
 __global__ void k_b_part_scatter(const PartPair *__restrict__ pairs, const int *__restrict__ blk_off, int np,
-                                 const uint8_t *__restrict__ cellC, int n, int thr) {
+                                 const uint8_t *__restrict__ cellC, int n, int thr, int wbit) {
     __shared__ int base[3 * 256];
     const int j = seg_of(blk_off, np, blockIdx.x);
     const PartPair &A = pairs[j];
@@ -418,7 +642,7 @@ __global__ void k_b_part_scatter(const PartPair *__restrict__ pairs, const int *
         const bool in = i < A.nP;
         const int p = in ? A.P[i] : 0;
         const int g = in ? A.gid[cm[p]] : 0;
-        const bool weak = in && A.weakcnt[p] >= thr;
+        const bool weak = in && meets(A.weakcnt[p], thr, wbit);
         const int sw = wave_add_by_key(cur + G, g, weak);
         if (weak) A.outW[base[G + g] + sw] = p;
     } else {
@@ -430,7 +654,7 @@ __global__ void k_b_part_scatter(const PartPair *__restrict__ pairs, const int *
             const int xr = A.spx[i];
             const int xo = origin(xr);
             g = A.gid[cm[xo]];
-            x = (xr >= 0 && A.weakcnt[xo] >= thr) ? xo : ~xo;
+            x = (xr >= 0 && meets(A.weakcnt[xo], thr, wbit)) ? xo : ~xo;
         }
         const int sx = wave_add_by_key(cur + 2 * G, g, mk != 0);
         if (mk) {
@@ -525,6 +749,80 @@ __global__ void k_b_path_check(DevData D, const PruneChild *__restrict__ C, cons
     }
 }
 
+// ------------------------------------------------------------ phi in leaf order
+// phi_S(p) exactly as the reference sums it: over p's neighbours in the order
+// of the neighbourhood tree's leaves (rank_order: points sorted by their eps
+// cells over all dimensions in tree level order, ties by id), float running
+// sum with a double term (ClusteringCpu.cu:116-124).  One block per point:
+// the block walks all points in that order, 256 at a time, finds the
+// neighbours (candidate cells within one in every dimension of S, then the
+// distance) and one thread adds their terms in order.
+__global__ __launch_bounds__(kBlock) void k_b_phi_exact(DevData D, const BatchSub *__restrict__ subs,
+                                                        const int2 *__restrict__ fb, const int *__restrict__ nfb,
+                                                        int fbcap, const int *__restrict__ rank_order, float eps,
+                                                        float s_star) {
+    __shared__ double terms[kBlock];
+    __shared__ int wcnt[kBlock / 64];
+    __shared__ float qv[kMaxDims];
+    __shared__ int qc[kMaxDims];
+    const int count = min(*nfb, fbcap);
+    const int n = D.n;
+    const int lane = __lane_id(), wave = threadIdx.x / 64;
+    for (int item = blockIdx.x; item < count; item += gridDim.x) {
+        const int2 it = fb[item];
+        const BatchSub &B = subs[it.x];
+        const int p = it.y;
+        const int k = B.S.k;
+        if ((int) threadIdx.x < k) {
+            const int s = B.S.dims[threadIdx.x];
+            qv[threadIdx.x] = D.Xt[(size_t) s * n + p];
+            qc[threadIdx.x] = D.cellE[(size_t) s * n + p];
+        }
+        __syncthreads();
+        float sum = 0.0f;
+        for (int base = 0; base < n; base += kBlock) {
+            const int rk = base + (int) threadIdx.x;
+            bool hit = false;
+            double term = 0.0;
+            if (rk < n) {
+                const int q = rank_order[rk];
+                bool ok = q != p;
+                for (int t = 0; t < k && ok; ++t) {
+                    const int de = (int) D.cellE[(size_t) B.S.dims[t] * n + q] - qc[t];
+                    ok = de >= -1 && de <= 1;
+                }
+                if (ok) {
+                    float acc = 0.0f;
+                    for (int t = 0; t < k; ++t) {
+                        const float diff = qv[t] - D.Xt[(size_t) B.S.dims[t] * n + q];
+                        const float sq = diff * diff;
+                        acc = acc + sq;
+                    }
+                    if (acc <= s_star) {
+                        hit = true;
+                        term = phi_term(acc, eps);
+                    }
+                }
+            }
+            const unsigned long long m = __ballot(hit);
+            if (lane == 0) wcnt[wave] = __popcll(m);
+            __syncthreads();
+            int off0 = 0, tot = 0;
+            for (int w = 0; w < kBlock / 64; ++w) {
+                if (w < wave) off0 += wcnt[w];
+                tot += wcnt[w];
+            }
+            if (hit) terms[off0 + __popcll(m & ((1ull << lane) - 1ull))] = term;
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (int t = 0; t < tot; ++t) sum = (float) ((double) sum + terms[t]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) B.cnt[p] = (sum >= B.phi_w ? 1 : 0) | (sum >= B.phi_d ? 2 : 0);
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ replay of pruneRedundancy and join
 // The depth-first order of the reference is replayed with one launch per
 // surviving tree and four per join.  `list` holds the result-table rows whose
@@ -532,32 +830,54 @@ __global__ void k_b_path_check(DevData D, const PruneChild *__restrict__ C, cons
 // its nkept > 0, which the joins earlier in stream order have set.  A tree is
 // known by its index; tstamp[p] is the tree of S whose weak-dense points hold p.
 
+// Long walks (at least kCoop items) are shared by every block of the launch,
+// short ones belong to block h % gridDim; each block reads every listed row.
+constexpr int kCoop = 4096;
+
+__device__ __forceinline__ int block_min(int v, int *wtmp) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_down(v, o));
+    if (__lane_id() == 0) wtmp[threadIdx.x / 64] = v;
+    __syncthreads();
+    int b = INT_MAX;
+    for (int w = 0; w < kBlock / 64; ++w) b = min(b, wtmp[w]);
+    __syncthreads();
+    return b;
+}
+
 // pruneRedundancy (ScyTreeNode.cu:669-734): per listed superspace H in the
-// result (one block per H), the smallest result cluster of H holding a point of
-// the tree; the largest of these minima decides.  The last block to finish
-// writes the decision.
+// result, the smallest result cluster of H holding a point of the tree; the
+// largest of these minima decides.  Per H the shorter of two walks is taken:
+// H's members (is the member a point of the tree?) or the tree's points (is
+// the point in a kept cluster of H?).  The last block to finish writes the
+// decision.
 __global__ void k_rep_redundancy(const Entry *__restrict__ E, const int *__restrict__ list, int nlist,
-                                 const int *__restrict__ tstamp, int tree, int nW, float r, int *__restrict__ mx,
-                                 unsigned *__restrict__ done, int *__restrict__ keepv) {
-    __shared__ int wmin[kBlock / 64];
+                                 const int *__restrict__ tstamp, const int *__restrict__ W, int tree, int nW, float r,
+                                 int *__restrict__ mx, unsigned *__restrict__ done, int *__restrict__ keepv) {
+    __shared__ int wtmp[kBlock / 64];
     __shared__ bool last;
-    for (int h = blockIdx.x; h < nlist; h += gridDim.x) {
+    for (int h = 0; h < nlist; ++h) {
         const Entry e = E[list[h]];
         if (e.nkept <= 0) continue;
+        const bool by_members = e.nmem <= nW;
+        const int len = by_members ? e.nmem : nW;
+        const bool coop = len >= kCoop;
+        if (!coop && (h % (int) gridDim.x) != (int) blockIdx.x) continue;
+        const int start = coop ? blockIdx.x * blockDim.x + threadIdx.x : threadIdx.x;
+        const int stride = coop ? gridDim.x * blockDim.x : blockDim.x;
         int best = INT_MAX;
-        for (int t = threadIdx.x; t < e.nmem; t += blockDim.x) {
-            const int q = e.mem[t];
-            if (tstamp[q] == tree) best = min(best, e.csize[e.comp[q]]);
+        if (by_members) {
+            for (int t = start; t < len; t += stride) {
+                const int q = e.mem[t];
+                if (tstamp[q] == tree) best = min(best, e.csize[e.comp[q]]);
+            }
+        } else {
+            for (int t = start; t < len; t += stride) {
+                const int K = e.comp[W[t]];
+                if (K >= 0 && e.inres[K]) best = min(best, e.csize[K]);
+            }
         }
-        for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_down(best, o));
-        if (__lane_id() == 0) wmin[threadIdx.x / 64] = best;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            int b = INT_MAX;
-            for (int w = 0; w < kBlock / 64; ++w) b = min(b, wmin[w]);
-            if (b != INT_MAX) atomicMax(mx, b);
-        }
-        __syncthreads();
+        best = block_min(best, wtmp);
+        if (threadIdx.x == 0 && best != INT_MAX) atomicMax(mx, best);
     }
     __threadfence();
     if (threadIdx.x == 0) last = atomicAdd(done, 1u) == gridDim.x - 1;
@@ -591,10 +911,14 @@ __global__ void k_rep_join_mark(const int *__restrict__ tstamp, int t_lo, int t_
 __global__ void k_rep_join_check(const Entry *__restrict__ E, const int *__restrict__ list, int nlist,
                                  const int *__restrict__ compS, const uint8_t *__restrict__ keepf,
                                  const int *__restrict__ csizeS, float r, uint8_t *__restrict__ drop) {
-    for (int h = blockIdx.x; h < nlist; h += gridDim.x) {
+    for (int h = 0; h < nlist; ++h) {
         const Entry e = E[list[h]];
         if (e.nkept <= 0) continue;
-        for (int t = threadIdx.x; t < e.nmem; t += blockDim.x) {
+        const bool coop = e.nmem >= kCoop;
+        if (!coop && (h % (int) gridDim.x) != (int) blockIdx.x) continue;
+        const int start = coop ? blockIdx.x * blockDim.x + threadIdx.x : threadIdx.x;
+        const int stride = coop ? gridDim.x * blockDim.x : blockDim.x;
+        for (int t = start; t < e.nmem; t += stride) {
             const int q = e.mem[t];
             const int K = compS[q];
             if (K < 0 || !keepf[K]) continue;
@@ -624,17 +948,49 @@ __global__ void k_rep_join_apply(uint8_t *__restrict__ keepf, uint8_t *__restric
     if (__lane_id() == 0 && added) atomicAdd(&E[eS].nkept, added);
 }
 
-// members of the newly kept clusters join S's member list
+// members of the newly kept clusters join S's member list: each block takes
+// kMemPerBlock points and reserves its slots with one atomic
+constexpr int kMemPerBlock = 4096;
+
 __global__ void k_rep_join_members(const int *__restrict__ compS, int n, const uint8_t *__restrict__ fresh,
                                    Entry *__restrict__ E, int eS) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool in = false;
-    if (i < n) {
-        const int K = compS[i];
-        in = K >= 0 && fresh[K];
+    __shared__ int wcount[kBlock / 64];
+    __shared__ int base;
+    constexpr int PER = kMemPerBlock / kBlock;
+    const int i0 = blockIdx.x * kMemPerBlock;
+    bool in[PER];
+    int mine = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int i = i0 + u * kBlock + threadIdx.x;
+        bool f = false;
+        if (i < n) {
+            const int K = compS[i];
+            f = K >= 0 && fresh[K];
+        }
+        in[u] = f;
+        mine += f ? 1 : 0;
     }
-    const int slot = wave_slot(in, &E[eS].nmem);
-    if (in) E[eS].mem[slot] = i;
+    // exclusive prefix of `mine` over the block
+    int incl = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (__lane_id() >= o) incl += v;
+    }
+    if (__lane_id() == 63) wcount[threadIdx.x / 64] = incl;
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < (int) threadIdx.x / 64) before += wcount[w];
+        total += wcount[w];
+    }
+    if (threadIdx.x == 0) base = total > 0 ? atomicAdd(&E[eS].nmem, total) : 0;
+    __syncthreads();
+    int slot = base + before + incl - mine;
+    int *mem = E[eS].mem;
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+        if (in[u]) mem[slot++] = i0 + u * kBlock + threadIdx.x;
 }
 
 __global__ void k_b_stamp(const CopySeg *__restrict__ segs, const int *__restrict__ blk_off, int ns) {
@@ -671,7 +1027,10 @@ __global__ void k_b_fill(T *p, T v, size_t count) {
 static_assert(kBlockThreads == kBlock && kQueriesPerBlock == QPB, "launch geometry constants disagree");
 
 static unsigned query_blocks_n(int n) { return (unsigned) ((n + QPB - 1) / QPB); }
-static size_t count_lds(int kmax) { return (sizeof(float) + sizeof(int)) * QPB * (size_t) (kmax > 0 ? kmax : 1); }
+static size_t count_lds(int kmax) {
+    return (sizeof(float) + sizeof(int)) * QPB * (size_t) (kmax > 0 ? kmax : 1) + 8 * QPB +
+           sizeof(int) * QPB * kRangeWords;
+}
 
 hipError_t launch_b_flags(const FlagTree *T, const int *blk_off, int nt, int nblk, const uint8_t *cellC, int n,
                           int cells, int *flags, hipStream_t st) {
@@ -688,7 +1047,7 @@ hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, cons
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(X.scan_temp, tb, X.hist, X.off, X.nhist, st);
     if (e != hipSuccess) return e;
     LAUNCH(k_b_scatter, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.keys, X.rank, X.off, X.order);
-    LAUNCH(k_b_gather, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.order, X.Ys, X.Es);
+    LAUNCH(k_b_gather, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.order, X.Ys, X.Es, X.sig);
     return hipGetLastError();
 }
 
@@ -716,18 +1075,32 @@ hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySe
 }
 
 hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, int nblk, int n, int kmax,
-                          const BatchIndex &X, float s_star, unsigned long long *stat, const KernelTimer &tm,
-                          hipStream_t st) {
+                          const BatchIndex &X, float s_star, float eps, bool phi, const DevData &D,
+                          const int *rank_order, unsigned long long *stat, const KernelTimer &tm, hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
     if (blk_off && nblk <= 0) return hipSuccess;
+    if (phi) LAUNCH(k_b_fill<int>, dim3(1), dim3(kBlock), 0, st, X.nfb, 0, (size_t) 1);
     if (tm.start) (void) hipEventRecord(tm.start, st);
-    if (blk_off)
-        LAUNCH(k_b_count<true>, dim3(nblk), dim3(kBlock), count_lds(kmax), st, subs, blk_off, nsub, n, X.off, X.order,
-               X.Ys, X.Es, X.qlist, X.qstart, s_star, X.nb, stat);
-    else
-        LAUNCH(k_b_count<false>, dim3(query_blocks_n(n), nsub), dim3(kBlock), count_lds(kmax), st, subs,
-               (const int *) nullptr, nsub, n, X.off, X.order, X.Ys, X.Es, X.qlist, X.qstart, s_star, X.nb, stat);
+    const size_t lds = count_lds(kmax);
+    if (blk_off) {
+        if (phi)
+            LAUNCH((k_b_count<true, true>), dim3(nblk), dim3(kBlock), lds, st, subs, blk_off, nsub, n, X.off, X.order,
+                   X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, X.nb, X.fb, X.nfb, X.fbcap, stat);
+        else
+            LAUNCH((k_b_count<true, false>), dim3(nblk), dim3(kBlock), lds, st, subs, blk_off, nsub, n, X.off,
+                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, X.nb, X.fb, X.nfb, X.fbcap, stat);
+    } else {
+        const dim3 grid(query_blocks_n(n), nsub);
+        if (phi)
+            LAUNCH((k_b_count<false, true>), grid, dim3(kBlock), lds, st, subs, (const int *) nullptr, nsub, n, X.off,
+                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, X.nb, X.fb, X.nfb, X.fbcap, stat);
+        else
+            LAUNCH((k_b_count<false, false>), grid, dim3(kBlock), lds, st, subs, (const int *) nullptr, nsub, n, X.off,
+                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, X.nb, X.fb, X.nfb, X.fbcap, stat);
+    }
     if (tm.stop) (void) hipEventRecord(tm.stop, st);
+    if (phi)
+        LAUNCH(k_b_phi_exact, dim3(1024), dim3(kBlock), 0, st, D, subs, X.fb, X.nfb, X.fbcap, rank_order, eps, s_star);
     return hipGetLastError();
 }
 
@@ -735,20 +1108,22 @@ hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, 
                                hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
     LAUNCH(k_b_dense_init, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.order, X.dsorted, X.par);
-    LAUNCH(k_b_link_sample, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.nb, X.par);
-    LAUNCH(k_b_uf_compress, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.par);
+    for (int round = 0; round < 2; ++round) {
+        LAUNCH(k_b_link_sample, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.nb, round, X.par);
+        LAUNCH(k_b_uf_compress, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.par);
+    }
     LAUNCH(k_b_giant, dim3(nsub), dim3(256), 0, st, subs, n, X.par, X.giant);
     LAUNCH(k_b_link_full, dim3(query_blocks_n(n), nsub), dim3(kBlock), count_lds(kmax), st, subs, n, X.off, X.order,
-           X.Ys, X.Es, X.dsorted, s_star, X.giant, X.par);
+           X.Ys, X.Es, X.sig, X.dsorted, s_star, X.giant, X.par);
     LAUNCH(k_b_uf_flatten, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.par);
     return hipGetLastError();
 }
 
 hipError_t launch_b_partition(const PartPair *pairs, const int *blk_off, int npairs, int nblk, const uint8_t *cellC,
-                              int n, int thr, hipStream_t st) {
+                              int n, int thr, int wbit, hipStream_t st) {
     if (nblk <= 0) return hipSuccess;
-    LAUNCH(k_b_part_count, dim3(nblk), dim3(kBlock), 0, st, pairs, blk_off, npairs, cellC, n, thr);
-    LAUNCH(k_b_part_scatter, dim3(nblk), dim3(kBlock), 0, st, pairs, blk_off, npairs, cellC, n, thr);
+    LAUNCH(k_b_part_count, dim3(nblk), dim3(kBlock), 0, st, pairs, blk_off, npairs, cellC, n, thr, wbit);
+    LAUNCH(k_b_part_scatter, dim3(nblk), dim3(kBlock), 0, st, pairs, blk_off, npairs, cellC, n, thr, wbit);
     return hipGetLastError();
 }
 
@@ -773,10 +1148,11 @@ hipError_t launch_b_stamp(const CopySeg *segs, const int *blk_off, int nseg, int
     return hipGetLastError();
 }
 
-hipError_t launch_rep_redundancy(const Entry *E, const int *list, int nlist, const int *tstamp, int tree, int nW,
-                                 float r, int *mx, unsigned *done, int *keepv, hipStream_t st) {
+hipError_t launch_rep_redundancy(const Entry *E, const int *list, int nlist, const int *tstamp, const int *W,
+                                 int tree, int nW, float r, int *mx, unsigned *done, int *keepv, hipStream_t st) {
     const int grid = std::max(1, std::min(nlist, 256));
-    LAUNCH(k_rep_redundancy, dim3(grid), dim3(kBlock), 0, st, E, list, nlist, tstamp, tree, nW, r, mx, done, keepv);
+    LAUNCH(k_rep_redundancy, dim3(grid), dim3(kBlock), 0, st, E, list, nlist, tstamp, W, tree, nW, r, mx, done,
+           keepv);
     return hipGetLastError();
 }
 
@@ -789,7 +1165,8 @@ hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *kee
         LAUNCH(k_rep_join_check, dim3(std::min(nlist, 256)), dim3(kBlock), 0, st, E, list, nlist, compS, keepf, csizeS,
                r, drop);
     LAUNCH(k_rep_join_apply, dim3(blocks_for(n)), dim3(kBlock), 0, st, keepf, drop, n, inresS, fresh, E, eS);
-    LAUNCH(k_rep_join_members, dim3(blocks_for(n)), dim3(kBlock), 0, st, compS, n, fresh, E, eS);
+    LAUNCH(k_rep_join_members, dim3((n + kMemPerBlock - 1) / kMemPerBlock), dim3(kBlock), 0, st, compS, n, fresh, E,
+           eS);
     return hipGetLastError();
 }
 

@@ -3,6 +3,7 @@ CPU INSCY) — exact for every integer output: cells, neighbour counts,
 components, subspace sets, canonical labels, per-tree decisions."""
 
 import ctypes
+import os
 import sys
 
 import numpy as np
@@ -192,6 +193,36 @@ def test_batched_small_cases_match_oracle(seed):
                       sigma=float(rng.choice([0.02, 0.05])))
     say("batched small", seed, n, d, cells, eps)
     compare_with_oracle(w, mode=_lib.MODE_BATCHED)
+
+
+@pytest.mark.parametrize("seed", list(range(10)))
+@pytest.mark.parametrize("mode", [_lib.MODE_BATCHED, _lib.MODE_EXACT])
+def test_phi_density_small_cases_match_oracle(seed, mode):
+    # rectangular=False (the reference wrapper's default): phi density
+    # against alpha/omega thresholds, summed in the neighbourhood tree's leaf
+    # order; both modes, traces included
+    import dataclasses
+    rng = np.random.default_rng(seed + 900)
+    n = int(rng.integers(100, 700))
+    d = int(rng.integers(2, 6))
+    cells = int(rng.integers(2, 7))
+    eps = float(rng.choice([0.03, 0.05, 0.1]))
+    w = datagen.small(seed + 40, n=n, d=d, n_clusters=int(rng.integers(1, 4)), eps=eps, cells=cells,
+                      num_obj=int(rng.integers(1, 5)), min_size=int(rng.integers(2, n // 10 + 3)),
+                      sigma=float(rng.choice([0.02, 0.05])))
+    w = dataclasses.replace(w, rectangular=False, F=float(rng.choice([0.5, 1.0, 2.0])))
+    say("phi small", seed, mode, n, d, cells, eps, w.F)
+    compare_with_oracle(w, mode=mode)
+
+
+def test_phi_density_config0_matches_golden():
+    import dataclasses
+    w = dataclasses.replace(datagen.config(0), rectangular=False)
+    say("phi config0", w.name)
+    got = compare_with_oracle(w, mode=_lib.MODE_BATCHED)
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "config0_phi.npz"))
+    assert [sum(1 << k for k in s) for s in got.subspaces] == list(g["subspace_masks"])
+    assert np.array_equal(np.stack(got.labels) if len(got.labels) else got.labels, g["labels"])
 
 
 def test_batched_config0_matches_oracle():
