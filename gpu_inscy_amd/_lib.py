@@ -115,6 +115,7 @@ def lib():
     L.inscy_profile.argtypes = [ctypes.c_int]
     L.inscy_profile.restype = ctypes.c_int
     L.inscy_debug_trace.argtypes = [ctypes.c_int]
+    L.inscy_debug_phi_exact.argtypes = [ctypes.c_int]
     L.inscy_debug_trace_rows.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]
     L.inscy_debug_trace_rows.restype = ctypes.c_int64
     _lib = L
@@ -194,6 +195,10 @@ def profile(on: bool) -> None:
 
 def debug_trace(on: bool) -> None:
     lib().inscy_debug_trace(int(on))
+
+
+def debug_phi_exact(on: bool) -> None:
+    lib().inscy_debug_phi_exact(int(on))
 
 
 def debug_trace_rows() -> np.ndarray:

@@ -144,9 +144,12 @@ __device__ __forceinline__ uint64_t mix64(uint64_t h) {
     return h;
 }
 
+// a and b share their c-grid cells on every unrestricted dimension at SCY-tree
+// levels 0..upto (the regular-node prefix of the tree)
 __device__ __forceinline__ bool same_prefix(const DevData &D, uint64_t smask, int a, int b, int upto) {
     if (b < 0 || b >= D.n) return false;
-    for (int k = 0; k <= upto; ++k) {
+    for (int t = 0; t <= upto; ++t) {
+        const int k = D.level[t];
         if ((smask >> k) & 1ull) continue;
         if (D.cellC[(size_t) k * D.n + a] != D.cellC[(size_t) k * D.n + b]) return false;
     }

@@ -255,7 +255,8 @@ public:
             (void) hipEventDestroy(t.stop);
         }
         for (void *p : {(void *) Xt, (void *) cellC, (void *) cellE, (void *) nearm, (void *) small, (void *) stamp,
-                        (void *) qmark, (void *) entries, (void *) scratch, (void *) rank_perm})
+                        (void *) qmark, (void *) entries, (void *) scratch, (void *) rank_perm,
+                        (void *) d_level})
             if (p) (void) hipFree(p);
         if (pin) (void) hipHostFree(pin);
     }
@@ -310,23 +311,14 @@ public:
         HIP_OK(launch_discretise(d_X, n, d, cells, prm.neighborhood_size, Xt, cellC, cellE, nearm,
                                  reinterpret_cast<int *>(small), st));
         HIP_OK(launch_hist1(cellE, n, d, ecells, hist, st));
-        if (phi_mode) {
-            // leaf order of the neighbourhood tree (levels in dimension order)
-            HIP_OK(hipMalloc(&rank_perm, (size_t) n * sizeof(int)));
-            std::vector<int> level(d);
-            for (int k = 0; k < d; ++k) level[k] = k;
-            const size_t tb = rank_order_temp_bytes(n);
-            int *tmp = arena.take<int>(n);
-            uint16_t *k1 = arena.take<uint16_t>(n);
-            uint16_t *k2 = arena.take<uint16_t>(n);
-            void *temp = arena.take<char>(tb);
-            HIP_OK(launch_rank_order(cellE, n, d, level.data(), rank_perm, tmp, k1, k2, temp, tb, st));
-        }
+        int *histc = arena.take<int>((size_t) d * cells);
+        HIP_OK(launch_histc(cellC, n, d, cells, histc, st));
         HIP_OK(launch_fill_i32(stamp, -1, n, st));
         HIP_OK(launch_fill_i32(qmark, -1, n, st));
-        std::vector<int> h((size_t) d * ecells);
+        std::vector<int> h((size_t) d * ecells), hc((size_t) d * cells);
         HIP_OK(hipMemcpyAsync(pin, small, sizeof(int), hipMemcpyDeviceToHost, st));
         HIP_OK(hipMemcpyAsync(h.data(), hist, h.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_OK(hipMemcpyAsync(hc.data(), histc, hc.size() * sizeof(int), hipMemcpyDeviceToHost, st));
         sync();
         if (reinterpret_cast<int *>(pin)[0] != 0)
             throw Failure(INSCY_ERR_ARGUMENT, "X must be finite with every value in [0, 1] (normalize first)");
@@ -340,14 +332,52 @@ public:
                 score[k] += (double) hk[c] * nb;
             }
         }
+        // SCY-tree level order (get_subspace_order, inscy_map.cpp:61-121):
+        // identity, or dimensions by the entropy of their c-grid occupancy
+        // (float p*log(p) sums, so an empty cell gives NaN as in the reference),
+        // ascending, reversed for entropy_order = -1
+        level.resize(d);
+        for (int k = 0; k < d; ++k) level[k] = k;
+        if (prm.entropy_order != 0) {
+            std::vector<std::pair<float, int>> ent(d);
+            for (int k = 0; k < d; ++k) {
+                float e = 0.0f;
+                for (int c = 0; c < cells; ++c) {
+                    const float p = (float) hc[(size_t) k * cells + c] / (float) n;
+                    e -= p * std::log(p);
+                }
+                ent[k] = {e, k};
+            }
+            std::sort(ent.begin(), ent.end());
+            for (int k = 0; k < d; ++k) level[k] = ent[k].second;
+            if (prm.entropy_order == -1) std::reverse(level.begin(), level.end());
+        }
+        lvl_of.assign(d, 0);
+        for (int t = 0; t < d; ++t) lvl_of[level[t]] = t;
+        HIP_OK(hipMalloc(&d_level, (size_t) 2 * d * sizeof(int)));
+        std::vector<int> both(level);
+        both.insert(both.end(), lvl_of.begin(), lvl_of.end());
+        HIP_OK(hipMemcpy(d_level, both.data(), both.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (phi_mode) {
+            // leaf order of the neighbourhood tree, whose levels follow the same order
+            HIP_OK(hipMalloc(&rank_perm, (size_t) n * sizeof(int)));
+            const size_t tb = rank_order_temp_bytes(n);
+            int *tmp = arena.take<int>(n);
+            uint16_t *k1 = arena.take<uint16_t>(n);
+            uint16_t *k2 = arena.take<uint16_t>(n);
+            void *temp = arena.take<char>(tb);
+            HIP_OK(launch_rank_order(cellE, n, d, level.data(), rank_perm, tmp, k1, k2, temp, tb, st));
+            sync();
+        }
         arena.release(0);
-        D = DevData{Xt, cellC, cellE, n, d, cells, ecells};
+        D = DevData{Xt, cellC, cellE, d_level, d_level + d, n, d, cells, ecells};
     }
 
     // ---------------------------------------------------------- run
     // level-synchronous driver (INSCY_MODE_BATCHED), same result
     void run_levels(inscy_result &out);
     bool levels_mode = false;
+    bool phi_all_exact = false;  // test hook: every phi by the leaf-order recomputation
 
     void run(inscy_result &out) {
         int *P = arena.take<int>(n);
@@ -464,19 +494,33 @@ private:
         return S;
     }
 
+    // positions of S that the neighbourhood tree walk matches: the longest
+    // prefix of S (ascending dimension ids) whose levels increase
+    // (ScyTreeNode.cu:910-915); every other dimension of S is tested by the
+    // distance alone
+    uint64_t match_of(const SubspaceArg &S) const {
+        uint64_t m = 0;
+        int si = 0;
+        for (int t = 0; t < d && si < S.k; ++t)
+            if (level[t] == S.dims[si]) m |= 1ull << si++;
+        return m;
+    }
+
     GridArg grid_of(const SubspaceArg &S) const {
         GridArg G{};
-        std::vector<int> ts(S.k);
-        for (int t = 0; t < S.k; ++t) ts[t] = t;
+        const uint64_t matched = match_of(S);
+        std::vector<int> ts;
+        for (int t = 0; t < S.k; ++t)
+            if ((matched >> t) & 1ull) ts.push_back(t);
         std::stable_sort(ts.begin(), ts.end(), [&](int a, int b) { return score[S.dims[a]] < score[S.dims[b]]; });
         G.C = ecells;
         G.j = 1;
         long long cellsj = ecells;
-        while (G.j < 3 && G.j < S.k && cellsj * ecells <= grid_cap) {
+        while (G.j < 3 && G.j < (int) ts.size() && cellsj * ecells <= grid_cap) {
             cellsj *= ecells;
             G.j++;
         }
-        G.checkmask = S.k >= 64 ? ~0ull : ((1ull << S.k) - 1ull);
+        G.checkmask = matched;
         for (int a = 0; a < G.j; ++a) {
             G.tI[a] = ts[a];
             G.checkmask &= ~(1ull << ts[a]);
@@ -872,6 +916,8 @@ private:
     float phi_weak = 0.f;
     std::vector<float> phi_dense;   // [k]
     int *rank_perm = nullptr;       // [n] points in the neighbourhood tree's leaf order
+    std::vector<int> level, lvl_of; // SCY-tree level -> dimension, dimension -> level
+    int *d_level = nullptr;         // both, on the device ([level | lvl_of])
     bool batch_trace = std::getenv("INSCY_TRACE_BATCHES") != nullptr;  // one stderr line per batch
     std::vector<int> lv_act, lv_next, lv_gidoff;
     std::vector<std::vector<std::vector<Group>>> lv_groups;

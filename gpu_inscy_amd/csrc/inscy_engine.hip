@@ -6,6 +6,7 @@ namespace inscy {
 static thread_local std::string g_error;
 static thread_local std::vector<Engine::TraceRow> g_trace;
 static bool g_trace_on = false;
+static bool g_phi_all_exact = false;
 static bool g_profile_on = false;
 
 static int fail(int code, const std::string &msg) {
@@ -21,6 +22,10 @@ static int check_params(int64_t n, int32_t d, const inscy_params *p, int32_t mod
         return fail(INSCY_ERR_ARGUMENT, "neighborhood_size must be > 0");
     if (p->number_of_cells < 1 || p->number_of_cells > 255)
         return fail(INSCY_ERR_ARGUMENT, "number_of_cells must be in [1, 255]");
+    // A permuted SCY-tree level order changes which s-connections restrict()
+    // copies (ScyTreeNode.cu:286-289 keeps every node whose dimension id
+    // exceeds the restricted one); the engine's path model covers the
+    // identity order only.
     if (p->entropy_order != 0) return fail(INSCY_ERR_UNSUPPORTED, "entropy_order != 0 is not implemented");
     if (mode != INSCY_MODE_EXACT && mode != INSCY_MODE_BATCHED) return fail(INSCY_ERR_ARGUMENT, "unknown mode");
     return INSCY_OK;
@@ -40,6 +45,7 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
         {
             Engine eng(*p, (int) n, d, st);
             eng.trace_on = g_trace_on;
+            eng.phi_all_exact = g_phi_all_exact;
             eng.profile_on = g_profile_on;
             // phi density (rectangular = false) is implemented in the
             // level-synchronous driver only; it serves both modes
@@ -142,6 +148,13 @@ int inscy_profile(int on) {
 // restricted tree — (mask, lo, hi, points, survivors, pass, redundant).
 int inscy_debug_trace(int on) {
     g_trace_on = on != 0;
+    return INSCY_OK;
+}
+
+// Test hook (not in the public header): decide every phi density by the
+// leaf-order recomputation instead of the double sum with its rounding band.
+int inscy_debug_phi_exact(int on) {
+    g_phi_all_exact = on != 0;
     return INSCY_OK;
 }
 

@@ -24,6 +24,8 @@ struct DevData {
     const float *Xt;        // [d][n] coordinates
     const uint8_t *cellC;   // [d][n] cell in the c-grid (ScyTreeNode.cu:100-102)
     const uint16_t *cellE;  // [d][n] cell in the ceil(1/eps)-grid (inscy_map.cpp:243-244)
+    const int *level;       // [d] dimension at each SCY-tree level (get_subspace_order, inscy_map.cpp:61-121)
+    const int *lvl_of;      // [d] level of each dimension
     int n, d, cells, ecells;
 };
 
@@ -102,6 +104,10 @@ struct BatchSub {
     // decision bits), thresholds of the weak and the dense test; count mode: dbit = 0
     int dbit;
     float phi_w, phi_d;
+    // positions of S whose eps cells restrict the candidates: the dimensions
+    // of S the neighbourhood tree walk matches in level order (all of S in
+    // the identity order; ScyTreeNode.cu:910-915)
+    uint64_t matchmask;
 };
 
 // Device buffers of one batch's grid indexes.
@@ -189,6 +195,8 @@ size_t select_temp_bytes(int items);
 hipError_t launch_discretise(const float *X, int n, int d, int cells, float eps, float *Xt, uint8_t *cellC,
                              uint16_t *cellE, uint64_t *nearm, int *bad, hipStream_t st);
 hipError_t launch_hist1(const uint16_t *cellE, int n, int d, int ecells, int *hist, hipStream_t st);
+// occupancy of the c-grid per dimension ([d][cells]), the input of the entropy order
+hipError_t launch_histc(const uint8_t *cellC, int n, int d, int cells, int *hist, hipStream_t st);
 // |N_S(p)| (capped at `cap`) of the points with cnt == -1 and, when mark is
 // given, mark[p] == markid (stat: candidates examined, points counted)
 hipError_t launch_subspace_counts(const DevData &D, const SubspaceArg &S, const GridArg &G, float s_star, int cap,
@@ -245,8 +253,9 @@ hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySe
 // phi: rectangular = false, the stored values are decision bits (1 weak-dense,
 // 2 dense); points near a threshold are recomputed in leaf order (rank_order)
 hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, int nblk, int n, int kmax,
-                          const BatchIndex &X, float s_star, float eps, bool phi, const DevData &D,
-                          const int *rank_order, unsigned long long *stat, const KernelTimer &tm, hipStream_t st);
+                          const BatchIndex &X, float s_star, float eps, bool phi, bool phi_all_exact,
+                          const DevData &D, const int *rank_order, unsigned long long *stat, const KernelTimer &tm,
+                          hipStream_t st);
 // dense components of nsub subspaces whose counts are complete
 hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, const BatchIndex &X, float s_star,
                                hipStream_t st);

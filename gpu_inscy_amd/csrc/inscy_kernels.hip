@@ -62,6 +62,13 @@ __global__ void k_hist1(const uint16_t *__restrict__ cellE, int n, int d, int ec
     atomicAdd(&hist[(size_t) k * ecells + cellE[i]], 1);
 }
 
+__global__ void k_histc(const uint8_t *__restrict__ cellC, int n, int d, int cells, int *__restrict__ hist) {
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t) n * d) return;
+    int k = (int) (i / n);
+    atomicAdd(&hist[(size_t) k * cells + cellC[i]], 1);
+}
+
 // ------------------------------------------------------------ expansion of one tree
 // mergeWithNeighbors decision input (ScyTreeNode.cu:837-851 with the flag of
 // restrict_node :262-267): cell c of dimension b is s-connected iff a path
@@ -430,7 +437,8 @@ __global__ void k_path_insert(DevData D, uint64_t smask, const int *__restrict__
     if (!need) return;
     int w = W[i];
     uint64_t h = 0x243F6A8885A308D3ull;
-    for (int k = 0; k < D.d; ++k) {
+    for (int t = 0; t < D.d; ++t) {
+        const int k = D.level[t];
         if ((smask >> k) & 1ull) continue;
         h = mix64(h + D.cellC[(size_t) k * D.n + w] + 1);
         if (!((need >> k) & 1ull)) continue;
@@ -444,7 +452,7 @@ __global__ void k_path_insert(DevData D, uint64_t smask, const int *__restrict__
                                 (unsigned long long) entry);
                 if (cur == kEmpty) break;
             }
-            if ((cur >> 32) == (entry >> 32) && same_prefix(D, smask, w, (int) (uint32_t) cur, k)) break;
+            if ((cur >> 32) == (entry >> 32) && same_prefix(D, smask, w, (int) (uint32_t) cur, t)) break;
             slot = (slot + 1) & tmask;
         }
     }
@@ -463,7 +471,8 @@ __global__ void k_path_check(DevData D, uint64_t smask, const int *__restrict__ 
         if (stamp[x] != tid) {
             uint64_t h = 0x243F6A8885A308D3ull;
             uint64_t keep = 0;
-            for (int k = 0; k < D.d; ++k) {
+            for (int t = 0; t < D.d; ++t) {
+                const int k = D.level[t];
                 if ((smask >> k) & 1ull) continue;
                 h = mix64(h + D.cellC[(size_t) k * D.n + x] + 1);
                 if (!((m >> k) & 1ull)) continue;
@@ -472,7 +481,7 @@ __global__ void k_path_check(DevData D, uint64_t smask, const int *__restrict__ 
                 while (true) {
                     uint64_t cur = __hip_atomic_load(&table[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (cur == kEmpty) break;
-                    if ((cur >> 32) == (e >> 32) && same_prefix(D, smask, x, (int) (uint32_t) cur, k)) {
+                    if ((cur >> 32) == (e >> 32) && same_prefix(D, smask, x, (int) (uint32_t) cur, t)) {
                         keep |= 1ull << k;
                         break;
                     }
@@ -715,6 +724,13 @@ hipError_t launch_hist1(const uint16_t *cellE, int n, int d, int ecells, int *hi
     size_t nd = (size_t) n * d;
     LAUNCH(k_fill<int>, dim3(blocks_for((size_t) d * ecells)), dim3(kBlock), 0, st, hist, 0, (size_t) d * ecells);
     LAUNCH(k_hist1, dim3(blocks_for(nd)), dim3(kBlock), 0, st, cellE, n, d, ecells, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_histc(const uint8_t *cellC, int n, int d, int cells, int *hist, hipStream_t st) {
+    size_t nd = (size_t) n * d;
+    LAUNCH(k_fill<int>, dim3(blocks_for((size_t) d * cells)), dim3(kBlock), 0, st, hist, 0, (size_t) d * cells);
+    LAUNCH(k_histc, dim3(blocks_for(nd)), dim3(kBlock), 0, st, cellC, n, d, cells, hist);
     return hipGetLastError();
 }
 

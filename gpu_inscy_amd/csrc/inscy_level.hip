@@ -276,6 +276,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         x.qmax = (int) qmax;
         set_bounds(c.s, x);
         x.dbit = phi_mode ? 2 : 0;
+        x.matchmask = match_of(x.S);
         x.phi_w = phi_weak;
         x.phi_d = phi_dense[x.S.k];
         x.nchk4 = 0;
@@ -346,7 +347,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         sync();
         tq0 = std::chrono::steady_clock::now();
     }
-    HIP_OK(launch_b_count(dsubs, B, dcb, cblk[B], n, kmax, X, s_star, prm.neighborhood_size, phi_mode, D, rank_perm,
+    HIP_OK(launch_b_count(dsubs, B, dcb, cblk[B], n, kmax, X, s_star, prm.neighborhood_size, phi_mode, phi_all_exact, D, rank_perm,
                           small + kStatAt, timer(), st));
     if (dbg) {
         sync();
@@ -557,7 +558,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
             sync();
             tc0 = std::chrono::steady_clock::now();
         }
-        HIP_OK(launch_b_count(dhc, ncomp, nullptr, 0, n, kmax, X, s_star, prm.neighborhood_size, phi_mode, D,
+        HIP_OK(launch_b_count(dhc, ncomp, nullptr, 0, n, kmax, X, s_star, prm.neighborhood_size, phi_mode, phi_all_exact, D,
                               rank_perm, small + kStatAt, timer(), st));
         if (dbg) {
             sync();

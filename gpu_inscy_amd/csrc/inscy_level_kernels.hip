@@ -216,7 +216,7 @@ __device__ __forceinline__ double phi_term(float acc, float eps) {
 // (each float rounding moves the running sum by at most 2^-24 of the final
 // value), so a threshold outside that band is decided exactly; points inside
 // it are recomputed in leaf order (k_b_phi_exact).
-constexpr double kPhiRel = 1.0 / 4194304.0;
+constexpr double kPhiRelDefault = 1.0 / 4194304.0;
 
 // density tests on a stored value: counts (bit == 0) or phi decision bits
 __device__ __forceinline__ bool meets(int v, int thr, int bit) { return bit ? (v >= 0 && (v & bit) != 0) : v >= thr; }
@@ -240,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
                                                     const int *__restrict__ order, const float *__restrict__ Ys,
                                                     const uint16_t *__restrict__ Es, const uint64_t *__restrict__ sig,
                                                     const int *__restrict__ qlist, const int *__restrict__ qstart,
-                                                    float s_star, float eps, int *__restrict__ nb,
+                                                    float s_star, float eps, double phirel, int *__restrict__ nb,
                                                     int2 *__restrict__ fb, int *__restrict__ nfb, int fbcap,
                                                     unsigned long long *__restrict__ stat) {
     constexpr int LPQ = kLanesPerQuery, U = 4, STEP = LPQ * U;
@@ -404,15 +404,15 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
             }
             seen += min(STEP, total - c0);
             if (PHI)
-                more = dsum * (1.0 - (double) total * kPhiRel) < thr_max;
+                more = dsum * (1.0 - (double) total * phirel) < thr_max;
             else
                 more = found < cap;
         }
         if (PHI) {
             // decision bits: 1 weak-dense, 2 dense; a threshold inside the
             // rounding band sends the point to the leaf-order recomputation
-            const double lo = dsum * (1.0 - (double) found * kPhiRel);
-            const double hi = dsum * (1.0 + (double) found * kPhiRel);
+            const double lo = dsum * (1.0 - (double) found * phirel);
+            const double hi = dsum * (1.0 + (double) found * phirel);
             int code = 0;
             bool unsure = false;
             if (!more) {
@@ -688,7 +688,8 @@ __global__ void k_b_path_insert(DevData D, const PruneChild *__restrict__ C, con
     const int w = P.W[i];
     const uint64_t smask = P.smask;
     uint64_t h = 0x243F6A8885A308D3ull;
-    for (int k = 0; k < D.d; ++k) {
+    for (int t = 0; t < D.d; ++t) {
+        const int k = D.level[t];
         if ((smask >> k) & 1ull) continue;
         h = mix64(h + D.cellC[(size_t) k * D.n + w] + 1);
         if (!((need >> k) & 1ull)) continue;
@@ -702,7 +703,7 @@ __global__ void k_b_path_insert(DevData D, const PruneChild *__restrict__ C, con
                                 (unsigned long long) entry);
                 if (cur == kEmpty) break;
             }
-            if ((cur >> 32) == (entry >> 32) && same_prefix(D, smask, w, (int) (uint32_t) cur, k)) break;
+            if ((cur >> 32) == (entry >> 32) && same_prefix(D, smask, w, (int) (uint32_t) cur, t)) break;
             slot = (slot + 1) & P.tmask;
         }
     }
@@ -722,7 +723,8 @@ __global__ void k_b_path_check(DevData D, const PruneChild *__restrict__ C, cons
             const uint64_t smask = P.smask;
             uint64_t h = 0x243F6A8885A308D3ull;
             uint64_t keep = 0;
-            for (int k = 0; k < D.d; ++k) {
+            for (int t = 0; t < D.d; ++t) {
+                const int k = D.level[t];
                 if ((smask >> k) & 1ull) continue;
                 h = mix64(h + D.cellC[(size_t) k * D.n + xo] + 1);
                 if (!((m >> k) & 1ull)) continue;
@@ -732,7 +734,7 @@ __global__ void k_b_path_check(DevData D, const PruneChild *__restrict__ C, cons
                     const uint64_t cur =
                         __hip_atomic_load(&P.table[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     if (cur == kEmpty) break;
-                    if ((cur >> 32) == (e >> 32) && same_prefix(D, smask, xo, (int) (uint32_t) cur, k)) {
+                    if ((cur >> 32) == (e >> 32) && same_prefix(D, smask, xo, (int) (uint32_t) cur, t)) {
                         keep |= 1ull << k;
                         break;
                     }
@@ -755,8 +757,8 @@ __global__ void k_b_path_check(DevData D, const PruneChild *__restrict__ C, cons
 // cells over all dimensions in tree level order, ties by id), float running
 // sum with a double term (ClusteringCpu.cu:116-124).  One block per point:
 // the block walks all points in that order, 256 at a time, finds the
-// neighbours (candidate cells within one in every dimension of S, then the
-// distance) and one thread adds their terms in order.
+// neighbours (candidate cells within one in every matched dimension of S,
+// then the distance) and one thread adds their terms in order.
 __global__ __launch_bounds__(kBlock) void k_b_phi_exact(DevData D, const BatchSub *__restrict__ subs,
                                                         const int2 *__restrict__ fb, const int *__restrict__ nfb,
                                                         int fbcap, const int *__restrict__ rank_order, float eps,
@@ -788,6 +790,7 @@ __global__ __launch_bounds__(kBlock) void k_b_phi_exact(DevData D, const BatchSu
                 const int q = rank_order[rk];
                 bool ok = q != p;
                 for (int t = 0; t < k && ok; ++t) {
+                    if (!((B.matchmask >> t) & 1ull)) continue;
                     const int de = (int) D.cellE[(size_t) B.S.dims[t] * n + q] - qc[t];
                     ok = de >= -1 && de <= 1;
                 }
@@ -1075,8 +1078,11 @@ hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySe
 }
 
 hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, int nblk, int n, int kmax,
-                          const BatchIndex &X, float s_star, float eps, bool phi, const DevData &D,
-                          const int *rank_order, unsigned long long *stat, const KernelTimer &tm, hipStream_t st) {
+                          const BatchIndex &X, float s_star, float eps, bool phi, bool phi_all_exact,
+                          const DevData &D, const int *rank_order, unsigned long long *stat, const KernelTimer &tm,
+                          hipStream_t st) {
+    // phi_all_exact: every point goes through the leaf-order recomputation (a test hook)
+    const double phirel = phi_all_exact ? 1e30 : kPhiRelDefault;
     if (nsub <= 0) return hipSuccess;
     if (blk_off && nblk <= 0) return hipSuccess;
     if (phi) LAUNCH(k_b_fill<int>, dim3(1), dim3(kBlock), 0, st, X.nfb, 0, (size_t) 1);
@@ -1085,18 +1091,18 @@ hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, in
     if (blk_off) {
         if (phi)
             LAUNCH((k_b_count<true, true>), dim3(nblk), dim3(kBlock), lds, st, subs, blk_off, nsub, n, X.off, X.order,
-                   X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, X.nb, X.fb, X.nfb, X.fbcap, stat);
+                   X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb, X.fbcap, stat);
         else
             LAUNCH((k_b_count<true, false>), dim3(nblk), dim3(kBlock), lds, st, subs, blk_off, nsub, n, X.off,
-                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, X.nb, X.fb, X.nfb, X.fbcap, stat);
+                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb, X.fbcap, stat);
     } else {
         const dim3 grid(query_blocks_n(n), nsub);
         if (phi)
             LAUNCH((k_b_count<false, true>), grid, dim3(kBlock), lds, st, subs, (const int *) nullptr, nsub, n, X.off,
-                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, X.nb, X.fb, X.nfb, X.fbcap, stat);
+                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb, X.fbcap, stat);
         else
             LAUNCH((k_b_count<false, false>), grid, dim3(kBlock), lds, st, subs, (const int *) nullptr, nsub, n, X.off,
-                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, X.nb, X.fb, X.nfb, X.fbcap, stat);
+                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb, X.fbcap, stat);
     }
     if (tm.stop) (void) hipEventRecord(tm.stop, st);
     if (phi)

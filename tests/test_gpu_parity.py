@@ -215,6 +215,28 @@ def test_phi_density_small_cases_match_oracle(seed, mode):
     compare_with_oracle(w, mode=mode)
 
 
+@pytest.mark.parametrize("seed", [0, 3, 7])
+def test_phi_leaf_order_recomputation_matches(seed):
+    # every phi decided by the exact leaf-order kernel (the band path off):
+    # the same result, which pins that kernel against the oracle directly
+    import dataclasses
+    rng = np.random.default_rng(seed + 900)
+    n = int(rng.integers(100, 700))
+    d = int(rng.integers(2, 6))
+    cells = int(rng.integers(2, 7))
+    eps = float(rng.choice([0.03, 0.05, 0.1]))
+    w = datagen.small(seed + 40, n=n, d=d, n_clusters=int(rng.integers(1, 4)), eps=eps, cells=cells,
+                      num_obj=int(rng.integers(1, 5)), min_size=int(rng.integers(2, n // 10 + 3)),
+                      sigma=float(rng.choice([0.02, 0.05])))
+    w = dataclasses.replace(w, rectangular=False, F=float(rng.choice([0.5, 1.0, 2.0])))
+    say("phi leaf order", seed)
+    _lib.debug_phi_exact(True)
+    try:
+        compare_with_oracle(w, mode=_lib.MODE_BATCHED)
+    finally:
+        _lib.debug_phi_exact(False)
+
+
 def test_phi_density_config0_matches_golden():
     import dataclasses
     w = dataclasses.replace(datagen.config(0), rectangular=False)
