@@ -48,7 +48,8 @@ def parse():
     ap.add_argument("--mode", choices=["exact", "batched"], default="batched")
     ap.add_argument("--depth-first-steps", type=int, default=2,
                     help="untimed runs of the depth-first driver for the comparison (0: skip)")
-    ap.add_argument("--config", type=int, default=1, help="workload index (1: n=25000 d=15)")
+    ap.add_argument("--config", type=int, default=1,
+                    help="workload index (1: n=25000 d=15; 2: n=100000 d=20; 3: 64 data sets)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
@@ -94,6 +95,8 @@ def main():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
     mode = _lib.MODE_EXACT if args.mode == "exact" else _lib.MODE_BATCHED
+    if args.config == 3:
+        return bench_many(args, mode)
     w = datagen.config(args.config)
     X = torch.from_numpy(w.X).to("cuda:0")
     torch.cuda.synchronize()
@@ -197,6 +200,67 @@ def main():
         "parity": parity,
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_many(args, mode):
+    """Config 3: 64 independent data sets through inscy_run_many (one step =
+    all of them); value = restricted trees per second over the 64 sets."""
+    import torch
+
+    from gpu_inscy_amd import _lib, datagen
+    from gpu_inscy_amd import inscy as INSCY
+
+    sets = datagen.multi_instance()
+    jobs = [(torch.from_numpy(w.X).to("cuda:0"), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular,
+             w.entropy_order) for w in sets]
+    torch.cuda.synchronize()
+    for _ in range(max(args.warmup, 1)):
+        first = INSCY.run_many(jobs, mode=mode)
+    _lib.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    runs = [INSCY.run_many(jobs, mode=mode) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    _lib.profile(False)
+    flat = [r for step in runs for r in step]
+    trees = sum(r.stats["trees"] for r in flat)
+    kernel_ms = sum(r.stats["count_kernel_ms"] for r in flat)
+    launches = sum(r.stats["count_launches"] for r in flat)
+    kbytes = sum(r.stats["count_bytes"] for r in flat)
+    avg_launch_ms = kernel_ms / max(launches, 1)
+    achieved = (kbytes / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
+    same = all(a.subspaces == b.subspaces for a, b in zip(first, runs[-1]))
+    out = {
+        "metric": "end-to-end INSCY seconds + subspaces/s; exact vs batched vs learned-prune (F1)",
+        "value": round(trees / elapsed, 2),
+        "unit": "subspaces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{len(sets)} independent data sets (n {min(w.n for w in sets)}..{max(w.n for w in sets)}, "
+                        f"d {min(w.d for w in sets)}..{max(w.d for w in sets)}), eps=0.015 c=6 mu=4 min_size=0.02n, "
+                        "inscy_run_many",
+            "mode": args.mode,
+            "trees_per_step": trees // max(args.steps, 1),
+            "subspaces": sum(r.stats["subspaces"] for r in runs[-1]),
+            "deterministic": same,
+            "parallelism": "single GPU, concurrent streams",
+        },
+        "parity": None,
+        "roofline": {"bound": "hbm", "kernel": "k_b_count" if args.mode == "batched" else "k_grid_count",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None},
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

@@ -24,11 +24,14 @@ MODE_EXACT = 1
 MODE_BATCHED = 2
 
 EXPORTED = [
-    "inscy_result_labels_all",
+    "inscy_result_labels_all", "inscy_run_many", "inscy_run_many_device", "inscy_release_memory",
     "inscy_run", "inscy_run_device", "inscy_result_num_subspaces", "inscy_result_subspace",
     "inscy_result_labels", "inscy_result_num_points", "inscy_result_stats", "inscy_result_free",
     "inscy_last_error", "inscy_discretise", "inscy_neighbour_counts", "inscy_dense_components", "inscy_profile",
 ]
+
+
+vp_t = ctypes.c_void_p
 
 
 class InscyError(RuntimeError):
@@ -88,6 +91,11 @@ def lib():
     L.inscy_run_device.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(Params), ctypes.c_int32, vp,
                                    ctypes.POINTER(vp)]
     L.inscy_run_device.restype = ctypes.c_int
+    L.inscy_run_many.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
+                                 ctypes.POINTER(Params), ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(vp)]
+    L.inscy_run_many.restype = ctypes.c_int
+    L.inscy_run_many_device.argtypes = L.inscy_run_many.argtypes
+    L.inscy_run_many_device.restype = ctypes.c_int
     L.inscy_result_num_subspaces.argtypes = [vp]
     L.inscy_result_num_subspaces.restype = ctypes.c_int32
     L.inscy_result_subspace.argtypes = [vp, ctypes.c_int32, i32p]
@@ -112,6 +120,8 @@ def lib():
     L.inscy_dense_components.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(Params), i32p,
                                          ctypes.c_int32, vp]
     L.inscy_dense_components.restype = ctypes.c_int
+    L.inscy_release_memory.argtypes = []
+    L.inscy_release_memory.restype = ctypes.c_int
     L.inscy_profile.argtypes = [ctypes.c_int]
     L.inscy_profile.restype = ctypes.c_int
     L.inscy_debug_trace.argtypes = [ctypes.c_int]
@@ -187,6 +197,20 @@ def run_device(ptr: int, n: int, d: int, params: Params, mode: int = MODE_EXACT,
     check(lib().inscy_run_device(ctypes.c_void_p(ptr), n, d, ctypes.byref(params), mode,
                                  ctypes.c_void_p(stream or None), ctypes.byref(h)))
     return Result(h)
+
+
+def run_many(ptrs, shapes, params, mode: int = MODE_BATCHED, device: bool = True) -> list:
+    """Runs independent data sets concurrently; ptrs are device (or host)
+    addresses of row-major float32 arrays with the given (n, d) shapes."""
+    count = len(ptrs)
+    P = (vp_t * count)(*[ctypes.c_void_p(p) for p in ptrs])
+    N = (ctypes.c_int64 * count)(*[int(s[0]) for s in shapes])
+    D = (ctypes.c_int32 * count)(*[int(s[1]) for s in shapes])
+    A = (Params * count)(*params)
+    out = (vp_t * count)()
+    fn = lib().inscy_run_many_device if device else lib().inscy_run_many
+    check(fn(P, N, D, A, count, mode, out))
+    return [Result(ctypes.c_void_p(out[i])) for i in range(count)]
 
 
 def profile(on: bool) -> None:

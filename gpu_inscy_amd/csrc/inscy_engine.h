@@ -40,6 +40,7 @@
 
 #include "../../include/inscy.h"
 #include "inscy_internal.h"
+#include "inscy_memory.h"
 
 // Host array for the result labels: 2 MiB aligned and advised for huge pages,
 // so that first touch of a multi-GB result costs few page faults.
@@ -121,11 +122,11 @@ struct Arena {
     char *base = nullptr;
     size_t cap = 0, top = 0;
     ~Arena() {
-        if (base) (void) hipFree(base);
+        if (base) dev_free(base);
     }
     void init(size_t bytes) {
         cap = bytes;
-        HIP_OK(hipMalloc(&base, bytes));
+        HIP_OK(dev_alloc(&base, bytes));
     }
     template <class T>
     T *take(size_t count) {
@@ -145,7 +146,7 @@ struct Slabs {
     char *cur = nullptr;
     size_t left = 0, chunk = 256u << 20;
     ~Slabs() {
-        for (void *p : chunks) (void) hipFree(p);
+        for (void *p : chunks) dev_free(p);
     }
     template <class T>
     T *take(size_t count) {
@@ -153,7 +154,7 @@ struct Slabs {
         if (bytes > left) {
             size_t sz = std::max(chunk, bytes);
             void *p = nullptr;
-            HIP_OK(hipMalloc(&p, sz));
+            HIP_OK(dev_alloc(&p, sz));
             chunks.push_back(p);
             cur = static_cast<char *>(p);
             left = sz;
@@ -176,7 +177,7 @@ struct Pool {
     std::vector<Chunk> chunks;
     size_t ci = 0, top = 0, chunk = 64u << 20;
     ~Pool() {
-        for (auto &c : chunks) (void) hipFree(c.p);
+        for (auto &c : chunks) dev_free(c.p);
     }
     template <class T>
     T *take(size_t count) {
@@ -188,7 +189,7 @@ struct Pool {
         if (ci == chunks.size()) {
             size_t sz = std::max(chunk, bytes);
             void *p = nullptr;
-            HIP_OK(hipMalloc(&p, sz));
+            HIP_OK(dev_alloc(&p, sz));
             chunks.push_back({static_cast<char *>(p), sz});
             top = 0;
         }
@@ -207,14 +208,14 @@ struct HostBuf {
     char *p = nullptr;
     size_t cap = 0;
     ~HostBuf() {
-        if (p) (void) hipHostFree(p);
+        if (p) host_free(p);
     }
     char *get(size_t bytes) {
         if (bytes > cap) {
-            if (p) HIP_OK(hipHostFree(p));
+            if (p) host_free(p);
             p = nullptr;
             size_t sz = std::max<size_t>(bytes, (size_t) 1 << 20);
-            HIP_OK(hipHostMalloc((void **) &p, sz));
+            HIP_OK(host_alloc(&p, sz));
             cap = sz;
         }
         return p;
@@ -250,6 +251,8 @@ public:
     Engine(const inscy_params &p, int n, int d, hipStream_t st) : prm(p), n(n), d(d), st(st) {}
 
     ~Engine() {
+        // blocks go back to the shared caches: nothing queued may still use them
+        (void) hipStreamSynchronize(st);
         for (auto &t : timers) {
             (void) hipEventDestroy(t.start);
             (void) hipEventDestroy(t.stop);
@@ -257,8 +260,8 @@ public:
         for (void *p : {(void *) Xt, (void *) cellC, (void *) cellE, (void *) nearm, (void *) small, (void *) stamp,
                         (void *) qmark, (void *) entries, (void *) scratch, (void *) rank_perm,
                         (void *) d_level})
-            if (p) (void) hipFree(p);
-        if (pin) (void) hipHostFree(pin);
+            if (p) dev_free(p);
+        if (pin) host_free(pin);
     }
 
     // ---------------------------------------------------------- set-up
@@ -280,14 +283,14 @@ public:
                 std::max(prm.F * alpha_thr(k, prm.neighborhood_size, n), (float) prm.num_obj * omega_thr(k)));
 
         size_t nd = (size_t) n * d;
-        HIP_OK(hipMalloc(&Xt, nd * sizeof(float)));
-        HIP_OK(hipMalloc(&cellC, nd));
-        HIP_OK(hipMalloc(&cellE, nd * sizeof(uint16_t)));
-        HIP_OK(hipMalloc(&nearm, (size_t) n * sizeof(uint64_t)));
-        HIP_OK(hipMalloc(&small, kSmallWords * sizeof(unsigned long long)));
-        HIP_OK(hipMalloc(&stamp, (size_t) n * sizeof(int)));
-        HIP_OK(hipMalloc(&qmark, (size_t) n * sizeof(int)));
-        HIP_OK(hipHostMalloc(&pin, pin_bytes()));
+        HIP_OK(dev_alloc(&Xt, nd * sizeof(float)));
+        HIP_OK(dev_alloc(&cellC, nd));
+        HIP_OK(dev_alloc(&cellE, nd * sizeof(uint16_t)));
+        HIP_OK(dev_alloc(&nearm, (size_t) n * sizeof(uint64_t)));
+        HIP_OK(dev_alloc(&small, kSmallWords * sizeof(unsigned long long)));
+        HIP_OK(dev_alloc(&stamp, (size_t) n * sizeof(int)));
+        HIP_OK(dev_alloc(&qmark, (size_t) n * sizeof(int)));
+        HIP_OK(host_alloc(&pin, pin_bytes()));
         grow_entries(1 << 12);
         // grid index size: C^j cells with j <= 3 and C^j <= max(8n, 4096)
         grid_cap = std::max<long long>(8LL * n, 4096);
@@ -354,13 +357,13 @@ public:
         }
         lvl_of.assign(d, 0);
         for (int t = 0; t < d; ++t) lvl_of[level[t]] = t;
-        HIP_OK(hipMalloc(&d_level, (size_t) 2 * d * sizeof(int)));
+        HIP_OK(dev_alloc(&d_level, (size_t) 2 * d * sizeof(int)));
         std::vector<int> both(level);
         both.insert(both.end(), lvl_of.begin(), lvl_of.end());
         HIP_OK(hipMemcpy(d_level, both.data(), both.size() * sizeof(int), hipMemcpyHostToDevice));
         if (phi_mode) {
             // leaf order of the neighbourhood tree, whose levels follow the same order
-            HIP_OK(hipMalloc(&rank_perm, (size_t) n * sizeof(int)));
+            HIP_OK(dev_alloc(&rank_perm, (size_t) n * sizeof(int)));
             const size_t tb = rank_order_temp_bytes(n);
             int *tmp = arena.take<int>(n);
             uint16_t *k1 = arena.take<uint16_t>(n);
@@ -462,14 +465,14 @@ private:
         int cap = std::max(want, 2 * ecap);
         Entry *ne = nullptr;
         int *ns = nullptr;
-        HIP_OK(hipMalloc(&ne, (size_t) cap * sizeof(Entry)));
+        HIP_OK(dev_alloc(&ne, (size_t) cap * sizeof(Entry)));
         // scratch: nE, nlist, keep, pad | list[cap] | mn[cap] | keepf[n], drop[n], fresh[n] (bytes)
-        HIP_OK(hipMalloc(&ns, (size_t) (8 + 2 * cap) * sizeof(int) + (size_t) 3 * n + 256));
+        HIP_OK(dev_alloc(&ns, (size_t) (8 + 2 * cap) * sizeof(int) + (size_t) 3 * n + 256));
         if (entries) {
             sync();
             HIP_OK(hipMemcpy(ne, entries, (size_t) nentries * sizeof(Entry), hipMemcpyDeviceToDevice));
-            HIP_OK(hipFree(entries));
-            HIP_OK(hipFree(scratch));
+            dev_free(entries);
+            dev_free(scratch);
         }
         entries = ne;
         scratch = ns;

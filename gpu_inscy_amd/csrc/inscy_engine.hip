@@ -1,6 +1,8 @@
 // C ABI of the MI355X INSCY engine (include/inscy.h).
 #include "inscy_engine.h"
 
+#include <atomic>
+
 namespace inscy {
 
 static thread_local std::string g_error;
@@ -72,11 +74,102 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
     }
 }
 
+// Many independent data sets: worker threads, each with its own stream and
+// engine, take the sets from a queue ordered by estimated cost (n^2 d,
+// largest first) so that the longest runs start first and the streams overlap
+// on the device.
+static int run_many_impl(const float *const *X, const int64_t *n, const int32_t *d, const inscy_params *params,
+                         int32_t count, int32_t mode, bool host_input, inscy_result **out) {
+    for (int32_t i = 0; i < count; ++i) out[i] = nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(INSCY_ERR_DEVICE, "no HIP device");
+    std::vector<int> order(count);
+    for (int32_t i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (double) n[a] * (double) n[a] * d[a] > (double) n[b] * (double) n[b] * d[b];
+    });
+    int workers = 4;
+    if (const char *w = std::getenv("INSCY_WORKERS")) workers = std::max(1, std::atoi(w));
+    workers = std::min<int>(workers, count);
+    std::atomic<int> next{0};
+    std::vector<int> rc(count, INSCY_OK);
+    std::vector<std::string> msg(count);
+    auto work = [&]() {
+        if (hipSetDevice(dev) != hipSuccess) return;
+        hipStream_t st = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return;
+        for (int q = next++; q < count; q = next++) {
+            const int i = order[q];
+            if (host_input) {
+                float *dX = nullptr;
+                const size_t bytes = (size_t) n[i] * d[i] * sizeof(float);
+                if (dev_alloc(&dX, bytes) != hipSuccess) {
+                    rc[i] = INSCY_ERR_MEMORY;
+                    msg[i] = "device allocation of the input failed";
+                    continue;
+                }
+                if (hipMemcpyAsync(dX, X[i], bytes, hipMemcpyHostToDevice, st) != hipSuccess) {
+                    rc[i] = INSCY_ERR_DEVICE;
+                    msg[i] = "input copy failed";
+                } else {
+                    rc[i] = run_impl(dX, n[i], d[i], &params[i], mode, st, &out[i]);
+                }
+                (void) hipStreamSynchronize(st);
+                dev_free(dX);
+            } else {
+                rc[i] = run_impl(X[i], n[i], d[i], &params[i], mode, st, &out[i]);
+            }
+            if (rc[i] != INSCY_OK) msg[i] = g_error;
+        }
+        (void) hipStreamDestroy(st);
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < workers; ++w) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+    for (int32_t i = 0; i < count; ++i) {
+        if (rc[i] != INSCY_OK || !out[i]) {
+            const int code = rc[i] != INSCY_OK ? rc[i] : INSCY_ERR_DEVICE;
+            const std::string m = "data set " + std::to_string(i) + ": " + (msg[i].empty() ? "not run" : msg[i]);
+            for (int32_t j = 0; j < count; ++j) {
+                delete out[j];
+                out[j] = nullptr;
+            }
+            return fail(code, m);
+        }
+    }
+    g_error.clear();
+    return INSCY_OK;
+}
+
 }  // namespace inscy
 
 using namespace inscy;
 
 extern "C" {
+
+int inscy_run_many(const float *const *X, const int64_t *n, const int32_t *d, const inscy_params *params,
+                   int32_t count, int32_t mode, inscy_result **out) {
+    if (!X || !n || !d || !params || !out || count < 1) return fail(INSCY_ERR_ARGUMENT, "invalid argument");
+    for (int32_t i = 0; i < count; ++i) {
+        int rc = check_params(n[i], d[i], &params[i], mode);
+        if (rc) return fail(rc, "data set " + std::to_string(i) + ": " + g_error);
+        if (!X[i]) return fail(INSCY_ERR_ARGUMENT, "data set " + std::to_string(i) + ": X is NULL");
+    }
+    return run_many_impl(X, n, d, params, count, mode, true, out);
+}
+
+int inscy_run_many_device(const float *const *d_X, const int64_t *n, const int32_t *d, const inscy_params *params,
+                          int32_t count, int32_t mode, inscy_result **out) {
+    if (!d_X || !n || !d || !params || !out || count < 1) return fail(INSCY_ERR_ARGUMENT, "invalid argument");
+    for (int32_t i = 0; i < count; ++i) {
+        int rc = check_params(n[i], d[i], &params[i], mode);
+        if (rc) return fail(rc, "data set " + std::to_string(i) + ": " + g_error);
+        if (!d_X[i]) return fail(INSCY_ERR_ARGUMENT, "data set " + std::to_string(i) + ": X is NULL");
+    }
+    return run_many_impl(d_X, n, d, params, count, mode, false, out);
+}
+
 
 int inscy_run_device(const float *d_X, int64_t n, int32_t d, const inscy_params *params, int32_t mode, void *stream,
                      inscy_result **out) {
@@ -96,15 +189,15 @@ int inscy_run(const float *X, int64_t n, int32_t d, const inscy_params *params, 
     if (!X) return fail(INSCY_ERR_ARGUMENT, "X is NULL");
     float *dX = nullptr;
     size_t bytes = (size_t) n * d * sizeof(float);
-    hipError_t e = hipMalloc(&dX, bytes);
-    if (e != hipSuccess) return fail(INSCY_ERR_MEMORY, std::string("hipMalloc: ") + hipGetErrorString(e));
+    hipError_t e = dev_alloc(&dX, bytes);
+    if (e != hipSuccess) return fail(INSCY_ERR_MEMORY, std::string("device allocation: ") + hipGetErrorString(e));
     e = hipMemcpy(dX, X, bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        (void) hipFree(dX);
+        dev_free(dX);
         return fail(INSCY_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
     }
     rc = run_impl(dX, n, d, params, mode, nullptr, out);
-    (void) hipFree(dX);
+    dev_free(dX);
     return rc;
 }
 
@@ -138,6 +231,12 @@ int inscy_result_stats(const inscy_result *res, inscy_stats *out) {
 void inscy_result_free(inscy_result *res) { delete res; }
 
 const char *inscy_last_error(void) { return g_error.c_str(); }
+
+int inscy_release_memory(void) {
+    device_cache().release(-1);
+    host_cache().release(-1);
+    return INSCY_OK;
+}
 
 int inscy_profile(int on) {
     g_profile_on = on != 0;

@@ -73,6 +73,37 @@ def run(X, neighborhood_size, F, num_obj, min_size, r=1.0, number_of_cells=3, re
     return _lib.run_host(np.asarray(X, dtype=np.float32), params, mode)
 
 
+def run_many(datasets, mode=_lib.MODE_BATCHED):
+    """Runs INSCY on many independent data sets at once; returns a list of
+    Results in input order.
+
+    datasets: iterable of (X, neighborhood_size, F, num_obj, min_size, r,
+    number_of_cells, rectangular, entropy_order) tuples.  All X must be CUDA
+    tensors on one device, or all host arrays.
+    """
+    import torch
+    xs, shapes, params = [], [], []
+    for (X, eps, F, num_obj, min_size, r, cells, rect, order) in datasets:
+        if isinstance(X, torch.Tensor):
+            X = X.detach().to(torch.float32).contiguous()
+        else:
+            X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.ndim != 2:
+            raise ValueError("every X must be [n, d]")
+        xs.append(X)
+        shapes.append(tuple(X.shape))
+        params.append(_lib.make_params(eps, F, num_obj, min_size, r, cells, rect, order))
+    on_device = [isinstance(X, torch.Tensor) and X.is_cuda for X in xs]
+    if any(on_device) and not all(on_device):
+        raise ValueError("mix of device and host inputs")
+    if all(on_device):
+        with torch.cuda.device(xs[0].device):
+            torch.cuda.current_stream().synchronize()
+            return _lib.run_many([X.data_ptr() for X in xs], shapes, params, mode, device=True)
+    hosts = [X.numpy() if isinstance(X, torch.Tensor) else X for X in xs]
+    return _lib.run_many([h.ctypes.data for h in hosts], shapes, params, mode, device=False)
+
+
 def _lists(res: _lib.Result):
     return [list(s) for s in res.subspaces], [row.tolist() for row in res.labels]
 

@@ -91,6 +91,20 @@ int inscy_run(const float *X, int64_t n, int32_t d, const inscy_params *params, 
 int inscy_run_device(const float *d_X, int64_t n, int32_t d, const inscy_params *params, int32_t mode,
                      void *stream, inscy_result **out);
 
+/* Runs INSCY on `count` independent data sets (config 3 of the measured
+ * workloads): X[i] is a row-major float32 [n[i], d[i]] host array, params[i]
+ * its parameters, out[i] receives its result.  Worker threads with one stream
+ * each (4, or the INSCY_WORKERS environment variable) take the sets in order
+ * of estimated cost n^2 d, largest first.  On an error no result is returned
+ * and the message names the data set.  The reference has no counterpart: its
+ * harness calls one op per data set in a loop (experiments/run.py:122-175). */
+int inscy_run_many(const float *const *X, const int64_t *n, const int32_t *d, const inscy_params *params,
+                   int32_t count, int32_t mode, inscy_result **out);
+
+/* Same with every X[i] already in device memory on the current device. */
+int inscy_run_many_device(const float *const *d_X, const int64_t *n, const int32_t *d, const inscy_params *params,
+                          int32_t count, int32_t mode, inscy_result **out);
+
 /* Number of subspaces in the result, in the reference's output order
  * (ascending dimension bitmask, util.cu:633-653 vec_cmp). */
 int32_t inscy_result_num_subspaces(const inscy_result *res);
@@ -117,6 +131,12 @@ void inscy_result_free(inscy_result *res);
  * the neighbour-count kernel with HIP events (no counterpart in the
  * reference; used by the benchmark's roofline figures). */
 int inscy_profile(int on);
+
+/* Device and pinned host blocks of finished runs are cached for reuse by
+ * later runs (freeing them would synchronise the whole device); this returns
+ * them to the runtime.  Call it only while no run is in progress.  No
+ * counterpart in the reference, which frees everything inside each call. */
+int inscy_release_memory(void);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *inscy_last_error(void);

@@ -294,6 +294,42 @@ def test_batched_config1_matches_golden():
     assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
 
 
+def test_run_many_equals_single_runs():
+    # independent data sets on concurrent streams: every result equals the
+    # single run of its data set (both density modes, both drivers)
+    sets = datagen.multi_instance(count=10)
+    jobs = []
+    for i, w in enumerate(sets):
+        rect = i % 3 != 2
+        jobs.append((torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, rect, 0))
+    say("run_many", len(jobs))
+    for mode in (_lib.MODE_BATCHED, _lib.MODE_EXACT):
+        many = INSCY.run_many(jobs, mode=mode)
+        assert len(many) == len(jobs)
+        for job, got in zip(jobs, many):
+            one = INSCY.run(*job, mode=mode)
+            assert got.subspaces == one.subspaces
+            assert np.array_equal(got.labels, one.labels)
+            assert got.stats["trees"] == one.stats["trees"]
+
+
+def test_run_many_host_input_and_errors():
+    say("run_many host")
+    sets = datagen.multi_instance(count=3)
+    jobs = [(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0) for w in sets]
+    many = INSCY.run_many(jobs)
+    for job, got in zip(jobs, many):
+        one = INSCY.run(*job, mode=_lib.MODE_BATCHED)
+        assert got.subspaces == one.subspaces and np.array_equal(got.labels, one.labels)
+    bad = list(jobs)
+    X = np.array(bad[1][0], copy=True)
+    X[3, 0] = 2.0
+    bad[1] = (X,) + bad[1][1:]
+    with pytest.raises(_lib.InscyError) as e:
+        INSCY.run_many(bad)
+    assert e.value.code == _lib.INSCY_ERR_ARGUMENT and "data set 1" in str(e.value)
+
+
 def test_out_of_range_input_is_an_error():
     say("out of range")
     X = np.full((50, 3), 0.5, dtype=np.float32)

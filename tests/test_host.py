@@ -52,6 +52,24 @@ def test_argument_errors_are_codes_not_crashes():
         _lib.check(_lib.INSCY_ERR_ARGUMENT)
 
 
+def test_run_many_rejects_bad_arguments_before_any_device_call():
+    L = _lib.lib()
+    X = np.zeros((4, 2), dtype=np.float32)
+    p = _lib.make_params(0.1, 1.0, 2, 1, 1.0, 3, True, 0)
+    ptrs = (ctypes.c_void_p * 1)(X.ctypes.data)
+    n = (ctypes.c_int64 * 1)(4)
+    d = (ctypes.c_int32 * 1)(2)
+    ps = (_lib.Params * 1)(p)
+    out = (ctypes.c_void_p * 1)()
+    assert L.inscy_run_many(ptrs, n, d, ps, 0, 2, out) == _lib.INSCY_ERR_ARGUMENT
+    assert L.inscy_run_many(None, n, d, ps, 1, 2, out) == _lib.INSCY_ERR_ARGUMENT
+    d0 = (ctypes.c_int32 * 1)(65)
+    assert L.inscy_run_many(ptrs, n, d0, ps, 1, 2, out) == _lib.INSCY_ERR_ARGUMENT
+    assert b"data set 0" in L.inscy_last_error()
+    ent = (_lib.Params * 1)(_lib.make_params(0.1, 1.0, 2, 1, 1.0, 3, True, 1))
+    assert L.inscy_run_many(ptrs, n, d, ent, 1, 2, out) == _lib.INSCY_ERR_UNSUPPORTED
+
+
 def test_result_accessors_tolerate_null():
     L = _lib.lib()
     assert L.inscy_result_num_subspaces(None) == 0
