@@ -330,6 +330,18 @@ def test_run_many_host_input_and_errors():
     assert e.value.code == _lib.INSCY_ERR_ARGUMENT and "data set 1" in str(e.value)
 
 
+def test_phi_density_config1_matches_golden():
+    import json
+    say("phi config1 golden")
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "config1_phi.json")))
+    w = datagen.config(1)
+    res = INSCY.run(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, False, 0,
+                    mode=_lib.MODE_BATCHED)
+    assert res.subspaces == [s for s, _, _, _ in g["subspaces"]]
+    assert digests(res) == [x[3] for x in g["subspaces"]]
+    assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
+
+
 def test_out_of_range_input_is_an_error():
     say("out of range")
     X = np.full((50, 3), 0.5, dtype=np.float32)
