@@ -50,6 +50,8 @@ def parse():
                     help="untimed runs of the depth-first driver for the comparison (0: skip)")
     ap.add_argument("--config", type=int, default=1,
                     help="workload index (1: n=25000 d=15; 2: n=100000 d=20; 3: 64 data sets)")
+    ap.add_argument("--real", choices=["glass", "vowel", "pendigits", "sky_0.5x0.5", "sky_1x1", "sky_1x2"],
+                    help="one of the reference's real-data workloads instead of --config")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
@@ -97,6 +99,8 @@ def main():
     mode = _lib.MODE_EXACT if args.mode == "exact" else _lib.MODE_BATCHED
     if args.config == 3:
         return bench_many(args, mode)
+    if args.real:
+        return bench_real(args, mode)
     w = datagen.config(args.config)
     X = torch.from_numpy(w.X).to("cuda:0")
     torch.cuda.synchronize()
@@ -200,6 +204,83 @@ def main():
         "parity": parity,
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+# The reference's published end-to-end seconds on its real-data workloads
+# (experiments/plot_real.py:13-14): INSCY CPU and GPU-INSCY (INSCY.GPU_memory),
+# hardware not stated.
+PUBLISHED_REAL = {
+    "glass": (2.9117519855499268, 0.061475515365600586),
+    "vowel": (36.15232849121094, 0.03729987144470215),
+    "pendigits": (31439.810033162434, 3.0369489987691245),
+    "sky_0.5x0.5": (77354.21990505855, 4.891563653945923),
+    "sky_1x1": (None, 62.529128074645996),
+    "sky_1x2": (None, 283.4412166277568),
+}
+
+
+def bench_real(args, mode):
+    """One of the reference's real-data workloads: end-to-end seconds per run
+    (lower is better) next to the reference's published seconds."""
+    import numpy as np
+    import torch
+
+    from gpu_inscy_amd import _lib, datagen
+    from gpu_inscy_amd import inscy as INSCY
+
+    w = datagen.real(args.real, os.path.join(ROOT, "tests", "golden", "real", "real_inputs.npz"))
+    X = torch.from_numpy(w.X).to("cuda:0")
+
+    def step():
+        return INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, w.entropy_order,
+                         mode=mode)
+
+    for _ in range(max(args.warmup, 1)):
+        res = step()
+    parity = None
+    gpath = os.path.join(ROOT, "tests", "golden", "real", "real_golden.json")
+    if os.path.exists(gpath):
+        g = json.load(open(gpath)).get(args.real)
+        if g:
+            dig = [hashlib.sha256(np.ascontiguousarray(l, dtype=np.int32).tobytes()).hexdigest() for l in res.labels]
+            parity = ([s for s, _, _, _ in g["subspaces"]] == res.subspaces and
+                      dig == [x[3] for x in g["subspaces"]] and res.stats["trees"] == g["trees"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    runs = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    cpu_pub, gpu_pub = PUBLISHED_REAL[args.real]
+    out = {
+        "metric": "end-to-end INSCY seconds (reference real-data workload)",
+        "value": round(sec, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": round(sec * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": round(sec / gpu_pub, 6),
+        "dtype": "f32",
+        "data": "real (reference's data/real inputs)",
+        "config": {
+            "workload": f"{w.name}: n={w.n} d={w.d}, eps={w.eps} c={w.cells} F={w.F} num_obj={w.num_obj} "
+                        f"min_size={w.min_size} r={w.r} rectangular=True",
+            "mode": args.mode,
+            "published_gpu_inscy_seconds": gpu_pub,
+            "published_cpu_inscy_seconds": cpu_pub,
+            "speedup_vs_published_gpu": round(gpu_pub / sec, 2),
+            "speedup_vs_published_cpu": round(cpu_pub / sec, 1) if cpu_pub else None,
+            "trees_per_step": runs[-1].stats["trees"],
+            "subspaces": runs[-1].stats["subspaces"],
+            "clusters": runs[-1].stats["clusters"],
+        },
+        "parity": parity,
+        "roofline": None,
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 

@@ -922,6 +922,7 @@ private:
     std::vector<int> level, lvl_of; // SCY-tree level -> dimension, dimension -> level
     int *d_level = nullptr;         // both, on the device ([level | lvl_of])
     bool batch_trace = std::getenv("INSCY_TRACE_BATCHES") != nullptr;  // one stderr line per batch
+    long long pair_stats[4] = {0, 0, 0, 0};
     std::vector<int> lv_act, lv_next, lv_gidoff;
     std::vector<std::vector<std::vector<Group>>> lv_groups;
     int *lv_gid = nullptr;

@@ -126,6 +126,10 @@ void Engine::run_levels(inscy_result &out) {
         trace_tree(0, keep);
     }
     auto t3 = std::chrono::steady_clock::now();
+    if (batch_trace)
+        std::fprintf(stderr, "[inscy pairs] %lld (tree, dimension) pairs, %lld with a surviving child; "
+                             "points counted %lld, in pairs with a surviving child %lld\n",
+                     pair_stats[0], pair_stats[1], pair_stats[2], pair_stats[3]);
     emit(out);
     stats.expand_seconds = std::chrono::duration<double>(t1 - t0).count();
     stats.replay_seconds = std::chrono::duration<double>(t2 - t1).count();
@@ -418,6 +422,14 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         const int tparent = lv_act[pi.a];
         const int first = lt[tparent].first;
         const int G = pi.G;
+        if (batch_trace) {
+            bool any = false;
+            for (int g = 0; g < G; ++g) any |= hcnt[pi.cnt_off + G + g] >= prm.min_size;
+            pair_stats[0] += 1;
+            pair_stats[1] += any;
+            pair_stats[2] += lt[tparent].nW;
+            pair_stats[3] += any ? lt[tparent].nW : 0;
+        }
         const int *c3 = hcnt.data() + pi.cnt_off;
         int offW = 0, offS = 0;
         for (int g = 0; g < G; ++g) {

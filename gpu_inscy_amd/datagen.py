@@ -145,6 +145,33 @@ def multi_instance(seed: int = 3000, count: int = 64):
     return out
 
 
+# The reference's real-data workloads (experiments/run_real.py:15-66): data set
+# -> (arrays in the inputs file, columns kept after normalisation).  Parameters
+# are run_real.py's: eps=0.01, c=4, F=1, num_obj=8, r=1, min_size=0.05 n,
+# rectangular, identity dimension order.
+REAL = {
+    "glass": (["glass"], slice(1, -1)),
+    "vowel": (["vowel"], slice(None)),
+    "pendigits": (["pendigits"], slice(None, -1)),
+    "sky_0.5x0.5": (["sky_tile0"], slice(None)),
+    "sky_1x1": ([f"sky_tile{i}" for i in range(4)], slice(None)),
+    "sky_1x2": ([f"sky_tile{i}" for i in range(8)], slice(None)),
+}
+
+
+def real(name: str, inputs_path: str) -> Workload:
+    """One of the reference's real-data workloads from the committed inputs
+    file (tests/golden/real/real_inputs.npz): concatenated, normalised per
+    column as python/inscy.py:37-42, columns selected as run_real.py."""
+    keys, cols = REAL[name]
+    with np.load(inputs_path, allow_pickle=False) as z:
+        raw = np.concatenate([z[k] for k in keys], axis=0).astype(np.float32)
+    X = np.ascontiguousarray(normalize(raw)[:, cols], dtype=np.float32)
+    n = X.shape[0]
+    return Workload(f"real_{name}", X, np.full(n, -1, dtype=np.int32), [], eps=0.01, cells=4, F=1.0, num_obj=8,
+                    min_size=int(n * 0.05))
+
+
 def small(seed: int, n: int = 200, d: int = 4, n_clusters: int = 2, eps: float = 0.05, cells: int = 3,
           num_obj: int = 3, min_size: int = 10, rectangular: bool = True, noise_frac: float = 0.2,
           sigma: float = 0.05, sub_dims=(2, 3)) -> Workload:

@@ -342,6 +342,34 @@ def test_phi_density_config1_matches_golden():
     assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
 
 
+REAL_DIR = os.path.join(os.path.dirname(__file__), "golden", "real")
+
+
+def _real_golden():
+    import json
+    path = os.path.join(REAL_DIR, "real_golden.json")
+    return json.load(open(path)) if os.path.exists(path) else {}
+
+
+@pytest.mark.parametrize("name", ["glass", "vowel", "pendigits", "sky_0.5x0.5", "sky_1x1", "sky_1x2"])
+def test_real_data_matches_golden(name):
+    # the reference's published real-data workloads (experiments/run_real.py)
+    g = _real_golden().get(name)
+    if g is None:
+        pytest.skip(f"no golden result for {name}")
+    say("real", name)
+    w = datagen.real(name, os.path.join(REAL_DIR, "real_inputs.npz"))
+    res = INSCY.run(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0,
+                    mode=_lib.MODE_BATCHED)
+    assert res.subspaces == [s for s, _, _, _ in g["subspaces"]]
+    assert digests(res) == [x[3] for x in g["subspaces"]]
+    assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
+    if w.n <= 1000:
+        ex = INSCY.run(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0,
+                       mode=_lib.MODE_EXACT)
+        assert ex.subspaces == res.subspaces and np.array_equal(ex.labels, res.labels)
+
+
 def test_out_of_range_input_is_an_error():
     say("out of range")
     X = np.full((50, 3), 0.5, dtype=np.float32)

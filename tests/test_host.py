@@ -90,6 +90,19 @@ def test_generator_is_deterministic_and_normalised():
     assert not np.array_equal(a.X, c.X)
 
 
+def test_real_data_loader_follows_run_real():
+    # experiments/run_real.py: glass drops its id and class columns,
+    # pendigits its class column, vowel keeps all; values normalised to [0, 1]
+    path = os.path.join(ROOT, "tests", "golden", "real", "real_inputs.npz")
+    shapes = {"glass": (214, 9), "vowel": (990, 10), "pendigits": (7494, 16), "sky_0.5x0.5": (7253, 17),
+              "sky_1x1": (7253 + 7598 + 7329 + 7447, 17)}
+    for name, shape in shapes.items():
+        w = datagen.real(name, path)
+        assert w.X.shape == shape and w.X.dtype == np.float32
+        assert w.X.min() == 0.0 and w.X.max() == 1.0
+        assert w.min_size == int(0.05 * shape[0]) and w.cells == 4 and w.num_obj == 8 and w.eps == 0.01
+
+
 def test_normalize_matches_reference_formula():
     rng = np.random.default_rng(0)
     x = rng.normal(size=(50, 3)).astype(np.float32)
