@@ -45,7 +45,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", choices=["exact", "batched"], default="batched")
+    ap.add_argument("--mode", choices=["exact", "batched", "learned"], default="batched",
+                    help="learned: the learned pruner (gpu_inscy_amd/pruner_weights.npz) against the exact run")
     ap.add_argument("--depth-first-steps", type=int, default=2,
                     help="untimed runs of the depth-first driver for the comparison (0: skip)")
     ap.add_argument("--config", type=int, default=1,
@@ -96,6 +97,8 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.mode == "learned":
+        return bench_learned(args)
     mode = _lib.MODE_EXACT if args.mode == "exact" else _lib.MODE_BATCHED
     if args.config == 3:
         return bench_many(args, mode)
@@ -279,6 +282,80 @@ def bench_real(args, mode):
             "clusters": runs[-1].stats["clusters"],
         },
         "parity": parity,
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_learned(args):
+    """Learned-prune mode against the exact (level-batched) run of the same
+    workload: end-to-end seconds per learned run (lower is better), the exact
+    run's seconds, and the cluster F1 of the learned result against the exact
+    one (gpu_inscy_amd.pruner.cluster_f1)."""
+    import torch
+
+    from gpu_inscy_amd import _lib, datagen, pruner
+    from gpu_inscy_amd import inscy as INSCY
+
+    w = datagen.config(args.config)
+    X = torch.from_numpy(w.X).to("cuda:0")
+    _, thr, min_depth, meta = pruner.load()
+    pruner.use()
+
+    def run(mode):
+        return INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, w.entropy_order,
+                         mode=mode)
+
+    for _ in range(max(args.warmup, 1)):
+        learned = run(_lib.MODE_LEARNED)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    exact = run(_lib.MODE_BATCHED)
+    torch.cuda.synchronize()
+    exact_sec = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    runs = [run(_lib.MODE_LEARNED) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / args.steps
+    learned = runs[-1]
+    f1 = pruner.cluster_f1(exact, learned)
+    st = learned.stats
+    out = {
+        "metric": "end-to-end INSCY seconds + subspaces/s; exact vs batched vs learned-prune (F1)",
+        "value": round(sec, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": max(args.warmup, 1),
+        "ms_per_step": round(sec * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (network: bf16 weights, f32 accumulation)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{w.name}: n={w.n} d={w.d}, eps={w.eps} c={w.cells} F={w.F} num_obj={w.num_obj} "
+                        f"min_size={w.min_size} r={w.r} rectangular=True",
+            "mode": "learned",
+            "exact_batched_seconds": round(exact_sec, 4),
+            "speedup_vs_exact_batched": round(exact_sec / sec, 3),
+            "cluster_f1": round(f1["f1"], 5),
+            "cluster_recall": round(f1["recall"], 5),
+            "cluster_precision": round(f1["precision"], 5),
+            "clusters_exact": f1["clusters_exact"],
+            "clusters_learned": f1["clusters_approx"],
+            "subspaces_exact": len(exact.subspaces),
+            "subspaces_learned": len(learned.subspaces),
+            "trees_exact": exact.stats["trees"],
+            "trees_learned": st["trees"],
+            "pairs_scored": st["pairs_scored"],
+            "pairs_skipped": st["pairs_skipped"],
+            "threshold": thr,
+            "min_depth": min_depth,
+            "trained_on": meta.get("generator"),
+        },
+        "parity": None,
         "roofline": None,
         "cpu_baseline": None,
     }

@@ -22,12 +22,20 @@ INSCY_ERR_UNSUPPORTED = 4
 
 MODE_EXACT = 1
 MODE_BATCHED = 2
+MODE_LEARNED = 3
+MODE_COLLECT = 4
+
+MLP_INPUTS = 64
+MLP_HIDDEN = 128
+MLP_PARAMS = 128 * 64 + 128 + 128 * 128 + 128 + 128 + 1
 
 EXPORTED = [
     "inscy_result_labels_all", "inscy_run_many", "inscy_run_many_device", "inscy_release_memory",
     "inscy_run", "inscy_run_device", "inscy_result_num_subspaces", "inscy_result_subspace",
     "inscy_result_labels", "inscy_result_num_points", "inscy_result_stats", "inscy_result_free",
     "inscy_last_error", "inscy_discretise", "inscy_neighbour_counts", "inscy_dense_components", "inscy_profile",
+    "inscy_set_pruner", "inscy_result_pairs", "inscy_mlp_forward", "inscy_mlp_gradient", "inscy_mlp_adam",
+    "inscy_mlp_round",
 ]
 
 
@@ -69,6 +77,8 @@ class Stats(ctypes.Structure):
         ("expand_seconds", ctypes.c_double),
         ("replay_seconds", ctypes.c_double),
         ("emit_seconds", ctypes.c_double),
+        ("pairs_scored", ctypes.c_int64),
+        ("pairs_skipped", ctypes.c_int64),
     ]
 
 
@@ -124,6 +134,20 @@ def lib():
     L.inscy_release_memory.restype = ctypes.c_int
     L.inscy_profile.argtypes = [ctypes.c_int]
     L.inscy_profile.restype = ctypes.c_int
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.inscy_set_pruner.argtypes = [fp, ctypes.c_float, ctypes.c_int32]
+    L.inscy_set_pruner.restype = ctypes.c_int
+    L.inscy_result_pairs.argtypes = [vp, ctypes.POINTER(fp), ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)),
+                                     ctypes.POINTER(i32p), ctypes.POINTER(ctypes.POINTER(ctypes.c_int64))]
+    L.inscy_result_pairs.restype = ctypes.c_int64
+    L.inscy_mlp_forward.argtypes = [vp, vp, ctypes.c_int64, vp, vp]
+    L.inscy_mlp_forward.restype = ctypes.c_int
+    L.inscy_mlp_gradient.argtypes = [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_float, vp, ctypes.c_int32, vp, vp]
+    L.inscy_mlp_gradient.restype = ctypes.c_int
+    L.inscy_mlp_adam.argtypes = [vp, vp, vp, vp, vp, ctypes.c_int32, ctypes.c_float, vp]
+    L.inscy_mlp_adam.restype = ctypes.c_int
+    L.inscy_mlp_round.argtypes = [vp, vp, vp]
+    L.inscy_mlp_round.restype = ctypes.c_int
     L.inscy_debug_trace.argtypes = [ctypes.c_int]
     L.inscy_debug_phi_exact.argtypes = [ctypes.c_int]
     L.inscy_debug_trace_rows.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]
@@ -181,6 +205,44 @@ class Result:
         st = Stats()
         check(L.inscy_result_stats(handle, ctypes.byref(st)))
         self.stats = {f: getattr(st, f) for f, _ in Stats._fields_}
+        self.pairs = _pairs(handle, owner)
+
+
+def _pairs(handle, owner):
+    """Training pairs of a collect-mode run (None otherwise): features
+    [P, 64] float32, labels [P] uint8, info [P, 4] int32 (|S| of the tree, m,
+    tree points, tree index), cost [P, 2] int64."""
+    L = lib()
+    f = ctypes.POINTER(ctypes.c_float)()
+    y = ctypes.POINTER(ctypes.c_uint8)()
+    info = ctypes.POINTER(ctypes.c_int32)()
+    cost = ctypes.POINTER(ctypes.c_int64)()
+    P = int(L.inscy_result_pairs(handle, ctypes.byref(f), ctypes.byref(y), ctypes.byref(info), ctypes.byref(cost)))
+    if P == 0:
+        return None
+
+    def view(ptr, ctype, count, dtype, shape):
+        buf = (ctype * count).from_address(ctypes.addressof(ptr.contents))
+        buf._owner = owner
+        return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    return {"features": view(f, ctypes.c_float, P * MLP_INPUTS, np.float32, (P, MLP_INPUTS)),
+            "labels": view(y, ctypes.c_uint8, P, np.uint8, (P,)),
+            "info": view(info, ctypes.c_int32, 4 * P, np.int32, (P, 4)),
+            "cost": view(cost, ctypes.c_int64, 2 * P, np.int64, (P, 2))}
+
+
+def set_pruner(weights, threshold: float, min_depth: int = 1) -> None:
+    """Loads the learned pruner's network (MLP_PARAMS float32 values) for
+    later learned-mode runs; weights=None unloads it."""
+    L = lib()
+    if weights is None:
+        check(L.inscy_set_pruner(None, 0.0, 0))
+        return
+    w = np.ascontiguousarray(weights, dtype=np.float32).ravel()
+    if w.size != MLP_PARAMS:
+        raise ValueError(f"the network has {MLP_PARAMS} parameters, got {w.size}")
+    check(L.inscy_set_pruner(w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), float(threshold), int(min_depth)))
 
 
 def run_host(X: np.ndarray, params: Params, mode: int = MODE_EXACT) -> Result:

@@ -19,7 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libinscy_hip.so")
-SOURCES = ["inscy_kernels.hip", "inscy_level_kernels.hip", "inscy_level.hip", "inscy_engine.hip"]
+SOURCES = ["inscy_kernels.hip", "inscy_level_kernels.hip", "inscy_pruner.hip", "inscy_mlp.hip", "inscy_level.hip",
+           "inscy_engine.hip"]
 ARCH = os.environ.get("INSCY_OFFLOAD_ARCH", "gfx950")
 
 
@@ -44,13 +45,17 @@ def build(verbose: bool = False) -> str:
     objs = []
     build_dir = os.path.join(HERE, "_build")
     os.makedirs(build_dir, exist_ok=True)
-    for src in SOURCES:
+    procs = []
+    for src in SOURCES:  # the translation units compile concurrently
         obj = os.path.join(build_dir, src.replace(".hip", ".o"))
         cmd = [cc, *flags(), "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
+        procs.append((cmd, subprocess.Popen(cmd)))
         objs.append(obj)
+    for cmd, proc in procs:
+        if proc.wait() != 0:
+            raise subprocess.CalledProcessError(proc.returncode, cmd)
     tmp = LIB + ".tmp"
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

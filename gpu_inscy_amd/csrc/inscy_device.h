@@ -23,6 +23,20 @@ __device__ __forceinline__ void st_relaxed(int *p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// segment owning block `blk`: the last s with blk_off[s] <= blk (empty
+// segments share an offset with their successor and are skipped)
+__device__ __forceinline__ int seg_of(const int *__restrict__ blk_off, int nseg, int blk) {
+    int lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (blk_off[mid] <= blk)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 // Wave-aggregated slot allocation for order-free stream compaction: one
 // atomic per wave.  Must be reached by every active lane of the wave.
 __device__ __forceinline__ int wave_slot(bool keep, int *counter) {

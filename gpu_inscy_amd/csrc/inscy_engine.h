@@ -61,6 +61,11 @@ struct inscy_result {
     std::vector<std::vector<int32_t>> subspaces;
     std::unique_ptr<int32_t[], HostFree> labels;  // [subspaces][n], canonical
     inscy_stats stats{};
+    // INSCY_MODE_COLLECT: the learned pruner's training pairs
+    std::vector<float> pair_feat;      // [pairs][kMlpIn]
+    std::vector<uint8_t> pair_label;   // [pairs]
+    std::vector<int32_t> pair_info;    // [pairs][4]: |S| of the tree, m, tree points, tree index
+    std::vector<int64_t> pair_cost;    // [pairs][2]: own points counted, with every pair below
 };
 
 namespace inscy {
@@ -380,6 +385,13 @@ public:
     // level-synchronous driver (INSCY_MODE_BATCHED), same result
     void run_levels(inscy_result &out);
     bool levels_mode = false;
+    // learned pruner (INSCY_MODE_LEARNED): device bf16 parameters, the score
+    // below which a (tree, dimension) pair is skipped, and the smallest |S| of
+    // a tree whose pairs may be skipped; pair collection (INSCY_MODE_COLLECT)
+    const uint16_t *pruner = nullptr;
+    float prune_thr = 0.f;
+    int prune_min_depth = 1;
+    bool collect = false;
     bool phi_all_exact = false;  // test hook: every phi by the leaf-order recomputation
 
     void run(inscy_result &out) {
@@ -922,7 +934,7 @@ private:
     std::vector<int> level, lvl_of; // SCY-tree level -> dimension, dimension -> level
     int *d_level = nullptr;         // both, on the device ([level | lvl_of])
     bool batch_trace = std::getenv("INSCY_TRACE_BATCHES") != nullptr;  // one stderr line per batch
-    long long pair_stats[4] = {0, 0, 0, 0};
+    long long pair_stats[6] = {0, 0, 0, 0, 0, 0};
     std::vector<int> lv_act, lv_next, lv_gidoff;
     std::vector<std::vector<std::vector<Group>>> lv_groups;
     int *lv_gid = nullptr;
@@ -948,6 +960,18 @@ private:
     void set_bounds(uint64_t s, BatchSub &x);
     void trace_tree(int f, const std::vector<int> &keep);
     std::vector<KernelTimer> timers;
+    // pairs of the level: first pair of each active tree, pruner decisions
+    std::vector<int> lv_pair0;
+    std::vector<char> lv_skip;
+    bool pair_on(int a, int m) const {
+        return lv_skip.empty() || !lv_skip[lv_pair0[a] + m - lt[lv_act[a]].first];
+    }
+    void score_pairs(int na);
+    // collected pairs (whole run): features, tree, dimension, children
+    long long col_base = 0;
+    std::vector<float> col_feat;
+    std::vector<int> col_tree, col_m, col_child0, col_G;
+    void finish_collect(inscy_result &out);
 };
 
 }  // namespace inscy

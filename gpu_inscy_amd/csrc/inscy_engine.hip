@@ -2,6 +2,7 @@
 #include "inscy_engine.h"
 
 #include <atomic>
+#include <mutex>
 
 namespace inscy {
 
@@ -29,8 +30,49 @@ static int check_params(int64_t n, int32_t d, const inscy_params *p, int32_t mod
     // exceeds the restricted one); the engine's path model covers the
     // identity order only.
     if (p->entropy_order != 0) return fail(INSCY_ERR_UNSUPPORTED, "entropy_order != 0 is not implemented");
-    if (mode != INSCY_MODE_EXACT && mode != INSCY_MODE_BATCHED) return fail(INSCY_ERR_ARGUMENT, "unknown mode");
+    if (mode < INSCY_MODE_EXACT || mode > INSCY_MODE_COLLECT) return fail(INSCY_ERR_ARGUMENT, "unknown mode");
     return INSCY_OK;
+}
+
+// The learned pruner's network: bf16 weights on the host and one copy per
+// device, uploaded on first use after each inscy_set_pruner.  The copies are
+// held until the process ends.
+struct PrunerState {
+    std::mutex mu;
+    std::vector<uint16_t> w;  // empty: no network loaded
+    float thr = 0.f;
+    int min_depth = 1;
+    int version = 0;
+    std::map<int, std::pair<int, uint16_t *>> dev;  // device -> (version, parameters)
+};
+static PrunerState &pruner_state() {
+    static PrunerState *p = new PrunerState;
+    return *p;
+}
+
+// float32 -> bf16, round to nearest even (the device kernels round alike)
+static uint16_t bf16_rne(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return (uint16_t) ((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+
+// the current device's copy of the network (null when none is loaded)
+static const uint16_t *pruner_on_device(float &thr, int &min_depth) {
+    PrunerState &P = pruner_state();
+    std::lock_guard<std::mutex> g(P.mu);
+    if (P.w.empty()) return nullptr;
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    auto &e = P.dev[dev];
+    if (!e.second) HIP_OK(hipMalloc(&e.second, P.w.size() * sizeof(uint16_t)));
+    if (e.first != P.version) {
+        HIP_OK(hipMemcpy(e.second, P.w.data(), P.w.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+        e.first = P.version;
+    }
+    thr = P.thr;
+    min_depth = P.min_depth;
+    return e.second;
 }
 
 static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *p, int32_t mode, hipStream_t user,
@@ -50,8 +92,13 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
             eng.phi_all_exact = g_phi_all_exact;
             eng.profile_on = g_profile_on;
             // phi density (rectangular = false) is implemented in the
-            // level-synchronous driver only; it serves both modes
-            eng.levels_mode = mode == INSCY_MODE_BATCHED || p->rectangular == 0;
+            // level-synchronous driver only; it serves every mode
+            eng.levels_mode = mode != INSCY_MODE_EXACT || p->rectangular == 0;
+            eng.collect = mode == INSCY_MODE_COLLECT;
+            if (mode == INSCY_MODE_LEARNED) {
+                eng.pruner = pruner_on_device(eng.prune_thr, eng.prune_min_depth);
+                if (!eng.pruner) throw Failure(INSCY_ERR_ARGUMENT, "INSCY_MODE_LEARNED needs a network (inscy_set_pruner)");
+            }
             eng.setup(d_X);
             if (eng.levels_mode)
                 eng.run_levels(*res);
@@ -231,6 +278,69 @@ int inscy_result_stats(const inscy_result *res, inscy_stats *out) {
 void inscy_result_free(inscy_result *res) { delete res; }
 
 const char *inscy_last_error(void) { return g_error.c_str(); }
+
+int inscy_set_pruner(const float *weights, float threshold, int32_t min_depth) {
+    if (weights && (!std::isfinite(threshold) || min_depth < 0))
+        return fail(INSCY_ERR_ARGUMENT, "threshold must be finite and min_depth >= 0");
+    PrunerState &P = pruner_state();
+    std::lock_guard<std::mutex> g(P.mu);
+    if (!weights) {
+        P.w.clear();
+        ++P.version;
+        return INSCY_OK;
+    }
+    std::vector<uint16_t> w(kMlpParams);
+    for (int i = 0; i < kMlpParams; ++i) {
+        if (!std::isfinite(weights[i])) return fail(INSCY_ERR_ARGUMENT, "weights must be finite");
+        w[i] = bf16_rne(weights[i]);
+    }
+    P.w.swap(w);
+    P.thr = threshold;
+    P.min_depth = min_depth;
+    ++P.version;
+    return INSCY_OK;
+}
+
+int64_t inscy_result_pairs(const inscy_result *res, const float **features, const uint8_t **labels,
+                           const int32_t **info, const int64_t **cost) {
+    if (!res) return 0;
+    if (features) *features = res->pair_feat.data();
+    if (labels) *labels = res->pair_label.data();
+    if (info) *info = res->pair_info.data();
+    if (cost) *cost = res->pair_cost.data();
+    return (int64_t) res->pair_label.size();
+}
+
+int inscy_mlp_forward(const uint16_t *d_params, const float *d_features, int64_t count, float *d_prob, void *stream) {
+    if (!d_params || !d_features || !d_prob || count < 0) return fail(INSCY_ERR_ARGUMENT, "invalid argument");
+    hipError_t e = launch_mlp_predict(d_features, count, d_params, d_prob, (hipStream_t) stream);
+    return e == hipSuccess ? INSCY_OK : fail(INSCY_ERR_DEVICE, hipGetErrorString(e));
+}
+
+int inscy_mlp_gradient(const uint16_t *d_params, const float *d_features, const float *d_labels,
+                       const float *d_weights, int64_t count, float inv_weight_sum, float *d_partials,
+                       int32_t nblocks, float *d_grad, void *stream) {
+    if (!d_params || !d_features || !d_labels || !d_weights || !d_partials || !d_grad || count < 1 || nblocks < 1 ||
+        !std::isfinite(inv_weight_sum))
+        return fail(INSCY_ERR_ARGUMENT, "invalid argument");
+    hipError_t e = launch_mlp_grad(d_features, d_labels, d_weights, count, inv_weight_sum, d_params, d_partials,
+                                   nblocks, d_grad, (hipStream_t) stream);
+    return e == hipSuccess ? INSCY_OK : fail(INSCY_ERR_DEVICE, hipGetErrorString(e));
+}
+
+int inscy_mlp_adam(float *d_master, const float *d_grad, float *d_m, float *d_v, uint16_t *d_params, int32_t step,
+                   float lr, void *stream) {
+    if (!d_master || !d_grad || !d_m || !d_v || !d_params || step < 1 || !(lr > 0.0f))
+        return fail(INSCY_ERR_ARGUMENT, "invalid argument");
+    hipError_t e = launch_mlp_adam(d_master, d_grad, d_m, d_v, d_params, step, lr, (hipStream_t) stream);
+    return e == hipSuccess ? INSCY_OK : fail(INSCY_ERR_DEVICE, hipGetErrorString(e));
+}
+
+int inscy_mlp_round(const float *d_master, uint16_t *d_params, void *stream) {
+    if (!d_master || !d_params) return fail(INSCY_ERR_ARGUMENT, "invalid argument");
+    hipError_t e = launch_mlp_round(d_master, d_params, (hipStream_t) stream);
+    return e == hipSuccess ? INSCY_OK : fail(INSCY_ERR_DEVICE, hipGetErrorString(e));
+}
 
 int inscy_release_memory(void) {
     device_cache().release(-1);

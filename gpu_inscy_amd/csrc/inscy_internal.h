@@ -282,6 +282,47 @@ hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *kee
                            const int *csizeS, int min_size, float r, int n, Entry *E, const int *list, int nlist,
                            int eS, uint8_t *inresS, uint8_t *keepf, uint8_t *drop, uint8_t *fresh, hipStream_t st);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t count, hipStream_t st);
+
+// ---- learned pruner
+constexpr int kMlpIn = 64, kMlpHidden = 128;
+constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpHidden * kMlpHidden + kMlpHidden + kMlpHidden + 1;
+constexpr int kHistBins = 16;  // value histogram of a tree's points in the candidate dimension
+
+// One (tree, dimension m) pair scored before its neighbour counts: the
+// tree's points W, the complete counts of its subspace S (null for the
+// root) and the cell -> merge group map of dimension m.
+struct FeatPair {
+    const int *W;
+    const int *cntS;  // counts of S, capped at cap
+    const int *gid;   // [cells]
+    uint64_t mask;    // S | {m}
+    int nW, pre, m, k, first, ngroups, bp, cap;
+};
+// acc: per pair kHistBins + cells + 2 ints (value histogram, cell histogram,
+// points with |N_S| >= cap and >= (weak_thr + cap) / 2); accs: per pair the
+// sum of |N_S| (u64).  feat: [npairs][kMlpIn].
+hipError_t launch_pair_features(const FeatPair *pairs, const int *blk_off, int npairs, int nblk, const float *Xt,
+                                const uint8_t *cellC, int n, int d, int cells, int min_size, int weak_thr, int *acc,
+                                unsigned long long *accs, float *feat, hipStream_t st);
+// network (bf16 parameters P, kMlpParams, layout W1 b1 W2 b2 w3 b3)
+hipError_t launch_mlp_predict(const float *X, long long N, const uint16_t *P, float *prob, hipStream_t st);
+// gradient of the weighted mean binary cross-entropy: part holds nblocks rows
+// of kMlpParams + 1 floats, grad receives kMlpParams + 1 (the last the loss)
+hipError_t launch_mlp_grad(const float *X, const float *Y, const float *WT, long long N, float inv_wsum,
+                           const uint16_t *P, float *part, int nblocks, float *grad, hipStream_t st);
+hipError_t launch_mlp_adam(float *master, const float *grad, float *m, float *v, uint16_t *P, int step, float lr,
+                           hipStream_t st);
+hipError_t launch_mlp_round(const float *master, uint16_t *P, hipStream_t st);
+
+// hit[t] |= 1 when a point of tree t's list lies in a kept cluster of its subspace
+struct TreeHit {
+    const int *W;
+    const int *comp;
+    const uint8_t *inres;
+    int nW, out;
+};
+hipError_t launch_tree_hits(const TreeHit *trees, const int *blk_off, int ntrees, int nblk, int *hit, hipStream_t st);
+
 constexpr int kRangeWords = 20;      // per query in LDS: 9 range starts, 10 prefix sums, count
 constexpr int kBlockThreads = 256;   // threads per block of every pass
 constexpr int kQueriesPerBlock = 16;  // queries per block of the neighbour passes

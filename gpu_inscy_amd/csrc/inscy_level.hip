@@ -41,6 +41,12 @@ void Engine::put(void *dst, const void *src, size_t bytes) {
 void Engine::run_levels(inscy_result &out) {
     lt.clear();
     emask.clear();
+    col_base = 0;
+    col_feat.clear();
+    col_tree.clear();
+    col_m.clear();
+    col_child0.clear();
+    col_G.clear();
     store.chunk = (size_t) 256 << 20;
     batch.chunk = (size_t) 256 << 20;
     up.get((size_t) 4 << 20);
@@ -116,6 +122,7 @@ void Engine::run_levels(inscy_result &out) {
     HIP_OK(launch_fill_i32(rep_words, 0, 2, st));
     HIP_OK(launch_fill_u8(keepf, 0, (size_t) 2 * n, st));  // keepf and dropf are adjacent
     replay_tree(0);
+    if (collect) finish_collect(out);
     sync();
     auto t2 = std::chrono::steady_clock::now();
     if (trace_on) {
@@ -128,8 +135,9 @@ void Engine::run_levels(inscy_result &out) {
     auto t3 = std::chrono::steady_clock::now();
     if (batch_trace)
         std::fprintf(stderr, "[inscy pairs] %lld (tree, dimension) pairs, %lld with a surviving child; "
-                             "points counted %lld, in pairs with a surviving child %lld\n",
-                     pair_stats[0], pair_stats[1], pair_stats[2], pair_stats[3]);
+                             "points counted %lld, in pairs with a surviving child %lld; pairs whose largest "
+                             "cell group holds fewer than min_size points %lld (points %lld)\n",
+                     pair_stats[0], pair_stats[1], pair_stats[2], pair_stats[3], pair_stats[4], pair_stats[5]);
     emit(out);
     stats.expand_seconds = std::chrono::duration<double>(t1 - t0).count();
     stats.replay_seconds = std::chrono::duration<double>(t2 - t1).count();
@@ -202,6 +210,8 @@ void Engine::expand_level(const std::vector<int> &frontier, const int *counts) {
         }
     }
     lv_gid = upload(cur, gid);
+    lv_skip.clear();
+    if (pruner || collect) score_pairs(na);
     lv_next_counts = nxt.take<int>(std::max<long long>(max_children, 1));
     HIP_OK(launch_fill_i32(lv_next_counts, 0, std::max<long long>(max_children, 1), st));
 
@@ -215,7 +225,11 @@ void Engine::expand_level(const std::vector<int> &frontier, const int *counts) {
         // the trees of one mask are created together, so each child subspace
         // is reached from exactly one run
         if (!seen.insert(t.smask).second) throw Failure(INSCY_ERR_DEVICE, "internal: frontier masks not grouped");
-        for (int m = t.first; m < d; ++m) cs.push_back({t.smask | (1ull << m), m, a, b});
+        for (int m = t.first; m < d; ++m) {
+            bool any = false;  // some pair of this child subspace is not skipped
+            for (int x = a; x < b && !any; ++x) any = pair_on(x, m);
+            if (any) cs.push_back({t.smask | (1ull << m), m, a, b});
+        }
         a = b;
     }
 
@@ -233,6 +247,7 @@ void Engine::expand_level(const std::vector<int> &frontier, const int *counts) {
             for (int q = 0; q < G.j; ++q) ncell *= G.C;
             size_t est = (size_t) n * (26 + 6 * (size_t) S.k) + (size_t) ncell * 8;
             for (int a = c.t0; a < c.t1; ++a) {
+                if (!pair_on(a, c.m)) continue;
                 const LTree &t = lt[lv_act[a]];
                 est += (size_t) t.nW * (8 + 32 * (size_t) (d - 1 - c.m)) + (size_t) t.nsp * 12 + 6 * 256 * 4;
             }
@@ -268,7 +283,8 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         long long ncell = 1;
         for (int q = 0; q < x.G.j; ++q) ncell *= x.G.C;
         long long qmax = 0;
-        for (int a = c.t0; a < c.t1; ++a) qmax += lt[lv_act[a]].nW;
+        for (int a = c.t0; a < c.t1; ++a)
+            if (pair_on(a, c.m)) qmax += lt[lv_act[a]].nW;
         x.base = (long long) b * n;
         x.coord = coord;
         x.hoff = hoff;
@@ -334,7 +350,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         const ChildSub &c = cs[i0 + b];
         for (int a = c.t0; a < c.t1; ++a) {
             const LTree &t = lt[lv_act[a]];
-            if (t.nW <= 0) continue;
+            if (t.nW <= 0 || !pair_on(a, c.m)) continue;
             qs.push_back({t.W, t.nW, b});
             qblk.push_back(qblk.back() + nblocks(t.nW));
         }
@@ -367,6 +383,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     for (int b = 0; b < B; ++b) {
         const ChildSub &c = cs[i0 + b];
         for (int a = c.t0; a < c.t1; ++a) {
+            if (!pair_on(a, c.m)) continue;
             const LTree &t = lt[lv_act[a]];
             const int G = (int) lv_groups[a][c.m - t.first].size();
             pinfo.push_back({a, b, G, pc_total});
@@ -423,12 +440,22 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         const int first = lt[tparent].first;
         const int G = pi.G;
         if (batch_trace) {
-            bool any = false;
-            for (int g = 0; g < G; ++g) any |= hcnt[pi.cnt_off + G + g] >= prm.min_size;
+            bool any = false, big = false;
+            for (int g = 0; g < G; ++g) {
+                any |= hcnt[pi.cnt_off + G + g] >= prm.min_size;
+                big |= hcnt[pi.cnt_off + g] >= prm.min_size;
+            }
             pair_stats[0] += 1;
             pair_stats[1] += any;
             pair_stats[2] += lt[tparent].nW;
             pair_stats[3] += any ? lt[tparent].nW : 0;
+            pair_stats[4] += big ? 0 : 1;
+            pair_stats[5] += big ? 0 : lt[tparent].nW;
+        }
+        if (collect) {
+            const long long gp = col_base + lv_pair0[pi.a] + (c.m - first);
+            col_child0[gp] = (int) lt.size();
+            col_G[gp] = G;
         }
         const int *c3 = hcnt.data() + pi.cnt_off;
         int offW = 0, offS = 0;
@@ -656,6 +683,159 @@ void Engine::replay_tree(int f) {
         }
         i = j;
     }
+}
+
+// Features of every (tree, dimension) pair of the level (learned and collect
+// modes) and, with a pruner loaded, its decisions: a pair whose score is below
+// the threshold is skipped, unless its tree has fewer than prune_min_depth
+// restricted dimensions.
+void Engine::score_pairs(int na) {
+    Pool &cur = lvl[cur_lvl];
+    lv_pair0.assign(na, 0);
+    std::vector<FeatPair> fp;
+    std::vector<int> blk{0};
+    for (int a = 0; a < na; ++a) {
+        const LTree &t = lt[lv_act[a]];
+        lv_pair0[a] = (int) fp.size();
+        // counts of S are complete for every subspace holding a surviving
+        // tree; in phi mode they are decision bits and are left out
+        const int *cntS = t.smask && !phi_mode ? subs.at(t.smask).cnt : nullptr;
+        const int k = __builtin_popcountll(t.smask);
+        for (int m = t.first; m < d; ++m) {
+            FeatPair f;
+            std::memset(&f, 0, sizeof(f));
+            f.W = t.W;
+            f.cntS = cntS;
+            f.gid = lv_gid + lv_gidoff[a] + (m - t.first) * cells;
+            f.mask = t.smask | (1ull << m);
+            f.nW = t.nW;
+            f.pre = t.pre;
+            f.m = m;
+            f.k = k;
+            f.first = t.first;
+            f.ngroups = (int) lv_groups[a][m - t.first].size();
+            f.bp = nblocks(t.nW);
+            f.cap = cap_of(k);
+            fp.push_back(f);
+            blk.push_back(blk.back() + f.bp);
+        }
+    }
+    const int np = (int) fp.size();
+    stats.pairs_scored += np;
+    if (np == 0) return;
+    const size_t stride = (size_t) kHistBins + cells + 2;
+    int *acc = cur.take<int>((size_t) np * stride);
+    unsigned long long *accs = cur.take<unsigned long long>(np);
+    float *feat = cur.take<float>((size_t) np * kMlpIn);
+    HIP_OK(launch_fill_i32(acc, 0, (size_t) np * stride, st));
+    HIP_OK(launch_fill_u64(reinterpret_cast<uint64_t *>(accs), 0ull, np, st));
+    FeatPair *dfp = upload(cur, fp);
+    int *dblk = upload(cur, blk);
+    HIP_OK(launch_pair_features(dfp, dblk, np, blk[np], D.Xt, cellC, n, d, cells, prm.min_size, weak_thr, acc, accs,
+                                feat, st));
+    if (collect) {
+        col_base = (long long) col_tree.size();
+        const size_t at = col_feat.size();
+        col_feat.resize(at + (size_t) np * kMlpIn);
+        float *h = reinterpret_cast<float *>(down.get((size_t) np * kMlpIn * sizeof(float)));
+        HIP_OK(hipMemcpyAsync(h, feat, (size_t) np * kMlpIn * sizeof(float), hipMemcpyDeviceToHost, st));
+        sync();
+        std::memcpy(col_feat.data() + at, h, (size_t) np * kMlpIn * sizeof(float));
+        for (int a = 0; a < na; ++a)
+            for (int m = lt[lv_act[a]].first; m < d; ++m) {
+                col_tree.push_back(lv_act[a]);
+                col_m.push_back(m);
+                col_child0.push_back(-1);
+                col_G.push_back(0);
+            }
+    }
+    if (pruner) {
+        float *prob = cur.take<float>(np);
+        HIP_OK(launch_mlp_predict(feat, np, pruner, prob, st));
+        const float *hp = reinterpret_cast<const float *>(down.get((size_t) np * sizeof(float)));
+        HIP_OK(hipMemcpyAsync(const_cast<float *>(hp), prob, (size_t) np * sizeof(float), hipMemcpyDeviceToHost, st));
+        sync();
+        lv_skip.assign(np, 0);
+        for (int a = 0; a < na; ++a) {
+            const LTree &t = lt[lv_act[a]];
+            if (__builtin_popcountll(t.smask) < prune_min_depth) continue;
+            for (int m = t.first; m < d; ++m) {
+                const int j = lv_pair0[a] + m - t.first;
+                if (hp[j] < prune_thr) {
+                    lv_skip[j] = 1;
+                    stats.pairs_skipped++;
+                }
+            }
+        }
+    }
+}
+
+// Labels of the collected pairs.  A surviving tree holds a result cluster
+// when it was kept by pruneRedundancy and one of its points lies in a cluster
+// of the result in its subspace; a pair is positive when a tree below it (a
+// child along its dimension, or any descendant of one) holds a result cluster.
+void Engine::finish_collect(inscy_result &out) {
+    const int nt = (int) lt.size();
+    std::vector<TreeHit> th;
+    std::vector<int> blk{0};
+    for (int i = 1; i < nt; ++i) {
+        const LTree &t = lt[i];
+        if (!t.pass || t.nW <= 0) continue;
+        const Subspace &sb = subs.at(t.smask);
+        th.push_back(TreeHit{t.W, sb.comp, sb.inres, t.nW, i});
+        blk.push_back(blk.back() + nblocks(t.nW));
+    }
+    int *hit = store.take<int>(std::max(nt, 1));
+    HIP_OK(launch_fill_i32(hit, 0, std::max(nt, 1), st));
+    if (!th.empty()) {
+        TreeHit *dth = upload(store, th);
+        int *dblk = upload(store, blk);
+        HIP_OK(launch_tree_hits(dth, dblk, (int) th.size(), blk.back(), hit, st));
+    }
+    std::vector<int> hh(nt), keep(nt);
+    HIP_OK(hipMemcpyAsync(hh.data(), hit, (size_t) nt * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(keep.data(), keepv, (size_t) nt * sizeof(int), hipMemcpyDeviceToHost, st));
+    sync();
+    const long long np = (long long) col_tree.size();
+    // pairs of each tree (contiguous in collection order)
+    std::vector<long long> p0(nt, -1), pn(nt, 0);
+    for (long long j = 0; j < np; ++j) {
+        if (p0[col_tree[j]] < 0) p0[col_tree[j]] = j;
+        pn[col_tree[j]]++;
+    }
+    // children have larger indices than their parents
+    std::vector<char> contrib(nt, 0);
+    std::vector<long long> tcost(nt, 0);
+    out.pair_label.assign(np, 0);
+    out.pair_cost.assign(2 * np, 0);
+    out.pair_info.assign(4 * np, 0);
+    for (int i = nt - 1; i >= 0; --i) {
+        const LTree &t = lt[i];
+        bool c = i > 0 && t.pass && keep[i] && hh[i];
+        long long tc = 0;
+        for (long long j = p0[i]; j >= 0 && j < p0[i] + pn[i]; ++j) {
+            bool pos = false;
+            long long cost = t.nW;
+            for (int u = 0; u < col_G[j]; ++u) {
+                const int ch = col_child0[j] + u;
+                pos |= contrib[ch] != 0;
+                cost += tcost[ch];
+            }
+            out.pair_label[j] = pos ? 1 : 0;
+            out.pair_cost[2 * j] = t.nW;
+            out.pair_cost[2 * j + 1] = cost;
+            out.pair_info[4 * j] = __builtin_popcountll(t.smask);
+            out.pair_info[4 * j + 1] = col_m[j];
+            out.pair_info[4 * j + 2] = t.nW;
+            out.pair_info[4 * j + 3] = i;
+            c |= pos;
+            tc += cost;
+        }
+        contrib[i] = c;
+        tcost[i] = tc;
+    }
+    out.pair_feat = std::move(col_feat);
+    col_feat.clear();
 }
 
 void Engine::trace_tree(int f, const std::vector<int> &keep) {

@@ -26,20 +26,6 @@ namespace {
 
 constexpr int QPB = kBlock / kLanesPerQuery;
 
-// segment owning block `blk`: the last s with blk_off[s] <= blk (empty
-// segments share an offset with their successor and are skipped)
-__device__ __forceinline__ int seg_of(const int *__restrict__ blk_off, int nseg, int blk) {
-    int lo = 0, hi = nseg;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (blk_off[mid] <= blk)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 __device__ __forceinline__ int origin(int x) { return x >= 0 ? x : ~x; }
 
 // ------------------------------------------------------------ merge flags of every tree of a level

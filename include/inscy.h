@@ -39,6 +39,8 @@ extern "C" {
 /* Engine modes. */
 #define INSCY_MODE_EXACT 1   /* one restricted tree at a time (reference GPU / GPU4 role) */
 #define INSCY_MODE_BATCHED 2 /* level-batched exact mode (reference GPU5 role); same results */
+#define INSCY_MODE_LEARNED 3 /* level-batched with the learned pruner (inscy_set_pruner): approximate */
+#define INSCY_MODE_COLLECT 4 /* level-batched exact, and records the pruner's training pairs (inscy_result_pairs) */
 
 /* Parameters of one run; field meaning as in the reference call
  * INSCY.CPU(X, neighborhood_size, F, num_obj, min_size, r, number_of_cells,
@@ -75,6 +77,10 @@ typedef struct inscy_stats {
     double expand_seconds;
     double replay_seconds;
     double emit_seconds;
+    /* learned and collect modes: (tree, dimension) pairs scored by the
+     * pruner's features, and pairs skipped (learned mode) */
+    int64_t pairs_scored;
+    int64_t pairs_skipped;
 } inscy_stats;
 
 typedef struct inscy_result inscy_result;
@@ -137,6 +143,56 @@ int inscy_profile(int on);
  * them to the runtime.  Call it only while no run is in progress.  No
  * counterpart in the reference, which frees everything inside each call. */
 int inscy_release_memory(void);
+
+/* ---- learned pruner (no counterpart in the reference) ----
+ *
+ * In INSCY_MODE_LEARNED the level-batched driver scores every (restricted
+ * tree T, dimension m) pair before the neighbour counts of T's points in
+ * S u {m}: a small network (64 -> 128 -> 128 -> 1, ReLU, bf16 weights, fp32
+ * accumulation) maps the pair's features to the probability that T's subtree
+ * along m reaches a cluster of the exact result.  Pairs scored below the
+ * threshold are skipped with everything below them; all other work, and all
+ * integer work, is exact, so a threshold of 0 gives the exact result.
+ *
+ * Features of a pair (64 float32): T's size and depth, m, the value and cell
+ * histograms of T's points in dimension m, T's points' counts in S, and the
+ * dimension bitmask of S u {m} (bit k at feature 32 + k % 32). */
+#define INSCY_MLP_INPUTS 64
+#define INSCY_MLP_HIDDEN 128
+#define INSCY_MLP_PARAMS (128 * 64 + 128 + 128 * 128 + 128 + 128 + 1)
+
+/* Network of later INSCY_MODE_LEARNED runs of this process: `weights` holds
+ * INSCY_MLP_PARAMS float32 values, W1[128][64] b1[128] W2[128][128] b2[128]
+ * w3[128] b3 (rounded to bf16 here).  Pairs scoring below `threshold` are
+ * skipped, except those of trees with fewer than `min_depth` restricted
+ * dimensions.  weights == NULL unloads the network. */
+int inscy_set_pruner(const float *weights, float threshold, int32_t min_depth);
+
+/* Training pairs recorded by an INSCY_MODE_COLLECT run: returns their count
+ * and sets (each optional) features [count][INSCY_MLP_INPUTS] float32,
+ * labels [count] (1: the exact result holds a cluster found in the pair's
+ * subtree), info [count][4] int32 (restricted dimensions of T, m, points of
+ * T, T's index), cost [count][2] int64 (points counted for the pair itself,
+ * and for the pair and every pair below it).  Arrays are owned by the result. */
+int64_t inscy_result_pairs(const inscy_result *res, const float **features, const uint8_t **labels,
+                           const int32_t **info, const int64_t **cost);
+
+/* The network's kernels on device arrays, for training (stream: hipStream_t or
+ * NULL).  params: INSCY_MLP_PARAMS bf16 values (uint16 bit patterns).
+ * forward: prob[i] = sigmoid(network(features[i])).
+ * gradient: grad[0..INSCY_MLP_PARAMS) = gradient, grad[INSCY_MLP_PARAMS] =
+ *   value, of inv_weight_sum * sum_i weights[i] * BCE(logit_i, labels[i]);
+ *   partials: nblocks * (INSCY_MLP_PARAMS + 1) float32 scratch.
+ * adam: one Adam step (beta 0.9, 0.999, eps 1e-8, bias-corrected at `step`
+ *   >= 1) on the float32 master weights, then params = bf16(master).
+ * round: params = bf16(master). */
+int inscy_mlp_forward(const uint16_t *d_params, const float *d_features, int64_t count, float *d_prob, void *stream);
+int inscy_mlp_gradient(const uint16_t *d_params, const float *d_features, const float *d_labels,
+                       const float *d_weights, int64_t count, float inv_weight_sum, float *d_partials,
+                       int32_t nblocks, float *d_grad, void *stream);
+int inscy_mlp_adam(float *d_master, const float *d_grad, float *d_m, float *d_v, uint16_t *d_params, int32_t step,
+                   float lr, void *stream);
+int inscy_mlp_round(const float *d_master, uint16_t *d_params, void *stream);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *inscy_last_error(void);

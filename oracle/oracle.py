@@ -241,3 +241,33 @@ def redundancy_test(result, n, pts, S, r=1.0):
     L.oracle_redundancy_test.restype = ctypes.c_int
     return bool(L.oracle_redundancy_test(n, len(result), _ip(ks), _ip(dims), _ip(labs) if len(result) else None,
                                          _ip(pts), len(pts), len(S), _ip(S), ctypes.c_float(r)))
+
+
+def mlp_forward(params: np.ndarray, X: np.ndarray) -> np.ndarray:
+    """FP64 logits of the learned pruner's network (oracle/mlp_ref.cpp)."""
+    L = lib()
+    L.oracle_mlp_forward.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                     ctypes.c_longlong, ctypes.POINTER(ctypes.c_double)]
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    out = np.zeros(X.shape[0], dtype=np.float64)
+    L.oracle_mlp_forward(_fp(P), _fp(X), X.shape[0], out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out
+
+
+def mlp_grad(params: np.ndarray, X: np.ndarray, Y: np.ndarray, WT: np.ndarray, inv_wsum: float):
+    """FP64 gradient and value of inv_wsum * sum_i WT_i * BCE(logit_i, Y_i)
+    (oracle/mlp_ref.cpp); returns (grad [params], loss)."""
+    L = lib()
+    L.oracle_mlp_grad.argtypes = [ctypes.POINTER(ctypes.c_float)] * 4 + [ctypes.c_longlong, ctypes.c_double,
+                                                                          ctypes.POINTER(ctypes.c_double)]
+    L.oracle_mlp_params.restype = ctypes.c_int
+    npar = int(L.oracle_mlp_params())
+    P = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.ascontiguousarray(Y, dtype=np.float32)
+    WT = np.ascontiguousarray(WT, dtype=np.float32)
+    out = np.zeros(npar + 1, dtype=np.float64)
+    L.oracle_mlp_grad(_fp(P), _fp(X), _fp(Y), _fp(WT), X.shape[0], float(inv_wsum),
+                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    return out[:npar], float(out[npar])
