@@ -53,6 +53,8 @@ def parse():
                     help="workload index (1: n=25000 d=15; 2: n=100000 d=20; 3: 64 data sets)")
     ap.add_argument("--real", choices=["glass", "vowel", "pendigits", "sky_0.5x0.5", "sky_1x1", "sky_1x2"],
                     help="one of the reference's real-data workloads instead of --config")
+    ap.add_argument("--threshold", type=float, default=None,
+                    help="learned mode: pruner threshold instead of the trained network's own")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
@@ -301,7 +303,9 @@ def bench_learned(args):
     w = datagen.config(args.config)
     X = torch.from_numpy(w.X).to("cuda:0")
     _, thr, min_depth, meta = pruner.load()
-    pruner.use()
+    if args.threshold is not None:
+        thr = args.threshold
+    pruner.use(threshold=thr)
 
     def run(mode):
         return INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, w.entropy_order,
