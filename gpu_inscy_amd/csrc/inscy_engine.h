@@ -956,7 +956,7 @@ private:
     std::unordered_map<uint64_t, std::pair<int, int>> sup_at;  // subspace -> (offset, count) in d_sup
     int *d_sup = nullptr;
     int *keepv = nullptr;
-    int *rep_words = nullptr;  // [0] running maximum, [1] finished blocks
+    int *rep_words = nullptr;  // [0] finished blocks, [1] clusters added, then 2 flags per result row
     void set_bounds(uint64_t s, BatchSub &x);
     void trace_tree(int f, const std::vector<int> &keep);
     std::vector<KernelTimer> timers;

@@ -274,13 +274,15 @@ hipError_t launch_b_copy(const CopySeg *segs, const int *blk_off, int nseg, int 
 hipError_t launch_b_entries(Entry *E, int at, const Entry *src, int count, int *nE, hipStream_t st);
 // dst[src[i]] = pad for every segment (the tree stamps of the replay)
 hipError_t launch_b_stamp(const CopySeg *segs, const int *blk_off, int nseg, int nblk, hipStream_t st);
-// replay: pruneRedundancy of one tree (keepv[tree] = decision; mx, done: two
-// zeroed words reset by the launch), join of one (parent, dimension) pair
+// replay: pruneRedundancy of one tree (keepv[tree] = decision; rowflag: 2 *
+// nlist zeroed words and done: one zeroed word, both left zeroed by the
+// launch), join of one (parent, dimension) pair (added: one scratch word)
 hipError_t launch_rep_redundancy(const Entry *E, const int *list, int nlist, const int *tstamp, const int *W,
-                                 int tree, int nW, float r, int *mx, unsigned *done, int *keepv, hipStream_t st);
+                                 int tree, int nW, float r, int *rowflag, unsigned *done, int *keepv, hipStream_t st);
 hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *keepv, const int *compS,
                            const int *csizeS, int min_size, float r, int n, Entry *E, const int *list, int nlist,
-                           int eS, uint8_t *inresS, uint8_t *keepf, uint8_t *drop, uint8_t *fresh, hipStream_t st);
+                           int eS, uint8_t *inresS, uint8_t *keepf, uint8_t *drop, uint8_t *fresh, int *added,
+                           hipStream_t st);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t count, hipStream_t st);
 
 // ---- learned pruner

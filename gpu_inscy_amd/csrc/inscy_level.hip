@@ -118,8 +118,10 @@ void Engine::run_levels(inscy_result &out) {
         d_sup = upload(store, flat);
     }
     keepv = store.take<int>(lt.size());
-    rep_words = store.take<int>(2);
-    HIP_OK(launch_fill_i32(rep_words, 0, 2, st));
+    // replay scratch: [0] finished blocks of a redundancy launch, [1] clusters
+    // added by a join, then two flags per result-table row
+    rep_words = store.take<int>(2 + 2 * std::max<size_t>(emask.size(), 1));
+    HIP_OK(launch_fill_i32(rep_words, 0, 2 + 2 * std::max<size_t>(emask.size(), 1), st));
     HIP_OK(launch_fill_u8(keepf, 0, (size_t) 2 * n, st));  // keepf and dropf are adjacent
     replay_tree(0);
     if (collect) finish_collect(out);
@@ -675,11 +677,11 @@ void Engine::replay_tree(int f) {
                 const int c = kids[u];
                 if (!lt[c].pass) continue;
                 replay_tree(c);
-                HIP_OK(launch_rep_redundancy(entries, list, sl.second, sub.tstamp, lt[c].W, c, lt[c].nW, prm.r, rep_words,
-                                             reinterpret_cast<unsigned *>(rep_words + 1), keepv, st));
+                HIP_OK(launch_rep_redundancy(entries, list, sl.second, sub.tstamp, lt[c].W, c, lt[c].nW, prm.r,
+                                             rep_words + 2, reinterpret_cast<unsigned *>(rep_words), keepv, st));
             }
             HIP_OK(launch_rep_join(sub.tstamp, t_lo, t_hi, keepv, sub.comp, sub.csize, prm.min_size, prm.r, n, entries,
-                                   list, sl.second, sub.entry, sub.inres, keepf, dropf, freshf, st));
+                                   list, sl.second, sub.entry, sub.inres, keepf, dropf, freshf, rep_words + 1, st));
         }
         i = j;
     }

@@ -818,76 +818,165 @@ __global__ __launch_bounds__(kBlock) void k_b_phi_exact(DevData D, const BatchSu
 // subspace contains S (fixed once the expansion is done); a row counts while
 // its nkept > 0, which the joins earlier in stream order have set.  A tree is
 // known by its index; tstamp[p] is the tree of S whose weak-dense points hold p.
+//
+// Both row walks (redundancy, join check) are planned in LDS: every block
+// reads a segment of up to kSegRows rows at once, cuts each row's walk into
+// chunks of kChunk items and takes chunks w = blockIdx.x (mod gridDim.x), so
+// no block walks the rows one after another.
+constexpr int kChunk = 2048;
+constexpr int kSegRows = 1024;
 
-// Long walks (at least kCoop items) are shared by every block of the launch,
-// short ones belong to block h % gridDim; each block reads every listed row.
-constexpr int kCoop = 4096;
+struct RowPlan {
+    int len[kSegRows];      // items of the row's walk (0: the row is not in the result)
+    int pre[kSegRows + 1];  // chunks before each row
+    uint8_t by_mem[kSegRows];
+};
 
-__device__ __forceinline__ int block_min(int v, int *wtmp) {
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_down(v, o));
-    if (__lane_id() == 0) wtmp[threadIdx.x / 64] = v;
+// loads rows [seg, seg + ns) of the list and their chunk prefix; returns the
+// number of chunks.  by_mem: walk the row's members (else the tree's points,
+// when nW is shorter; nW < 0 walks members always)
+__device__ int plan_rows(const Entry *__restrict__ E, const int *__restrict__ list, int seg, int ns, int nW,
+                         RowPlan &R, int *wtmp) {
+    for (int i = threadIdx.x; i < ns; i += blockDim.x) {
+        const Entry e = E[list[seg + i]];
+        const bool bm = nW < 0 || e.nmem <= nW;
+        R.len[i] = e.nkept > 0 ? (bm ? e.nmem : nW) : 0;
+        R.by_mem[i] = bm ? 1 : 0;
+    }
     __syncthreads();
-    int b = INT_MAX;
-    for (int w = 0; w < kBlock / 64; ++w) b = min(b, wtmp[w]);
+    // exclusive prefix of chunk counts: 4 rows per thread, then a block scan
+    constexpr int PER = kSegRows / kBlock;
+    int c[PER], sum = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int i = threadIdx.x * PER + u;
+        c[u] = i < ns ? (R.len[i] + kChunk - 1) / kChunk : 0;
+        sum += c[u];
+    }
+    int incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (__lane_id() >= o) incl += v;
+    }
+    if (__lane_id() == 63) wtmp[threadIdx.x / 64] = incl;
     __syncthreads();
-    return b;
+    int before = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < (int) threadIdx.x / 64) before += wtmp[w];
+        total += wtmp[w];
+    }
+    int at = before + incl - sum;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int i = threadIdx.x * PER + u;
+        if (i < ns) R.pre[i] = at;
+        at += c[u];
+    }
+    if (threadIdx.x == 0) R.pre[ns] = total;
+    __syncthreads();
+    return total;
 }
 
-// pruneRedundancy (ScyTreeNode.cu:669-734): per listed superspace H in the
-// result, the smallest result cluster of H holding a point of the tree; the
-// largest of these minima decides.  Per H the shorter of two walks is taken:
-// H's members (is the member a point of the tree?) or the tree's points (is
-// the point in a kept cluster of H?).  The last block to finish writes the
-// decision.
+// row of chunk w: the last i with pre[i] <= w (rows without chunks share
+// their successor's offset)
+__device__ __forceinline__ int row_of(const RowPlan &R, int ns, int w) {
+    int lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (R.pre[mid] <= w)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// pruneRedundancy (ScyTreeNode.cu:669-734) keeps the tree when no listed
+// superspace H of the result has a cluster holding a tree point, or when |T|*r
+// exceeds the largest over H of the smallest such cluster.  Equivalently it
+// drops the tree when some H touches it with clusters of size >= |T|*r only.
+// rowflag[2h] (H touches the tree) and rowflag[2h+1] (H holds a touching
+// cluster smaller than |T|*r) are set by the chunks; a row known to hold a
+// small cluster cannot drop the tree and its remaining chunks are skipped.  The
+// last block decides and clears the flags.
 __global__ void k_rep_redundancy(const Entry *__restrict__ E, const int *__restrict__ list, int nlist,
                                  const int *__restrict__ tstamp, const int *__restrict__ W, int tree, int nW, float r,
-                                 int *__restrict__ mx, unsigned *__restrict__ done, int *__restrict__ keepv) {
+                                 int *__restrict__ rowflag, unsigned *__restrict__ done, int *__restrict__ keepv) {
+    __shared__ RowPlan R;
     __shared__ int wtmp[kBlock / 64];
+    __shared__ int skip;
     __shared__ bool last;
-    for (int h = 0; h < nlist; ++h) {
-        const Entry e = E[list[h]];
-        if (e.nkept <= 0) continue;
-        const bool by_members = e.nmem <= nW;
-        const int len = by_members ? e.nmem : nW;
-        const bool coop = len >= kCoop;
-        if (!coop && (h % (int) gridDim.x) != (int) blockIdx.x) continue;
-        const int start = coop ? blockIdx.x * blockDim.x + threadIdx.x : threadIdx.x;
-        const int stride = coop ? gridDim.x * blockDim.x : blockDim.x;
-        int best = INT_MAX;
-        if (by_members) {
-            for (int t = start; t < len; t += stride) {
-                const int q = e.mem[t];
-                if (tstamp[q] == tree) best = min(best, e.csize[e.comp[q]]);
+    const double lhs = (double) ((float) nW * r);
+    for (int seg = 0; seg < nlist; seg += kSegRows) {
+        const int ns = min(kSegRows, nlist - seg);
+        const int total = plan_rows(E, list, seg, ns, nW, R, wtmp);
+        for (int w = blockIdx.x; w < total; w += gridDim.x) {
+            const int i = row_of(R, ns, w);
+            const int h = seg + i;
+            if (threadIdx.x == 0) skip = ld_relaxed(&rowflag[2 * h + 1]);
+            __syncthreads();
+            const bool sk = skip != 0;
+            __syncthreads();
+            if (sk) continue;
+            const Entry e = E[list[h]];
+            const int t0 = (w - R.pre[i]) * kChunk, t1 = min(R.len[i], t0 + kChunk);
+            bool touch = false, small = false;
+            if (R.by_mem[i]) {
+                for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+                    const int q = e.mem[t];
+                    if (tstamp[q] == tree) {
+                        touch = true;
+                        small |= lhs > (double) e.csize[e.comp[q]];
+                    }
+                }
+            } else {
+                for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+                    const int K = e.comp[W[t]];
+                    if (K >= 0 && e.inres[K]) {
+                        touch = true;
+                        small |= lhs > (double) e.csize[K];
+                    }
+                }
             }
-        } else {
-            for (int t = start; t < len; t += stride) {
-                const int K = e.comp[W[t]];
-                if (K >= 0 && e.inres[K]) best = min(best, e.csize[K]);
+            touch = __syncthreads_or(touch);
+            small = __syncthreads_or(small);
+            if (threadIdx.x == 0) {
+                if (touch && !ld_relaxed(&rowflag[2 * h])) atomicOr(&rowflag[2 * h], 1);
+                if (small) atomicOr(&rowflag[2 * h + 1], 1);
             }
         }
-        best = block_min(best, wtmp);
-        if (threadIdx.x == 0 && best != INT_MAX) atomicMax(mx, best);
+        __syncthreads();
     }
     __threadfence();
     if (threadIdx.x == 0) last = atomicAdd(done, 1u) == gridDim.x - 1;
     __syncthreads();
-    if (last && threadIdx.x == 0) {
-        const int m = __hip_atomic_load(mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const float lhs = (float) nW * r;
-        keepv[tree] = (m == 0 || (double) lhs > (double) m) ? 1 : 0;
-        __hip_atomic_store(mx, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (last) {
+        __threadfence();
+        int drop = 0;
+        for (int h = threadIdx.x; h < nlist; h += blockDim.x) {
+            const int t = ld_relaxed(&rowflag[2 * h]), sm = ld_relaxed(&rowflag[2 * h + 1]);
+            drop |= (t && !sm) ? 1 : 0;
+            st_relaxed(&rowflag[2 * h], 0);
+            st_relaxed(&rowflag[2 * h + 1], 0);
+        }
+        drop = __syncthreads_or(drop);
+        if (threadIdx.x == 0) {
+            keepv[tree] = drop ? 0 : 1;
+            __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
 // join (util.cu:655-743) of the clusters of the kept trees [t_lo, t_hi) of one
 // (parent, dimension) pair: cluster (ClusteringCpu.cu:317-389) labels the dense
 // component of every point of a kept tree; components of at least min_size
-// are candidates
+// are candidates.  added[0] counts the roots the apply step adds; it is
+// cleared here for this join.
 __global__ void k_rep_join_mark(const int *__restrict__ tstamp, int t_lo, int t_hi, const int *__restrict__ keepv,
                                 const int *__restrict__ comp, const int *__restrict__ csize, int min_size, int n,
-                                uint8_t *__restrict__ keepf) {
+                                uint8_t *__restrict__ keepf, int *__restrict__ added) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) *added = 0;
     if (p >= n) return;
     const int t = tstamp[p];
     if (t < t_lo || t >= t_hi || !keepv[t]) return;
@@ -896,30 +985,34 @@ __global__ void k_rep_join_mark(const int *__restrict__ tstamp, int t_lo, int t_
 }
 
 // a candidate K is dropped when one of its points lies in a result cluster of
-// a superspace H larger than |K| * r (one block per H)
+// a superspace H larger than |K| * r (chunks of the rows' member lists)
 __global__ void k_rep_join_check(const Entry *__restrict__ E, const int *__restrict__ list, int nlist,
                                  const int *__restrict__ compS, const uint8_t *__restrict__ keepf,
                                  const int *__restrict__ csizeS, float r, uint8_t *__restrict__ drop) {
-    for (int h = 0; h < nlist; ++h) {
-        const Entry e = E[list[h]];
-        if (e.nkept <= 0) continue;
-        const bool coop = e.nmem >= kCoop;
-        if (!coop && (h % (int) gridDim.x) != (int) blockIdx.x) continue;
-        const int start = coop ? blockIdx.x * blockDim.x + threadIdx.x : threadIdx.x;
-        const int stride = coop ? gridDim.x * blockDim.x : blockDim.x;
-        for (int t = start; t < e.nmem; t += stride) {
-            const int q = e.mem[t];
-            const int K = compS[q];
-            if (K < 0 || !keepf[K]) continue;
-            if ((float) csizeS[K] * r < (float) e.csize[e.comp[q]]) drop[K] = 1;
+    __shared__ RowPlan R;
+    __shared__ int wtmp[kBlock / 64];
+    for (int seg = 0; seg < nlist; seg += kSegRows) {
+        const int ns = min(kSegRows, nlist - seg);
+        const int total = plan_rows(E, list, seg, ns, -1, R, wtmp);
+        for (int w = blockIdx.x; w < total; w += gridDim.x) {
+            const int i = row_of(R, ns, w);
+            const Entry e = E[list[seg + i]];
+            const int t0 = (w - R.pre[i]) * kChunk, t1 = min(R.len[i], t0 + kChunk);
+            for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+                const int q = e.mem[t];
+                const int K = compS[q];
+                if (K < 0 || !keepf[K]) continue;
+                if ((float) csizeS[K] * r < (float) e.csize[e.comp[q]]) drop[K] = 1;
+            }
         }
+        __syncthreads();
     }
 }
 
 // survivors join the result of S; the candidate flags are cleared for the next join
 __global__ void k_rep_join_apply(uint8_t *__restrict__ keepf, uint8_t *__restrict__ drop, int n,
                                  uint8_t *__restrict__ inres, uint8_t *__restrict__ fresh, Entry *__restrict__ E,
-                                 int eS) {
+                                 int eS, int *__restrict__ added_out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     int added = 0;
     if (k < n) {
@@ -934,7 +1027,10 @@ __global__ void k_rep_join_apply(uint8_t *__restrict__ keepf, uint8_t *__restric
         drop[k] = 0;
     }
     for (int o = 32; o > 0; o >>= 1) added += __shfl_down(added, o);
-    if (__lane_id() == 0 && added) atomicAdd(&E[eS].nkept, added);
+    if (__lane_id() == 0 && added) {
+        atomicAdd(&E[eS].nkept, added);
+        atomicAdd(added_out, added);
+    }
 }
 
 // members of the newly kept clusters join S's member list: each block takes
@@ -942,9 +1038,10 @@ __global__ void k_rep_join_apply(uint8_t *__restrict__ keepf, uint8_t *__restric
 constexpr int kMemPerBlock = 4096;
 
 __global__ void k_rep_join_members(const int *__restrict__ compS, int n, const uint8_t *__restrict__ fresh,
-                                   Entry *__restrict__ E, int eS) {
+                                   Entry *__restrict__ E, int eS, const int *__restrict__ added) {
     __shared__ int wcount[kBlock / 64];
     __shared__ int base;
+    if (*added == 0) return;  // no cluster joined: nothing to list (uniform over the grid)
     constexpr int PER = kMemPerBlock / kBlock;
     const int i0 = blockIdx.x * kMemPerBlock;
     bool in[PER];
@@ -1141,24 +1238,25 @@ hipError_t launch_b_stamp(const CopySeg *segs, const int *blk_off, int nseg, int
 }
 
 hipError_t launch_rep_redundancy(const Entry *E, const int *list, int nlist, const int *tstamp, const int *W,
-                                 int tree, int nW, float r, int *mx, unsigned *done, int *keepv, hipStream_t st) {
-    const int grid = std::max(1, std::min(nlist, 256));
-    LAUNCH(k_rep_redundancy, dim3(grid), dim3(kBlock), 0, st, E, list, nlist, tstamp, W, tree, nW, r, mx, done,
+                                 int tree, int nW, float r, int *rowflag, unsigned *done, int *keepv, hipStream_t st) {
+    const int grid = std::max(1, std::min(nlist + nW / kChunk, 256));
+    LAUNCH(k_rep_redundancy, dim3(grid), dim3(kBlock), 0, st, E, list, nlist, tstamp, W, tree, nW, r, rowflag, done,
            keepv);
     return hipGetLastError();
 }
 
 hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *keepv, const int *compS,
                            const int *csizeS, int min_size, float r, int n, Entry *E, const int *list, int nlist,
-                           int eS, uint8_t *inresS, uint8_t *keepf, uint8_t *drop, uint8_t *fresh, hipStream_t st) {
+                           int eS, uint8_t *inresS, uint8_t *keepf, uint8_t *drop, uint8_t *fresh, int *added,
+                           hipStream_t st) {
     LAUNCH(k_rep_join_mark, dim3(blocks_for(n)), dim3(kBlock), 0, st, tstamp, t_lo, t_hi, keepv, compS, csizeS,
-           min_size, n, keepf);
+           min_size, n, keepf, added);
     if (nlist > 0)
-        LAUNCH(k_rep_join_check, dim3(std::min(nlist, 256)), dim3(kBlock), 0, st, E, list, nlist, compS, keepf, csizeS,
-               r, drop);
-    LAUNCH(k_rep_join_apply, dim3(blocks_for(n)), dim3(kBlock), 0, st, keepf, drop, n, inresS, fresh, E, eS);
+        LAUNCH(k_rep_join_check, dim3(std::min(nlist + n / kChunk, 256)), dim3(kBlock), 0, st, E, list, nlist, compS,
+               keepf, csizeS, r, drop);
+    LAUNCH(k_rep_join_apply, dim3(blocks_for(n)), dim3(kBlock), 0, st, keepf, drop, n, inresS, fresh, E, eS, added);
     LAUNCH(k_rep_join_members, dim3((n + kMemPerBlock - 1) / kMemPerBlock), dim3(kBlock), 0, st, compS, n, fresh, E,
-           eS);
+           eS, added);
     return hipGetLastError();
 }
 

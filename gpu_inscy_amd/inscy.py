@@ -17,6 +17,8 @@ Mapping of the reference's entry points:
   GPU3, GPU_star, GPU_memory, GPU_memory2 (:183-214) -> batched mode
   run_cpu, run_cpu_weak, run_gpu, run_gpu_* (7-argument ops, :110-186) -> exact
       mode with rectangular=False, entropy_order=0, the defaults those ops had
+  run_cmp (:133-134, inscy_map.cpp:273-320)  -> exact mode, tensor output
+  GPU_learned (no counterpart)            -> the learned pruner (approximate)
 """
 
 from __future__ import annotations
@@ -31,7 +33,7 @@ __all__ = [
     "GPU_memory", "GPU_memory2", "run_cpu", "run_cpu_weak", "run_gpu", "run_gpu_weak", "run_gpu_reduced",
     "run_gpu_multi", "run_gpu_multi2", "run_gpu_multi2_cl_all", "run_gpu_multi2_cl_re_all",
     "run_gpu_multi2_cl_multi", "run_gpu_multi2_cl_multi_mem", "run_cpu_gpu_mix", "run_cpu_gpu_mix_cl_steam",
-    "run_gpu_stream", "load_synt",
+    "run_gpu_stream", "run_cmp", "GPU_learned", "load_synt",
 ]
 
 
@@ -146,6 +148,35 @@ run_gpu_multi2_cl_multi = _seven
 run_gpu_multi2_cl_multi_mem = _seven
 run_cpu_gpu_mix = _seven
 run_cpu_gpu_mix_cl_steam = _seven
+
+
+def GPU_learned(X, neighborhood_size, F, num_obj, min_size, r=1., number_of_cells=3, rectangular=False,
+                entropy_order=0):
+    """Level-batched INSCY with the learned pruner (approximate; the shipped
+    network gpu_inscy_amd/pruner_weights.npz unless another is loaded with
+    gpu_inscy_amd.pruner.use)."""
+    from . import pruner
+    if not getattr(GPU_learned, "_loaded", False):
+        pruner.use()
+        GPU_learned._loaded = True
+    return _lists(run(X, neighborhood_size, F, num_obj, min_size, r, number_of_cells, rectangular, entropy_order,
+                      _lib.MODE_LEARNED))
+
+
+def run_cmp(X, neighborhood_size, F, num_obj, min_size, number_of_cells=3):
+    """python/inscy.py:133-134 -> inscy_map.cpp:273-320: the result as two
+    int32 tensors, subspace masks [#subspaces, d] (1 at the subspace's
+    dimensions) and labels [#subspaces, n].  The reference computes it with
+    INSCYCompare, a development driver that checks its GPU trees against its
+    CPU trees on the way and passes r = 0.5 to its clustering; here it is the
+    exact engine with r = 0.5 and the 7-argument ops' rectangular=False."""
+    import torch
+    res = run(X, neighborhood_size, F, num_obj, min_size, 0.5, number_of_cells, False, 0, _lib.MODE_EXACT)
+    d = int(X.shape[1])
+    masks = torch.zeros((len(res.subspaces), d), dtype=torch.int32)
+    for i, s in enumerate(res.subspaces):
+        masks[i, s] = 1
+    return [masks, torch.from_numpy(np.array(res.labels, dtype=np.int32))]
 
 
 def run_gpu_stream(X, neighborhood_size, F, num_obj, min_size, number_of_cells=3):

@@ -175,7 +175,7 @@ def test_trained_network_against_no_pruning():
         pytest.skip("no trained network (python -m gpu_inscy_amd.train_pruner)")
     say("trained network")
     import time
-    w = datagen.config(2, 10)
+    w = datagen.config(2, 12)  # trained on repeats 1-8, threshold from 9-11
     X = torch.from_numpy(w.X).cuda()
     torch.cuda.synchronize()
     t = time.perf_counter()

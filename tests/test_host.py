@@ -136,3 +136,16 @@ def test_count_number_of_clusters_and_clean_up():
     assert [list(x) for x in s2] == [[0], [0, 1]]
     assert c2[0].tolist() == [0, 0, -1, -1, -2]
     assert c2[1].tolist() == [3, 3, 3, 3, -1]
+
+
+def test_experiment_grids_follow_the_reference_harness():
+    from gpu_inscy_amd import experiments
+    name, p = experiments.grid("n", 4)
+    assert name == "_n" and p["n"] == [500, 1000, 2000, 4000, 8000] and p["d"] == [15] and p["real_cls"] == [4]
+    name, p = experiments.grid("d", 10, None, True)
+    assert name == "_d_large" and p["n"] == [25000] and p["d"] == [5 * i for i in range(1, 21)]
+    name, p = experiments.grid("F", 4, "c")
+    assert name == "_F_c" and p["F"] == [.5, 1., 1.5, 2., 2.5] and p["c"] == [2, 4, 6, 8, 10]
+    assert experiments.run_file("GPU_INSCY", 1500, 15, 4, 0.01, 1.0, 1.0, 8, 0.05, 0, 4) == \
+        "GPU_INSCYn1500d15c4N_size0.01F1.0r1.0num_obj8min_size0.05order0.npz"
+    assert experiments.run_file("INSCY", 1500, 15, 4, 0.01, 1.0, 1.0, 8, 0.05, 0, 10).endswith("order0cl10.npz")
