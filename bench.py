@@ -200,7 +200,8 @@ def main():
             "speedup_vs_depth_first": (round(other / ms_step, 3) if args.mode == "batched" else
                                        round(ms_step / other, 3)) if other else None,
             "modes_agree": same if other else None,
-            "phase_seconds": {k: round(runs[-1].stats[k + "_seconds"], 4) for k in ("expand", "replay", "emit")},
+            "phase_seconds": dict({k: round(runs[-1].stats[k + "_seconds"], 4) for k in ("expand", "replay", "emit")},
+                                  library=round(runs[-1].stats["seconds"], 4)),
             "trees_per_step": trees // max(args.steps, 1),
             "subspaces": res.stats["subspaces"],
             "clusters": res.stats["clusters"],

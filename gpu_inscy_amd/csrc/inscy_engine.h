@@ -319,14 +319,16 @@ public:
         HIP_OK(launch_discretise(d_X, n, d, cells, prm.neighborhood_size, Xt, cellC, cellE, nearm,
                                  reinterpret_cast<int *>(small), st));
         HIP_OK(launch_hist1(cellE, n, d, ecells, hist, st));
+        // c-grid occupancy: the input of the entropy order only
         int *histc = arena.take<int>((size_t) d * cells);
-        HIP_OK(launch_histc(cellC, n, d, cells, histc, st));
+        if (prm.entropy_order != 0) HIP_OK(launch_histc(cellC, n, d, cells, histc, st));
         HIP_OK(launch_fill_i32(stamp, -1, n, st));
         HIP_OK(launch_fill_i32(qmark, -1, n, st));
         std::vector<int> h((size_t) d * ecells), hc((size_t) d * cells);
         HIP_OK(hipMemcpyAsync(pin, small, sizeof(int), hipMemcpyDeviceToHost, st));
         HIP_OK(hipMemcpyAsync(h.data(), hist, h.size() * sizeof(int), hipMemcpyDeviceToHost, st));
-        HIP_OK(hipMemcpyAsync(hc.data(), histc, hc.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+        if (prm.entropy_order != 0)
+            HIP_OK(hipMemcpyAsync(hc.data(), histc, hc.size() * sizeof(int), hipMemcpyDeviceToHost, st));
         sync();
         if (reinterpret_cast<int *>(pin)[0] != 0)
             throw Failure(INSCY_ERR_ARGUMENT, "X must be finite with every value in [0, 1] (normalize first)");
@@ -365,7 +367,8 @@ public:
         HIP_OK(dev_alloc(&d_level, (size_t) 2 * d * sizeof(int)));
         std::vector<int> both(level);
         both.insert(both.end(), lvl_of.begin(), lvl_of.end());
-        HIP_OK(hipMemcpy(d_level, both.data(), both.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpyAsync(d_level, both.data(), both.size() * sizeof(int), hipMemcpyHostToDevice, st));
+        sync();  // `both` is pageable and goes out of scope
         if (phi_mode) {
             // leaf order of the neighbourhood tree, whose levels follow the same order
             HIP_OK(dev_alloc(&rank_perm, (size_t) n * sizeof(int)));
@@ -956,7 +959,9 @@ private:
     std::unordered_map<uint64_t, std::pair<int, int>> sup_at;  // subspace -> (offset, count) in d_sup
     int *d_sup = nullptr;
     int *keepv = nullptr;
-    int *rep_words = nullptr;  // [0] finished blocks, [1] clusters added, then 2 flags per result row
+    int *rep_words = nullptr;  // [0] finished blocks, [1] unused, then 2 flags per result row
+    JoinScratch js{};          // join flags and candidate list
+    int rep_join = 0;          // number of the last join
     void set_bounds(uint64_t s, BatchSub &x);
     void trace_tree(int f, const std::vector<int> &keep);
     std::vector<KernelTimer> timers;

@@ -122,7 +122,13 @@ void Engine::run_levels(inscy_result &out) {
     // added by a join, then two flags per result-table row
     rep_words = store.take<int>(2 + 2 * std::max<size_t>(emask.size(), 1));
     HIP_OK(launch_fill_i32(rep_words, 0, 2 + 2 * std::max<size_t>(emask.size(), 1), st));
-    HIP_OK(launch_fill_u8(keepf, 0, (size_t) 2 * n, st));  // keepf and dropf are adjacent
+    // join flags tagged with the join number (first join: 1), two words
+    rep_join = 0;
+    {
+        int *jsb = store.take<int>((size_t) 3 * n + 2);
+        HIP_OK(launch_fill_i32(jsb, 0, (size_t) 3 * n + 2, st));
+        js = JoinScratch{jsb, jsb + n, jsb + 2 * (size_t) n, jsb + 3 * (size_t) n};
+    }
     replay_tree(0);
     if (collect) finish_collect(out);
     sync();
@@ -657,31 +663,42 @@ void Engine::set_bounds(uint64_t s, BatchSub &x) {
 void Engine::replay_tree(int f) {
     const std::vector<int> &kids = lt[f].kids;
     size_t i = 0;
+    std::vector<int> pass;
     while (i < kids.size()) {
         const int m = lt[kids[i]].m;
         size_t j = i;
         int t_lo = INT_MAX, t_hi = -1;
-        bool any = false;
+        pass.clear();
         while (j < kids.size() && lt[kids[j]].m == m) {
             t_lo = std::min(t_lo, kids[j]);
             t_hi = std::max(t_hi, kids[j] + 1);
-            any |= lt[kids[j]].pass;
+            if (lt[kids[j]].pass) pass.push_back(kids[j]);
             ++j;
         }
-        if (any) {
+        if (!pass.empty()) {
             const uint64_t s = lt[kids[i]].smask;
             const Subspace &sub = subs[s];
             const auto sl = sup_at.at(s);
             const int *list = d_sup + sl.first;
-            for (size_t u = i; u < j; ++u) {
-                const int c = kids[u];
-                if (!lt[c].pass) continue;
+            int *rowflag = rep_words + 2;
+            unsigned *done = reinterpret_cast<unsigned *>(rep_words);
+            if (pass.size() == 1) {
+                // the only surviving child: its subtree, its pruneRedundancy,
+                // then the join of its own clusters
+                const int c = pass[0];
                 replay_tree(c);
-                HIP_OK(launch_rep_redundancy(entries, list, sl.second, sub.tstamp, lt[c].W, c, lt[c].nW, prm.r,
-                                             rep_words + 2, reinterpret_cast<unsigned *>(rep_words), keepv, st));
+                HIP_OK(launch_rep_single(entries, list, sl.second, sub.tstamp, lt[c].W, c, lt[c].nW, prm.r, rowflag,
+                                         done, keepv, sub.comp, sub.csize, prm.min_size, n, ++rep_join, js, sub.inres,
+                                         entries, sub.entry, st));
+            } else {
+                for (int c : pass) {
+                    replay_tree(c);
+                    HIP_OK(launch_rep_redundancy(entries, list, sl.second, sub.tstamp, lt[c].W, c, lt[c].nW, prm.r,
+                                                 rowflag, done, keepv, st));
+                }
+                HIP_OK(launch_rep_join(sub.tstamp, t_lo, t_hi, keepv, sub.comp, sub.csize, prm.min_size, prm.r, n,
+                                       entries, list, sl.second, sub.entry, sub.inres, ++rep_join, js, st));
             }
-            HIP_OK(launch_rep_join(sub.tstamp, t_lo, t_hi, keepv, sub.comp, sub.csize, prm.min_size, prm.r, n, entries,
-                                   list, sl.second, sub.entry, sub.inres, keepf, dropf, freshf, rep_words + 1, st));
         }
         i = j;
     }

@@ -967,30 +967,56 @@ __global__ void k_rep_redundancy(const Entry *__restrict__ E, const int *__restr
     }
 }
 
-// join (util.cu:655-743) of the clusters of the kept trees [t_lo, t_hi) of one
-// (parent, dimension) pair: cluster (ClusteringCpu.cu:317-389) labels the dense
-// component of every point of a kept tree; components of at least min_size
-// are candidates.  added[0] counts the roots the apply step adds; it is
-// cleared here for this join.
-__global__ void k_rep_join_mark(const int *__restrict__ tstamp, int t_lo, int t_hi, const int *__restrict__ keepv,
-                                const int *__restrict__ comp, const int *__restrict__ csize, int min_size, int n,
-                                uint8_t *__restrict__ keepf, int *__restrict__ added) {
+// ---- joins (util.cu:655-743)
+// cluster (ClusteringCpu.cu:317-389) labels the dense component of every
+// point of a kept tree; components of at least min_size are the candidates of
+// the join.  Flags in JoinScratch carry the number J of the join that set them
+// (cand[K] == J: K is a candidate; drop[K] == J: K is dropped; fresh[K] == J:
+// K joined the result), so no flag is ever cleared.  words[0] is set when the
+// join has a candidate (cleared by the member pass), words[1] counts the
+// clusters it added (cleared by the candidate pass).
+
+__device__ __forceinline__ void mark_candidate(const JoinScratch &js, int K, bool want, int J) {
+    if (want && js.cand[K] != J) js.cand[K] = J;  // every writer stores the same value
+    if (__ballot(want) && __lane_id() == 0) js.words[0] = 1;
+}
+
+// candidates of one kept tree (the pair's only surviving child)
+__global__ void k_rep_cand_tree(const int *__restrict__ W, int nW, int tree, const int *__restrict__ keepv,
+                                const int *__restrict__ comp, const int *__restrict__ csize, int min_size, int J,
+                                JoinScratch js) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) js.words[1] = 0;
+    if (!keepv[tree]) return;  // uniform
+    int K = -1;
+    if (i < nW) K = comp[W[i]];
+    const bool want = K >= 0 && csize[K] >= min_size;
+    mark_candidate(js, want ? K : 0, want, J);
+}
+
+// candidates of the kept trees [t_lo, t_hi) of one (parent, dimension) pair
+__global__ void k_rep_cand_group(const int *__restrict__ tstamp, int t_lo, int t_hi, const int *__restrict__ keepv,
+                                 const int *__restrict__ comp, const int *__restrict__ csize, int min_size, int n,
+                                 int J, JoinScratch js) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) *added = 0;
-    if (p >= n) return;
-    const int t = tstamp[p];
-    if (t < t_lo || t >= t_hi || !keepv[t]) return;
-    const int c = comp[p];
-    if (c >= 0 && csize[c] >= min_size) keepf[c] = 1;
+    if (p == 0) js.words[1] = 0;
+    int K = -1;
+    if (p < n) {
+        const int t = tstamp[p];
+        if (t >= t_lo && t < t_hi && keepv[t]) K = comp[p];
+    }
+    const bool want = K >= 0 && csize[K] >= min_size;
+    mark_candidate(js, want ? K : 0, want, J);
 }
 
 // a candidate K is dropped when one of its points lies in a result cluster of
 // a superspace H larger than |K| * r (chunks of the rows' member lists)
 __global__ void k_rep_join_check(const Entry *__restrict__ E, const int *__restrict__ list, int nlist,
-                                 const int *__restrict__ compS, const uint8_t *__restrict__ keepf,
-                                 const int *__restrict__ csizeS, float r, uint8_t *__restrict__ drop) {
+                                 const int *__restrict__ compS, const int *__restrict__ csizeS, float r, int J,
+                                 JoinScratch js) {
     __shared__ RowPlan R;
     __shared__ int wtmp[kBlock / 64];
+    if (ld_relaxed(&js.words[0]) == 0) return;  // no candidate (uniform over the grid)
     for (int seg = 0; seg < nlist; seg += kSegRows) {
         const int ns = min(kSegRows, nlist - seg);
         const int total = plan_rows(E, list, seg, ns, -1, R, wtmp);
@@ -1001,47 +1027,41 @@ __global__ void k_rep_join_check(const Entry *__restrict__ E, const int *__restr
             for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
                 const int q = e.mem[t];
                 const int K = compS[q];
-                if (K < 0 || !keepf[K]) continue;
-                if ((float) csizeS[K] * r < (float) e.csize[e.comp[q]]) drop[K] = 1;
+                if (K < 0 || js.cand[K] != J) continue;
+                if ((float) csizeS[K] * r < (float) e.csize[e.comp[q]]) js.drop[K] = J;
             }
         }
         __syncthreads();
     }
 }
 
-// survivors join the result of S; the candidate flags are cleared for the next join
-__global__ void k_rep_join_apply(uint8_t *__restrict__ keepf, uint8_t *__restrict__ drop, int n,
-                                 uint8_t *__restrict__ inres, uint8_t *__restrict__ fresh, Entry *__restrict__ E,
-                                 int eS, int *__restrict__ added_out) {
+// the surviving candidates join the result of S
+__global__ void k_rep_join_apply(int n, int J, JoinScratch js, uint8_t *__restrict__ inres, Entry *__restrict__ E,
+                                 int eS) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     int added = 0;
-    if (k < n) {
-        uint8_t f = 0;
-        if (keepf[k] && !drop[k] && !inres[k]) {
-            inres[k] = 1;
-            f = 1;
-            added = 1;
-        }
-        fresh[k] = f;
-        keepf[k] = 0;
-        drop[k] = 0;
+    if (k < n && js.cand[k] == J && js.drop[k] != J && !inres[k]) {
+        inres[k] = 1;
+        js.fresh[k] = J;
+        added = 1;
     }
     for (int o = 32; o > 0; o >>= 1) added += __shfl_down(added, o);
     if (__lane_id() == 0 && added) {
         atomicAdd(&E[eS].nkept, added);
-        atomicAdd(added_out, added);
+        atomicAdd(&js.words[1], added);
     }
 }
 
 // members of the newly kept clusters join S's member list: each block takes
 // kMemPerBlock points and reserves its slots with one atomic
-constexpr int kMemPerBlock = 4096;
+constexpr int kMemPerBlock = 1024;
 
-__global__ void k_rep_join_members(const int *__restrict__ compS, int n, const uint8_t *__restrict__ fresh,
-                                   Entry *__restrict__ E, int eS, const int *__restrict__ added) {
+__global__ void k_rep_join_members(const int *__restrict__ compS, int n, Entry *__restrict__ E, int eS, int J,
+                                   JoinScratch js) {
     __shared__ int wcount[kBlock / 64];
     __shared__ int base;
-    if (*added == 0) return;  // no cluster joined: nothing to list (uniform over the grid)
+    if (blockIdx.x == 0 && threadIdx.x == 0) js.words[0] = 0;  // read by no other block of this launch
+    if (js.words[1] == 0) return;  // no cluster joined: nothing to list (uniform over the grid)
     constexpr int PER = kMemPerBlock / kBlock;
     const int i0 = blockIdx.x * kMemPerBlock;
     bool in[PER];
@@ -1052,7 +1072,7 @@ __global__ void k_rep_join_members(const int *__restrict__ compS, int n, const u
         bool f = false;
         if (i < n) {
             const int K = compS[i];
-            f = K >= 0 && fresh[K];
+            f = K >= 0 && js.fresh[K] == J;
         }
         in[u] = f;
         mine += f ? 1 : 0;
@@ -1245,19 +1265,40 @@ hipError_t launch_rep_redundancy(const Entry *E, const int *list, int nlist, con
     return hipGetLastError();
 }
 
-hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *keepv, const int *compS,
-                           const int *csizeS, int min_size, float r, int n, Entry *E, const int *list, int nlist,
-                           int eS, uint8_t *inresS, uint8_t *keepf, uint8_t *drop, uint8_t *fresh, int *added,
-                           hipStream_t st) {
-    LAUNCH(k_rep_join_mark, dim3(blocks_for(n)), dim3(kBlock), 0, st, tstamp, t_lo, t_hi, keepv, compS, csizeS,
-           min_size, n, keepf, added);
+#define HIP_RETURN_IF(expr)                \
+    do {                                   \
+        hipError_t e_ = (expr);            \
+        if (e_ != hipSuccess) return e_;   \
+    } while (0)
+
+// check, apply and member pass of join J (the candidates are marked)
+static hipError_t join_tail(const Entry *E, const int *list, int nlist, const int *compS, const int *csizeS, float r,
+                            int n, int J, const JoinScratch &js, uint8_t *inresS, Entry *Ew, int eS, hipStream_t st) {
     if (nlist > 0)
         LAUNCH(k_rep_join_check, dim3(std::min(nlist + n / kChunk, 256)), dim3(kBlock), 0, st, E, list, nlist, compS,
-               keepf, csizeS, r, drop);
-    LAUNCH(k_rep_join_apply, dim3(blocks_for(n)), dim3(kBlock), 0, st, keepf, drop, n, inresS, fresh, E, eS, added);
-    LAUNCH(k_rep_join_members, dim3((n + kMemPerBlock - 1) / kMemPerBlock), dim3(kBlock), 0, st, compS, n, fresh, E,
-           eS, added);
+               csizeS, r, J, js);
+    LAUNCH(k_rep_join_apply, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, J, js, inresS, Ew, eS);
+    LAUNCH(k_rep_join_members, dim3((n + kMemPerBlock - 1) / kMemPerBlock), dim3(kBlock), 0, st, compS, n, Ew, eS, J,
+           js);
     return hipGetLastError();
+}
+
+hipError_t launch_rep_single(const Entry *E, const int *list, int nlist, const int *tstamp, const int *W, int tree,
+                            int nW, float r, int *rowflag, unsigned *done, int *keepv, const int *compS,
+                            const int *csizeS, int min_size, int n, int J, const JoinScratch &js, uint8_t *inresS,
+                            Entry *Ew, int eS, hipStream_t st) {
+    HIP_RETURN_IF(launch_rep_redundancy(E, list, nlist, tstamp, W, tree, nW, r, rowflag, done, keepv, st));
+    LAUNCH(k_rep_cand_tree, dim3(std::max(1u, blocks_for(nW))), dim3(kBlock), 0, st, W, nW, tree, keepv, compS, csizeS,
+           min_size, J, js);
+    return join_tail(E, list, nlist, compS, csizeS, r, n, J, js, inresS, Ew, eS, st);
+}
+
+hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *keepv, const int *compS,
+                           const int *csizeS, int min_size, float r, int n, Entry *E, const int *list, int nlist,
+                           int eS, uint8_t *inresS, int J, const JoinScratch &js, hipStream_t st) {
+    LAUNCH(k_rep_cand_group, dim3(blocks_for(n)), dim3(kBlock), 0, st, tstamp, t_lo, t_hi, keepv, compS, csizeS,
+           min_size, n, J, js);
+    return join_tail(E, list, nlist, compS, csizeS, r, n, J, js, inresS, E, eS, st);
 }
 
 hipError_t launch_b_entries(Entry *E, int at, const Entry *src, int count, int *nE, hipStream_t st) {
