@@ -259,8 +259,8 @@ public:
         // blocks go back to the shared caches: nothing queued may still use them
         (void) hipStreamSynchronize(st);
         for (auto &t : timers) {
-            (void) hipEventDestroy(t.start);
-            (void) hipEventDestroy(t.stop);
+            event_pool().put(t.start);
+            event_pool().put(t.stop);
         }
         for (void *p : {(void *) Xt, (void *) cellC, (void *) cellE, (void *) nearm, (void *) small, (void *) stamp,
                         (void *) qmark, (void *) entries, (void *) scratch, (void *) rank_perm,
@@ -468,8 +468,8 @@ private:
     KernelTimer timer() {
         if (!profile_on) return KernelTimer{};
         KernelTimer t;
-        HIP_OK(hipEventCreate(&t.start));
-        HIP_OK(hipEventCreate(&t.stop));
+        HIP_OK(event_pool().get(&t.start));
+        HIP_OK(event_pool().get(&t.stop));
         timers.push_back(t);
         return t;
     }
@@ -524,12 +524,14 @@ private:
         return m;
     }
 
-    GridArg grid_of(const SubspaceArg &S) const {
+    // grid index of S: the (up to 3) most selective matched dimensions, not
+    // counting position `exclude` (the child's own dimension of a shared index)
+    GridArg grid_of(const SubspaceArg &S, int exclude = -1) const {
         GridArg G{};
         const uint64_t matched = match_of(S);
         std::vector<int> ts;
         for (int t = 0; t < S.k; ++t)
-            if ((matched >> t) & 1ull) ts.push_back(t);
+            if (((matched >> t) & 1ull) && t != exclude) ts.push_back(t);
         std::stable_sort(ts.begin(), ts.end(), [&](int a, int b) { return score[S.dims[a]] < score[S.dims[b]]; });
         G.C = ecells;
         G.j = 1;
@@ -809,8 +811,8 @@ private:
             const size_t stage_bytes = (size_t) R * n * sizeof(int32_t) + (size_t) R * sizeof(int) + 256;
             char *stage[2] = {emit_stage[0].get(stage_bytes), emit_stage[1].get(stage_bytes)};
             hipEvent_t done[2];
-            HIP_OK(hipEventCreateWithFlags(&done[0], hipEventDisableTiming));
-            HIP_OK(hipEventCreateWithFlags(&done[1], hipEventDisableTiming));
+            HIP_OK(event_pool().get(&done[0]));
+            HIP_OK(event_pool().get(&done[1]));
             const unsigned nthr = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
             auto copy_out = [&](int c0, int r, const char *src) {
                 int32_t *dst = out.labels.get() + (size_t) c0 * n;
@@ -854,8 +856,8 @@ private:
             }
             HIP_OK(hipEventSynchronize(done[prev_buf]));
             copy_out(prev_c0, prev_r, stage[prev_buf]);
-            (void) hipEventDestroy(done[0]);
-            (void) hipEventDestroy(done[1]);
+            event_pool().put(done[0]);
+            event_pool().put(done[1]);
         }
         stats.subspaces = (int64_t) m;
         std::vector<unsigned long long> h2(16 * 64);

@@ -81,14 +81,22 @@ struct KernelTimer {
 // tree that holds the path, ~x when it is not (the s-connection of a point
 // pruned by the weak-density test).
 
-// One subspace of a batch: its grid index lives in the batch buffers at sorted
-// positions [base, base + n), coordinates at [coord, coord + k*n), grid cells
-// at [hoff, hoff + ncell).
+// One subspace of a batch.  Its per-subspace state (query flags, dense flags,
+// packed cells, sample edges, union-find parents) lives at [base, base + n).
+// Its grid index (keys, ranks, sorted order) lives at [obase, obase + n) with
+// grid cells at [hoff, hoff + ncell); the children S + {m} of one frontier
+// mask S (m is always the last dimension) share one index, built by the first
+// of them (rep = 1), and the coordinates of S's dimensions in sorted order.
+// Column t of the sorted coordinates and cells starts at mcoord for t == pos_m
+// (the child's own dimension), else at coord + t*n; pos_m = k when nothing is
+// shared.
 constexpr int kMaxBounds = 8;
 struct BatchSub {
     SubspaceArg S;
     GridArg G;
     long long base, coord, hoff, qoff;
+    long long obase, mcoord;
+    int pos_m, rep;
     int *cnt;           // [n] neighbour counts (capped), -1 = not counted
     const int *bound[kMaxBounds];  // [n] complete counts of subspaces S minus one dimension
     int *comp;          // [n] component roots (components pass)
@@ -125,6 +133,9 @@ struct BatchIndex {
     size_t select_bytes;
     int *nb;              // [B*n*2] first two neighbours found by the count passes
     uint64_t *sig;        // [B*n] packed cells of the checked dimensions per sorted position
+    uint64_t *sigS;       // [B*n] (index space) the same over the shared dimensions only
+    const int *reps;      // subspaces that build an index, in index order
+    int nrep;
     int2 *fb;             // phi mode: (subspace, point) pairs to recompute in leaf order
     int *nfb;             // [1] their number
     int fbcap;

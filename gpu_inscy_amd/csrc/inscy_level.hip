@@ -278,24 +278,68 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     const int QPB = kQueriesPerBlock;
 
     // ---- grid indexes of the batch's subspaces
+    // The children S + {m} of one frontier run whose S has at least three
+    // matched dimensions share one grid index over S's most selective ones
+    // (m, the last dimension, is then checked per candidate) and the sorted
+    // coordinates of S's dimensions; each adds its own column for m.
     std::vector<BatchSub> hs(B);
+    std::vector<int> reps;
     long long coord = 0, hoff = 0, qoff = 0;
     int kmax = 1;
+    int builder = -1;  // the index builder the next child of the same run may share
     int *cntB = batch.take<int>((size_t) B * n);
     for (int b = 0; b < B; ++b) {
         const ChildSub &c = cs[i0 + b];
         BatchSub &x = hs[b];
         std::memset(&x, 0, sizeof(x));
         x.S = arg_of(c.s);
-        x.G = grid_of(x.S);
+        const int kS = x.S.k - 1;
+        // shared where it costs the neighbour counts little: S's index
+        // dimensions are at most 25% less selective than the best index of
+        // S + {m} (expected candidates per query scale with the product of the
+        // index dimensions' selectivity scores)
+        bool shareable = kS >= 3 && __builtin_popcountll(match_of(x.S) & ((1ull << kS) - 1)) >= 3;
+        GridArg gbest{};
+        if (shareable) {
+            gbest = grid_of(x.S, x.S.k - 1);
+            const GridArg gopt = grid_of(x.S);
+            double cs_ = 1.0, co_ = 1.0;
+            for (int a = 0; a < gbest.j; ++a) cs_ *= std::max(score[x.S.dims[gbest.tI[a]]], 1.0);
+            for (int a = 0; a < gopt.j; ++a) co_ *= std::max(score[x.S.dims[gopt.tI[a]]], 1.0);
+            shareable = gbest.j == gopt.j && cs_ <= 1.25 * co_;
+        }
         long long ncell = 1;
-        for (int q = 0; q < x.G.j; ++q) ncell *= x.G.C;
+        if (shareable && builder >= 0 && cs[i0 + builder].t0 == c.t0 && cs[i0 + builder].t1 == c.t1 &&
+            hs[builder].G.j == gbest.j &&
+            std::equal(gbest.tI, gbest.tI + gbest.j, hs[builder].G.tI)) {
+            const BatchSub &r = hs[builder];
+            x.G = r.G;
+            x.obase = r.obase;
+            x.hoff = r.hoff;
+            x.coord = r.coord;
+            x.pos_m = x.S.k - 1;
+            x.mcoord = coord;
+            x.rep = 0;
+            coord += n;
+            for (int q = 0; q < x.G.j; ++q) ncell *= x.G.C;
+        } else {
+            x.G = shareable ? grid_of(x.S, x.S.k - 1) : grid_of(x.S);
+            for (int q = 0; q < x.G.j; ++q) ncell *= x.G.C;
+            x.obase = (long long) reps.size() * n;
+            x.hoff = hoff;
+            hoff += ncell;
+            x.coord = coord;
+            x.pos_m = shareable ? x.S.k - 1 : x.S.k;
+            x.mcoord = shareable ? coord + (long long) (x.S.k - 1) * n : 0;
+            x.rep = 1;
+            coord += (long long) x.S.k * n;
+            reps.push_back(b);
+            if (shareable) builder = b;  // a later child of the same run may share it
+        }
         long long qmax = 0;
         for (int a = c.t0; a < c.t1; ++a)
             if (pair_on(a, c.m)) qmax += lt[lv_act[a]].nW;
         x.base = (long long) b * n;
-        x.coord = coord;
-        x.hoff = hoff;
         x.qoff = qoff;
         x.cnt = cntB + (size_t) b * n;
         x.ncell = (int) ncell;
@@ -316,8 +360,6 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
             else
                 x.restmask |= 1ull << t;
         }
-        coord += (long long) x.S.k * n;
-        hoff += ncell;
         qoff += qmax;
         kmax = std::max(kmax, x.S.k);
     }
@@ -338,6 +380,9 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.select_temp = batch.take<char>(X.select_bytes);
     X.nb = batch.take<int>((size_t) 2 * B * n);
     X.sig = batch.take<uint64_t>((size_t) B * n);
+    X.sigS = batch.take<uint64_t>((size_t) reps.size() * n);
+    X.reps = upload(batch, reps);
+    X.nrep = (int) reps.size();
     X.nfb = batch.take<int>(1);
     X.fbcap = phi_mode ? B * n : 1;  // each point of each subspace is sent at most once per pass
     X.fb = batch.take<int2>(X.fbcap);

@@ -47,43 +47,77 @@ __global__ void k_b_flags(const FlagTree *__restrict__ T, const int *__restrict_
 }
 
 // ------------------------------------------------------------ grid indexes of a batch of subspaces
-__global__ void k_b_keys(DevData D, const BatchSub *__restrict__ subs, int *__restrict__ keys, int *__restrict__ rank,
-                         int *__restrict__ hist) {
-    const BatchSub &B = subs[blockIdx.y];
+// column t of subspace B's sorted coordinates / cells
+__device__ __forceinline__ long long col_of(const BatchSub &B, int t, int n) {
+    return t == B.pos_m ? B.mcoord : B.coord + (long long) t * n;
+}
+
+// grid keys and in-cell ranks of the subspaces that build an index (grid.y =
+// position in X.reps); one counting sort per such subspace
+__global__ void k_b_keys(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ reps,
+                         int *__restrict__ keys, int *__restrict__ rank, int *__restrict__ hist) {
+    const BatchSub &B = subs[reps[blockIdx.y]];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D.n) return;
     int key = 0;
     for (int a = 0; a < B.G.j; ++a) key = key * B.G.C + D.cellE[(size_t) B.S.dims[B.G.tI[a]] * D.n + p];
-    keys[B.base + p] = key;
-    rank[B.base + p] = atomicAdd(&hist[B.hoff + key], 1);
+    keys[B.obase + p] = key;
+    rank[B.obase + p] = atomicAdd(&hist[B.hoff + key], 1);
 }
 
-// `off` is one exclusive scan over the grid cells of every subspace of the
-// batch, so positions are global: subspace b holds [b*n, (b+1)*n).  The
-// scatter writes one int per point; the coordinates and cells are then
-// gathered in sorted order, so their writes are coalesced.
-__global__ void k_b_scatter(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ keys,
-                            const int *__restrict__ rank, const int *__restrict__ off, int *__restrict__ order) {
-    const BatchSub &B = subs[blockIdx.y];
+// `off` is one exclusive scan over the grid cells of every index of the
+// batch, so positions are global: index r holds [r*n, (r+1)*n).  The scatter
+// writes one int per point; the coordinates and cells are then gathered in
+// sorted order, so their writes are coalesced.
+__global__ void k_b_scatter(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ reps,
+                            const int *__restrict__ keys, const int *__restrict__ rank, const int *__restrict__ off,
+                            int *__restrict__ order) {
+    const BatchSub &B = subs[reps[blockIdx.y]];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D.n) return;
-    order[off[B.hoff + keys[B.base + p]] + rank[B.base + p]] = p;
+    order[off[B.hoff + keys[B.obase + p]] + rank[B.obase + p]] = p;
 }
 
-__global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ order,
-                           float *__restrict__ Ys, uint16_t *__restrict__ Es, uint64_t *__restrict__ sig) {
-    const BatchSub &B = subs[blockIdx.y];
+// the index builder's columns: every dimension but the child's own (all of
+// them when nothing is shared), and the packed cells of the checked ones
+__global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ reps,
+                           const int *__restrict__ order, float *__restrict__ Ys, uint16_t *__restrict__ Es,
+                           uint64_t *__restrict__ sigS) {
+    const BatchSub &B = subs[reps[blockIdx.y]];
     const int n = D.n;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int p = order[B.base + i];
+    const int p = order[B.obase + i];
     for (int t = 0; t < B.S.k; ++t) {
+        if (t == B.pos_m) continue;
         const int s = B.S.dims[t];
         Ys[B.coord + (long long) t * n + i] = D.Xt[(size_t) s * n + p];
         Es[B.coord + (long long) t * n + i] = D.cellE[(size_t) s * n + p];
     }
     uint64_t w = 0;
-    for (int f = 0; f < B.nchk4; ++f) w |= (uint64_t) D.cellE[(size_t) B.S.dims[B.chk[f]] * n + p] << (16 * f);
+    for (int f = 0; f < B.nchk4; ++f)
+        if (B.chk[f] != B.pos_m) w |= (uint64_t) D.cellE[(size_t) B.S.dims[B.chk[f]] * n + p] << (16 * f);
+    sigS[B.obase + i] = w;
+}
+
+// every subspace: its own column and its packed cells
+__global__ void k_b_gather_own(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ order,
+                               float *__restrict__ Ys, uint16_t *__restrict__ Es, const uint64_t *__restrict__ sigS,
+                               uint64_t *__restrict__ sig) {
+    const BatchSub &B = subs[blockIdx.y];
+    const int n = D.n;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t w = sigS[B.obase + i];
+    if (B.pos_m < B.S.k) {
+        const int p = order[B.obase + i];
+        const int s = B.S.dims[B.pos_m];
+        const uint16_t e = D.cellE[(size_t) s * n + p];
+        Ys[B.mcoord + i] = D.Xt[(size_t) s * n + p];
+        Es[B.mcoord + i] = e;
+        for (int f = 0; f < B.nchk4; ++f)
+            if (B.chk[f] == B.pos_m) w |= (uint64_t) e << (16 * f);
+    }
     sig[B.base + i] = w;
 }
 
@@ -97,7 +131,8 @@ __global__ void k_b_qmark(const BatchSub *__restrict__ subs, const QuerySeg *__r
     if (i >= Q.len) return;
     const BatchSub &B = subs[Q.b];
     const int p = Q.P[i];
-    qflag[off[B.hoff + keys[B.base + p]] + rank[B.base + p]] = 1;
+    const long long at = off[B.hoff + keys[B.obase + p]] + rank[B.obase + p];
+    qflag[B.base + (at - B.obase)] = 1;
 }
 
 // qstart[b] = first index of the selected (ascending global) positions that is
@@ -164,8 +199,8 @@ __device__ __forceinline__ bool cells_near(uint64_t w, uint64_t qsig, int nc) {
 // Candidate at local sorted position jj against the staged query: cells of the
 // checked dimensions within one (packed word first), then the distance summed
 // in subspace order with separate multiplies and adds, compared with s_star.
-__device__ __forceinline__ bool cand_test(const BatchSub &B, int n, const float *__restrict__ Yb,
-                                          const uint16_t *__restrict__ Eb, const uint64_t *__restrict__ sigb,
+__device__ __forceinline__ bool cand_test(const BatchSub &B, int n, const float *__restrict__ Ys,
+                                          const uint16_t *__restrict__ Es, const uint64_t *__restrict__ sigb,
                                           uint64_t qsig, const float *qy, const int *qe, int stride, int jj,
                                           float s_star, float &acc_out) {
     const int nc = B.nchk4;
@@ -174,11 +209,12 @@ __device__ __forceinline__ bool cand_test(const BatchSub &B, int n, const float 
     const int k = B.S.k;
     float acc = 0.0f;
     for (int t = 0; t < k; ++t) {
+        const long long col = col_of(B, t, n);
         if ((rest >> t) & 1ull) {
-            const int de = (int) Eb[(size_t) t * n + jj] - qe[t * stride];
+            const int de = (int) Es[col + jj] - qe[t * stride];
             if (de < -1 || de > 1) return false;
         }
-        const float diff = qy[t * stride] - Yb[(size_t) t * n + jj];
+        const float diff = qy[t * stride] - Ys[col + jj];
         const float sq = diff * diff;
         acc = acc + sq;
         if (acc > s_star) return false;
@@ -214,6 +250,17 @@ __device__ __forceinline__ void lds_layout(char *smem, int k, float *&sy, int *&
     se = reinterpret_cast<int *>(smem + sizeof(float) * QPB * k);
     qs = reinterpret_cast<uint64_t *>(smem + 8 * (size_t) QPB * k);
     qr = reinterpret_cast<int *>(smem + 8 * (size_t) QPB * k + 8 * QPB);
+}
+
+// the query's coordinates and cells into LDS
+__device__ __forceinline__ void stage_query_b(const BatchSub &B, int n, const float *__restrict__ Ys,
+                                              const uint16_t *__restrict__ Es, int i, float *sy, int *se,
+                                              int stride) {
+    for (int t = 0; t < B.S.k; ++t) {
+        const long long col = col_of(B, t, n);
+        sy[t * stride] = Ys[col + i];
+        se[t * stride] = Es[col + i];
+    }
 }
 
 // ------------------------------------------------------------ neighbour counts
@@ -256,11 +303,9 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
         i = blk * QPB + lq;
         active = i < n;
     }
-    const float *Yb = Ys + B.coord;
-    const uint16_t *Eb = Es + B.coord;
     const uint64_t *sigb = sig + B.base;
     int *cnt = B.cnt;
-    const int p = active ? order[B.base + i] : 0;
+    const int p = active ? order[B.obase + i] : 0;
     active = active && cnt[p] < 0;
     // N_S(p) is a subset of N_S'(p) for S' a subset of S: the smallest
     // complete count of the subspaces S minus one dimension, when below both
@@ -276,10 +321,10 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
             active = false;
         }
     }
-    const int base0 = (int) B.base;
+    const int base0 = (int) B.obase;
     int *q = qr + lq * kRangeWords;
     if (sub == 0 && active) {
-        stage_query(B.S, n, Yb, Eb, i, sy + lq, se + lq, QPB);
+        stage_query_b(B, n, Ys, Es, i, sy + lq, se + lq, QPB);
         qs[lq] = sigb[i];
         build_runs(B.G, off + B.hoff, se + lq, QPB, base0, q);
     }
@@ -309,6 +354,9 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
         float qv[kFastDims];
 #pragma unroll
         for (int t = 0; t < kFastDims; ++t) qv[t] = t < k ? qy[t * QPB] : 0.0f;
+        // column bases (uniform over the block: the subspace is)
+        const int pm = B.pos_m;
+        const long long cb = B.coord, mc = B.mcoord;
         int r = 0;
         // phi: running double sum of the terms (the same on all lanes of the
         // query); it may stop once it is certainly above both thresholds
@@ -347,7 +395,7 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
                 for (int u = 0; u < U; ++u)
 #pragma unroll
                     for (int t = 0; t < kFastDims; ++t)
-                        y[u][t] = (ok[u] && t < k) ? Yb[(size_t) t * n + jj[u]] : 0.0f;
+                        y[u][t] = (ok[u] && t < k) ? Ys[(t == pm ? mc : cb + (long long) t * n) + jj[u]] : 0.0f;
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     float acc = 0.0f;
@@ -367,7 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
                 for (int u = 0; u < U; ++u) {
                     accs[u] = 0.0f;
                     hit[u] = jj[u] >= 0 && jj[u] != i &&
-                             cand_test(B, n, Yb, Eb, sigb, qsig, qy, qe, QPB, jj[u], s_star, accs[u]);
+                             cand_test(B, n, Ys, Es, sigb, qsig, qy, qe, QPB, jj[u], s_star, accs[u]);
                 }
             }
             if (PHI) {
@@ -384,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
                 // the first two neighbours found: the sample edges of the components pass
                 if (hit[u] && found < 2) {
                     const int at = found + __popcll(mine & below);
-                    if (at < 2) nbp[at] = order[B.base + jj[u]];
+                    if (at < 2) nbp[at] = order[B.obase + jj[u]];
                 }
                 found += __popcll(mine);
             }
@@ -450,7 +498,7 @@ __global__ void k_b_dense_init(const BatchSub *__restrict__ subs, int n, const i
     const BatchSub &B = subs[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int p = order[B.base + i];
+    const int p = order[B.obase + i];
     const bool dense = meets(B.cnt[p], B.thr_dense, B.dbit);
     dsorted[B.base + i] = dense ? 1 : 0;
     par[B.base + p] = dense ? p : -1;
@@ -529,19 +577,17 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
     const int lq = threadIdx.x / LPQ, sub = threadIdx.x % LPQ;
     const int i = blockIdx.x * QPB + lq;
     const uint8_t *ds = dsorted + B.base;
-    const int *ord = order + B.base;
+    const int *ord = order + B.obase;
     const uint64_t *sigb = sig + B.base;
     int *pr = par + B.base;
-    const float *Yb = Ys + B.coord;
-    const uint16_t *Eb = Es + B.coord;
     bool active = i < n && ds[i];
     const int p = active ? ord[i] : 0;
     if (active && ld_relaxed(&pr[p]) == giant[blockIdx.y]) active = false;
     int *q = qr + lq * kRangeWords;
     if (sub == 0 && active) {
-        stage_query(B.S, n, Yb, Eb, i, sy + lq, se + lq, QPB);
+        stage_query_b(B, n, Ys, Es, i, sy + lq, se + lq, QPB);
         qs[lq] = sigb[i];
-        build_runs(B.G, off + B.hoff, se + lq, QPB, (int) B.base, q);
+        build_runs(B.G, off + B.hoff, se + lq, QPB, (int) B.obase, q);
     }
     __syncthreads();
     if (!active) return;
@@ -554,7 +600,7 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
     for (int c = sub; c < total; c += LPQ) {
         const int jj = run_pos(q, c, r);
         float acc;
-        if (jj == i || !ds[jj] || !cand_test(B, n, Yb, Eb, sigb, qsig, qy, qe, QPB, jj, s_star, acc)) continue;
+        if (jj == i || !ds[jj] || !cand_test(B, n, Ys, Es, sigb, qsig, qy, qe, QPB, jj, s_star, acc)) continue;
         const int qq = ord[jj];
         if (uf_find(pr, qq) == rp) continue;
         uf_union(pr, p, qq);
@@ -1148,12 +1194,16 @@ hipError_t launch_b_flags(const FlagTree *T, const int *blk_off, int nt, int nbl
 hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
     LAUNCH(k_b_fill<int>, dim3(blocks_for(X.nhist)), dim3(kBlock), 0, st, X.hist, 0, (size_t) X.nhist);
-    LAUNCH(k_b_keys, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.keys, X.rank, X.hist);
+    LAUNCH(k_b_keys, dim3(blocks_for(D.n), X.nrep), dim3(kBlock), 0, st, D, subs, X.reps, X.keys, X.rank, X.hist);
     size_t tb = X.scan_bytes;
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(X.scan_temp, tb, X.hist, X.off, X.nhist, st);
     if (e != hipSuccess) return e;
-    LAUNCH(k_b_scatter, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.keys, X.rank, X.off, X.order);
-    LAUNCH(k_b_gather, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.order, X.Ys, X.Es, X.sig);
+    LAUNCH(k_b_scatter, dim3(blocks_for(D.n), X.nrep), dim3(kBlock), 0, st, D, subs, X.reps, X.keys, X.rank, X.off,
+           X.order);
+    LAUNCH(k_b_gather, dim3(blocks_for(D.n), X.nrep), dim3(kBlock), 0, st, D, subs, X.reps, X.order, X.Ys, X.Es,
+           X.sigS);
+    LAUNCH(k_b_gather_own, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.order, X.Ys, X.Es, X.sigS,
+           X.sig);
     return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@
 #include <mutex>
 #include <unordered_map>
 #include <utility>
+#include <vector>
 
 namespace inscy {
 
@@ -105,6 +106,43 @@ inline BlockCache &device_cache() {
 inline BlockCache &host_cache() {
     static BlockCache *c = new BlockCache(true);
     return *c;
+}
+
+// Timing events of the profiling switch, reused across runs: creating and
+// destroying two events per timed launch costs more than the launches.
+class EventPool {
+public:
+    hipError_t get(hipEvent_t *e) {
+        const int dev = device();
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            auto &fl = free_[dev];
+            if (!fl.empty()) {
+                *e = fl.back();
+                fl.pop_back();
+                return hipSuccess;
+            }
+        }
+        return hipEventCreate(e);
+    }
+    void put(hipEvent_t e) {
+        if (!e) return;
+        std::lock_guard<std::mutex> g(mu_);
+        free_[device()].push_back(e);
+    }
+
+private:
+    static int device() {
+        int d = 0;
+        (void) hipGetDevice(&d);
+        return d;
+    }
+    std::mutex mu_;
+    std::unordered_map<int, std::vector<hipEvent_t>> free_;
+};
+inline EventPool &event_pool() {
+    static EventPool *p = new EventPool;
+    return *p;
 }
 
 template <class T>
