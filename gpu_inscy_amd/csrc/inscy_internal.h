@@ -270,9 +270,10 @@ hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, in
 // dense components of nsub subspaces whose counts are complete
 hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, const BatchIndex &X, float s_star,
                                hipStream_t st);
-// weak test: count >= thr, or bit wbit of the phi decision bits when wbit != 0
+// weak test: count >= thr, or bit wbit of the phi decision bits when wbit != 0;
+// lists are written only for groups with at least min_size weak-dense points
 hipError_t launch_b_partition(const PartPair *pairs, const int *blk_off, int npairs, int nblk, const uint8_t *cellC,
-                              int n, int thr, int wbit, hipStream_t st);
+                              int n, int thr, int wbit, int min_size, hipStream_t st);
 // points sorted by their eps cells over all dimensions in `level` order, ties
 // by id: the leaf order of the reference's neighbourhood tree (d stable passes)
 hipError_t launch_rank_order(const uint16_t *cellE, int n, int d, const int *level, int *perm, int *tmp,

@@ -472,7 +472,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     }
     PartPair *dpairs = upload(batch, pairs);
     int *dpb = upload(batch, pblk);
-    HIP_OK(launch_b_partition(dpairs, dpb, np, pblk[np], cellC, n, weak_thr, phi_mode ? 1 : 0, st));
+    HIP_OK(launch_b_partition(dpairs, dpb, np, pblk[np], cellC, n, weak_thr, phi_mode ? 1 : 0, prm.min_size, st));
     char *h = down.get((size_t) std::max(pc_total, 1) * sizeof(int));
     HIP_OK(hipMemcpyAsync(h, pcnt, (size_t) std::max(pc_total, 1) * sizeof(int), hipMemcpyDeviceToHost, st));
     sync();

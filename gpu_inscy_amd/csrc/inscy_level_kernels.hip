@@ -652,14 +652,26 @@ __global__ void k_b_part_count(const PartPair *__restrict__ pairs, const int *__
         if (h[t]) atomicAdd(&A.cnt[t], h[t]);
 }
 
+// Only the groups whose weak-dense points reach min_size become surviving
+// trees; the lists of the others are never read, so they are not written (a
+// pair without such a group does nothing).  Slots: LDS ranks within the block,
+// then one global reservation per (block, group).
 __global__ void k_b_part_scatter(const PartPair *__restrict__ pairs, const int *__restrict__ blk_off, int np,
-                                 const uint8_t *__restrict__ cellC, int n, int thr, int wbit) {
+                                 const uint8_t *__restrict__ cellC, int n, int thr, int wbit, int min_size) {
     __shared__ int base[3 * 256];
+    __shared__ int lcnt[3 * 256];
+    __shared__ int lres[3 * 256];
+    __shared__ int any_pass;
     const int j = seg_of(blk_off, np, blockIdx.x);
     const PartPair &A = pairs[j];
     const int blk = blockIdx.x - blk_off[j];
     const int G = A.G;
     const uint8_t *cm = cellC + (size_t) A.m * n;
+    if (threadIdx.x == 0) any_pass = 0;
+    for (int t = threadIdx.x; t < 3 * G; t += blockDim.x) lcnt[t] = 0;
+    __syncthreads();
+    for (int g = threadIdx.x; g < G; g += blockDim.x)
+        if (A.cnt[G + g] >= min_size) any_pass = 1;
     if (threadIdx.x < 3) {
         int acc = 0;
         for (int g = 0; g < G; ++g) {
@@ -668,30 +680,45 @@ __global__ void k_b_part_scatter(const PartPair *__restrict__ pairs, const int *
         }
     }
     __syncthreads();
+    if (!any_pass) return;  // uniform over the block
     int *cur = A.cnt + 3 * G;
-    if (blk < A.bp) {
+    const bool pts = blk < A.bp;
+    int key = -1, rnk = 0, val = 0;
+    uint64_t mk = 0;
+    if (pts) {
         const int i = blk * blockDim.x + threadIdx.x;
-        const bool in = i < A.nP;
-        const int p = in ? A.P[i] : 0;
-        const int g = in ? A.gid[cm[p]] : 0;
-        const bool weak = in && meets(A.weakcnt[p], thr, wbit);
-        const int sw = wave_add_by_key(cur + G, g, weak);
-        if (weak) A.outW[base[G + g] + sw] = p;
+        if (i < A.nP) {
+            const int p = A.P[i];
+            const int g = A.gid[cm[p]];
+            if (A.cnt[G + g] >= min_size && meets(A.weakcnt[p], thr, wbit)) {
+                key = G + g;
+                val = p;
+            }
+        }
     } else {
         const int i = (blk - A.bp) * blockDim.x + threadIdx.x;
-        uint64_t mk = 0;
-        int x = 0, g = 0;
         if (i < A.nsp) {
             mk = A.m >= 63 ? 0ull : (A.spm[i] & (~0ull << (A.m + 1)));
             const int xr = A.spx[i];
             const int xo = origin(xr);
-            g = A.gid[cm[xo]];
-            x = (xr >= 0 && meets(A.weakcnt[xo], thr, wbit)) ? xo : ~xo;
+            const int g = A.gid[cm[xo]];
+            if (mk && A.cnt[G + g] >= min_size) {
+                key = 2 * G + g;
+                val = (xr >= 0 && meets(A.weakcnt[xo], thr, wbit)) ? xo : ~xo;
+            }
         }
-        const int sx = wave_add_by_key(cur + 2 * G, g, mk != 0);
-        if (mk) {
-            const int pos = base[2 * G + g] + sx;
-            A.outx[pos] = x;
+    }
+    if (key >= 0) rnk = atomicAdd(&lcnt[key], 1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < 3 * G; t += blockDim.x)
+        if (lcnt[t]) lres[t] = atomicAdd(&cur[t], lcnt[t]);
+    __syncthreads();
+    if (key >= 0) {
+        const int pos = base[key] + lres[key] + rnk;
+        if (pts) {
+            A.outW[pos] = val;
+        } else {
+            A.outx[pos] = val;
             A.outm[pos] = mk;
         }
     }
@@ -1279,10 +1306,10 @@ hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, 
 }
 
 hipError_t launch_b_partition(const PartPair *pairs, const int *blk_off, int npairs, int nblk, const uint8_t *cellC,
-                              int n, int thr, int wbit, hipStream_t st) {
+                              int n, int thr, int wbit, int min_size, hipStream_t st) {
     if (nblk <= 0) return hipSuccess;
     LAUNCH(k_b_part_count, dim3(nblk), dim3(kBlock), 0, st, pairs, blk_off, npairs, cellC, n, thr, wbit);
-    LAUNCH(k_b_part_scatter, dim3(nblk), dim3(kBlock), 0, st, pairs, blk_off, npairs, cellC, n, thr, wbit);
+    LAUNCH(k_b_part_scatter, dim3(nblk), dim3(kBlock), 0, st, pairs, blk_off, npairs, cellC, n, thr, wbit, min_size);
     return hipGetLastError();
 }
 
