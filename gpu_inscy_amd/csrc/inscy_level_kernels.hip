@@ -164,12 +164,29 @@ __global__ void k_b_qstart(const int *__restrict__ qlist, const int *__restrict_
 // [0, total): q[0..8] run starts (local positions), q[9..18] prefix sums of
 // the run lengths, q[19] the number of runs.  A lane maps its candidate
 // number to a position with a run pointer that only moves forward.
+// q[20]: the run of the query's own cells in the first index dimensions (it
+// holds the query, so it is not empty)
 __device__ __forceinline__ void build_runs(const GridArg &G, const int *__restrict__ off, const int *qe, int stride,
                                            int base0, int *q) {
+    const int C = G.C;
+    const int c0 = qe[G.tI[0] * stride];
+    int ca, ce;
+    if (G.j == 1) {
+        ca = off[max(c0 - 1, 0)];
+        ce = off[min(c0 + 1, C - 1) + 1];
+    } else {
+        const int cl = qe[G.tI[G.j - 1] * stride];
+        const int lo = max(cl - 1, 0), hi = min(cl + 1, C - 1);
+        const int row = G.j == 2 ? c0 * C : (c0 * C + qe[G.tI[1] * stride]) * C;
+        ca = off[row + lo];
+        ce = off[row + hi + 1];
+    }
     int nr = 0, acc = 0;
     q[9] = 0;
+    q[20] = 0;
     for_ranges(G, off, qe, stride, [&](int a, int e) {
         if (e > a) {
+            if (a == ca && e == ce) q[20] = nr;
             q[nr] = a - base0;
             acc += e - a;
             q[10 + nr] = acc;
@@ -343,7 +360,10 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
         // the first two neighbours found are then spread over the candidates
         // (random sample edges for the components pass, as Afforest draws
         // them) instead of the same few points at the start of a shared run
-        const int rot = total > 0 ? (int) (((uint32_t) p * 2654435761u) % (uint32_t) total) : 0;
+        // the walk starts at a pseudo-random candidate of the run of the
+        // query's own cells (the densest part of its neighbourhood) and wraps
+        const int cs0 = q[9 + q[20]], clen = q[10 + q[20]] - cs0;
+        const int rot = total > 0 ? cs0 + (int) (((uint32_t) p * 2654435761u) % (uint32_t) max(clen, 1)) : 0;
         bool wrapped = false;
         int *nbp = nb + 2 * (B.base + p);
         // k <= 8 with every checked dimension packed: all loads of a step are
