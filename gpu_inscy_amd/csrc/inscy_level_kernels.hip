@@ -24,7 +24,9 @@ namespace inscy {
 
 namespace {
 
-constexpr int QPB = kBlock / kLanesPerQuery;
+constexpr int kBLanes = 8;  // lanes per query of the batched neighbour passes
+constexpr int QPB = kBlock / kBLanes;
+static_assert(QPB == kQueriesPerBlock, "queries per block of the host and the kernels");
 
 __device__ __forceinline__ int origin(int x) { return x >= 0 ? x : ~x; }
 
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
                                                     float s_star, float eps, double phirel, int *__restrict__ nb,
                                                     int2 *__restrict__ fb, int *__restrict__ nfb, int fbcap,
                                                     unsigned long long *__restrict__ stat) {
-    constexpr int LPQ = kLanesPerQuery, U = 4, STEP = LPQ * U;
+    constexpr int LPQ = kBLanes, U = 4, STEP = LPQ * U;
     extern __shared__ char smem[];
     int b, blk;
     if (LIST) {
@@ -586,7 +588,7 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
                                                         const uint64_t *__restrict__ sig,
                                                         const uint8_t *__restrict__ dsorted, float s_star,
                                                         const int *__restrict__ giant, int *__restrict__ par) {
-    constexpr int LPQ = kLanesPerQuery;
+    constexpr int LPQ = kBLanes;
     extern __shared__ char smem[];
     const BatchSub &B = subs[blockIdx.y];
     const int k = B.S.k;
