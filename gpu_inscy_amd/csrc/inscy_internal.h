@@ -350,6 +350,6 @@ hipError_t launch_tree_hits(const TreeHit *trees, const int *blk_off, int ntrees
 
 constexpr int kRangeWords = 21;      // per query in LDS: 9 range starts, 10 prefix sums, count, own run
 constexpr int kBlockThreads = 256;   // threads per block of every pass
-constexpr int kQueriesPerBlock = 32;  // queries per block of the batched neighbour passes
+constexpr int kQueriesPerBlock = 64;  // queries per block of the batched neighbour passes
 
 }  // namespace inscy

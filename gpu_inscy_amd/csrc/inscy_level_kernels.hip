@@ -24,7 +24,7 @@ namespace inscy {
 
 namespace {
 
-constexpr int kBLanes = 8;  // lanes per query of the batched neighbour passes
+constexpr int kBLanes = 4;  // lanes per query of the batched neighbour passes
 constexpr int QPB = kBlock / kBLanes;
 static_assert(QPB == kQueriesPerBlock, "queries per block of the host and the kernels");
 
