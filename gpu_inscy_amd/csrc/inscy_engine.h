@@ -955,7 +955,9 @@ private:
     }
     void expand_level(const std::vector<int> &frontier, const int *counts);
     void process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1);
-    void replay_tree(int f);
+    void replay_staged();
+    std::vector<int> sup_flat;  // superset rows of every subspace (host copy of d_sup)
+    static unsigned blocks_for_host(long long count) { return (unsigned) ((count + kBlockThreads - 1) / kBlockThreads); }
     // replay state: superset rows per subspace, decisions per tree
     std::vector<uint64_t> emask;                 // mask of every result-table row
     std::unordered_map<uint64_t, std::pair<int, int>> sup_at;  // subspace -> (offset, count) in d_sup

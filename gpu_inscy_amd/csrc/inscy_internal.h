@@ -291,6 +291,37 @@ hipError_t launch_b_stamp(const CopySeg *segs, const int *blk_off, int nseg, int
 // launch), join of one (parent, dimension) pair (added: one scratch word)
 hipError_t launch_rep_redundancy(const Entry *E, const int *list, int nlist, const int *tstamp, const int *W,
                                  int tree, int nW, float r, int *rowflag, unsigned *done, int *keepv, hipStream_t st);
+// One step of the staged replay: pruneRedundancy of `tree` (kind 0 and 1),
+// and/or the join J of the kept trees [t_lo, t_hi) into subspace S (kind 0:
+// the pair's only surviving child is `tree`; kind 2: a pair with several).
+// Steps of one stage touch pairwise incomparable subspaces, so they commute.
+// flag: S's count array, reused for the join's per-component flags
+// (candidate -(2+3J), dropped -(3+3J), joined -(4+3J)); clist: candidate
+// roots (capacity ccap); words: [0] candidates listed, [1] clusters added.
+struct RepOp {
+    const int *list;
+    const int *tstamp;
+    const int *W;
+    int *flag;
+    const int *comp;
+    const int *csize;
+    uint8_t *inres;
+    int *rowflag;
+    unsigned *done;
+    int *words;
+    int *clist;
+    int nlist, nW, tree, t_lo, t_hi, entry, J, kind, ccap, pad;
+};
+// the staged replay's passes over nops steps; each takes blk_off (per-step
+// block offsets) over the steps it serves
+hipError_t launch_rep_stage(const RepOp *ops, int nops, const int *blk_red, int nblk_red, const int *blk_cand,
+                            int nblk_cand, const int *blk_chk, int nblk_chk, const int *blk_mem, int nblk_mem,
+                            Entry *E, float r, int min_size, int n, int *keepv, hipStream_t st);
+// blocks per step of each pass (the host sizes the offsets with these)
+int rep_red_blocks(int nlist, int nW);
+int rep_check_blocks(int nlist, int n);
+int rep_mem_blocks(int n);
+
 // join scratch: per-component flags tagged with the join number (cand, drop,
 // fresh: [n] ints, zero before the first join), words[0] "has a candidate"
 // (zero between joins), words[1] clusters added

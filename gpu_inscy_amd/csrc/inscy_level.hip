@@ -17,6 +17,8 @@
 
 #include "inscy_engine.h"
 
+#include <functional>
+
 namespace inscy {
 
 namespace {
@@ -116,6 +118,7 @@ void Engine::run_levels(inscy_result &out) {
             sup_at[sm] = {at, (int) flat.size() - at};
         }
         d_sup = upload(store, flat);
+        sup_flat.swap(flat);
     }
     keepv = store.take<int>(lt.size());
     // replay scratch: [0] finished blocks of a redundancy launch, [1] clusters
@@ -129,7 +132,7 @@ void Engine::run_levels(inscy_result &out) {
         HIP_OK(launch_fill_i32(jsb, 0, (size_t) 3 * n + 2, st));
         js = JoinScratch{jsb, jsb + n, jsb + 2 * (size_t) n, jsb + 3 * (size_t) n};
     }
-    replay_tree(0);
+    replay_staged();
     if (collect) finish_collect(out);
     sync();
     auto t2 = std::chrono::steady_clock::now();
@@ -702,50 +705,144 @@ void Engine::set_bounds(uint64_t s, BatchSub &x) {
     x.lo = lo;
 }
 
-// pruneRedundancy and join of the subtree of f, in the reference's order:
-// per dimension, every surviving child's subtree first, then its redundancy
-// decision, then the join of the clusters of the kept children.
-void Engine::replay_tree(int f) {
-    const std::vector<int> &kids = lt[f].kids;
-    size_t i = 0;
+// pruneRedundancy and join, replayed in stages.  The reference's order visits
+// per dimension every surviving child's subtree, then its pruneRedundancy,
+// then the join of the kept children's clusters.  A step on subspace S reads
+// the result rows of the supersets of S and writes the row of S, so two steps
+// must keep their order exactly when one subspace contains the other; each
+// step goes to the stage after the last earlier step it depends on, and the
+// steps of one stage run together, a handful of launches per stage.
+void Engine::replay_staged() {
+    struct Step {
+        uint64_t s;
+        int kind, tree, t_lo, t_hi;
+    };
+    std::vector<Step> steps;
     std::vector<int> pass;
-    while (i < kids.size()) {
-        const int m = lt[kids[i]].m;
-        size_t j = i;
-        int t_lo = INT_MAX, t_hi = -1;
-        pass.clear();
-        while (j < kids.size() && lt[kids[j]].m == m) {
-            t_lo = std::min(t_lo, kids[j]);
-            t_hi = std::max(t_hi, kids[j] + 1);
-            if (lt[kids[j]].pass) pass.push_back(kids[j]);
-            ++j;
-        }
-        if (!pass.empty()) {
-            const uint64_t s = lt[kids[i]].smask;
-            const Subspace &sub = subs[s];
-            const auto sl = sup_at.at(s);
-            const int *list = d_sup + sl.first;
-            int *rowflag = rep_words + 2;
-            unsigned *done = reinterpret_cast<unsigned *>(rep_words);
-            if (pass.size() == 1) {
-                // the only surviving child: its subtree, its pruneRedundancy,
-                // then the join of its own clusters
-                const int c = pass[0];
-                replay_tree(c);
-                HIP_OK(launch_rep_single(entries, list, sl.second, sub.tstamp, lt[c].W, c, lt[c].nW, prm.r, rowflag,
-                                         done, keepv, sub.comp, sub.csize, prm.min_size, n, ++rep_join, js, sub.inres,
-                                         entries, sub.entry, st));
-            } else {
-                for (int c : pass) {
-                    replay_tree(c);
-                    HIP_OK(launch_rep_redundancy(entries, list, sl.second, sub.tstamp, lt[c].W, c, lt[c].nW, prm.r,
-                                                 rowflag, done, keepv, st));
-                }
-                HIP_OK(launch_rep_join(sub.tstamp, t_lo, t_hi, keepv, sub.comp, sub.csize, prm.min_size, prm.r, n,
-                                       entries, list, sl.second, sub.entry, sub.inres, ++rep_join, js, st));
+    std::function<void(int)> walk = [&](int f) {
+        const std::vector<int> &kids = lt[f].kids;
+        size_t i = 0;
+        while (i < kids.size()) {
+            const int m = lt[kids[i]].m;
+            size_t j = i;
+            int t_lo = INT_MAX, t_hi = -1;
+            std::vector<int> pk;
+            while (j < kids.size() && lt[kids[j]].m == m) {
+                t_lo = std::min(t_lo, kids[j]);
+                t_hi = std::max(t_hi, kids[j] + 1);
+                if (lt[kids[j]].pass) pk.push_back(kids[j]);
+                ++j;
             }
+            const uint64_t s = lt[kids[i]].smask;
+            if (pk.size() == 1) {
+                walk(pk[0]);
+                steps.push_back({s, 0, pk[0], pk[0], pk[0] + 1});
+            } else if (!pk.empty()) {
+                for (int c : pk) {
+                    walk(c);
+                    steps.push_back({s, 1, c, c, c + 1});
+                }
+                steps.push_back({s, 2, -1, t_lo, t_hi});
+            }
+            i = j;
         }
-        i = j;
+    };
+    walk(0);
+    // stages: last_on[r] = last stage of a step on row r, last_sub[r] = last
+    // stage of a step on a subspace of row r
+    const int ne = (int) emask.size();
+    std::vector<int> last_on(ne, 0), last_sub(ne, 0), stage(steps.size());
+    int nst = 0;
+    for (size_t i = 0; i < steps.size(); ++i) {
+        const Subspace &sb = subs.at(steps[i].s);
+        const auto sl = sup_at.at(steps[i].s);
+        int st0 = last_sub[sb.entry];
+        for (int u = 0; u < sl.second; ++u) st0 = std::max(st0, last_on[sup_flat[sl.first + u]]);
+        const int sti = st0 + 1;
+        stage[i] = sti;
+        nst = std::max(nst, sti);
+        last_on[sb.entry] = std::max(last_on[sb.entry], sti);
+        for (int u = 0; u < sl.second; ++u) {
+            int &x = last_sub[sup_flat[sl.first + u]];
+            x = std::max(x, sti);
+        }
+    }
+    std::vector<std::vector<int>> by_stage(nst + 1);
+    for (size_t i = 0; i < steps.size(); ++i) by_stage[stage[i]].push_back((int) i);
+    if (batch_trace)
+        std::fprintf(stderr, "[inscy replay] %zu steps in %d stages\n", steps.size(), nst);
+
+    // per step: row flags (2 per superset row) and a block counter
+    // (redundancy), the join's two words and its candidate list
+    const size_t kScratchInts = (size_t) 64 << 20;  // per launch group
+    const int ccap = prm.min_size > 0 ? std::min(n, n / prm.min_size + 1) : n;
+    for (int sg = 1; sg <= nst; ++sg) {
+        const std::vector<int> &ids = by_stage[sg];
+        size_t a = 0;
+        while (a < ids.size()) {
+            // a chunk of the stage whose scratch fits the budget
+            size_t b = a, ints = 0;
+            while (b < ids.size()) {
+                const Step &x = steps[ids[b]];
+                const size_t need = 2 * (size_t) sup_at.at(x.s).second + 3 + (x.kind == 1 ? 0 : (size_t) ccap);
+                if (b > a && ints + need > kScratchInts) break;
+                ints += need;
+                ++b;
+            }
+            batch.reset();
+            int *scr = batch.take<int>(std::max<size_t>(ints, 1));
+            HIP_OK(launch_fill_i32(scr, 0, std::max<size_t>(ints, 1), st));
+            std::vector<RepOp> ops(b - a);
+            std::vector<int> br{0}, bc{0}, bk{0}, bm{0};
+            size_t at = 0;
+            for (size_t u = a; u < b; ++u) {
+                const Step &x = steps[ids[u]];
+                const Subspace &sb = subs.at(x.s);
+                const auto sl = sup_at.at(x.s);
+                RepOp &o = ops[u - a];
+                std::memset(&o, 0, sizeof(o));
+                o.list = d_sup + sl.first;
+                o.nlist = sl.second;
+                o.tstamp = sb.tstamp;
+                o.flag = sb.cnt;
+                o.comp = sb.comp;
+                o.csize = sb.csize;
+                o.inres = sb.inres;
+                o.entry = sb.entry;
+                o.kind = x.kind;
+                o.tree = x.tree;
+                o.t_lo = x.t_lo;
+                o.t_hi = x.t_hi;
+                o.J = x.kind == 1 ? 0 : ++rep_join;
+                o.ccap = ccap;
+                o.rowflag = scr + at;
+                at += 2 * (size_t) sl.second;
+                o.done = reinterpret_cast<unsigned *>(scr + at);
+                at += 1;
+                o.words = scr + at;
+                at += 2;
+                if (x.kind != 1) {
+                    o.clist = scr + at;
+                    at += ccap;
+                }
+                if (x.tree >= 0) {
+                    o.W = lt[x.tree].W;
+                    o.nW = lt[x.tree].nW;
+                }
+                br.push_back(br.back() + (x.kind != 2 ? rep_red_blocks(o.nlist, o.nW) : 0));
+                bc.push_back(bc.back() + (x.kind == 0 ? (int) std::max(1u, blocks_for_host(o.nW))
+                                                      : x.kind == 2 ? (int) blocks_for_host(n) : 0));
+                bk.push_back(bk.back() + (x.kind != 1 ? rep_check_blocks(o.nlist, n) : 0));
+                bm.push_back(bm.back() + (x.kind != 1 ? rep_mem_blocks(n) : 0));
+            }
+            const int no = (int) ops.size();
+            RepOp *dops = upload(batch, ops);
+            int *dbr = upload(batch, br), *dbc = upload(batch, bc), *dbk = upload(batch, bk), *dbm = upload(batch, bm);
+            HIP_OK(launch_rep_stage(dops, no, dbr, br.back(), dbc, bc.back(), dbk, bk.back(), dbm, bm.back(), entries,
+                                    prm.r, prm.min_size, n, keepv, st));
+            sync();  // the chunk's scratch and descriptors are reused by the next one
+            a = b;
+        }
     }
 }
 

@@ -1062,6 +1062,217 @@ __global__ void k_rep_redundancy(const Entry *__restrict__ E, const int *__restr
     }
 }
 
+// ---- the staged replay: every pass serves all steps of one stage
+__device__ __forceinline__ int flag_cand(int J) { return -(2 + 3 * J); }
+__device__ __forceinline__ int flag_drop(int J) { return -(3 + 3 * J); }
+__device__ __forceinline__ int flag_fresh(int J) { return -(4 + 3 * J); }
+
+// pruneRedundancy of every step with a tree (the body of k_rep_redundancy,
+// with the step's own block range standing for the grid)
+__global__ void k_rep_red_multi(const RepOp *__restrict__ ops, const int *__restrict__ blk_off, int nops,
+                                const Entry *__restrict__ E, float r, int *__restrict__ keepv) {
+    __shared__ RowPlan R;
+    __shared__ int wtmp[kBlock / 64];
+    __shared__ int skip;
+    __shared__ bool last;
+    const int o = seg_of(blk_off, nops, blockIdx.x);
+    const RepOp &Op = ops[o];
+    const int nb = blk_off[o + 1] - blk_off[o], bi = blockIdx.x - blk_off[o];
+    const int nlist = Op.nlist, nW = Op.nW, tree = Op.tree;
+    const int *list = Op.list;
+    int *rowflag = Op.rowflag;
+    const double lhs = (double) ((float) nW * r);
+    for (int seg = 0; seg < nlist; seg += kSegRows) {
+        const int ns = min(kSegRows, nlist - seg);
+        const int total = plan_rows(E, list, seg, ns, nW, R, wtmp);
+        for (int w = bi; w < total; w += nb) {
+            const int i = row_of(R, ns, w);
+            const int h = seg + i;
+            if (threadIdx.x == 0) skip = ld_relaxed(&rowflag[2 * h + 1]);
+            __syncthreads();
+            const bool sk = skip != 0;
+            __syncthreads();
+            if (sk) continue;
+            const Entry e = E[list[h]];
+            const int t0 = (w - R.pre[i]) * kChunk, t1 = min(R.len[i], t0 + kChunk);
+            bool touch = false, small = false;
+            if (R.by_mem[i]) {
+                for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+                    const int q = e.mem[t];
+                    if (Op.tstamp[q] == tree) {
+                        touch = true;
+                        small |= lhs > (double) e.csize[e.comp[q]];
+                    }
+                }
+            } else {
+                for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+                    const int K = e.comp[Op.W[t]];
+                    if (K >= 0 && e.inres[K]) {
+                        touch = true;
+                        small |= lhs > (double) e.csize[K];
+                    }
+                }
+            }
+            touch = __syncthreads_or(touch);
+            small = __syncthreads_or(small);
+            if (threadIdx.x == 0) {
+                if (touch && !ld_relaxed(&rowflag[2 * h])) atomicOr(&rowflag[2 * h], 1);
+                if (small) atomicOr(&rowflag[2 * h + 1], 1);
+            }
+        }
+        __syncthreads();
+    }
+    __threadfence();
+    if (threadIdx.x == 0) last = atomicAdd(Op.done, 1u) == (unsigned) nb - 1;
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        int drop = 0;
+        for (int h = threadIdx.x; h < nlist; h += blockDim.x) {
+            const int t = ld_relaxed(&rowflag[2 * h]), sm = ld_relaxed(&rowflag[2 * h + 1]);
+            drop |= (t && !sm) ? 1 : 0;
+        }
+        drop = __syncthreads_or(drop);
+        if (threadIdx.x == 0) keepv[tree] = drop ? 0 : 1;
+    }
+}
+
+// candidates of every join step: the components (of at least min_size
+// points) of the points of its kept trees; one list entry per component
+__global__ void k_rep_cand_multi(const RepOp *__restrict__ ops, const int *__restrict__ blk_off, int nops,
+                                 const int *__restrict__ keepv, int min_size, int n) {
+    const int o = seg_of(blk_off, nops, blockIdx.x);
+    const RepOp &Op = ops[o];
+    const int i = (blockIdx.x - blk_off[o]) * blockDim.x + threadIdx.x;
+    int K = -1;
+    if (Op.kind == 0) {
+        if (i < Op.nW && keepv[Op.tree]) K = Op.comp[Op.W[i]];
+    } else if (i < n) {
+        const int t = Op.tstamp[i];
+        if (t >= Op.t_lo && t < Op.t_hi && keepv[t]) K = Op.comp[i];
+    }
+    const int cj = flag_cand(Op.J);
+    bool want = K >= 0 && Op.csize[K] >= min_size && Op.flag[K] != cj;
+    // one claim per distinct component in the wave
+    unsigned long long todo = __ballot(want);
+    while (todo) {
+        const int leader = __ffsll((long long) todo) - 1;
+        const int lk = __shfl(K, leader);
+        todo &= ~__ballot(want && K == lk);
+        if (__lane_id() == leader) {
+            const int old = atomicExch(&Op.flag[lk], cj);
+            if (old != cj) {
+                const int at = atomicAdd(&Op.words[0], 1);
+                if (at < Op.ccap) Op.clist[at] = lk;
+            }
+        }
+    }
+}
+
+// a candidate K is dropped when one of its points lies in a result cluster of
+// a superspace H larger than |K| * r
+__global__ void k_rep_check_multi(const RepOp *__restrict__ ops, const int *__restrict__ blk_off, int nops,
+                                  const Entry *__restrict__ E, float r) {
+    __shared__ RowPlan R;
+    __shared__ int wtmp[kBlock / 64];
+    const int o = seg_of(blk_off, nops, blockIdx.x);
+    const RepOp &Op = ops[o];
+    if (ld_relaxed(&Op.words[0]) == 0) return;  // no candidate (uniform over the step's blocks)
+    const int nb = blk_off[o + 1] - blk_off[o], bi = blockIdx.x - blk_off[o];
+    const int cj = flag_cand(Op.J), dj = flag_drop(Op.J);
+    for (int seg = 0; seg < Op.nlist; seg += kSegRows) {
+        const int ns = min(kSegRows, Op.nlist - seg);
+        const int total = plan_rows(E, Op.list, seg, ns, -1, R, wtmp);
+        for (int w = bi; w < total; w += nb) {
+            const int i = row_of(R, ns, w);
+            const Entry e = E[Op.list[seg + i]];
+            const int t0 = (w - R.pre[i]) * kChunk, t1 = min(R.len[i], t0 + kChunk);
+            for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
+                const int q = e.mem[t];
+                const int K = Op.comp[q];
+                if (K < 0 || Op.flag[K] != cj) continue;
+                if ((float) Op.csize[K] * r < (float) e.csize[e.comp[q]]) Op.flag[K] = dj;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// the surviving candidates of every join step join the result of its
+// subspace (one block per step)
+__global__ void k_rep_apply_multi(const RepOp *__restrict__ ops, Entry *__restrict__ E) {
+    __shared__ int wtmp[kBlock / 64];
+    const RepOp &Op = ops[blockIdx.x];
+    if (Op.kind == 1) return;
+    const int nc = min(Op.words[0], Op.ccap);
+    const int cj = flag_cand(Op.J), fj = flag_fresh(Op.J);
+    int added = 0;
+    for (int k = threadIdx.x; k < nc; k += blockDim.x) {
+        const int K = Op.clist[k];
+        if (Op.flag[K] == cj && !Op.inres[K]) {
+            Op.inres[K] = 1;
+            Op.flag[K] = fj;
+            ++added;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) added += __shfl_down(added, o);
+    if (__lane_id() == 0) wtmp[threadIdx.x / 64] = added;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < kBlock / 64; ++w) tot += wtmp[w];
+        if (tot) E[Op.entry].nkept += tot;
+        Op.words[1] = tot;
+    }
+}
+
+constexpr int kMemPerBlock = 1024;
+
+// members of the newly joined clusters of every join step
+__global__ void k_rep_members_multi(const RepOp *__restrict__ ops, const int *__restrict__ blk_off, int nops,
+                                    Entry *__restrict__ E, int n) {
+    __shared__ int wcount[kBlock / 64];
+    __shared__ int base;
+    const int o = seg_of(blk_off, nops, blockIdx.x);
+    const RepOp &Op = ops[o];
+    if (Op.words[1] == 0) return;  // nothing joined (uniform over the step's blocks)
+    const int fj = flag_fresh(Op.J);
+    constexpr int PER = kMemPerBlock / kBlock;
+    const int i0 = (blockIdx.x - blk_off[o]) * kMemPerBlock;
+    bool in[PER];
+    int mine = 0;
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int i = i0 + u * kBlock + threadIdx.x;
+        bool f = false;
+        if (i < n) {
+            const int K = Op.comp[i];
+            f = K >= 0 && Op.flag[K] == fj;
+        }
+        in[u] = f;
+        mine += f ? 1 : 0;
+    }
+    int incl = mine;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d);
+        if (__lane_id() >= d) incl += v;
+    }
+    if (__lane_id() == 63) wcount[threadIdx.x / 64] = incl;
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; ++w) {
+        if (w < (int) threadIdx.x / 64) before += wcount[w];
+        total += wcount[w];
+    }
+    if (threadIdx.x == 0) base = total > 0 ? atomicAdd(&E[Op.entry].nmem, total) : 0;
+    __syncthreads();
+    int slot = base + before + incl - mine;
+    int *mem = E[Op.entry].mem;
+#pragma unroll
+    for (int u = 0; u < PER; ++u)
+        if (in[u]) mem[slot++] = i0 + u * kBlock + threadIdx.x;
+}
+
 // ---- joins (util.cu:655-743)
 // cluster (ClusteringCpu.cu:317-389) labels the dense component of every
 // point of a kept tree; components of at least min_size are the candidates of
@@ -1149,7 +1360,6 @@ __global__ void k_rep_join_apply(int n, int J, JoinScratch js, uint8_t *__restri
 
 // members of the newly kept clusters join S's member list: each block takes
 // kMemPerBlock points and reserves its slots with one atomic
-constexpr int kMemPerBlock = 1024;
 
 __global__ void k_rep_join_members(const int *__restrict__ compS, int n, Entry *__restrict__ E, int eS, int J,
                                    JoinScratch js) {
@@ -1398,6 +1608,23 @@ hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *kee
     LAUNCH(k_rep_cand_group, dim3(blocks_for(n)), dim3(kBlock), 0, st, tstamp, t_lo, t_hi, keepv, compS, csizeS,
            min_size, n, J, js);
     return join_tail(E, list, nlist, compS, csizeS, r, n, J, js, inresS, E, eS, st);
+}
+
+int rep_red_blocks(int nlist, int nW) { return std::max(1, std::min(nlist + nW / kChunk, 256)); }
+int rep_check_blocks(int nlist, int n) { return nlist > 0 ? std::min(nlist + n / kChunk, 256) : 0; }
+int rep_mem_blocks(int n) { return (n + kMemPerBlock - 1) / kMemPerBlock; }
+
+hipError_t launch_rep_stage(const RepOp *ops, int nops, const int *blk_red, int nblk_red, const int *blk_cand,
+                            int nblk_cand, const int *blk_chk, int nblk_chk, const int *blk_mem, int nblk_mem,
+                            Entry *E, float r, int min_size, int n, int *keepv, hipStream_t st) {
+    if (nops <= 0) return hipSuccess;
+    if (nblk_red > 0) LAUNCH(k_rep_red_multi, dim3(nblk_red), dim3(kBlock), 0, st, ops, blk_red, nops, E, r, keepv);
+    if (nblk_cand > 0)
+        LAUNCH(k_rep_cand_multi, dim3(nblk_cand), dim3(kBlock), 0, st, ops, blk_cand, nops, keepv, min_size, n);
+    if (nblk_chk > 0) LAUNCH(k_rep_check_multi, dim3(nblk_chk), dim3(kBlock), 0, st, ops, blk_chk, nops, E, r);
+    LAUNCH(k_rep_apply_multi, dim3(nops), dim3(kBlock), 0, st, ops, E);
+    if (nblk_mem > 0) LAUNCH(k_rep_members_multi, dim3(nblk_mem), dim3(kBlock), 0, st, ops, blk_mem, nops, E, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_b_entries(Entry *E, int at, const Entry *src, int count, int *nE, hipStream_t st) {
