@@ -963,9 +963,7 @@ private:
     std::unordered_map<uint64_t, std::pair<int, int>> sup_at;  // subspace -> (offset, count) in d_sup
     int *d_sup = nullptr;
     int *keepv = nullptr;
-    int *rep_words = nullptr;  // [0] finished blocks, [1] unused, then 2 flags per result row
-    JoinScratch js{};          // join flags and candidate list
-    int rep_join = 0;          // number of the last join
+    int rep_join = 0;  // number of the last join
     void set_bounds(uint64_t s, BatchSub &x);
     void trace_tree(int f, const std::vector<int> &keep);
     std::vector<KernelTimer> timers;

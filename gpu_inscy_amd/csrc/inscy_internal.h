@@ -322,21 +322,6 @@ int rep_red_blocks(int nlist, int nW);
 int rep_check_blocks(int nlist, int n);
 int rep_mem_blocks(int n);
 
-// join scratch: per-component flags tagged with the join number (cand, drop,
-// fresh: [n] ints, zero before the first join), words[0] "has a candidate"
-// (zero between joins), words[1] clusters added
-struct JoinScratch {
-    int *cand, *drop, *fresh, *words;
-};
-// join J of the kept trees [t_lo, t_hi) of one (parent, dimension) pair
-hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *keepv, const int *compS,
-                           const int *csizeS, int min_size, float r, int n, Entry *E, const int *list, int nlist,
-                           int eS, uint8_t *inresS, int J, const JoinScratch &js, hipStream_t st);
-// pruneRedundancy and join J of a pair whose only surviving child is `tree`
-hipError_t launch_rep_single(const Entry *E, const int *list, int nlist, const int *tstamp, const int *W, int tree,
-                            int nW, float r, int *rowflag, unsigned *done, int *keepv, const int *compS,
-                            const int *csizeS, int min_size, int n, int J, const JoinScratch &js, uint8_t *inresS,
-                            Entry *Ew, int eS, hipStream_t st);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t count, hipStream_t st);
 
 // ---- learned pruner

@@ -12,8 +12,10 @@
 //     every (tree, dimension) pair and one read-back of the group sizes;
 //   - the surviving children of a batch get their path prune, and their new
 //     subspaces their dense components, a handful of launches for all of them.
-// pruneRedundancy and join are then replayed on the device in the reference's
-// depth-first order, which gives the depth-first driver's result exactly.
+// pruneRedundancy and join are then replayed on the device in stages that keep
+// the reference's depth-first order wherever it matters (steps on subspaces
+// one of which contains the other), which gives the depth-first driver's
+// result exactly.
 
 #include "inscy_engine.h"
 
@@ -121,17 +123,7 @@ void Engine::run_levels(inscy_result &out) {
         sup_flat.swap(flat);
     }
     keepv = store.take<int>(lt.size());
-    // replay scratch: [0] finished blocks of a redundancy launch, [1] clusters
-    // added by a join, then two flags per result-table row
-    rep_words = store.take<int>(2 + 2 * std::max<size_t>(emask.size(), 1));
-    HIP_OK(launch_fill_i32(rep_words, 0, 2 + 2 * std::max<size_t>(emask.size(), 1), st));
-    // join flags tagged with the join number (first join: 1), two words
     rep_join = 0;
-    {
-        int *jsb = store.take<int>((size_t) 3 * n + 2);
-        HIP_OK(launch_fill_i32(jsb, 0, (size_t) 3 * n + 2, st));
-        js = JoinScratch{jsb, jsb + n, jsb + 2 * (size_t) n, jsb + 3 * (size_t) n};
-    }
     replay_staged();
     if (collect) finish_collect(out);
     sync();

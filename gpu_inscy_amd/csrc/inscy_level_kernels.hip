@@ -908,16 +908,17 @@ __global__ __launch_bounds__(kBlock) void k_b_phi_exact(DevData D, const BatchSu
 }
 
 // ------------------------------------------------------------ replay of pruneRedundancy and join
-// The depth-first order of the reference is replayed with one launch per
-// surviving tree and four per join.  `list` holds the result-table rows whose
-// subspace contains S (fixed once the expansion is done); a row counts while
-// its nkept > 0, which the joins earlier in stream order have set.  A tree is
-// known by its index; tstamp[p] is the tree of S whose weak-dense points hold p.
+// The reference's depth-first order is replayed in stages: one launch per
+// pass serves every step of a stage (the host groups the steps).  `list`
+// holds the result-table rows whose subspace contains S (fixed once the
+// expansion is done); a row counts while its nkept > 0, which the joins of
+// earlier stages have set.  A tree is known by its index; tstamp[p] is the
+// tree of S whose weak-dense points hold p.
 //
-// Both row walks (redundancy, join check) are planned in LDS: every block
-// reads a segment of up to kSegRows rows at once, cuts each row's walk into
-// chunks of kChunk items and takes chunks w = blockIdx.x (mod gridDim.x), so
-// no block walks the rows one after another.
+// Both row walks (redundancy, join check) are planned in LDS: every block of
+// a step reads a segment of up to kSegRows rows at once, cuts each row's walk
+// into chunks of kChunk items and takes every nb-th chunk (nb: the step's
+// blocks), so no block walks the rows one after another.
 constexpr int kChunk = 2048;
 constexpr int kSegRows = 1024;
 
@@ -984,82 +985,6 @@ __device__ __forceinline__ int row_of(const RowPlan &R, int ns, int w) {
             hi = mid;
     }
     return lo;
-}
-
-// pruneRedundancy (ScyTreeNode.cu:669-734) keeps the tree when no listed
-// superspace H of the result has a cluster holding a tree point, or when |T|*r
-// exceeds the largest over H of the smallest such cluster.  Equivalently it
-// drops the tree when some H touches it with clusters of size >= |T|*r only.
-// rowflag[2h] (H touches the tree) and rowflag[2h+1] (H holds a touching
-// cluster smaller than |T|*r) are set by the chunks; a row known to hold a
-// small cluster cannot drop the tree and its remaining chunks are skipped.  The
-// last block decides and clears the flags.
-__global__ void k_rep_redundancy(const Entry *__restrict__ E, const int *__restrict__ list, int nlist,
-                                 const int *__restrict__ tstamp, const int *__restrict__ W, int tree, int nW, float r,
-                                 int *__restrict__ rowflag, unsigned *__restrict__ done, int *__restrict__ keepv) {
-    __shared__ RowPlan R;
-    __shared__ int wtmp[kBlock / 64];
-    __shared__ int skip;
-    __shared__ bool last;
-    const double lhs = (double) ((float) nW * r);
-    for (int seg = 0; seg < nlist; seg += kSegRows) {
-        const int ns = min(kSegRows, nlist - seg);
-        const int total = plan_rows(E, list, seg, ns, nW, R, wtmp);
-        for (int w = blockIdx.x; w < total; w += gridDim.x) {
-            const int i = row_of(R, ns, w);
-            const int h = seg + i;
-            if (threadIdx.x == 0) skip = ld_relaxed(&rowflag[2 * h + 1]);
-            __syncthreads();
-            const bool sk = skip != 0;
-            __syncthreads();
-            if (sk) continue;
-            const Entry e = E[list[h]];
-            const int t0 = (w - R.pre[i]) * kChunk, t1 = min(R.len[i], t0 + kChunk);
-            bool touch = false, small = false;
-            if (R.by_mem[i]) {
-                for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-                    const int q = e.mem[t];
-                    if (tstamp[q] == tree) {
-                        touch = true;
-                        small |= lhs > (double) e.csize[e.comp[q]];
-                    }
-                }
-            } else {
-                for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-                    const int K = e.comp[W[t]];
-                    if (K >= 0 && e.inres[K]) {
-                        touch = true;
-                        small |= lhs > (double) e.csize[K];
-                    }
-                }
-            }
-            touch = __syncthreads_or(touch);
-            small = __syncthreads_or(small);
-            if (threadIdx.x == 0) {
-                if (touch && !ld_relaxed(&rowflag[2 * h])) atomicOr(&rowflag[2 * h], 1);
-                if (small) atomicOr(&rowflag[2 * h + 1], 1);
-            }
-        }
-        __syncthreads();
-    }
-    __threadfence();
-    if (threadIdx.x == 0) last = atomicAdd(done, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        int drop = 0;
-        for (int h = threadIdx.x; h < nlist; h += blockDim.x) {
-            const int t = ld_relaxed(&rowflag[2 * h]), sm = ld_relaxed(&rowflag[2 * h + 1]);
-            drop |= (t && !sm) ? 1 : 0;
-            st_relaxed(&rowflag[2 * h], 0);
-            st_relaxed(&rowflag[2 * h + 1], 0);
-        }
-        drop = __syncthreads_or(drop);
-        if (threadIdx.x == 0) {
-            keepv[tree] = drop ? 0 : 1;
-            __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
 }
 
 // ---- the staged replay: every pass serves all steps of one stage
@@ -1273,137 +1198,6 @@ __global__ void k_rep_members_multi(const RepOp *__restrict__ ops, const int *__
         if (in[u]) mem[slot++] = i0 + u * kBlock + threadIdx.x;
 }
 
-// ---- joins (util.cu:655-743)
-// cluster (ClusteringCpu.cu:317-389) labels the dense component of every
-// point of a kept tree; components of at least min_size are the candidates of
-// the join.  Flags in JoinScratch carry the number J of the join that set them
-// (cand[K] == J: K is a candidate; drop[K] == J: K is dropped; fresh[K] == J:
-// K joined the result), so no flag is ever cleared.  words[0] is set when the
-// join has a candidate (cleared by the member pass), words[1] counts the
-// clusters it added (cleared by the candidate pass).
-
-__device__ __forceinline__ void mark_candidate(const JoinScratch &js, int K, bool want, int J) {
-    if (want && js.cand[K] != J) js.cand[K] = J;  // every writer stores the same value
-    if (__ballot(want) && __lane_id() == 0) js.words[0] = 1;
-}
-
-// candidates of one kept tree (the pair's only surviving child)
-__global__ void k_rep_cand_tree(const int *__restrict__ W, int nW, int tree, const int *__restrict__ keepv,
-                                const int *__restrict__ comp, const int *__restrict__ csize, int min_size, int J,
-                                JoinScratch js) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) js.words[1] = 0;
-    if (!keepv[tree]) return;  // uniform
-    int K = -1;
-    if (i < nW) K = comp[W[i]];
-    const bool want = K >= 0 && csize[K] >= min_size;
-    mark_candidate(js, want ? K : 0, want, J);
-}
-
-// candidates of the kept trees [t_lo, t_hi) of one (parent, dimension) pair
-__global__ void k_rep_cand_group(const int *__restrict__ tstamp, int t_lo, int t_hi, const int *__restrict__ keepv,
-                                 const int *__restrict__ comp, const int *__restrict__ csize, int min_size, int n,
-                                 int J, JoinScratch js) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) js.words[1] = 0;
-    int K = -1;
-    if (p < n) {
-        const int t = tstamp[p];
-        if (t >= t_lo && t < t_hi && keepv[t]) K = comp[p];
-    }
-    const bool want = K >= 0 && csize[K] >= min_size;
-    mark_candidate(js, want ? K : 0, want, J);
-}
-
-// a candidate K is dropped when one of its points lies in a result cluster of
-// a superspace H larger than |K| * r (chunks of the rows' member lists)
-__global__ void k_rep_join_check(const Entry *__restrict__ E, const int *__restrict__ list, int nlist,
-                                 const int *__restrict__ compS, const int *__restrict__ csizeS, float r, int J,
-                                 JoinScratch js) {
-    __shared__ RowPlan R;
-    __shared__ int wtmp[kBlock / 64];
-    if (ld_relaxed(&js.words[0]) == 0) return;  // no candidate (uniform over the grid)
-    for (int seg = 0; seg < nlist; seg += kSegRows) {
-        const int ns = min(kSegRows, nlist - seg);
-        const int total = plan_rows(E, list, seg, ns, -1, R, wtmp);
-        for (int w = blockIdx.x; w < total; w += gridDim.x) {
-            const int i = row_of(R, ns, w);
-            const Entry e = E[list[seg + i]];
-            const int t0 = (w - R.pre[i]) * kChunk, t1 = min(R.len[i], t0 + kChunk);
-            for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) {
-                const int q = e.mem[t];
-                const int K = compS[q];
-                if (K < 0 || js.cand[K] != J) continue;
-                if ((float) csizeS[K] * r < (float) e.csize[e.comp[q]]) js.drop[K] = J;
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// the surviving candidates join the result of S
-__global__ void k_rep_join_apply(int n, int J, JoinScratch js, uint8_t *__restrict__ inres, Entry *__restrict__ E,
-                                 int eS) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    int added = 0;
-    if (k < n && js.cand[k] == J && js.drop[k] != J && !inres[k]) {
-        inres[k] = 1;
-        js.fresh[k] = J;
-        added = 1;
-    }
-    for (int o = 32; o > 0; o >>= 1) added += __shfl_down(added, o);
-    if (__lane_id() == 0 && added) {
-        atomicAdd(&E[eS].nkept, added);
-        atomicAdd(&js.words[1], added);
-    }
-}
-
-// members of the newly kept clusters join S's member list: each block takes
-// kMemPerBlock points and reserves its slots with one atomic
-
-__global__ void k_rep_join_members(const int *__restrict__ compS, int n, Entry *__restrict__ E, int eS, int J,
-                                   JoinScratch js) {
-    __shared__ int wcount[kBlock / 64];
-    __shared__ int base;
-    if (blockIdx.x == 0 && threadIdx.x == 0) js.words[0] = 0;  // read by no other block of this launch
-    if (js.words[1] == 0) return;  // no cluster joined: nothing to list (uniform over the grid)
-    constexpr int PER = kMemPerBlock / kBlock;
-    const int i0 = blockIdx.x * kMemPerBlock;
-    bool in[PER];
-    int mine = 0;
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-        const int i = i0 + u * kBlock + threadIdx.x;
-        bool f = false;
-        if (i < n) {
-            const int K = compS[i];
-            f = K >= 0 && js.fresh[K] == J;
-        }
-        in[u] = f;
-        mine += f ? 1 : 0;
-    }
-    // exclusive prefix of `mine` over the block
-    int incl = mine;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int v = __shfl_up(incl, o);
-        if (__lane_id() >= o) incl += v;
-    }
-    if (__lane_id() == 63) wcount[threadIdx.x / 64] = incl;
-    __syncthreads();
-    int before = 0, total = 0;
-    for (int w = 0; w < kBlock / 64; ++w) {
-        if (w < (int) threadIdx.x / 64) before += wcount[w];
-        total += wcount[w];
-    }
-    if (threadIdx.x == 0) base = total > 0 ? atomicAdd(&E[eS].nmem, total) : 0;
-    __syncthreads();
-    int slot = base + before + incl - mine;
-    int *mem = E[eS].mem;
-#pragma unroll
-    for (int u = 0; u < PER; ++u)
-        if (in[u]) mem[slot++] = i0 + u * kBlock + threadIdx.x;
-}
-
 __global__ void k_b_stamp(const CopySeg *__restrict__ segs, const int *__restrict__ blk_off, int ns) {
     const int s = seg_of(blk_off, ns, blockIdx.x);
     const CopySeg S = segs[s];
@@ -1564,50 +1358,6 @@ hipError_t launch_b_stamp(const CopySeg *segs, const int *blk_off, int nseg, int
     if (nblk <= 0) return hipSuccess;
     LAUNCH(k_b_stamp, dim3(nblk), dim3(kBlock), 0, st, segs, blk_off, nseg);
     return hipGetLastError();
-}
-
-hipError_t launch_rep_redundancy(const Entry *E, const int *list, int nlist, const int *tstamp, const int *W,
-                                 int tree, int nW, float r, int *rowflag, unsigned *done, int *keepv, hipStream_t st) {
-    const int grid = std::max(1, std::min(nlist + nW / kChunk, 256));
-    LAUNCH(k_rep_redundancy, dim3(grid), dim3(kBlock), 0, st, E, list, nlist, tstamp, W, tree, nW, r, rowflag, done,
-           keepv);
-    return hipGetLastError();
-}
-
-#define HIP_RETURN_IF(expr)                \
-    do {                                   \
-        hipError_t e_ = (expr);            \
-        if (e_ != hipSuccess) return e_;   \
-    } while (0)
-
-// check, apply and member pass of join J (the candidates are marked)
-static hipError_t join_tail(const Entry *E, const int *list, int nlist, const int *compS, const int *csizeS, float r,
-                            int n, int J, const JoinScratch &js, uint8_t *inresS, Entry *Ew, int eS, hipStream_t st) {
-    if (nlist > 0)
-        LAUNCH(k_rep_join_check, dim3(std::min(nlist + n / kChunk, 256)), dim3(kBlock), 0, st, E, list, nlist, compS,
-               csizeS, r, J, js);
-    LAUNCH(k_rep_join_apply, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, J, js, inresS, Ew, eS);
-    LAUNCH(k_rep_join_members, dim3((n + kMemPerBlock - 1) / kMemPerBlock), dim3(kBlock), 0, st, compS, n, Ew, eS, J,
-           js);
-    return hipGetLastError();
-}
-
-hipError_t launch_rep_single(const Entry *E, const int *list, int nlist, const int *tstamp, const int *W, int tree,
-                            int nW, float r, int *rowflag, unsigned *done, int *keepv, const int *compS,
-                            const int *csizeS, int min_size, int n, int J, const JoinScratch &js, uint8_t *inresS,
-                            Entry *Ew, int eS, hipStream_t st) {
-    HIP_RETURN_IF(launch_rep_redundancy(E, list, nlist, tstamp, W, tree, nW, r, rowflag, done, keepv, st));
-    LAUNCH(k_rep_cand_tree, dim3(std::max(1u, blocks_for(nW))), dim3(kBlock), 0, st, W, nW, tree, keepv, compS, csizeS,
-           min_size, J, js);
-    return join_tail(E, list, nlist, compS, csizeS, r, n, J, js, inresS, Ew, eS, st);
-}
-
-hipError_t launch_rep_join(const int *tstamp, int t_lo, int t_hi, const int *keepv, const int *compS,
-                           const int *csizeS, int min_size, float r, int n, Entry *E, const int *list, int nlist,
-                           int eS, uint8_t *inresS, int J, const JoinScratch &js, hipStream_t st) {
-    LAUNCH(k_rep_cand_group, dim3(blocks_for(n)), dim3(kBlock), 0, st, tstamp, t_lo, t_hi, keepv, compS, csizeS,
-           min_size, n, J, js);
-    return join_tail(E, list, nlist, compS, csizeS, r, n, J, js, inresS, E, eS, st);
 }
 
 int rep_red_blocks(int nlist, int nW) { return std::max(1, std::min(nlist + nW / kChunk, 256)); }
