@@ -43,8 +43,8 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", choices=["exact", "batched", "learned"], default="batched",
                     help="learned: the learned pruner (gpu_inscy_amd/pruner_weights.npz) against the exact run")
     ap.add_argument("--depth-first-steps", type=int, default=2,
@@ -202,6 +202,7 @@ def main():
             "modes_agree": same if other else None,
             "phase_seconds": dict({k: round(runs[-1].stats[k + "_seconds"], 4) for k in ("expand", "replay", "emit")},
                                   library=round(runs[-1].stats["seconds"], 4)),
+            "library_ms_per_step": round(sum(r.stats["seconds"] for r in runs) / args.steps * 1e3, 3),
             "trees_per_step": trees // max(args.steps, 1),
             "subspaces": res.stats["subspaces"],
             "clusters": res.stats["clusters"],

@@ -188,13 +188,11 @@ class Result:
     def __init__(self, handle):
         L = lib()
         owner = _Handle(handle)
+        self._owner = owner
+        self._subspaces = None
         n = int(L.inscy_result_num_points(handle))
         m = int(L.inscy_result_num_subspaces(handle))
-        dims = (ctypes.c_int32 * 64)()
-        self.subspaces = []
-        for i in range(m):
-            k = L.inscy_result_subspace(handle, i, dims)
-            self.subspaces.append([int(dims[j]) for j in range(k)])
+        self._m = m
         if m > 0 and n > 0:
             ptr = L.inscy_result_labels_all(handle)
             buf = (ctypes.c_int32 * (m * n)).from_address(ctypes.addressof(ptr.contents))
@@ -206,6 +204,19 @@ class Result:
         check(L.inscy_result_stats(handle, ctypes.byref(st)))
         self.stats = {f: getattr(st, f) for f, _ in Stats._fields_}
         self.pairs = _pairs(handle, owner)
+
+    @property
+    def subspaces(self):
+        """Dimension lists of the result subspaces (built on first access)."""
+        if self._subspaces is None:
+            L = lib()
+            dims = (ctypes.c_int32 * 64)()
+            out = []
+            for i in range(self._m):
+                k = L.inscy_result_subspace(self._owner.handle, i, dims)
+                out.append(list(dims[:k]))
+            self._subspaces = out
+        return self._subspaces
 
 
 def _pairs(handle, owner):
