@@ -544,7 +544,13 @@ __global__ void k_b_link_sample(const BatchSub *__restrict__ subs, int n, const 
     // the recorded neighbours: min(count, 2) of them (phi mode records them too)
     const int q = nb[2 * (B.base + p) + round];
     if (B.dbit == 0 && c <= round) return;
-    if (q >= 0 && q < n && meets(cnt[q], B.thr_dense, B.dbit)) uf_union(par + B.base, p, q);
+    if (q < 0 || q >= n || !meets(cnt[q], B.thr_dense, B.dbit)) return;
+    int *pr = par + B.base;
+    // after the first round's compression every point's parent is its root:
+    // equal parents are one set, decided without reading the root's own entry
+    // (which every point of a large component would read at once)
+    if (ld_relaxed(&pr[p]) == ld_relaxed(&pr[q])) return;
+    uf_union(pr, p, q);
 }
 
 __global__ void k_b_uf_compress(const BatchSub *__restrict__ subs, int n, int *__restrict__ par) {
@@ -624,7 +630,8 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
         float acc;
         if (jj == i || !ds[jj] || !cand_test(B, n, Ys, Es, sigb, qsig, qy, qe, QPB, jj, s_star, acc)) continue;
         const int qq = ord[jj];
-        if (uf_find(pr, qq) == rp) continue;
+        // parents only move up to roots: a parent equal to p's root is p's set
+        if (ld_relaxed(&pr[qq]) == rp || uf_find(pr, qq) == rp) continue;
         uf_union(pr, p, qq);
         rp = uf_find(pr, p);
     }
