@@ -263,7 +263,9 @@ hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySe
 // uncounted point when blk_off is null (grid.y = subspace)
 // phi: rectangular = false, the stored values are decision bits (1 weak-dense,
 // 2 dense); points near a threshold are recomputed in leaf order (rank_order)
-hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, int nblk, int n, int kmax,
+// kfast in 1..7: every subspace of the launch has k = kfast and at most four
+// checked dimensions (the count kernel is then compiled for that k); else 0
+hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, int nblk, int n, int kmax, int kfast,
                           const BatchIndex &X, float s_star, float eps, bool phi, bool phi_all_exact,
                           const DevData &D, const int *rank_order, unsigned long long *stat, const KernelTimer &tm,
                           hipStream_t st);

@@ -280,7 +280,8 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     std::vector<BatchSub> hs(B);
     std::vector<int> reps;
     long long coord = 0, hoff = 0, qoff = 0;
-    int kmax = 1;
+    int kmax = 1, kmin = 64;
+    bool all_packed = true;
     int builder = -1;  // the index builder the next child of the same run may share
     int *cntB = batch.take<int>((size_t) B * n);
     for (int b = 0; b < B; ++b) {
@@ -357,7 +358,10 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         }
         qoff += qmax;
         kmax = std::max(kmax, x.S.k);
+        kmin = std::min(kmin, x.S.k);
+        all_packed = all_packed && x.restmask == 0;
     }
+    const int kfast = (kmin == kmax && kmax <= 7 && all_packed) ? kmax : 0;
     BatchIndex X{};
     X.nhist = (int) (hoff + 1);
     X.keys = batch.take<int>((size_t) B * n);
@@ -415,7 +419,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         sync();
         tq0 = std::chrono::steady_clock::now();
     }
-    HIP_OK(launch_b_count(dsubs, B, dcb, cblk[B], n, kmax, X, s_star, prm.neighborhood_size, phi_mode, phi_all_exact, D, rank_perm,
+    HIP_OK(launch_b_count(dsubs, B, dcb, cblk[B], n, kmax, kfast, X, s_star, prm.neighborhood_size, phi_mode, phi_all_exact, D, rank_perm,
                           small + kStatAt, timer(), st));
     if (dbg) {
         sync();
@@ -645,7 +649,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
             sync();
             tc0 = std::chrono::steady_clock::now();
         }
-        HIP_OK(launch_b_count(dhc, ncomp, nullptr, 0, n, kmax, X, s_star, prm.neighborhood_size, phi_mode, phi_all_exact, D,
+        HIP_OK(launch_b_count(dhc, ncomp, nullptr, 0, n, kmax, kfast, X, s_star, prm.neighborhood_size, phi_mode, phi_all_exact, D,
                               rank_perm, small + kStatAt, timer(), st));
         if (dbg) {
             sync();

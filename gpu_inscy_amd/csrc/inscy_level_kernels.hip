@@ -282,11 +282,284 @@ __device__ __forceinline__ void stage_query_b(const BatchSub &B, int n, const fl
     }
 }
 
+// ------------------------------------------------------------ candidate walks of one query
+// The LPQ lanes of a query take candidates c0 + u*LPQ + sub of each step.
+// The walk starts at a pseudo-random candidate of the run of the query's own
+// cells (the densest part of its neighbourhood) and wraps: the first two
+// neighbours found are then spread over the candidates (random sample edges
+// for the components pass, as Afforest draws them) instead of the same few
+// points at the start of a shared run.
+struct QueryWalk {
+    const BatchSub &B;
+    int n, i, p;
+    const int *q;    // the query's runs (LDS)
+    const float *qy; // its coordinates (LDS, stride QPB)
+    const int *qe;   // its cells (LDS, stride QPB)
+    uint64_t qsig;   // its packed cells
+    const int *__restrict__ order;
+    const float *__restrict__ Ys;
+    const uint16_t *__restrict__ Es;
+    const uint64_t *__restrict__ sigb;
+    float s_star, eps;
+    double phirel;
+    int *nbp;        // its two recorded neighbours
+};
+
+__device__ __forceinline__ int walk_rot(const QueryWalk &W, int total) {
+    const int *q = W.q;
+    const int cs0 = q[9 + q[20]], clen = q[10 + q[20]] - cs0;
+    return total > 0 ? cs0 + (int) (((uint32_t) W.p * 2654435761u) % (uint32_t) max(clen, 1)) : 0;
+}
+
+// per 16-bit field (w - q + 1) mod 2^16 is 0, 1 or 2 exactly when the cells
+// differ by at most one (cells < 65535); fields beyond the checked
+// dimensions are zero in both words and pass
+__device__ __forceinline__ bool cells_near_packed(uint64_t w, uint64_t qsig) {
+    typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+    const u16x2 one = {1, 1};
+    const u16x2 d0 = __builtin_bit_cast(u16x2, (uint32_t) w) - __builtin_bit_cast(u16x2, (uint32_t) qsig) + one;
+    const u16x2 d1 =
+        __builtin_bit_cast(u16x2, (uint32_t) (w >> 32)) - __builtin_bit_cast(u16x2, (uint32_t) (qsig >> 32)) + one;
+    const u16x2 m = __builtin_elementwise_max(d0, d1);
+    return m.x <= 2 && m.y <= 2;
+}
+
+// neighbours found in one step: the first two are recorded (the sample edges
+// of the components pass); phi adds the terms of the step
+template <int U, bool PHI>
+__device__ __forceinline__ void walk_tally(const QueryWalk &W, const int (&jj)[U], const bool (&hit)[U],
+                                           const float (&accs)[U], int &found, double &dsum) {
+    constexpr int LPQ = kBLanes;
+    const int sub = __lane_id() % LPQ;
+    const int gshift = (__lane_id() / LPQ) * LPQ;
+    const unsigned long long below = (1ull << sub) - 1ull;
+    if (PHI) {
+        double part = 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (hit[u]) part += phi_term(accs[u], W.eps);
+        for (int o = LPQ / 2; o > 0; o >>= 1) part += __shfl_xor(part, o);
+        dsum += part;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const unsigned long long mine = (__ballot(hit[u]) >> gshift) & ((1ull << LPQ) - 1ull);
+        if (hit[u] && found < 2) {
+            const int at = found + __popcll(mine & below);
+            if (at < 2) W.nbp[at] = W.order[W.B.obase + jj[u]];
+        }
+        found += __popcll(mine);
+    }
+}
+
+__device__ __forceinline__ bool walk_more(const QueryWalk &W, bool phi, int found, double dsum, int total) {
+    if (phi) {
+        // phi may stop once its running double sum is certainly above both thresholds
+        const double thr_max = (double) fmaxf(W.B.phi_w, W.B.phi_d);
+        return dsum * (1.0 - (double) total * W.phirel) < thr_max;
+    }
+    return found < W.B.cap;
+}
+
+// k = K dimensions, every checked one packed.  The walk has no branches
+// around its loads: every lane loads the packed cells and the K coordinates
+// of its candidates at once (a lane past the end re-reads the last one and
+// is masked), so one memory round trip serves a step, and the full sum
+// compared once equals the early-exit test (the partial sums never
+// decrease).  The run bounds stay in registers while the lanes move through
+// a run.
+template <int K, bool PHI>
+__device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double &dsum, bool &more, long long &seen) {
+    constexpr int LPQ = kBLanes, U = 4, STEP = LPQ * U;
+    const BatchSub &B = W.B;
+    const int *q = W.q;
+    const int sub = __lane_id() % LPQ;
+    const int total = q[9 + q[19]];
+    const int rot = walk_rot(W, total);
+    const float *col[K];
+    float qv[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        col[t] = W.Ys + (t == B.pos_m ? B.mcoord : B.coord + (long long) t * W.n);
+        qv[t] = W.qy[t * QPB];
+    }
+    const uint64_t *sigb = B.nchk4 > 0 ? W.sigb : nullptr;
+    int r = 0, rend = q[10], roff = q[0] - q[9];
+    bool wrapped = false;
+    for (int c0 = 0; c0 < total && more; c0 += STEP) {
+        int jj[U];
+        bool valid[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c0 + u * LPQ + sub;
+            valid[u] = c < total;
+            int cr = min(c, total - 1) + rot;
+            if (cr >= total) {
+                cr -= total;
+                if (!wrapped) {
+                    wrapped = true;
+                    r = 0;
+                    rend = q[10];
+                    roff = q[0] - q[9];
+                }
+            }
+            while (cr >= rend) {
+                ++r;
+                rend = q[10 + r];
+                roff = q[r] - q[9 + r];
+            }
+            jj[u] = cr + roff;
+        }
+        uint64_t w[U];
+        float y[U][K];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            // 32-bit byte offsets from the column bases (a subspace's columns hold n < 2^29 entries)
+            const uint32_t at4 = (uint32_t) jj[u] << 2;
+            w[u] = sigb ? *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(sigb) + 2 * at4) : 0ull;
+#pragma unroll
+            for (int t = 0; t < K; ++t) y[u][t] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(col[t]) + at4);
+        }
+        bool hit[U];
+        float accs[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int t = 0; t < K; ++t) {
+                const float diff = qv[t] - y[u][t];
+                const float sq = diff * diff;
+                acc = acc + sq;
+            }
+            hit[u] = valid[u] && jj[u] != W.i && cells_near_packed(w[u], W.qsig) && acc <= W.s_star;
+            accs[u] = acc;
+        }
+        walk_tally<U, PHI>(W, jj, hit, accs, found, dsum);
+        seen += min(STEP, total - c0);
+        more = walk_more(W, PHI, found, dsum, total);
+    }
+}
+
+// k <= kFastDims with every checked dimension packed, k known only at run
+// time (batches that mix dimensionalities): loads predicated on the packed
+// cell test, the sum over the first k of kFastDims terms
+template <bool PHI>
+__device__ __forceinline__ void walk_fast_rt(const QueryWalk &W, int &found, double &dsum, bool &more,
+                                             long long &seen) {
+    constexpr int LPQ = kBLanes, U = 4, STEP = LPQ * U;
+    const BatchSub &B = W.B;
+    const int *q = W.q;
+    const int sub = __lane_id() % LPQ;
+    const int total = q[9 + q[19]];
+    const int rot = walk_rot(W, total);
+    const int k = B.S.k, nc = B.nchk4, pm = B.pos_m;
+    const long long cb = B.coord, mc = B.mcoord;
+    float qv[kFastDims];
+#pragma unroll
+    for (int t = 0; t < kFastDims; ++t) qv[t] = t < k ? W.qy[t * QPB] : 0.0f;
+    int r = 0;
+    bool wrapped = false;
+    for (int c0 = 0; c0 < total && more; c0 += STEP) {
+        int jj[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c0 + u * LPQ + sub;
+            jj[u] = -1;
+            if (c < total) {
+                int cr = c + rot;
+                if (cr >= total) {
+                    cr -= total;
+                    if (!wrapped) {
+                        wrapped = true;
+                        r = 0;
+                    }
+                }
+                jj[u] = run_pos(q, cr, r);
+            }
+        }
+        uint64_t w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) w[u] = (jj[u] >= 0 && nc > 0) ? W.sigb[jj[u]] : 0ull;
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) ok[u] = jj[u] >= 0 && jj[u] != W.i && cells_near(w[u], W.qsig, nc);
+        float y[U][kFastDims];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int t = 0; t < kFastDims; ++t)
+                y[u][t] = (ok[u] && t < k) ? W.Ys[(t == pm ? mc : cb + (long long) t * W.n) + jj[u]] : 0.0f;
+        bool hit[U];
+        float accs[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            float acc = 0.0f;
+#pragma unroll
+            for (int t = 0; t < kFastDims; ++t) {
+                if (t < k) {
+                    const float diff = qv[t] - y[u][t];
+                    const float sq = diff * diff;
+                    acc = acc + sq;
+                }
+            }
+            hit[u] = ok[u] && acc <= W.s_star;
+            accs[u] = acc;
+        }
+        walk_tally<U, PHI>(W, jj, hit, accs, found, dsum);
+        seen += min(STEP, total - c0);
+        more = walk_more(W, PHI, found, dsum, total);
+    }
+}
+
+// any k, cells of the checked dimensions beyond the packed four compared one
+// by one (cand_test)
+template <bool PHI>
+__device__ __forceinline__ void walk_generic(const QueryWalk &W, int &found, double &dsum, bool &more,
+                                             long long &seen) {
+    constexpr int LPQ = kBLanes, U = 4, STEP = LPQ * U;
+    const int *q = W.q;
+    const int sub = __lane_id() % LPQ;
+    const int total = q[9 + q[19]];
+    const int rot = walk_rot(W, total);
+    int r = 0;
+    bool wrapped = false;
+    for (int c0 = 0; c0 < total && more; c0 += STEP) {
+        int jj[U];
+        bool hit[U];
+        float accs[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c0 + u * LPQ + sub;
+            jj[u] = -1;
+            if (c < total) {
+                int cr = c + rot;
+                if (cr >= total) {
+                    cr -= total;
+                    if (!wrapped) {
+                        wrapped = true;
+                        r = 0;
+                    }
+                }
+                jj[u] = run_pos(q, cr, r);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            accs[u] = 0.0f;
+            hit[u] = jj[u] >= 0 && jj[u] != W.i &&
+                     cand_test(W.B, W.n, W.Ys, W.Es, W.sigb, W.qsig, W.qy, W.qe, QPB, jj[u], W.s_star, accs[u]);
+        }
+        walk_tally<U, PHI>(W, jj, hit, accs, found, dsum);
+        seen += min(STEP, total - c0);
+        more = walk_more(W, PHI, found, dsum, total);
+    }
+}
+
 // ------------------------------------------------------------ neighbour counts
 // k_grid_count over a batch: LIST = the compacted queries of each subspace
 // (blocks by segment), else every sorted position of subspace blockIdx.y whose
 // count is still missing (components pass, with the subset bound).
-template <bool LIST, bool PHI>
+template <bool LIST, bool PHI, int K>
 __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__ subs, const int *__restrict__ blk_off,
                                                     int nsub, int n, const int *__restrict__ off,
                                                     const int *__restrict__ order, const float *__restrict__ Ys,
@@ -351,119 +624,18 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
     long long seen = 0;
     int found = 0;
     if (active) {
-        const float *qy = sy + lq;
-        const int *qe = se + lq;
-        const uint64_t qsig = qs[lq];
-        const int gshift = (__lane_id() / LPQ) * LPQ;
-        const unsigned long long below = (1ull << sub) - 1ull;
-        const int cap = B.cap;
-        const int total = q[9 + q[19]];
-        // the walk starts at a pseudo-random candidate of the query and wraps:
-        // the first two neighbours found are then spread over the candidates
-        // (random sample edges for the components pass, as Afforest draws
-        // them) instead of the same few points at the start of a shared run
-        // the walk starts at a pseudo-random candidate of the run of the
-        // query's own cells (the densest part of its neighbourhood) and wraps
-        const int cs0 = q[9 + q[20]], clen = q[10 + q[20]] - cs0;
-        const int rot = total > 0 ? cs0 + (int) (((uint32_t) p * 2654435761u) % (uint32_t) max(clen, 1)) : 0;
-        bool wrapped = false;
-        int *nbp = nb + 2 * (B.base + p);
-        // k <= 8 with every checked dimension packed: all loads of a step are
-        // issued before any is used; the full sum compared once equals the
-        // early-exit test (the partial sums never decrease)
-        const bool fast = k <= kFastDims && B.restmask == 0;
-        const int nc = B.nchk4;
-        float qv[kFastDims];
-#pragma unroll
-        for (int t = 0; t < kFastDims; ++t) qv[t] = t < k ? qy[t * QPB] : 0.0f;
-        // column bases (uniform over the block: the subspace is)
-        const int pm = B.pos_m;
-        const long long cb = B.coord, mc = B.mcoord;
-        int r = 0;
-        // phi: running double sum of the terms (the same on all lanes of the
-        // query); it may stop once it is certainly above both thresholds
         double dsum = 0.0;
-        const double thr_max = PHI ? (double) fmaxf(B.phi_w, B.phi_d) : 0.0;
         bool more = true;
-        for (int c0 = 0; c0 < total && more; c0 += STEP) {
-            int jj[U];
-            bool hit[U];
-            float accs[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int c = c0 + u * LPQ + sub;
-                jj[u] = -1;
-                if (c < total) {
-                    int cr = c + rot;
-                    if (cr >= total) {
-                        cr -= total;
-                        if (!wrapped) {
-                            wrapped = true;
-                            r = 0;
-                        }
-                    }
-                    jj[u] = run_pos(q, cr, r);
-                }
-            }
-            if (fast) {
-                uint64_t w[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) w[u] = (jj[u] >= 0 && nc > 0) ? sigb[jj[u]] : 0ull;
-                bool ok[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) ok[u] = jj[u] >= 0 && jj[u] != i && cells_near(w[u], qsig, nc);
-                float y[U][kFastDims];
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-#pragma unroll
-                    for (int t = 0; t < kFastDims; ++t)
-                        y[u][t] = (ok[u] && t < k) ? Ys[(t == pm ? mc : cb + (long long) t * n) + jj[u]] : 0.0f;
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    float acc = 0.0f;
-#pragma unroll
-                    for (int t = 0; t < kFastDims; ++t) {
-                        if (t < k) {
-                            const float diff = qv[t] - y[u][t];
-                            const float sq = diff * diff;
-                            acc = acc + sq;
-                        }
-                    }
-                    hit[u] = ok[u] && acc <= s_star;
-                    accs[u] = acc;
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    accs[u] = 0.0f;
-                    hit[u] = jj[u] >= 0 && jj[u] != i &&
-                             cand_test(B, n, Ys, Es, sigb, qsig, qy, qe, QPB, jj[u], s_star, accs[u]);
-                }
-            }
-            if (PHI) {
-                double part = 0.0;
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (hit[u]) part += phi_term(accs[u], eps);
-                for (int o = LPQ / 2; o > 0; o >>= 1) part += __shfl_xor(part, o);
-                dsum += part;
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const unsigned long long mine = (__ballot(hit[u]) >> gshift) & ((1ull << LPQ) - 1ull);
-                // the first two neighbours found: the sample edges of the components pass
-                if (hit[u] && found < 2) {
-                    const int at = found + __popcll(mine & below);
-                    if (at < 2) nbp[at] = order[B.obase + jj[u]];
-                }
-                found += __popcll(mine);
-            }
-            seen += min(STEP, total - c0);
-            if (PHI)
-                more = dsum * (1.0 - (double) total * phirel) < thr_max;
-            else
-                more = found < cap;
-        }
+        const QueryWalk W{B, n, i, p, q, sy + lq, se + lq, qs[lq], order, Ys, Es, sigb, s_star, eps, phirel,
+                          nb + 2 * (B.base + p)};
+        // K > 0: every subspace of the launch has k = K with its checked
+        // dimensions packed (the host checks), so the walk is compiled for it
+        if constexpr (K > 0)
+            walk_fast<K, PHI>(W, found, dsum, more, seen);
+        else if (k <= kFastDims && B.restmask == 0)
+            walk_fast_rt<PHI>(W, found, dsum, more, seen);
+        else
+            walk_generic<PHI>(W, found, dsum, more, seen);
         if (PHI) {
             // decision bits: 1 weak-dense, 2 dense; a threshold inside the
             // rounding band sends the point to the leaf-order recomputation
@@ -1290,7 +1462,31 @@ hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySe
     return hipGetLastError();
 }
 
-hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, int nblk, int n, int kmax,
+template <int K>
+static void launch_count_k(const BatchSub *subs, int nsub, const int *blk_off, int nblk, int n, size_t lds,
+                           const BatchIndex &X, float s_star, float eps, double phirel, bool phi,
+                           unsigned long long *stat, hipStream_t st) {
+    if (blk_off) {
+        if (phi)
+            LAUNCH((k_b_count<true, true, K>), dim3(nblk), dim3(kBlock), lds, st, subs, blk_off, nsub, n, X.off,
+                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb, X.fbcap, stat);
+        else
+            LAUNCH((k_b_count<true, false, K>), dim3(nblk), dim3(kBlock), lds, st, subs, blk_off, nsub, n, X.off,
+                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb, X.fbcap, stat);
+    } else {
+        const dim3 grid(query_blocks_n(n), nsub);
+        if (phi)
+            LAUNCH((k_b_count<false, true, K>), grid, dim3(kBlock), lds, st, subs, (const int *) nullptr, nsub, n,
+                   X.off, X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb,
+                   X.fbcap, stat);
+        else
+            LAUNCH((k_b_count<false, false, K>), grid, dim3(kBlock), lds, st, subs, (const int *) nullptr, nsub, n,
+                   X.off, X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb,
+                   X.fbcap, stat);
+    }
+}
+
+hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, int nblk, int n, int kmax, int kfast,
                           const BatchIndex &X, float s_star, float eps, bool phi, bool phi_all_exact,
                           const DevData &D, const int *rank_order, unsigned long long *stat, const KernelTimer &tm,
                           hipStream_t st) {
@@ -1301,21 +1497,15 @@ hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, in
     if (phi) LAUNCH(k_b_fill<int>, dim3(1), dim3(kBlock), 0, st, X.nfb, 0, (size_t) 1);
     if (tm.start) (void) hipEventRecord(tm.start, st);
     const size_t lds = count_lds(kmax);
-    if (blk_off) {
-        if (phi)
-            LAUNCH((k_b_count<true, true>), dim3(nblk), dim3(kBlock), lds, st, subs, blk_off, nsub, n, X.off, X.order,
-                   X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb, X.fbcap, stat);
-        else
-            LAUNCH((k_b_count<true, false>), dim3(nblk), dim3(kBlock), lds, st, subs, blk_off, nsub, n, X.off,
-                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb, X.fbcap, stat);
-    } else {
-        const dim3 grid(query_blocks_n(n), nsub);
-        if (phi)
-            LAUNCH((k_b_count<false, true>), grid, dim3(kBlock), lds, st, subs, (const int *) nullptr, nsub, n, X.off,
-                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb, X.fbcap, stat);
-        else
-            LAUNCH((k_b_count<false, false>), grid, dim3(kBlock), lds, st, subs, (const int *) nullptr, nsub, n, X.off,
-                   X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb, X.fbcap, stat);
+    switch (kfast) {
+    case 1: launch_count_k<1>(subs, nsub, blk_off, nblk, n, lds, X, s_star, eps, phirel, phi, stat, st); break;
+    case 2: launch_count_k<2>(subs, nsub, blk_off, nblk, n, lds, X, s_star, eps, phirel, phi, stat, st); break;
+    case 3: launch_count_k<3>(subs, nsub, blk_off, nblk, n, lds, X, s_star, eps, phirel, phi, stat, st); break;
+    case 4: launch_count_k<4>(subs, nsub, blk_off, nblk, n, lds, X, s_star, eps, phirel, phi, stat, st); break;
+    case 5: launch_count_k<5>(subs, nsub, blk_off, nblk, n, lds, X, s_star, eps, phirel, phi, stat, st); break;
+    case 6: launch_count_k<6>(subs, nsub, blk_off, nblk, n, lds, X, s_star, eps, phirel, phi, stat, st); break;
+    case 7: launch_count_k<7>(subs, nsub, blk_off, nblk, n, lds, X, s_star, eps, phirel, phi, stat, st); break;
+    default: launch_count_k<0>(subs, nsub, blk_off, nblk, n, lds, X, s_star, eps, phirel, phi, stat, st); break;
     }
     if (tm.stop) (void) hipEventRecord(tm.stop, st);
     if (phi)
