@@ -270,8 +270,8 @@ hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, in
                           const DevData &D, const int *rank_order, unsigned long long *stat, const KernelTimer &tm,
                           hipStream_t st);
 // dense components of nsub subspaces whose counts are complete
-hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, const BatchIndex &X, float s_star,
-                               hipStream_t st);
+hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, int kfast, const BatchIndex &X,
+                               float s_star, hipStream_t st);
 // weak test: count >= thr, or bit wbit of the phi decision bits when wbit != 0;
 // lists are written only for groups with at least min_size weak-dense points
 hipError_t launch_b_partition(const PartPair *pairs, const int *blk_off, int npairs, int nblk, const uint8_t *cellC,

@@ -655,7 +655,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
             sync();
             tc1 = std::chrono::steady_clock::now();
         }
-        HIP_OK(launch_b_components(dhc, ncomp, n, kmax, X, s_star, st));
+        HIP_OK(launch_b_components(dhc, ncomp, n, kmax, kfast, X, s_star, st));
         if (dbg) {
             sync();
             tc2 = std::chrono::steady_clock::now();

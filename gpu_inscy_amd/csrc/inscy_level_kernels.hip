@@ -568,7 +568,7 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
                                                     float s_star, float eps, double phirel, int *__restrict__ nb,
                                                     int2 *__restrict__ fb, int *__restrict__ nfb, int fbcap,
                                                     unsigned long long *__restrict__ stat) {
-    constexpr int LPQ = kBLanes, U = 4, STEP = LPQ * U;
+    constexpr int LPQ = kBLanes;
     extern __shared__ char smem[];
     int b, blk;
     if (LIST) {
@@ -760,6 +760,12 @@ __global__ void k_b_giant(const BatchSub *__restrict__ subs, int n, const int *_
     if (t == 0) giant[blockIdx.x] = best_c[0] > 0 ? best_r[0] : -1;
 }
 
+// Full phase: every dense point outside the giant component links to all its
+// dense neighbours.  K > 0: every subspace of the launch has k = K with its
+// checked dimensions packed, and a step loads the dense flags, packed cells,
+// point ids and K coordinates of U candidates per lane at once (a lane past
+// the end re-reads the last candidate and is masked).
+template <int K>
 __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restrict__ subs, int n,
                                                         const int *__restrict__ off, const int *__restrict__ order,
                                                         const float *__restrict__ Ys, const uint16_t *__restrict__ Es,
@@ -796,16 +802,80 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
     const uint64_t qsig = qs[lq];
     const int total = q[9 + q[19]];
     int rp = uf_find(pr, p);
-    int r = 0;
-    for (int c = sub; c < total; c += LPQ) {
-        const int jj = run_pos(q, c, r);
-        float acc;
-        if (jj == i || !ds[jj] || !cand_test(B, n, Ys, Es, sigb, qsig, qy, qe, QPB, jj, s_star, acc)) continue;
-        const int qq = ord[jj];
-        // parents only move up to roots: a parent equal to p's root is p's set
-        if (ld_relaxed(&pr[qq]) == rp || uf_find(pr, qq) == rp) continue;
-        uf_union(pr, p, qq);
-        rp = uf_find(pr, p);
+    if constexpr (K > 0) {
+        constexpr int U = 4, STEP = LPQ * U;
+        const float *col[K];
+        float qv[K];
+#pragma unroll
+        for (int t = 0; t < K; ++t) {
+            col[t] = Ys + (t == B.pos_m ? B.mcoord : B.coord + (long long) t * n);
+            qv[t] = qy[t * QPB];
+        }
+        const bool packed = B.nchk4 > 0;
+        int r = 0, rend = q[10], roff = q[0] - q[9];
+        for (int c0 = 0; c0 < total; c0 += STEP) {
+            int jj[U];
+            bool valid[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int c = c0 + u * LPQ + sub;
+                valid[u] = c < total;
+                const int cr = min(c, total - 1);
+                while (cr >= rend) {
+                    ++r;
+                    rend = q[10 + r];
+                    roff = q[r] - q[9 + r];
+                }
+                jj[u] = cr + roff;
+            }
+            uint64_t w[U];
+            uint8_t dn[U];
+            int qq[U];
+            float y[U][K];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint32_t at = (uint32_t) jj[u];
+                dn[u] = ds[at];
+                qq[u] = ord[at];
+                w[u] = packed ? sigb[at] : 0ull;
+#pragma unroll
+                for (int t = 0; t < K; ++t) y[u][t] = col[t][at];
+            }
+            bool hit[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int t = 0; t < K; ++t) {
+                    const float diff = qv[t] - y[u][t];
+                    const float sq = diff * diff;
+                    acc = acc + sq;
+                }
+                hit[u] = valid[u] && jj[u] != i && dn[u] && cells_near_packed(w[u], qsig) && acc <= s_star;
+            }
+            // parents only move up to roots: a parent equal to p's root is p's set
+            int pq[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) pq[u] = hit[u] ? ld_relaxed(&pr[qq[u]]) : rp;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (!hit[u] || pq[u] == rp || uf_find(pr, qq[u]) == rp) continue;
+                uf_union(pr, p, qq[u]);
+                rp = uf_find(pr, p);
+            }
+        }
+    } else {
+        int r = 0;
+        for (int c = sub; c < total; c += LPQ) {
+            const int jj = run_pos(q, c, r);
+            float acc;
+            if (jj == i || !ds[jj] || !cand_test(B, n, Ys, Es, sigb, qsig, qy, qe, QPB, jj, s_star, acc)) continue;
+            const int qq = ord[jj];
+            // parents only move up to roots: a parent equal to p's root is p's set
+            if (ld_relaxed(&pr[qq]) == rp || uf_find(pr, qq) == rp) continue;
+            uf_union(pr, p, qq);
+            rp = uf_find(pr, p);
+        }
     }
 }
 
@@ -1513,8 +1583,8 @@ hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, in
     return hipGetLastError();
 }
 
-hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, const BatchIndex &X, float s_star,
-                               hipStream_t st) {
+hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, int kfast, const BatchIndex &X,
+                               float s_star, hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
     LAUNCH(k_b_dense_init, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.order, X.dsorted, X.par);
     for (int round = 0; round < 2; ++round) {
@@ -1522,8 +1592,22 @@ hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, 
         LAUNCH(k_b_uf_compress, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.par);
     }
     LAUNCH(k_b_giant, dim3(nsub), dim3(256), 0, st, subs, n, X.par, X.giant);
-    LAUNCH(k_b_link_full, dim3(query_blocks_n(n), nsub), dim3(kBlock), count_lds(kmax), st, subs, n, X.off, X.order,
-           X.Ys, X.Es, X.sig, X.dsorted, s_star, X.giant, X.par);
+    const dim3 grid(query_blocks_n(n), nsub);
+    const size_t lds = count_lds(kmax);
+#define LINK_FULL(K)                                                                                                   \
+    LAUNCH(k_b_link_full<K>, grid, dim3(kBlock), lds, st, subs, n, X.off, X.order, X.Ys, X.Es, X.sig, X.dsorted,      \
+           s_star, X.giant, X.par)
+    switch (kfast) {
+    case 1: LINK_FULL(1); break;
+    case 2: LINK_FULL(2); break;
+    case 3: LINK_FULL(3); break;
+    case 4: LINK_FULL(4); break;
+    case 5: LINK_FULL(5); break;
+    case 6: LINK_FULL(6); break;
+    case 7: LINK_FULL(7); break;
+    default: LINK_FULL(0); break;
+    }
+#undef LINK_FULL
     LAUNCH(k_b_uf_flatten, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.par);
     return hipGetLastError();
 }
