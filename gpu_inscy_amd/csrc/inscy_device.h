@@ -120,11 +120,19 @@ __device__ __forceinline__ void stage_query(const SubspaceArg &S, int n, const f
     }
 }
 
+// Union-find over parents that only move towards smaller ids (par[x] <= x,
+// x a root iff par[x] == x; unions link the larger root under the smaller
+// with a device-scope compare-and-swap).  Every value ever held by par[x] is
+// an ancestor of x, so the finds use plain loads and stores: a copy that an
+// XCD's L2 still holds after another XCD changed the word yields an older
+// ancestor, never a wrong set.  uf_union retries with the value its failed
+// swap returns (always current, and smaller), so it ends; a find with no
+// union running beside it (compression, flattening) returns the current root.
 __device__ __forceinline__ int uf_find(int *par, int x) {
-    int p = ld_relaxed(&par[x]);
+    int p = par[x];
     while (p != x) {
-        int g = ld_relaxed(&par[p]);
-        if (g != p) st_relaxed(&par[x], g);  // path halving: g is an ancestor of x
+        int g = par[p];
+        if (g != p) par[x] = g;  // path halving: g is an ancestor of x
         x = p;
         p = g;
     }

@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_link_sample(int n, SubspaceArg S, Gr
 
 __global__ void k_uf_compress(int *__restrict__ par, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && ld_relaxed(&par[i]) >= 0) st_relaxed(&par[i], uf_find(par, i));
+    if (i < n && par[i] >= 0) par[i] = uf_find(par, i);
 }
 
 // one block of 256: the most frequent root among 256 evenly spaced points
@@ -345,7 +345,7 @@ __global__ void k_giant(const int *__restrict__ par, int n, int *__restrict__ gi
     __shared__ int best_r[256];
     const int t = threadIdx.x;
     int p = (int) (((long long) t * n) / 256);
-    roots[t] = p < n ? ld_relaxed(&par[p]) : -1;
+    roots[t] = p < n ? par[p] : -1;
     __syncthreads();
     int r = roots[t], c = 0;
     if (r >= 0)
@@ -377,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_link_full(int n, SubspaceArg S, Grid
     const int i = blockIdx.x * QPB + lq;
     bool active = i < n && dsorted[i];
     int p = active ? order[i] : 0;
-    if (active && ld_relaxed(&par[p]) == *giant) active = false;
+    if (active && par[p] == *giant) active = false;
     if (sub == 0 && active) stage_query(S, n, Ys, Es, i, sy + lq, se + lq, QPB);
     __syncthreads();
     if (!active) return;
@@ -398,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_link_full(int n, SubspaceArg S, Grid
 __global__ void k_uf_flatten(int *__restrict__ par, int n, int *__restrict__ comp, int *__restrict__ csize) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int c = -1;
-    if (i < n && ld_relaxed(&par[i]) >= 0) c = uf_find(par, i);
+    if (i < n && par[i] >= 0) c = uf_find(par, i);
     wave_add_by_key(csize, c, c >= 0);
     if (i < n) comp[i] = c;
 }

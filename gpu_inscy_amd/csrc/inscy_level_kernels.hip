@@ -721,14 +721,14 @@ __global__ void k_b_link_sample(const BatchSub *__restrict__ subs, int n, const 
     // after the first round's compression every point's parent is its root:
     // equal parents are one set, decided without reading the root's own entry
     // (which every point of a large component would read at once)
-    if (ld_relaxed(&pr[p]) == ld_relaxed(&pr[q])) return;
+    if (pr[p] == pr[q]) return;
     uf_union(pr, p, q);
 }
 
 __global__ void k_b_uf_compress(const BatchSub *__restrict__ subs, int n, int *__restrict__ par) {
     int *pr = par + subs[blockIdx.y].base;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && ld_relaxed(&pr[i]) >= 0) st_relaxed(&pr[i], uf_find(pr, i));
+    if (i < n && pr[i] >= 0) pr[i] = uf_find(pr, i);
 }
 
 // one block of 256 per subspace: the most frequent root among 256 evenly
@@ -741,7 +741,7 @@ __global__ void k_b_giant(const BatchSub *__restrict__ subs, int n, const int *_
     const int *pr = par + subs[blockIdx.x].base;
     const int t = threadIdx.x;
     const int p = (int) (((long long) t * n) / 256);
-    roots[t] = p < n ? ld_relaxed(&pr[p]) : -1;
+    roots[t] = p < n ? pr[p] : -1;
     __syncthreads();
     int r = roots[t], c = 0;
     if (r >= 0)
@@ -788,7 +788,7 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
     int *pr = par + B.base;
     bool active = i < n && ds[i];
     const int p = active ? ord[i] : 0;
-    if (active && ld_relaxed(&pr[p]) == giant[blockIdx.y]) active = false;
+    if (active && pr[p] == giant[blockIdx.y]) active = false;
     int *q = qr + lq * kRangeWords;
     if (sub == 0 && active) {
         stage_query_b(B, n, Ys, Es, i, sy + lq, se + lq, QPB);
@@ -856,7 +856,7 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
             // parents only move up to roots: a parent equal to p's root is p's set
             int pq[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) pq[u] = hit[u] ? ld_relaxed(&pr[qq[u]]) : rp;
+            for (int u = 0; u < U; ++u) pq[u] = hit[u] ? pr[qq[u]] : rp;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (!hit[u] || pq[u] == rp || uf_find(pr, qq[u]) == rp) continue;
@@ -872,7 +872,7 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
             if (jj == i || !ds[jj] || !cand_test(B, n, Ys, Es, sigb, qsig, qy, qe, QPB, jj, s_star, acc)) continue;
             const int qq = ord[jj];
             // parents only move up to roots: a parent equal to p's root is p's set
-            if (ld_relaxed(&pr[qq]) == rp || uf_find(pr, qq) == rp) continue;
+            if (pr[qq] == rp || uf_find(pr, qq) == rp) continue;
             uf_union(pr, p, qq);
             rp = uf_find(pr, p);
         }
@@ -884,7 +884,7 @@ __global__ void k_b_uf_flatten(const BatchSub *__restrict__ subs, int n, int *__
     int *pr = par + B.base;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int c = -1;
-    if (i < n && ld_relaxed(&pr[i]) >= 0) c = uf_find(pr, i);
+    if (i < n && pr[i] >= 0) c = uf_find(pr, i);
     wave_add_by_key(B.csize, c, c >= 0);
     if (i < n) B.comp[i] = c;
 }
