@@ -24,8 +24,9 @@ namespace inscy {
 
 namespace {
 
-constexpr int kBLanes = 4;  // lanes per query of the batched neighbour passes
+constexpr int kBLanes = 4;  // lanes per query of the batched neighbour passes (one quad: the walks sum over it with DPP)
 constexpr int QPB = kBlock / kBLanes;
+static_assert(kBLanes == 4, "the walks' tally sums a query's lanes as one DPP quad");
 static_assert(QPB == kQueriesPerBlock, "queries per block of the host and the kernels");
 
 __device__ __forceinline__ int origin(int x) { return x >= 0 ? x : ~x; }
@@ -383,7 +384,7 @@ __device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double
         col[t] = W.Ys + (t == B.pos_m ? B.mcoord : B.coord + (long long) t * W.n);
         qv[t] = W.qy[t * QPB];
     }
-    const uint64_t *sigb = B.nchk4 > 0 ? W.sigb : nullptr;
+    const bool packed = B.nchk4 > 0;
     int r = 0, rend = q[10], roff = q[0] - q[9];
     bool wrapped = false;
     for (int c0 = 0; c0 < total && more; c0 += STEP) {
@@ -416,7 +417,7 @@ __device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double
         for (int u = 0; u < U; ++u) {
             // 32-bit byte offsets from the column bases (a subspace's columns hold n < 2^29 entries)
             const uint32_t at4 = (uint32_t) jj[u] << 2;
-            w[u] = sigb ? *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(sigb) + 2 * at4) : 0ull;
+            w[u] = packed ? *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(W.sigb) + 2 * at4) : 0ull;
 #pragma unroll
             for (int t = 0; t < K; ++t) y[u][t] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(col[t]) + at4);
         }
@@ -431,10 +432,26 @@ __device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double
                 const float sq = diff * diff;
                 acc = acc + sq;
             }
-            hit[u] = valid[u] && jj[u] != W.i && cells_near_packed(w[u], W.qsig) && acc <= W.s_star;
+            // every term evaluated (no short cut): the test compiles without branches
+            hit[u] = valid[u] & (jj[u] != W.i) & cells_near_packed(w[u], W.qsig) & (acc <= W.s_star);
             accs[u] = acc;
         }
-        walk_tally<U, PHI>(W, jj, hit, accs, found, dsum);
+        if (PHI) {
+            walk_tally<U, PHI>(W, jj, hit, accs, found, dsum);
+        } else {
+            // the query's hits of the step summed over its 4 lanes (one quad) with
+            // two DPP moves; the ballots only while the two sample edges are open
+            int h = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) h += hit[u] ? 1 : 0;
+            h += __builtin_amdgcn_update_dpp(0, h, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+            h += __builtin_amdgcn_update_dpp(0, h, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+            if (found < 2 && h > 0) {
+                int f = found;
+                walk_tally<U, false>(W, jj, hit, accs, f, dsum);
+            }
+            found += h;
+        }
         seen += min(STEP, total - c0);
         more = walk_more(W, PHI, found, dsum, total);
     }
@@ -851,7 +868,7 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
                     const float sq = diff * diff;
                     acc = acc + sq;
                 }
-                hit[u] = valid[u] && jj[u] != i && dn[u] && cells_near_packed(w[u], qsig) && acc <= s_star;
+                hit[u] = valid[u] & (jj[u] != i) & (dn[u] != 0) & cells_near_packed(w[u], qsig) & (acc <= s_star);
             }
             // parents only move up to roots: a parent equal to p's root is p's set
             int pq[U];
