@@ -446,7 +446,9 @@ __global__ void k_path_insert(DevData D, uint64_t smask, const int *__restrict__
         uint64_t entry = (e & 0xFFFFFFFF00000000ull) | (uint32_t) w;
         uint64_t slot = e & tmask;
         while (true) {
-            uint64_t cur = __hip_atomic_load(&table[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // a slot is written once, by the swap from empty: a plain load that
+            // still sees it empty is corrected by the failed swap, a full one is final
+            uint64_t cur = table[slot];
             if (cur == kEmpty) {
                 cur = atomicCAS((unsigned long long *) &table[slot], (unsigned long long) kEmpty,
                                 (unsigned long long) entry);
@@ -479,7 +481,7 @@ __global__ void k_path_check(DevData D, uint64_t smask, const int *__restrict__ 
                 uint64_t e = mix64(h ^ (0x9E3779B97F4A7C15ull * (uint64_t) (k + 1)));
                 uint64_t slot = e & tmask;
                 while (true) {
-                    uint64_t cur = __hip_atomic_load(&table[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    uint64_t cur = table[slot];  // complete: the inserts ran in an earlier launch
                     if (cur == kEmpty) break;
                     if ((cur >> 32) == (e >> 32) && same_prefix(D, smask, x, (int) (uint32_t) cur, t)) {
                         keep |= 1ull << k;

@@ -1044,7 +1044,9 @@ __global__ void k_b_path_insert(DevData D, const PruneChild *__restrict__ C, con
         const uint64_t entry = (e & 0xFFFFFFFF00000000ull) | (uint32_t) w;
         uint64_t slot = e & P.tmask;
         while (true) {
-            uint64_t cur = __hip_atomic_load(&P.table[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // a slot is written once, by the swap from empty: a plain load that
+            // still sees it empty is corrected by the failed swap, a full one is final
+            uint64_t cur = P.table[slot];
             if (cur == kEmpty) {
                 cur = atomicCAS((unsigned long long *) &P.table[slot], (unsigned long long) kEmpty,
                                 (unsigned long long) entry);
@@ -1078,8 +1080,7 @@ __global__ void k_b_path_check(DevData D, const PruneChild *__restrict__ C, cons
                 const uint64_t e = mix64(h ^ (0x9E3779B97F4A7C15ull * (uint64_t) (k + 1)));
                 uint64_t slot = e & P.tmask;
                 while (true) {
-                    const uint64_t cur =
-                        __hip_atomic_load(&P.table[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint64_t cur = P.table[slot];  // complete: the inserts ran in an earlier launch
                     if (cur == kEmpty) break;
                     if ((cur >> 32) == (e >> 32) && same_prefix(D, smask, xo, (int) (uint32_t) cur, t)) {
                         keep |= 1ull << k;
