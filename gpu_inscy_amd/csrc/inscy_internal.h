@@ -116,6 +116,7 @@ struct BatchSub {
     // of S the neighbourhood tree walk matches in level order (all of S in
     // the identity order; ScyTreeNode.cu:910-915)
     uint64_t matchmask;
+    const uint16_t *cellE;  // [d][n] eps-grid cells by point (the query's own cells are read here)
 };
 
 // Device buffers of one batch's grid indexes.
@@ -124,7 +125,7 @@ struct BatchIndex {
     int *hist, *off;      // [nhist]
     int *order;           // [B*n] point id at each sorted position
     float *Ys;            // coordinates in sorted order
-    uint16_t *Es;         // eps-grid cells in sorted order
+    uint16_t *Es;         // eps-grid cells in sorted order (gathered only when a walk tests cells one by one)
     uint8_t *qflag;       // [B*n] query flag per (global) sorted position
     int *qlist;           // global sorted positions of the queries, ascending
     int *nsel;            // [1] number of queries
@@ -255,7 +256,10 @@ hipError_t launch_root_paths(const uint64_t *nearm, int n, int *outx, uint64_t *
 hipError_t launch_b_flags(const FlagTree *T, const int *blk_off, int nt, int nblk, const uint8_t *cellC, int n,
                           int cells, int *flags, hipStream_t st);
 // grid indexes of nsub subspaces: keys, one scan over all grid cells, scatter
-hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, hipStream_t st);
+// need_es: gather the sorted cell columns Es (a launch of the batch runs the
+// walk for any k, which tests cells one by one)
+hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, bool need_es,
+                          hipStream_t st);
 // queries in sorted order: flag the points of the segments, compact per subspace
 hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySeg *segs, const int *blk_off,
                             int nseg, int nblk, const BatchIndex &X, hipStream_t st);

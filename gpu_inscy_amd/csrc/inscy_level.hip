@@ -345,6 +345,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         set_bounds(c.s, x);
         x.dbit = phi_mode ? 2 : 0;
         x.matchmask = match_of(x.S);
+        x.cellE = D.cellE;
         x.phi_w = phi_weak;
         x.phi_d = phi_dense[x.S.k];
         x.nchk4 = 0;
@@ -393,7 +394,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.scan_temp = batch.take<char>(X.scan_bytes);
     HIP_OK(launch_fill_i32(cntB, -1, (size_t) B * n, st));
     BatchSub *dsubs = upload(batch, hs);
-    HIP_OK(launch_b_index(D, dsubs, B, X, st));
+    HIP_OK(launch_b_index(D, dsubs, B, X, kfast == 0, st));
 
     // ---- queries: the points of the trees each subspace refines, in sorted order
     std::vector<QuerySeg> qs;

@@ -85,7 +85,7 @@ __global__ void k_b_scatter(DevData D, const BatchSub *__restrict__ subs, const 
 // them when nothing is shared), and the packed cells of the checked ones
 __global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ reps,
                            const int *__restrict__ order, float *__restrict__ Ys, uint16_t *__restrict__ Es,
-                           uint64_t *__restrict__ sigS) {
+                           uint64_t *__restrict__ sigS, int need_es) {
     const BatchSub &B = subs[reps[blockIdx.y]];
     const int n = D.n;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -95,7 +95,7 @@ __global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const i
         if (t == B.pos_m) continue;
         const int s = B.S.dims[t];
         Ys[B.coord + (long long) t * n + i] = D.Xt[(size_t) s * n + p];
-        Es[B.coord + (long long) t * n + i] = D.cellE[(size_t) s * n + p];
+        if (need_es) Es[B.coord + (long long) t * n + i] = D.cellE[(size_t) s * n + p];
     }
     uint64_t w = 0;
     for (int f = 0; f < B.nchk4; ++f)
@@ -106,7 +106,7 @@ __global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const i
 // every subspace: its own column and its packed cells
 __global__ void k_b_gather_own(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ order,
                                float *__restrict__ Ys, uint16_t *__restrict__ Es, const uint64_t *__restrict__ sigS,
-                               uint64_t *__restrict__ sig) {
+                               uint64_t *__restrict__ sig, int need_es) {
     const BatchSub &B = subs[blockIdx.y];
     const int n = D.n;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -117,7 +117,7 @@ __global__ void k_b_gather_own(DevData D, const BatchSub *__restrict__ subs, con
         const int s = B.S.dims[B.pos_m];
         const uint16_t e = D.cellE[(size_t) s * n + p];
         Ys[B.mcoord + i] = D.Xt[(size_t) s * n + p];
-        Es[B.mcoord + i] = e;
+        if (need_es) Es[B.mcoord + i] = e;
         for (int f = 0; f < B.nchk4; ++f)
             if (B.chk[f] == B.pos_m) w |= (uint64_t) e << (16 * f);
     }
@@ -272,14 +272,12 @@ __device__ __forceinline__ void lds_layout(char *smem, int k, float *&sy, int *&
     qr = reinterpret_cast<int *>(smem + 8 * (size_t) QPB * k + 8 * QPB);
 }
 
-// the query's coordinates and cells into LDS
-__device__ __forceinline__ void stage_query_b(const BatchSub &B, int n, const float *__restrict__ Ys,
-                                              const uint16_t *__restrict__ Es, int i, float *sy, int *se,
-                                              int stride) {
+// the query's coordinates (sorted position i) and cells (point p) into LDS
+__device__ __forceinline__ void stage_query_b(const BatchSub &B, int n, const float *__restrict__ Ys, int p, int i,
+                                              float *sy, int *se, int stride) {
     for (int t = 0; t < B.S.k; ++t) {
-        const long long col = col_of(B, t, n);
-        sy[t * stride] = Ys[col + i];
-        se[t * stride] = Es[col + i];
+        sy[t * stride] = Ys[col_of(B, t, n) + i];
+        se[t * stride] = B.cellE[(size_t) B.S.dims[t] * n + p];
     }
 }
 
@@ -633,7 +631,7 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
     const int base0 = (int) B.obase;
     int *q = qr + lq * kRangeWords;
     if (sub == 0 && active) {
-        stage_query_b(B, n, Ys, Es, i, sy + lq, se + lq, QPB);
+        stage_query_b(B, n, Ys, p, i, sy + lq, se + lq, QPB);
         qs[lq] = sigb[i];
         build_runs(B.G, off + B.hoff, se + lq, QPB, base0, q);
     }
@@ -808,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
     if (active && pr[p] == giant[blockIdx.y]) active = false;
     int *q = qr + lq * kRangeWords;
     if (sub == 0 && active) {
-        stage_query_b(B, n, Ys, Es, i, sy + lq, se + lq, QPB);
+        stage_query_b(B, n, Ys, p, i, sy + lq, se + lq, QPB);
         qs[lq] = sigb[i];
         build_runs(B.G, off + B.hoff, se + lq, QPB, (int) B.obase, q);
     }
@@ -1511,7 +1509,8 @@ hipError_t launch_b_flags(const FlagTree *T, const int *blk_off, int nt, int nbl
     return hipGetLastError();
 }
 
-hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, hipStream_t st) {
+hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, bool need_es,
+                          hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
     LAUNCH(k_b_fill<int>, dim3(blocks_for(X.nhist)), dim3(kBlock), 0, st, X.hist, 0, (size_t) X.nhist);
     LAUNCH(k_b_keys, dim3(blocks_for(D.n), X.nrep), dim3(kBlock), 0, st, D, subs, X.reps, X.keys, X.rank, X.hist);
@@ -1521,9 +1520,9 @@ hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, cons
     LAUNCH(k_b_scatter, dim3(blocks_for(D.n), X.nrep), dim3(kBlock), 0, st, D, subs, X.reps, X.keys, X.rank, X.off,
            X.order);
     LAUNCH(k_b_gather, dim3(blocks_for(D.n), X.nrep), dim3(kBlock), 0, st, D, subs, X.reps, X.order, X.Ys, X.Es,
-           X.sigS);
+           X.sigS, (int) need_es);
     LAUNCH(k_b_gather_own, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.order, X.Ys, X.Es, X.sigS,
-           X.sig);
+           X.sig, (int) need_es);
     return hipGetLastError();
 }
 
