@@ -150,6 +150,7 @@ def lib():
     L.inscy_mlp_round.restype = ctypes.c_int
     L.inscy_debug_trace.argtypes = [ctypes.c_int]
     L.inscy_debug_phi_exact.argtypes = [ctypes.c_int]
+    L.inscy_debug_generic_walk.argtypes = [ctypes.c_int]
     L.inscy_debug_trace_rows.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]
     L.inscy_debug_trace_rows.restype = ctypes.c_int64
     _lib = L
@@ -296,6 +297,11 @@ def debug_trace(on: bool) -> None:
 
 def debug_phi_exact(on: bool) -> None:
     lib().inscy_debug_phi_exact(int(on))
+
+
+def debug_generic_walk(on: bool) -> None:
+    """Test hook: run the neighbour walks for k known only at run time."""
+    lib().inscy_debug_generic_walk(int(on))
 
 
 def debug_trace_rows() -> np.ndarray:

@@ -396,6 +396,7 @@ public:
     int prune_min_depth = 1;
     bool collect = false;
     bool phi_all_exact = false;  // test hook: every phi by the leaf-order recomputation
+    bool generic_walk = false;   // test hook: no neighbour walk compiled for its k
 
     void run(inscy_result &out) {
         int *P = arena.take<int>(n);

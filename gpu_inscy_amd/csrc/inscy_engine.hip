@@ -10,6 +10,7 @@ static thread_local std::string g_error;
 static thread_local std::vector<Engine::TraceRow> g_trace;
 static bool g_trace_on = false;
 static bool g_phi_all_exact = false;
+static bool g_generic_walk = false;
 static bool g_profile_on = false;
 
 static int fail(int code, const std::string &msg) {
@@ -90,6 +91,7 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
             Engine eng(*p, (int) n, d, st);
             eng.trace_on = g_trace_on;
             eng.phi_all_exact = g_phi_all_exact;
+            eng.generic_walk = g_generic_walk;
             eng.profile_on = g_profile_on;
             // phi density (rectangular = false) is implemented in the
             // level-synchronous driver only; it serves every mode
@@ -364,6 +366,14 @@ int inscy_debug_trace(int on) {
 // leaf-order recomputation instead of the double sum with its rounding band.
 int inscy_debug_phi_exact(int on) {
     g_phi_all_exact = on != 0;
+    return INSCY_OK;
+}
+
+// Test hook (not in the public header): run every neighbour walk of the
+// level-synchronous driver for k known only at run time, as for a batch that
+// mixes dimensionalities (no launch compiled for its k).
+int inscy_debug_generic_walk(int on) {
+    g_generic_walk = on != 0;
     return INSCY_OK;
 }
 

@@ -362,7 +362,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         kmin = std::min(kmin, x.S.k);
         all_packed = all_packed && x.restmask == 0;
     }
-    const int kfast = (kmin == kmax && kmax <= 7 && all_packed) ? kmax : 0;
+    const int kfast = (!generic_walk && kmin == kmax && kmax <= 7 && all_packed) ? kmax : 0;
     BatchIndex X{};
     X.nhist = (int) (hoff + 1);
     X.keys = batch.take<int>((size_t) B * n);

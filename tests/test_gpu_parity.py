@@ -342,6 +342,31 @@ def test_phi_density_config1_matches_golden():
     assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
 
 
+@pytest.mark.parametrize("rect", [True, False])
+def test_runtime_k_walks_match_golden(rect):
+    # the walks compiled for a launch's k against the walks for k known only
+    # at run time (the path of batches that mix dimensionalities): both give
+    # the golden result, with the same counters
+    import json
+    say("runtime-k walks", rect)
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                    "config1.json" if rect else "config1_phi.json")))
+    w = datagen.config(1)
+    X = torch.from_numpy(w.X).cuda()
+    run = lambda: INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, rect, 0, mode=_lib.MODE_BATCHED)
+    compiled = run()
+    _lib.debug_generic_walk(True)
+    try:
+        generic = run()
+    finally:
+        _lib.debug_generic_walk(False)
+    for res in (compiled, generic):
+        assert res.subspaces == [s for s, _, _, _ in g["subspaces"]]
+        assert digests(res) == [x[3] for x in g["subspaces"]]
+        assert res.stats["trees"] == g["trees"]
+    assert compiled.stats["queries"] == generic.stats["queries"]
+
+
 REAL_DIR = os.path.join(os.path.dirname(__file__), "golden", "real")
 
 
