@@ -79,6 +79,7 @@ class Stats(ctypes.Structure):
         ("emit_seconds", ctypes.c_double),
         ("pairs_scored", ctypes.c_int64),
         ("pairs_skipped", ctypes.c_int64),
+        ("sorts_kept", ctypes.c_int64),
     ]
 
 

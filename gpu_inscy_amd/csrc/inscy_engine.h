@@ -310,8 +310,7 @@ public:
         // the level-synchronous driver keeps its lists in pools; its arena holds
         // the replay's label flags (one per open dimension) and the emit buffers
         if (levels_mode) want = (size_t) n * (d + 8) + pass + (64u << 20);
-        size_t free_b = 0, total_b = 0;
-        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) want = std::min(want, free_b / 3);
+        if (const size_t free_b = device_available_bytes()) want = std::min(want, free_b / 3);
         arena.init(want);
 
         int *hist = arena.take<int>((size_t) d * ecells);
@@ -928,6 +927,13 @@ private:
     Pool store;     // weak-dense point lists of surviving trees, keep log (whole run)
     Pool lvl[2];    // frontier paths and counters of the current / next level
     Pool batch;     // one batch of child subspaces
+    // grid index sorts kept for later batches, by index dimensions (whole run)
+    struct KeptSort {
+        int *order, *pos, *off;
+    };
+    std::unordered_map<uint64_t, KeptSort> kept;
+    Pool keptp;
+    size_t kept_used = 0, kept_budget = 0;
     int cur_lvl = 0;
     HostBuf up, down;
     HostBuf emit_stage[2];

@@ -256,10 +256,25 @@ hipError_t launch_root_paths(const uint64_t *nearm, int n, int *outx, uint64_t *
 hipError_t launch_b_flags(const FlagTree *T, const int *blk_off, int nt, int nblk, const uint8_t *cellC, int n,
                           int cells, int *flags, hipStream_t st);
 // grid indexes of nsub subspaces: keys, one scan over all grid cells, scatter
+// A grid index sort kept for later batches of the run (the same index
+// dimensions sort the points the same way): sorted point ids, each point's
+// sorted position and the cell offsets, all local to the index; with the
+// batch-side place of one index (obase, hoff) and its cell count.
+struct SortCopy {
+    int *order, *pos, *off;
+    long long obase, hoff;
+    int ncell;
+};
+
 // need_es: gather the sorted cell columns Es (a launch of the batch runs the
-// walk for any k, which tests cells one by one)
+// walk for any k, which tests cells one by one).  miss_reps: the indexes to
+// sort (positions in subs); stores: sorts to keep (filled from the batch after
+// sorting); hits: kept sorts copied in (after the stores, so an index repeated
+// within the batch reads the sort just kept).  After it X.keys holds every
+// point's local sorted position.
 hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, bool need_es,
-                          hipStream_t st);
+                          const int *miss_reps, int nmiss, const SortCopy *stores, int nstore, const SortCopy *hits,
+                          int nhit, int max_cell, hipStream_t st);
 // queries in sorted order: flag the points of the segments, compact per subspace
 hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySeg *segs, const int *blk_off,
                             int nseg, int nblk, const BatchIndex &X, hipStream_t st);

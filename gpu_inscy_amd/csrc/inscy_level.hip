@@ -54,10 +54,13 @@ void Engine::run_levels(inscy_result &out) {
     store.chunk = (size_t) 256 << 20;
     batch.chunk = (size_t) 256 << 20;
     up.get((size_t) 4 << 20);
-    size_t free_b = 0, total_b = 0;
     batch_budget = (size_t) 2 << 30;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+    if (const size_t free_b = device_available_bytes())
         batch_budget = std::min<size_t>(std::max<size_t>(free_b / 8, (size_t) 256 << 20), (size_t) 8 << 30);
+    kept.clear();
+    keptp.reset();
+    kept_used = 0;
+    kept_budget = batch_budget;
 
     // the root tree: every point, the root s-connection paths
     Pool &L0 = lvl[cur_lvl];
@@ -394,7 +397,38 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.scan_temp = batch.take<char>(X.scan_bytes);
     HIP_OK(launch_fill_i32(cntB, -1, (size_t) B * n, st));
     BatchSub *dsubs = upload(batch, hs);
-    HIP_OK(launch_b_index(D, dsubs, B, X, kfast == 0, st));
+    // index sorts kept from earlier batches of the run: the same index
+    // dimensions (in index order) sort the points the same way
+    std::vector<int> miss;
+    std::vector<SortCopy> hits, stores;
+    int max_cell = 0;
+    for (int r : reps) {
+        const BatchSub &x = hs[r];
+        int ncell = 1;
+        for (int q = 0; q < x.G.j; ++q) ncell *= x.G.C;
+        max_cell = std::max(max_cell, ncell);
+        uint64_t key = (uint64_t) x.G.j;
+        for (int a = 0; a < x.G.j; ++a) key |= (uint64_t) (x.S.dims[x.G.tI[a]] + 1) << (8 * (a + 1));
+        auto it = kept.find(key);
+        if (it != kept.end()) {
+            hits.push_back({it->second.order, it->second.pos, it->second.off, x.obase, x.hoff, ncell});
+            continue;
+        }
+        miss.push_back(r);
+        const size_t need = ((size_t) 2 * n + ncell + 1) * sizeof(int) + 768;
+        if (kept_used + need <= kept_budget) {
+            KeptSort ks{keptp.take<int>(n), keptp.take<int>(n), keptp.take<int>((size_t) ncell + 1)};
+            kept.emplace(key, ks);
+            kept_used += need;
+            stores.push_back({ks.order, ks.pos, ks.off, x.obase, x.hoff, ncell});
+        }
+    }
+    stats.sorts_kept += (int64_t) hits.size();
+    const int *dmiss = upload(batch, miss);
+    const SortCopy *dstores = upload(batch, stores);
+    const SortCopy *dhits = upload(batch, hits);
+    HIP_OK(launch_b_index(D, dsubs, B, X, kfast == 0, dmiss, (int) miss.size(), dstores, (int) stores.size(), dhits,
+                          (int) hits.size(), max_cell, st));
 
     // ---- queries: the points of the trees each subspace refines, in sorted order
     std::vector<QuerySeg> qs;

@@ -73,12 +73,14 @@ __global__ void k_b_keys(DevData D, const BatchSub *__restrict__ subs, const int
 // writes one int per point; the coordinates and cells are then gathered in
 // sorted order, so their writes are coalesced.
 __global__ void k_b_scatter(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ reps,
-                            const int *__restrict__ keys, const int *__restrict__ rank, const int *__restrict__ off,
+                            int *__restrict__ keys, const int *__restrict__ rank, const int *__restrict__ off,
                             int *__restrict__ order) {
     const BatchSub &B = subs[reps[blockIdx.y]];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D.n) return;
-    order[off[B.hoff + keys[B.obase + p]] + rank[B.obase + p]] = p;
+    const int at = off[B.hoff + keys[B.obase + p]] + rank[B.obase + p];
+    order[at] = p;
+    keys[B.obase + p] = at - (int) B.obase;  // from here on: the point's local sorted position
 }
 
 // the index builder's columns: every dimension but the child's own (all of
@@ -125,17 +127,47 @@ __global__ void k_b_gather_own(DevData D, const BatchSub *__restrict__ subs, con
 }
 
 // ------------------------------------------------------------ query lists in sorted order
+// pos: every point's local sorted position in its index
 __global__ void k_b_qmark(const BatchSub *__restrict__ subs, const QuerySeg *__restrict__ segs,
-                          const int *__restrict__ blk_off, int nseg, const int *__restrict__ keys,
-                          const int *__restrict__ rank, const int *__restrict__ off, uint8_t *__restrict__ qflag) {
+                          const int *__restrict__ blk_off, int nseg, const int *__restrict__ pos,
+                          uint8_t *__restrict__ qflag) {
     const int s = seg_of(blk_off, nseg, blockIdx.x);
     const QuerySeg Q = segs[s];
     const int i = (blockIdx.x - blk_off[s]) * blockDim.x + threadIdx.x;
     if (i >= Q.len) return;
     const BatchSub &B = subs[Q.b];
     const int p = Q.P[i];
-    const long long at = off[B.hoff + keys[B.obase + p]] + rank[B.obase + p];
-    qflag[B.base + (at - B.obase)] = 1;
+    qflag[B.base + pos[B.obase + p]] = 1;
+}
+
+// kept sorts: a kept index's last cell holds all n points in the histogram
+// (so the batch's scan places the next index right), its sort is copied in
+// after the scan; sorts to keep are copied out of the batch
+__global__ void k_b_hist_kept(const SortCopy *__restrict__ c, int nc, int n, int *__restrict__ hist) {
+    const int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < nc) hist[c[h].hoff + c[h].ncell - 1] = n;
+}
+
+__global__ void k_b_sort_in(const SortCopy *__restrict__ cs, int n, int *__restrict__ order, int *__restrict__ pos,
+                            int *__restrict__ off) {
+    const SortCopy &c = cs[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        order[c.obase + i] = c.order[i];
+        pos[c.obase + i] = c.pos[i];
+    }
+    if (i <= c.ncell) off[c.hoff + i] = c.off[i] + (int) c.obase;
+}
+
+__global__ void k_b_sort_out(const SortCopy *__restrict__ cs, int n, const int *__restrict__ order,
+                             const int *__restrict__ pos, const int *__restrict__ off) {
+    const SortCopy &c = cs[blockIdx.y];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        c.order[i] = order[c.obase + i];
+        c.pos[i] = pos[c.obase + i];
+    }
+    if (i <= c.ncell) c.off[i] = off[c.hoff + i] - (int) c.obase;
 }
 
 // qstart[b] = first index of the selected (ascending global) positions that is
@@ -1510,15 +1542,24 @@ hipError_t launch_b_flags(const FlagTree *T, const int *blk_off, int nt, int nbl
 }
 
 hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, bool need_es,
-                          hipStream_t st) {
+                          const int *miss_reps, int nmiss, const SortCopy *stores, int nstore, const SortCopy *hits,
+                          int nhit, int max_cell, hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
     LAUNCH(k_b_fill<int>, dim3(blocks_for(X.nhist)), dim3(kBlock), 0, st, X.hist, 0, (size_t) X.nhist);
-    LAUNCH(k_b_keys, dim3(blocks_for(D.n), X.nrep), dim3(kBlock), 0, st, D, subs, X.reps, X.keys, X.rank, X.hist);
+    if (nmiss > 0)
+        LAUNCH(k_b_keys, dim3(blocks_for(D.n), nmiss), dim3(kBlock), 0, st, D, subs, miss_reps, X.keys, X.rank,
+               X.hist);
+    if (nhit > 0) LAUNCH(k_b_hist_kept, dim3(blocks_for(nhit)), dim3(kBlock), 0, st, hits, nhit, D.n, X.hist);
     size_t tb = X.scan_bytes;
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(X.scan_temp, tb, X.hist, X.off, X.nhist, st);
     if (e != hipSuccess) return e;
-    LAUNCH(k_b_scatter, dim3(blocks_for(D.n), X.nrep), dim3(kBlock), 0, st, D, subs, X.reps, X.keys, X.rank, X.off,
-           X.order);
+    if (nmiss > 0)
+        LAUNCH(k_b_scatter, dim3(blocks_for(D.n), nmiss), dim3(kBlock), 0, st, D, subs, miss_reps, X.keys, X.rank,
+               X.off, X.order);
+    const unsigned span = blocks_for((size_t) std::max(D.n, max_cell + 1));
+    if (nstore > 0)
+        LAUNCH(k_b_sort_out, dim3(span, nstore), dim3(kBlock), 0, st, stores, D.n, X.order, X.keys, X.off);
+    if (nhit > 0) LAUNCH(k_b_sort_in, dim3(span, nhit), dim3(kBlock), 0, st, hits, D.n, X.order, X.keys, X.off);
     LAUNCH(k_b_gather, dim3(blocks_for(D.n), X.nrep), dim3(kBlock), 0, st, D, subs, X.reps, X.order, X.Ys, X.Es,
            X.sigS, (int) need_es);
     LAUNCH(k_b_gather_own, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.order, X.Ys, X.Es, X.sigS,
@@ -1540,7 +1581,7 @@ hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySe
     const size_t nb = (size_t) nsub * n;
     LAUNCH(k_b_fill<uint8_t>, dim3(blocks_for(nb)), dim3(kBlock), 0, st, X.qflag, (uint8_t) 0, nb);
     if (nblk > 0)
-        LAUNCH(k_b_qmark, dim3(nblk), dim3(kBlock), 0, st, subs, segs, blk_off, nseg, X.keys, X.rank, X.off, X.qflag);
+        LAUNCH(k_b_qmark, dim3(nblk), dim3(kBlock), 0, st, subs, segs, blk_off, nseg, X.keys, X.qflag);
     size_t tb = X.select_bytes;
     hipError_t e = hipcub::DeviceSelect::Flagged(X.select_temp, tb, hipcub::CountingInputIterator<int>(0), X.qflag,
                                                  X.qlist, X.nsel, (int) nb, st);

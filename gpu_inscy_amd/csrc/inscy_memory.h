@@ -61,6 +61,17 @@ public:
         free_[it->second.second].insert({it->second.first, p});
     }
 
+    // bytes of the current device's cached blocks that no run holds
+    size_t idle_bytes() {
+        const int dev = device();
+        std::lock_guard<std::mutex> g(mu_);
+        size_t sum = 0;
+        auto it = free_.find(dev);
+        if (it != free_.end())
+            for (const auto &b : it->second) sum += b.first;
+        return sum;
+    }
+
     // frees the cached (unused) blocks of one device, or of all (dev < 0)
     void release(int dev) {
         std::lock_guard<std::mutex> g(mu_);
@@ -143,6 +154,15 @@ private:
 inline EventPool &event_pool() {
     static EventPool *p = new EventPool;
     return *p;
+}
+
+// Device memory a run can still take: what the runtime reports free plus the
+// cached blocks no run holds (an earlier run's buffers stay cached and would
+// otherwise shrink every later run's budgets).
+inline size_t device_available_bytes() {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
+    return free_b + device_cache().idle_bytes();
 }
 
 template <class T>

@@ -81,6 +81,9 @@ typedef struct inscy_stats {
      * pruner's features, and pairs skipped (learned mode) */
     int64_t pairs_scored;
     int64_t pairs_skipped;
+    /* level-synchronous driver: grid indexes whose sort was taken from an
+     * earlier batch of the run (same index dimensions) instead of sorted */
+    int64_t sorts_kept;
 } inscy_stats;
 
 typedef struct inscy_result inscy_result;
