@@ -132,7 +132,7 @@ struct BatchIndex {
     int *qstart;          // [B+1] first query of each subspace in qlist
     void *select_temp;
     size_t select_bytes;
-    int *nb;              // [B*n*2] first two neighbours found by the count passes
+    int *nb;              // [B*n*kSampleEdges] first neighbours found by the count passes
     uint64_t *sig;        // [B*n] packed cells of the checked dimensions per sorted position
     uint64_t *sigS;       // [B*n] (index space) the same over the shared dimensions only
     const int *reps;      // subspaces that build an index, in index order
@@ -388,5 +388,6 @@ hipError_t launch_tree_hits(const TreeHit *trees, const int *blk_off, int ntrees
 constexpr int kRangeWords = 21;      // per query in LDS: 9 range starts, 10 prefix sums, count, own run
 constexpr int kBlockThreads = 256;   // threads per block of every pass
 constexpr int kQueriesPerBlock = 64;  // queries per block of the batched neighbour passes
+constexpr int kSampleEdges = 2;       // neighbours recorded per point by the count walk (Afforest's sample rounds)
 
 }  // namespace inscy

@@ -381,7 +381,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.qstart = batch.take<int>(B + 1);
     X.select_bytes = select_temp_bytes(B * n);
     X.select_temp = batch.take<char>(X.select_bytes);
-    X.nb = batch.take<int>((size_t) 2 * B * n);
+    X.nb = batch.take<int>((size_t) kSampleEdges * B * n);
     X.sig = batch.take<uint64_t>((size_t) B * n);
     X.sigS = batch.take<uint64_t>((size_t) reps.size() * n);
     X.reps = upload(batch, reps);
@@ -389,7 +389,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.nfb = batch.take<int>(1);
     X.fbcap = phi_mode ? B * n : 1;  // each point of each subspace is sent at most once per pass
     X.fb = batch.take<int2>(X.fbcap);
-    HIP_OK(launch_fill_i32(X.nb, -1, (size_t) 2 * B * n, st));
+    HIP_OK(launch_fill_i32(X.nb, -1, (size_t) kSampleEdges * B * n, st));
     X.dsorted = batch.take<uint8_t>((size_t) B * n);
     X.par = batch.take<int>((size_t) B * n);
     X.giant = batch.take<int>(B);

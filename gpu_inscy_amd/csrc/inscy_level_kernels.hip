@@ -375,9 +375,9 @@ __device__ __forceinline__ void walk_tally(const QueryWalk &W, const int (&jj)[U
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         const unsigned long long mine = (__ballot(hit[u]) >> gshift) & ((1ull << LPQ) - 1ull);
-        if (hit[u] && found < 2) {
+        if (hit[u] && found < kSampleEdges) {
             const int at = found + __popcll(mine & below);
-            if (at < 2) W.nbp[at] = W.order[W.B.obase + jj[u]];
+            if (at < kSampleEdges) W.nbp[at] = W.order[W.B.obase + jj[u]];
         }
         found += __popcll(mine);
     }
@@ -476,7 +476,7 @@ __device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double
             for (int u = 0; u < U; ++u) h += hit[u] ? 1 : 0;
             h += __builtin_amdgcn_update_dpp(0, h, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
             h += __builtin_amdgcn_update_dpp(0, h, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-            if (found < 2 && h > 0) {
+            if (found < kSampleEdges && h > 0) {
                 int f = found;
                 walk_tally<U, false>(W, jj, hit, accs, f, dsum);
             }
@@ -674,7 +674,7 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
         double dsum = 0.0;
         bool more = true;
         const QueryWalk W{B, n, i, p, q, sy + lq, se + lq, qs[lq], order, Ys, Es, sigb, s_star, eps, phirel,
-                          nb + 2 * (B.base + p)};
+                          nb + kSampleEdges * (B.base + p)};
         // K > 0: every subspace of the launch has k = K with its checked
         // dimensions packed (the host checks), so the walk is compiled for it
         if constexpr (K > 0)
@@ -761,7 +761,7 @@ __global__ void k_b_link_sample(const BatchSub *__restrict__ subs, int n, const 
     const int c = cnt[p];
     if (!meets(c, B.thr_dense, B.dbit)) return;
     // the recorded neighbours: min(count, 2) of them (phi mode records them too)
-    const int q = nb[2 * (B.base + p) + round];
+    const int q = nb[kSampleEdges * (B.base + p) + round];
     if (B.dbit == 0 && c <= round) return;
     if (q < 0 || q >= n || !meets(cnt[q], B.thr_dense, B.dbit)) return;
     int *pr = par + B.base;
@@ -1645,7 +1645,7 @@ hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, 
                                float s_star, hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
     LAUNCH(k_b_dense_init, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.order, X.dsorted, X.par);
-    for (int round = 0; round < 2; ++round) {
+    for (int round = 0; round < kSampleEdges; ++round) {
         LAUNCH(k_b_link_sample, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.nb, round, X.par);
         LAUNCH(k_b_uf_compress, dim3(blocks_for(n), nsub), dim3(kBlock), 0, st, subs, n, X.par);
     }
