@@ -268,13 +268,14 @@ struct SortCopy {
 
 // need_es: gather the sorted cell columns Es (a launch of the batch runs the
 // walk for any k, which tests cells one by one).  miss_reps: the indexes to
-// sort (positions in subs); stores: sorts to keep (filled from the batch after
+// sort (positions in subs), placed first: their cells are the histogram's
+// first hmiss entries; stores: sorts to keep (filled from the batch after
 // sorting); hits: kept sorts copied in (after the stores, so an index repeated
 // within the batch reads the sort just kept).  After it X.keys holds every
 // point's local sorted position.
 hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, bool need_es,
-                          const int *miss_reps, int nmiss, const SortCopy *stores, int nstore, const SortCopy *hits,
-                          int nhit, int max_cell, hipStream_t st);
+                          const int *miss_reps, int nmiss, int hmiss, const SortCopy *stores, int nstore,
+                          const SortCopy *hits, int nhit, int max_cell, hipStream_t st);
 // queries in sorted order: flag the points of the segments, compact per subspace
 hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySeg *segs, const int *blk_off,
                             int nseg, int nblk, const BatchIndex &X, hipStream_t st);

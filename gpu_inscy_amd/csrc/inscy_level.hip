@@ -286,6 +286,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     int kmax = 1, kmin = 64;
     bool all_packed = true;
     int builder = -1;  // the index builder the next child of the same run may share
+    std::vector<int> shared_of(B, -1);  // the builder whose index a child shares
     int *cntB = batch.take<int>((size_t) B * n);
     for (int b = 0; b < B; ++b) {
         const ChildSub &c = cs[i0 + b];
@@ -312,6 +313,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
             hs[builder].G.j == gbest.j &&
             std::equal(gbest.tI, gbest.tI + gbest.j, hs[builder].G.tI)) {
             const BatchSub &r = hs[builder];
+            shared_of[b] = builder;
             x.G = r.G;
             x.obase = r.obase;
             x.hoff = r.hoff;
@@ -366,6 +368,73 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         all_packed = all_packed && x.restmask == 0;
     }
     const int kfast = (!generic_walk && kmin == kmax && kmax <= 7 && all_packed) ? kmax : 0;
+
+    // index sorts kept from earlier batches of the run: the same index
+    // dimensions (in index order) sort the points the same way.  The indexes
+    // still to sort take the first places (obase, hoff), so the histogram
+    // scan covers only them; kept sorts are copied in behind.  A sort kept in
+    // this batch serves a later index of the same batch too.
+    std::vector<int> miss, hit_reps;
+    std::vector<KeptSort> hit_src, store_src;
+    int max_cell = 0;
+    auto cells_of = [&](const BatchSub &x) {
+        int c = 1;
+        for (int q = 0; q < x.G.j; ++q) c *= x.G.C;
+        return c;
+    };
+    for (int r : reps) {
+        const BatchSub &x = hs[r];
+        const int ncell = cells_of(x);
+        max_cell = std::max(max_cell, ncell);
+        uint64_t key = (uint64_t) x.G.j;
+        for (int a = 0; a < x.G.j; ++a) key |= (uint64_t) (x.S.dims[x.G.tI[a]] + 1) << (8 * (a + 1));
+        auto it = kept.find(key);
+        if (it != kept.end()) {
+            hit_reps.push_back(r);
+            hit_src.push_back(it->second);
+            continue;
+        }
+        miss.push_back(r);
+        const size_t need = ((size_t) 2 * n + ncell + 1) * sizeof(int) + 768;
+        KeptSort ks{nullptr, nullptr, nullptr};
+        if (kept_used + need <= kept_budget) {
+            ks = KeptSort{keptp.take<int>(n), keptp.take<int>(n), keptp.take<int>((size_t) ncell + 1)};
+            kept.emplace(key, ks);
+            kept_used += need;
+        }
+        store_src.push_back(ks);
+    }
+    long long hmiss = 0;
+    {
+        long long slot = 0, hpos = 0;
+        for (int r : miss) {
+            hs[r].obase = slot++ * n;
+            hs[r].hoff = hpos;
+            hpos += cells_of(hs[r]);
+        }
+        hmiss = hpos;
+        for (int r : hit_reps) {
+            hs[r].obase = slot++ * n;
+            hs[r].hoff = hpos;
+            hpos += cells_of(hs[r]);
+        }
+        for (int b = 0; b < B; ++b)
+            if (shared_of[b] >= 0) {
+                hs[b].obase = hs[shared_of[b]].obase;
+                hs[b].hoff = hs[shared_of[b]].hoff;
+            }
+    }
+    std::vector<SortCopy> hits, stores;
+    for (size_t u = 0; u < miss.size(); ++u)
+        if (store_src[u].order)
+            stores.push_back({store_src[u].order, store_src[u].pos, store_src[u].off, hs[miss[u]].obase,
+                              hs[miss[u]].hoff, cells_of(hs[miss[u]])});
+    for (size_t u = 0; u < hit_reps.size(); ++u)
+        hits.push_back({hit_src[u].order, hit_src[u].pos, hit_src[u].off, hs[hit_reps[u]].obase,
+                        hs[hit_reps[u]].hoff, cells_of(hs[hit_reps[u]])});
+    stats.sorts_kept += (int64_t) hits.size();
+    reps = miss;
+    reps.insert(reps.end(), hit_reps.begin(), hit_reps.end());
     BatchIndex X{};
     X.nhist = (int) (hoff + 1);
     X.keys = batch.take<int>((size_t) B * n);
@@ -397,38 +466,12 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.scan_temp = batch.take<char>(X.scan_bytes);
     HIP_OK(launch_fill_i32(cntB, -1, (size_t) B * n, st));
     BatchSub *dsubs = upload(batch, hs);
-    // index sorts kept from earlier batches of the run: the same index
-    // dimensions (in index order) sort the points the same way
-    std::vector<int> miss;
-    std::vector<SortCopy> hits, stores;
-    int max_cell = 0;
-    for (int r : reps) {
-        const BatchSub &x = hs[r];
-        int ncell = 1;
-        for (int q = 0; q < x.G.j; ++q) ncell *= x.G.C;
-        max_cell = std::max(max_cell, ncell);
-        uint64_t key = (uint64_t) x.G.j;
-        for (int a = 0; a < x.G.j; ++a) key |= (uint64_t) (x.S.dims[x.G.tI[a]] + 1) << (8 * (a + 1));
-        auto it = kept.find(key);
-        if (it != kept.end()) {
-            hits.push_back({it->second.order, it->second.pos, it->second.off, x.obase, x.hoff, ncell});
-            continue;
-        }
-        miss.push_back(r);
-        const size_t need = ((size_t) 2 * n + ncell + 1) * sizeof(int) + 768;
-        if (kept_used + need <= kept_budget) {
-            KeptSort ks{keptp.take<int>(n), keptp.take<int>(n), keptp.take<int>((size_t) ncell + 1)};
-            kept.emplace(key, ks);
-            kept_used += need;
-            stores.push_back({ks.order, ks.pos, ks.off, x.obase, x.hoff, ncell});
-        }
-    }
-    stats.sorts_kept += (int64_t) hits.size();
     const int *dmiss = upload(batch, miss);
     const SortCopy *dstores = upload(batch, stores);
     const SortCopy *dhits = upload(batch, hits);
-    HIP_OK(launch_b_index(D, dsubs, B, X, kfast == 0, dmiss, (int) miss.size(), dstores, (int) stores.size(), dhits,
-                          (int) hits.size(), max_cell, st));
+    HIP_OK(launch_b_index(D, dsubs, B, X, kfast == 0, dmiss, (int) miss.size(), (int) hmiss, dstores,
+                          (int) stores.size(), dhits, (int) hits.size(), max_cell, st));
+
 
     // ---- queries: the points of the trees each subspace refines, in sorted order
     std::vector<QuerySeg> qs;

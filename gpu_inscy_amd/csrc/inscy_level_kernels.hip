@@ -140,14 +140,8 @@ __global__ void k_b_qmark(const BatchSub *__restrict__ subs, const QuerySeg *__r
     qflag[B.base + pos[B.obase + p]] = 1;
 }
 
-// kept sorts: a kept index's last cell holds all n points in the histogram
-// (so the batch's scan places the next index right), its sort is copied in
-// after the scan; sorts to keep are copied out of the batch
-__global__ void k_b_hist_kept(const SortCopy *__restrict__ c, int nc, int n, int *__restrict__ hist) {
-    const int h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h < nc) hist[c[h].hoff + c[h].ncell - 1] = n;
-}
-
+// kept sorts: copied in after the scan of the indexes still to sort; sorts
+// to keep are copied out of the batch after their scatter
 __global__ void k_b_sort_in(const SortCopy *__restrict__ cs, int n, int *__restrict__ order, int *__restrict__ pos,
                             int *__restrict__ off) {
     const SortCopy &c = cs[blockIdx.y];
@@ -1542,16 +1536,18 @@ hipError_t launch_b_flags(const FlagTree *T, const int *blk_off, int nt, int nbl
 }
 
 hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, bool need_es,
-                          const int *miss_reps, int nmiss, const SortCopy *stores, int nstore, const SortCopy *hits,
-                          int nhit, int max_cell, hipStream_t st) {
+                          const int *miss_reps, int nmiss, int hmiss, const SortCopy *stores, int nstore,
+                          const SortCopy *hits, int nhit, int max_cell, hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
-    LAUNCH(k_b_fill<int>, dim3(blocks_for(X.nhist)), dim3(kBlock), 0, st, X.hist, 0, (size_t) X.nhist);
+    // the scan over the sorted indexes' cells ends with their total, the
+    // first position of the kept ones
+    const int nscan = hmiss + 1;
+    LAUNCH(k_b_fill<int>, dim3(blocks_for(nscan)), dim3(kBlock), 0, st, X.hist, 0, (size_t) nscan);
     if (nmiss > 0)
         LAUNCH(k_b_keys, dim3(blocks_for(D.n), nmiss), dim3(kBlock), 0, st, D, subs, miss_reps, X.keys, X.rank,
                X.hist);
-    if (nhit > 0) LAUNCH(k_b_hist_kept, dim3(blocks_for(nhit)), dim3(kBlock), 0, st, hits, nhit, D.n, X.hist);
     size_t tb = X.scan_bytes;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(X.scan_temp, tb, X.hist, X.off, X.nhist, st);
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(X.scan_temp, tb, X.hist, X.off, nscan, st);
     if (e != hipSuccess) return e;
     if (nmiss > 0)
         LAUNCH(k_b_scatter, dim3(blocks_for(D.n), nmiss), dim3(kBlock), 0, st, D, subs, miss_reps, X.keys, X.rank,
