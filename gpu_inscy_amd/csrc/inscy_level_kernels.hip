@@ -310,7 +310,7 @@ __device__ __forceinline__ void stage_query_b(const BatchSub &B, int n, const fl
 // ------------------------------------------------------------ candidate walks of one query
 // The LPQ lanes of a query take candidates c0 + u*LPQ + sub of each step.
 // The walk starts at a pseudo-random candidate of the run of the query's own
-// cells (the densest part of its neighbourhood) and wraps: the first two
+// cells (the densest part of its neighbourhood) and wraps: the first
 // neighbours found are then spread over the candidates (random sample edges
 // for the components pass, as Afforest draws them) instead of the same few
 // points at the start of a shared run.
@@ -349,7 +349,7 @@ __device__ __forceinline__ bool cells_near_packed(uint64_t w, uint64_t qsig) {
     return m.x <= 2 && m.y <= 2;
 }
 
-// neighbours found in one step: the first two are recorded (the sample edges
+// neighbours found in one step: the first kSampleEdges are recorded (the sample edges
 // of the components pass); phi adds the terms of the step
 template <int U, bool PHI>
 __device__ __forceinline__ void walk_tally(const QueryWalk &W, const int (&jj)[U], const bool (&hit)[U],
@@ -741,7 +741,7 @@ __global__ void k_b_dense_init(const BatchSub *__restrict__ subs, int n, const i
 }
 
 // Sample phase of the components: every dense point links to the dense ones
-// among the first two neighbours its count pass recorded (any edges would do:
+// among the first kSampleEdges neighbours its count pass recorded (any edges would do:
 // the full phase below links every point outside the giant component to all
 // its dense neighbours, so the result does not depend on the sample).
 // One round per recorded neighbour, with the roots compressed in between, so
@@ -754,7 +754,7 @@ __global__ void k_b_link_sample(const BatchSub *__restrict__ subs, int n, const 
     const int *cnt = B.cnt;
     const int c = cnt[p];
     if (!meets(c, B.thr_dense, B.dbit)) return;
-    // the recorded neighbours: min(count, 2) of them (phi mode records them too)
+    // the recorded neighbours: min(count, kSampleEdges) of them (phi mode records them too)
     const int q = nb[kSampleEdges * (B.base + p) + round];
     if (B.dbit == 0 && c <= round) return;
     if (q < 0 || q >= n || !meets(cnt[q], B.thr_dense, B.dbit)) return;
