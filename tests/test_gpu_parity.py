@@ -365,6 +365,8 @@ def test_runtime_k_walks_match_golden(rect):
         assert digests(res) == [x[3] for x in g["subspaces"]]
         assert res.stats["trees"] == g["trees"]
     assert compiled.stats["queries"] == generic.stats["queries"]
+    # most grid indexes of the run reuse a sort kept from an earlier batch
+    assert compiled.stats["sorts_kept"] > 0
 
 
 REAL_DIR = os.path.join(os.path.dirname(__file__), "golden", "real")
