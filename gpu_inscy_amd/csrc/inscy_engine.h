@@ -934,6 +934,12 @@ private:
     std::unordered_map<uint64_t, KeptSort> kept;
     Pool keptp;
     size_t kept_used = 0, kept_budget = 0;
+    // sorted coordinate columns kept the same way, by index dimensions and
+    // dimension: (column, number of the batch that gathers it)
+    std::unordered_map<uint64_t, std::pair<float *, int>> kept_cols;
+    Pool colp;
+    size_t cols_used = 0;
+    int batch_no = 0;
     int cur_lvl = 0;
     HostBuf up, down;
     HostBuf emit_stage[2];

@@ -273,9 +273,20 @@ struct SortCopy {
 // sorting); hits: kept sorts copied in (after the stores, so an index repeated
 // within the batch reads the sort just kept).  After it X.keys holds every
 // point's local sorted position.
+// Sorted coordinate columns kept across batches (same index dimensions and
+// dimension): per index column (rep position * kmax + t) and per subspace's
+// own column, a kept column to read (src) or a slot to keep a gathered one
+// (store), either null.
+struct ColumnKeep {
+    const float *const *src;
+    float *const *store;
+    const float *const *own_src;
+    float *const *own_store;
+    int kmax;
+};
 hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, bool need_es,
                           const int *miss_reps, int nmiss, int hmiss, const SortCopy *stores, int nstore,
-                          const SortCopy *hits, int nhit, int max_cell, hipStream_t st);
+                          const SortCopy *hits, int nhit, int max_cell, const ColumnKeep &cols, hipStream_t st);
 // queries in sorted order: flag the points of the segments, compact per subspace
 hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySeg *segs, const int *blk_off,
                             int nseg, int nblk, const BatchIndex &X, hipStream_t st);

@@ -61,6 +61,10 @@ void Engine::run_levels(inscy_result &out) {
     keptp.reset();
     kept_used = 0;
     kept_budget = batch_budget;
+    kept_cols.clear();
+    colp.reset();
+    cols_used = 0;
+    batch_no = 0;
 
     // the root tree: every point, the root s-connection paths
     Pool &L0 = lvl[cur_lvl];
@@ -466,11 +470,41 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.scan_temp = batch.take<char>(X.scan_bytes);
     HIP_OK(launch_fill_i32(cntB, -1, (size_t) B * n, st));
     BatchSub *dsubs = upload(batch, hs);
+    // sorted columns kept from earlier batches: the same index dimensions and
+    // dimension give the same column (a column kept in this batch is read only
+    // from the next one on: the batch's gathers run in one launch)
+    ++batch_no;
+    auto sort_key = [&](const BatchSub &x) {
+        uint64_t key = (uint64_t) x.G.j;
+        for (int a = 0; a < x.G.j; ++a) key |= (uint64_t) (x.S.dims[x.G.tI[a]] + 1) << (8 * (a + 1));
+        return key;
+    };
+    auto column = [&](const BatchSub &x, int t, const float *&src, float *&dst) {
+        const uint64_t ck = sort_key(x) | (uint64_t) (x.S.dims[t] + 1) << 32;
+        auto it = kept_cols.find(ck);
+        if (it != kept_cols.end()) {
+            if (it->second.second < batch_no) src = it->second.first;
+        } else if (cols_used + (size_t) n * sizeof(float) + 256 <= kept_budget) {
+            dst = colp.take<float>(n);
+            kept_cols.emplace(ck, std::make_pair(dst, batch_no));
+            cols_used += (size_t) n * sizeof(float) + 256;
+        }
+    };
+    std::vector<const float *> ysrc(reps.size() * kmax, nullptr), own_src(B, nullptr);
+    std::vector<float *> ystore(reps.size() * kmax, nullptr), own_store(B, nullptr);
+    for (size_t u = 0; u < reps.size(); ++u) {
+        const BatchSub &x = hs[reps[u]];
+        for (int t = 0; t < x.S.k; ++t)
+            if (t != x.pos_m) column(x, t, ysrc[u * kmax + t], ystore[u * kmax + t]);
+    }
+    for (int b = 0; b < B; ++b)
+        if (hs[b].pos_m < hs[b].S.k) column(hs[b], hs[b].pos_m, own_src[b], own_store[b]);
+    ColumnKeep cols{upload(batch, ysrc), upload(batch, ystore), upload(batch, own_src), upload(batch, own_store), kmax};
     const int *dmiss = upload(batch, miss);
     const SortCopy *dstores = upload(batch, stores);
     const SortCopy *dhits = upload(batch, hits);
     HIP_OK(launch_b_index(D, dsubs, B, X, kfast == 0, dmiss, (int) miss.size(), (int) hmiss, dstores,
-                          (int) stores.size(), dhits, (int) hits.size(), max_cell, st));
+                          (int) stores.size(), dhits, (int) hits.size(), max_cell, cols, st));
 
 
     // ---- queries: the points of the trees each subspace refines, in sorted order

@@ -85,9 +85,14 @@ __global__ void k_b_scatter(DevData D, const BatchSub *__restrict__ subs, const 
 
 // the index builder's columns: every dimension but the child's own (all of
 // them when nothing is shared), and the packed cells of the checked ones
+// Sorted columns kept across batches: ysrc[c] (c = rep position * kmax + t
+// for the index columns, the subspace for its own column) is a kept copy of
+// the column, read in order instead of gathered; ystore[c] receives a newly
+// gathered column to keep.  Either may be null.
 __global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ reps,
                            const int *__restrict__ order, float *__restrict__ Ys, uint16_t *__restrict__ Es,
-                           uint64_t *__restrict__ sigS, int need_es) {
+                           uint64_t *__restrict__ sigS, int need_es, const float *const *__restrict__ ysrc,
+                           float *const *__restrict__ ystore, int kmax) {
     const BatchSub &B = subs[reps[blockIdx.y]];
     const int n = D.n;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -96,7 +101,16 @@ __global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const i
     for (int t = 0; t < B.S.k; ++t) {
         if (t == B.pos_m) continue;
         const int s = B.S.dims[t];
-        Ys[B.coord + (long long) t * n + i] = D.Xt[(size_t) s * n + p];
+        const float *src = ysrc[blockIdx.y * kmax + t];
+        float v;
+        if (src) {
+            v = src[i];
+        } else {
+            v = D.Xt[(size_t) s * n + p];
+            float *dst = ystore[blockIdx.y * kmax + t];
+            if (dst) dst[i] = v;
+        }
+        Ys[B.coord + (long long) t * n + i] = v;
         if (need_es) Es[B.coord + (long long) t * n + i] = D.cellE[(size_t) s * n + p];
     }
     uint64_t w = 0;
@@ -108,7 +122,8 @@ __global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const i
 // every subspace: its own column and its packed cells
 __global__ void k_b_gather_own(DevData D, const BatchSub *__restrict__ subs, const int *__restrict__ order,
                                float *__restrict__ Ys, uint16_t *__restrict__ Es, const uint64_t *__restrict__ sigS,
-                               uint64_t *__restrict__ sig, int need_es) {
+                               uint64_t *__restrict__ sig, int need_es, const float *const *__restrict__ ysrc,
+                               float *const *__restrict__ ystore) {
     const BatchSub &B = subs[blockIdx.y];
     const int n = D.n;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -117,11 +132,23 @@ __global__ void k_b_gather_own(DevData D, const BatchSub *__restrict__ subs, con
     if (B.pos_m < B.S.k) {
         const int p = order[B.obase + i];
         const int s = B.S.dims[B.pos_m];
-        const uint16_t e = D.cellE[(size_t) s * n + p];
-        Ys[B.mcoord + i] = D.Xt[(size_t) s * n + p];
-        if (need_es) Es[B.mcoord + i] = e;
-        for (int f = 0; f < B.nchk4; ++f)
-            if (B.chk[f] == B.pos_m) w |= (uint64_t) e << (16 * f);
+        const float *src = ysrc[blockIdx.y];
+        float v;
+        if (src) {
+            v = src[i];
+        } else {
+            v = D.Xt[(size_t) s * n + p];
+            if (float *dst = ystore[blockIdx.y]) dst[i] = v;
+        }
+        Ys[B.mcoord + i] = v;
+        bool own_checked = false;
+        for (int f = 0; f < B.nchk4; ++f) own_checked = own_checked || B.chk[f] == B.pos_m;
+        if (need_es || own_checked) {
+            const uint16_t e = D.cellE[(size_t) s * n + p];
+            if (need_es) Es[B.mcoord + i] = e;
+            for (int f = 0; f < B.nchk4; ++f)
+                if (B.chk[f] == B.pos_m) w |= (uint64_t) e << (16 * f);
+        }
     }
     sig[B.base + i] = w;
 }
@@ -1537,7 +1564,7 @@ hipError_t launch_b_flags(const FlagTree *T, const int *blk_off, int nt, int nbl
 
 hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, const BatchIndex &X, bool need_es,
                           const int *miss_reps, int nmiss, int hmiss, const SortCopy *stores, int nstore,
-                          const SortCopy *hits, int nhit, int max_cell, hipStream_t st) {
+                          const SortCopy *hits, int nhit, int max_cell, const ColumnKeep &cols, hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
     // the scan over the sorted indexes' cells ends with their total, the
     // first position of the kept ones
@@ -1557,9 +1584,9 @@ hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, cons
         LAUNCH(k_b_sort_out, dim3(span, nstore), dim3(kBlock), 0, st, stores, D.n, X.order, X.keys, X.off);
     if (nhit > 0) LAUNCH(k_b_sort_in, dim3(span, nhit), dim3(kBlock), 0, st, hits, D.n, X.order, X.keys, X.off);
     LAUNCH(k_b_gather, dim3(blocks_for(D.n), X.nrep), dim3(kBlock), 0, st, D, subs, X.reps, X.order, X.Ys, X.Es,
-           X.sigS, (int) need_es);
+           X.sigS, (int) need_es, cols.src, cols.store, cols.kmax);
     LAUNCH(k_b_gather_own, dim3(blocks_for(D.n), nsub), dim3(kBlock), 0, st, D, subs, X.order, X.Ys, X.Es, X.sigS,
-           X.sig, (int) need_es);
+           X.sig, (int) need_es, cols.own_src, cols.own_store);
     return hipGetLastError();
 }
 
