@@ -133,6 +133,7 @@ def main():
     kernel_ms = sum(r.stats["count_kernel_ms"] for r in runs)
     launches = sum(r.stats["count_launches"] for r in runs)
     kbytes = sum(r.stats["count_bytes"] for r in runs)
+    ebytes = sum(r.stats["count_examined_bytes"] for r in runs)
     avg_launch_ms = kernel_ms / max(launches, 1)
     achieved = (kbytes / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
     kernel = "k_b_count" if args.mode == "batched" else "k_grid_count"
@@ -147,6 +148,11 @@ def main():
         "algorithmic_bytes_per_launch": round(kbytes / max(launches, 1), 1),
         "avg_launch_us": round(avg_launch_ms * 1e3, 3),
         "launches_per_step": launches // max(args.steps, 1),
+        # the candidates' packed cells and coordinates the walks read (mostly
+        # L2 hits): what the kernel moves through the caches, beside the
+        # algorithmic bytes of the neighbours it establishes
+        "examined_bytes_per_launch": round(ebytes / max(launches, 1), 1),
+        "examined_GBs": round((ebytes / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e9, 2) if avg_launch_ms > 0 else 0.0,
         "kernel_share_of_step": round(kernel_ms / (elapsed * 1e3), 4),
     }
 

@@ -80,6 +80,7 @@ class Stats(ctypes.Structure):
         ("pairs_scored", ctypes.c_int64),
         ("pairs_skipped", ctypes.c_int64),
         ("sorts_kept", ctypes.c_int64),
+        ("count_examined_bytes", ctypes.c_int64),
     ]
 
 

@@ -866,6 +866,7 @@ private:
             stats.candidates += (int64_t) h2[16 * b];
             stats.queries += (int64_t) h2[16 * b + 1];
             stats.count_bytes += (int64_t) h2[16 * b + 2];
+            stats.count_examined_bytes += (int64_t) h2[16 * b + 3];
         }
         // device time of the neighbour-count kernel, from the event pairs
         for (size_t i = 0; i < timers.size(); ++i) {

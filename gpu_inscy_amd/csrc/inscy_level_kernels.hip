@@ -734,21 +734,25 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
     if (stat) {
         // per wave into one of 64 slot groups: [0] candidates examined, [1]
         // points counted, [2] algorithmic bytes (4k + 4 per point, 4 + 4k per
-        // neighbour established)
+        // neighbour established), [3] bytes of the examined candidates (their
+        // packed cells when packed, and k coordinates)
         const bool mine = sub == 0 && active;
         long long done = mine ? 1 : 0;
         long long abytes = mine ? (4ll * k + 4) + (long long) found * (4 + 4ll * k) : 0;
         if (sub != 0) seen = 0;
+        long long ebytes = seen * ((B.nchk4 > 0 ? 8ll : 0ll) + 4ll * k);
         for (int o = 32; o > 0; o >>= 1) {
             seen += __shfl_down(seen, o);
             done += __shfl_down(done, o);
             abytes += __shfl_down(abytes, o);
+            ebytes += __shfl_down(ebytes, o);
         }
         if (__lane_id() == 0 && done) {
             unsigned long long *slot = stat + 16 * (blockIdx.x & 63);
             atomicAdd(&slot[0], (unsigned long long) seen);
             atomicAdd(&slot[1], (unsigned long long) done);
             atomicAdd(&slot[2], (unsigned long long) abytes);
+            atomicAdd(&slot[3], (unsigned long long) ebytes);
         }
     }
 }

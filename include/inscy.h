@@ -84,6 +84,9 @@ typedef struct inscy_stats {
     /* level-synchronous driver: grid indexes whose sort was taken from an
      * earlier batch of the run (same index dimensions) instead of sorted */
     int64_t sorts_kept;
+    /* level-synchronous driver: bytes of the candidates the neighbour-count
+     * kernel examined (packed cells and coordinates), most of them cache hits */
+    int64_t count_examined_bytes;
 } inscy_stats;
 
 typedef struct inscy_result inscy_result;
