@@ -499,10 +499,24 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     }
     for (int b = 0; b < B; ++b)
         if (hs[b].pos_m < hs[b].S.k) column(hs[b], hs[b].pos_m, own_src[b], own_store[b]);
-    ColumnKeep cols{upload(batch, ysrc), upload(batch, ystore), upload(batch, own_src), upload(batch, own_store), kmax};
-    const int *dmiss = upload(batch, miss);
-    const SortCopy *dstores = upload(batch, stores);
-    const SortCopy *dhits = upload(batch, hits);
+    // the index tables of the batch in one copy
+    std::vector<char> blob;
+    auto add = [&blob](const auto &v) {
+        const size_t at = (blob.size() + 255) & ~(size_t) 255;
+        const size_t bytes = v.size() * sizeof(v[0]);
+        blob.resize(at + bytes);
+        if (bytes) std::memcpy(blob.data() + at, v.data(), bytes);
+        return at;
+    };
+    const size_t o_ysrc = add(ysrc), o_ystore = add(ystore), o_osrc = add(own_src), o_ostore = add(own_store);
+    const size_t o_miss = add(miss), o_stores = add(stores), o_hits = add(hits);
+    char *dblob = upload(batch, blob);
+    ColumnKeep cols{reinterpret_cast<const float *const *>(dblob + o_ysrc), reinterpret_cast<float *const *>(dblob + o_ystore),
+                    reinterpret_cast<const float *const *>(dblob + o_osrc),
+                    reinterpret_cast<float *const *>(dblob + o_ostore), kmax};
+    const int *dmiss = reinterpret_cast<const int *>(dblob + o_miss);
+    const SortCopy *dstores = reinterpret_cast<const SortCopy *>(dblob + o_stores);
+    const SortCopy *dhits = reinterpret_cast<const SortCopy *>(dblob + o_hits);
     HIP_OK(launch_b_index(D, dsubs, B, X, kfast == 0, dmiss, (int) miss.size(), (int) hmiss, dstores,
                           (int) stores.size(), dhits, (int) hits.size(), max_cell, cols, st));
 
