@@ -153,6 +153,7 @@ def lib():
     L.inscy_debug_trace.argtypes = [ctypes.c_int]
     L.inscy_debug_phi_exact.argtypes = [ctypes.c_int]
     L.inscy_debug_generic_walk.argtypes = [ctypes.c_int]
+    L.inscy_debug_keep_budget.argtypes = [ctypes.c_int64]
     L.inscy_debug_trace_rows.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]
     L.inscy_debug_trace_rows.restype = ctypes.c_int64
     _lib = L
@@ -304,6 +305,11 @@ def debug_phi_exact(on: bool) -> None:
 def debug_generic_walk(on: bool) -> None:
     """Test hook: run the neighbour walks for k known only at run time."""
     lib().inscy_debug_generic_walk(int(on))
+
+
+def debug_keep_budget(nbytes: int) -> None:
+    """Test hook: bytes a run keeps for index sorts and for sorted columns (each; < 0: default)."""
+    lib().inscy_debug_keep_budget(int(nbytes))
 
 
 def debug_trace_rows() -> np.ndarray:

@@ -396,6 +396,7 @@ public:
     bool collect = false;
     bool phi_all_exact = false;  // test hook: every phi by the leaf-order recomputation
     bool generic_walk = false;   // test hook: no neighbour walk compiled for its k
+    long long keep_budget_override = -1;  // test hook: bytes kept across batches (< 0: the batch budget)
 
     void run(inscy_result &out) {
         int *P = arena.take<int>(n);

@@ -11,6 +11,7 @@ static thread_local std::vector<Engine::TraceRow> g_trace;
 static bool g_trace_on = false;
 static bool g_phi_all_exact = false;
 static bool g_generic_walk = false;
+static long long g_keep_budget = -1;
 static bool g_profile_on = false;
 
 static int fail(int code, const std::string &msg) {
@@ -92,6 +93,7 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
             eng.trace_on = g_trace_on;
             eng.phi_all_exact = g_phi_all_exact;
             eng.generic_walk = g_generic_walk;
+            eng.keep_budget_override = g_keep_budget;
             eng.profile_on = g_profile_on;
             // phi density (rectangular = false) is implemented in the
             // level-synchronous driver only; it serves every mode
@@ -374,6 +376,14 @@ int inscy_debug_phi_exact(int on) {
 // mixes dimensionalities (no launch compiled for its k).
 int inscy_debug_generic_walk(int on) {
     g_generic_walk = on != 0;
+    return INSCY_OK;
+}
+
+// Test hook (not in the public header): bytes for the grid-index sorts and
+// for the sorted columns a run keeps across batches (each); < 0: the batch
+// budget.
+int inscy_debug_keep_budget(int64_t bytes) {
+    g_keep_budget = bytes;
     return INSCY_OK;
 }
 

@@ -60,7 +60,7 @@ void Engine::run_levels(inscy_result &out) {
     kept.clear();
     keptp.reset();
     kept_used = 0;
-    kept_budget = batch_budget;
+    kept_budget = keep_budget_override >= 0 ? (size_t) keep_budget_override : batch_budget;
     kept_cols.clear();
     colp.reset();
     cols_used = 0;
@@ -471,8 +471,11 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     HIP_OK(launch_fill_i32(cntB, -1, (size_t) B * n, st));
     BatchSub *dsubs = upload(batch, hs);
     // sorted columns kept from earlier batches: the same index dimensions and
-    // dimension give the same column (a column kept in this batch is read only
-    // from the next one on: the batch's gathers run in one launch)
+    // dimension give the same column, in the order of the index's kept sort,
+    // so only an index whose sort is kept (copied in, or kept in this batch)
+    // keeps or reads columns: an index sorted again orders the points of a
+    // cell differently.  A column kept in this batch is read only from the
+    // next one on (the batch's gathers run in one launch).
     ++batch_no;
     auto sort_key = [&](const BatchSub &x) {
         uint64_t key = (uint64_t) x.G.j;
@@ -480,6 +483,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         return key;
     };
     auto column = [&](const BatchSub &x, int t, const float *&src, float *&dst) {
+        if (!kept.count(sort_key(x))) return;
         const uint64_t ck = sort_key(x) | (uint64_t) (x.S.dims[t] + 1) << 32;
         auto it = kept_cols.find(ck);
         if (it != kept_cols.end()) {

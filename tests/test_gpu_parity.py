@@ -369,6 +369,30 @@ def test_runtime_k_walks_match_golden(rect):
     assert compiled.stats["sorts_kept"] > 0
 
 
+@pytest.mark.parametrize("budget", [0, 3 << 20, 64 << 20])
+def test_kept_sorts_and_columns_budget(budget):
+    # nothing kept, or the budget spent part way through the run (later
+    # indexes are sorted and their columns gathered per batch, and a column
+    # is never kept for an index whose sort was not): the golden config-1
+    # result either way
+    import json
+    say("keep budget", budget)
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "config1.json")))
+    w = datagen.config(1)
+    _lib.debug_keep_budget(budget)
+    try:
+        res = INSCY.run(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0,
+                        mode=_lib.MODE_BATCHED)
+    finally:
+        _lib.debug_keep_budget(-1)
+    assert res.subspaces == [s for s, _, _, _ in g["subspaces"]]
+    assert digests(res) == [x[3] for x in g["subspaces"]]
+    say("sorts kept", res.stats["sorts_kept"])
+    assert res.stats["trees"] == g["trees"]
+    if budget == 0:
+        assert res.stats["sorts_kept"] == 0
+
+
 REAL_DIR = os.path.join(os.path.dirname(__file__), "golden", "real")
 
 
