@@ -448,7 +448,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.off = batch.take<int>(X.nhist);
     X.Ys = batch.take<float>(coord);
     X.Es = batch.take<uint16_t>(coord);
-    X.qflag = batch.take<uint8_t>((size_t) B * n);
+    X.qflag = batch.take<uint8_t>(((size_t) B * n + 15) & ~(size_t) 15);  // read 16 bytes a lane
     X.qlist = batch.take<int>(std::max<long long>(qoff, 1));
     X.nsel = batch.take<int>(1);
     X.qstart = batch.take<int>(B + 1);

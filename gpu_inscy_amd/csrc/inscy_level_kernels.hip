@@ -1594,12 +1594,96 @@ hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, cons
     return hipGetLastError();
 }
 
+// Ordered compaction of the query flags (one byte per sorted position, 0 or
+// 1) into the positions that are set: a count per tile of 4,096 flags, an
+// exclusive scan of the tile counts, and a write pass that ranks each lane's
+// 16 flags inside its tile.  Every lane reads its flags as one 16-byte load;
+// the flag array is padded to a multiple of 16 bytes and bytes at or past nb
+// are ignored.
+constexpr int kFlagsPerLane = 16;
+constexpr int kFlagTile = kBlock * kFlagsPerLane;
+
+__device__ __forceinline__ int set_bytes(uint32_t w) {
+    w |= w >> 4;  // bit 0 of each byte: the OR of the byte's bits
+    w |= w >> 2;
+    w |= w >> 1;
+    return __builtin_popcount(w & 0x01010101u);
+}
+
+__device__ __forceinline__ uint4 lane_flags(const uint8_t *__restrict__ qflag, size_t at, size_t nb) {
+    if (at >= nb) return make_uint4(0, 0, 0, 0);
+    uint4 v = *reinterpret_cast<const uint4 *>(qflag + at);
+    if (at + kFlagsPerLane > nb) {
+        uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        for (int q = 0; q < 4; ++q)
+            for (int c = 0; c < 4; ++c)
+                if (at + 4 * q + c >= nb) w[q] &= ~(0xffu << (8 * c));
+        v = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_b_flag_count(const uint8_t *__restrict__ qflag, size_t nb, int ntile,
+                                                          int *__restrict__ tcnt) {
+    __shared__ int wsum[kBlock / 64];
+    const size_t at = ((size_t) blockIdx.x * kBlock + threadIdx.x) * kFlagsPerLane;
+    const uint4 v = lane_flags(qflag, at, nb);
+    int c = set_bytes(v.x) + set_bytes(v.y) + set_bytes(v.z) + set_bytes(v.w);
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if (__lane_id() == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int q = 0; q < kBlock / 64; ++q) t += wsum[q];
+        tcnt[blockIdx.x] = t;
+        if (blockIdx.x == 0) tcnt[ntile] = 0;  // the scan's last entry is the total
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_b_flag_write(const uint8_t *__restrict__ qflag, size_t nb,
+                                                          const int *__restrict__ toff, int ntile,
+                                                          int *__restrict__ qlist, int *__restrict__ nsel) {
+    __shared__ int wsum[kBlock / 64];
+    const size_t at = ((size_t) blockIdx.x * kBlock + threadIdx.x) * kFlagsPerLane;
+    const uint4 v = lane_flags(qflag, at, nb);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    int m[4], c = 0;
+    for (int q = 0; q < 4; ++q) {
+        uint32_t x = w[q];
+        x |= x >> 4;
+        x |= x >> 2;
+        x |= x >> 1;
+        m[q] = (int) (x & 0x01010101u);
+        c += __builtin_popcount((uint32_t) m[q]);
+    }
+    const int lane = __lane_id();
+    int x = c;  // inclusive scan over the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    int o = toff[blockIdx.x] + x - c;
+    for (int q = 0; q < (int) (threadIdx.x >> 6); ++q) o += wsum[q];
+    for (int q = 0; q < 4; ++q) {
+        uint32_t mm = (uint32_t) m[q];
+        while (mm) {
+            const int bit = __builtin_ctz(mm);
+            qlist[o++] = (int) (at + 4 * q + (bit >> 3));
+            mm &= mm - 1;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *nsel = toff[ntile];
+}
+
+static int flag_tiles(size_t items) { return (int) ((items + kFlagTile - 1) / kFlagTile); }
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t) 255; }
+
 size_t select_temp_bytes(int items) {
-    size_t bytes = 0;
-    if (hipcub::DeviceSelect::Flagged(nullptr, bytes, hipcub::CountingInputIterator<int>(0), (const uint8_t *) nullptr,
-                                      (int *) nullptr, (int *) nullptr, items) != hipSuccess)
-        return 1u << 20;
-    return bytes;
+    const int nt = flag_tiles((size_t) std::max(items, 1));
+    return 2 * align256((size_t) (nt + 1) * sizeof(int)) + align256(scan_temp_bytes(nt + 1));
 }
 
 hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySeg *segs, const int *blk_off,
@@ -1609,10 +1693,16 @@ hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySe
     LAUNCH(k_b_fill<uint8_t>, dim3(blocks_for(nb)), dim3(kBlock), 0, st, X.qflag, (uint8_t) 0, nb);
     if (nblk > 0)
         LAUNCH(k_b_qmark, dim3(nblk), dim3(kBlock), 0, st, subs, segs, blk_off, nseg, X.keys, X.qflag);
-    size_t tb = X.select_bytes;
-    hipError_t e = hipcub::DeviceSelect::Flagged(X.select_temp, tb, hipcub::CountingInputIterator<int>(0), X.qflag,
-                                                 X.qlist, X.nsel, (int) nb, st);
+    const int nt = flag_tiles(nb);
+    char *base = static_cast<char *>(X.select_temp);
+    int *tcnt = reinterpret_cast<int *>(base);
+    int *toff = reinterpret_cast<int *>(base + align256((size_t) (nt + 1) * sizeof(int)));
+    char *temp = base + 2 * align256((size_t) (nt + 1) * sizeof(int));
+    size_t tb = X.select_bytes - 2 * align256((size_t) (nt + 1) * sizeof(int));
+    LAUNCH(k_b_flag_count, dim3(nt), dim3(kBlock), 0, st, X.qflag, nb, nt, tcnt);
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, tb, tcnt, toff, nt + 1, st);
     if (e != hipSuccess) return e;
+    LAUNCH(k_b_flag_write, dim3(nt), dim3(kBlock), 0, st, X.qflag, nb, toff, nt, X.qlist, X.nsel);
     LAUNCH(k_b_qstart, dim3(blocks_for(nsub + 1)), dim3(kBlock), 0, st, X.qlist, X.nsel, nsub, n, X.qstart);
     return hipGetLastError();
 }
