@@ -154,6 +154,9 @@ def lib():
     L.inscy_debug_phi_exact.argtypes = [ctypes.c_int]
     L.inscy_debug_generic_walk.argtypes = [ctypes.c_int]
     L.inscy_debug_keep_budget.argtypes = [ctypes.c_int64]
+    L.inscy_debug_flag_compact_temp.argtypes = [ctypes.c_int64]
+    L.inscy_debug_flag_compact_temp.restype = ctypes.c_int64
+    L.inscy_debug_flag_compact.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64, vp, vp, vp]
     L.inscy_debug_trace_rows.argtypes = [ctypes.POINTER(ctypes.c_int64), ctypes.c_int64]
     L.inscy_debug_trace_rows.restype = ctypes.c_int64
     _lib = L
@@ -310,6 +313,19 @@ def debug_generic_walk(on: bool) -> None:
 def debug_keep_budget(nbytes: int) -> None:
     """Test hook: bytes a run keeps for index sorts and for sorted columns (each; < 0: default)."""
     lib().inscy_debug_keep_budget(int(nbytes))
+
+
+def debug_flag_compact_temp(nb: int) -> int:
+    """Test hook: device scratch bytes of debug_flag_compact over nb flags."""
+    return int(lib().inscy_debug_flag_compact_temp(int(nb)))
+
+
+def debug_flag_compact(flags_ptr: int, nb: int, temp_ptr: int, temp_bytes: int, out_ptr: int, nsel_ptr: int,
+                       stream: int = 0) -> None:
+    """Test hook: the ordered compaction of the query flags (device pointers)."""
+    check(lib().inscy_debug_flag_compact(ctypes.c_void_p(flags_ptr), int(nb), ctypes.c_void_p(temp_ptr),
+                                         int(temp_bytes), ctypes.c_void_p(out_ptr), ctypes.c_void_p(nsel_ptr),
+                                         ctypes.c_void_p(stream or None)))
 
 
 def debug_trace_rows() -> np.ndarray:

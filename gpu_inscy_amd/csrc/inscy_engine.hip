@@ -387,6 +387,25 @@ int inscy_debug_keep_budget(int64_t bytes) {
     return INSCY_OK;
 }
 
+// Test hook (not in the public header): the ordered compaction of the query
+// flags.  out[0, *nsel) receives the positions of the nonzero bytes of
+// flags[0, nb) in ascending order.  Device pointers on `stream`: flags
+// readable up to nb rounded up to 16 bytes, out of nb ints, temp of
+// inscy_debug_flag_compact_temp(nb) bytes.
+int64_t inscy_debug_flag_compact_temp(int64_t nb) {
+    if (nb <= 0 || nb >= INT_MAX) return -1;
+    return (int64_t) select_temp_bytes((int) nb);
+}
+
+int inscy_debug_flag_compact(const uint8_t *flags, int64_t nb, void *temp, int64_t temp_bytes, int *out, int *nsel,
+                             void *stream) {
+    if (!flags || !temp || !out || !nsel || nb <= 0 || nb >= INT_MAX ||
+        temp_bytes < inscy_debug_flag_compact_temp(nb))
+        return fail(INSCY_ERR_ARGUMENT, "invalid argument");
+    hipError_t e = launch_flag_compact(flags, (size_t) nb, temp, (size_t) temp_bytes, out, nsel, (hipStream_t) stream);
+    return e == hipSuccess ? INSCY_OK : fail(INSCY_ERR_DEVICE, hipGetErrorString(e));
+}
+
 int64_t inscy_debug_trace_rows(int64_t *out, int64_t cap) {
     int64_t m = (int64_t) g_trace.size();
     if (out)

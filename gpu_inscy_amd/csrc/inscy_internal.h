@@ -202,6 +202,11 @@ struct CopySeg {
 float sqrt_threshold(float eps);
 size_t scan_temp_bytes(int items);
 size_t select_temp_bytes(int items);
+// positions of the nonzero bytes of flags[0, nb) in order into out, their
+// number into *nsel; flags readable up to nb rounded up to 16 bytes, temp of
+// select_temp_bytes(nb) bytes
+hipError_t launch_flag_compact(const uint8_t *flags, size_t nb, void *temp, size_t temp_bytes, int *out, int *nsel,
+                               hipStream_t st);
 
 // Launchers (all asynchronous on `st`; return the hipError_t of the launches).
 hipError_t launch_discretise(const float *X, int n, int d, int cells, float eps, float *Xt, uint8_t *cellC,

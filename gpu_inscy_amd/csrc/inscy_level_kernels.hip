@@ -1686,6 +1686,21 @@ size_t select_temp_bytes(int items) {
     return 2 * align256((size_t) (nt + 1) * sizeof(int)) + align256(scan_temp_bytes(nt + 1));
 }
 
+hipError_t launch_flag_compact(const uint8_t *flags, size_t nb, void *temp, size_t temp_bytes, int *out, int *nsel,
+                               hipStream_t st) {
+    const int nt = flag_tiles(nb);
+    const size_t cnt_bytes = align256((size_t) (nt + 1) * sizeof(int));
+    char *base = static_cast<char *>(temp);
+    int *tcnt = reinterpret_cast<int *>(base);
+    int *toff = reinterpret_cast<int *>(base + cnt_bytes);
+    size_t tb = temp_bytes - 2 * cnt_bytes;
+    LAUNCH(k_b_flag_count, dim3(nt), dim3(kBlock), 0, st, flags, nb, nt, tcnt);
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(base + 2 * cnt_bytes, tb, tcnt, toff, nt + 1, st);
+    if (e != hipSuccess) return e;
+    LAUNCH(k_b_flag_write, dim3(nt), dim3(kBlock), 0, st, flags, nb, toff, nt, out, nsel);
+    return hipGetLastError();
+}
+
 hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySeg *segs, const int *blk_off,
                             int nseg, int nblk, const BatchIndex &X, hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
@@ -1693,16 +1708,8 @@ hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySe
     LAUNCH(k_b_fill<uint8_t>, dim3(blocks_for(nb)), dim3(kBlock), 0, st, X.qflag, (uint8_t) 0, nb);
     if (nblk > 0)
         LAUNCH(k_b_qmark, dim3(nblk), dim3(kBlock), 0, st, subs, segs, blk_off, nseg, X.keys, X.qflag);
-    const int nt = flag_tiles(nb);
-    char *base = static_cast<char *>(X.select_temp);
-    int *tcnt = reinterpret_cast<int *>(base);
-    int *toff = reinterpret_cast<int *>(base + align256((size_t) (nt + 1) * sizeof(int)));
-    char *temp = base + 2 * align256((size_t) (nt + 1) * sizeof(int));
-    size_t tb = X.select_bytes - 2 * align256((size_t) (nt + 1) * sizeof(int));
-    LAUNCH(k_b_flag_count, dim3(nt), dim3(kBlock), 0, st, X.qflag, nb, nt, tcnt);
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(temp, tb, tcnt, toff, nt + 1, st);
+    hipError_t e = launch_flag_compact(X.qflag, nb, X.select_temp, X.select_bytes, X.qlist, X.nsel, st);
     if (e != hipSuccess) return e;
-    LAUNCH(k_b_flag_write, dim3(nt), dim3(kBlock), 0, st, X.qflag, nb, toff, nt, X.qlist, X.nsel);
     LAUNCH(k_b_qstart, dim3(blocks_for(nsub + 1)), dim3(kBlock), 0, st, X.qlist, X.nsel, nsub, n, X.qstart);
     return hipGetLastError();
 }
