@@ -87,8 +87,9 @@ def traffic_from_profile(kernel, cfg):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the multi-GPU design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: the engine measures "
+              "one GPU's share of the work (independent subspace branches per GPU), with no "
+              "collectives", file=sys.stderr)
         sys.exit(2)
 
     import torch
