@@ -41,9 +41,3 @@ def test_flag_compaction_matches_nonzero(nb, density):
     assert k == want.numel()
     assert torch.equal(got, want)
     assert bool((rest == -7).all())  # nothing written past the count
-
-
-def test_flag_compaction_rejects_bad_sizes():
-    assert _lib.debug_flag_compact_temp(0) < 0
-    with pytest.raises(_lib.InscyError):
-        _lib.debug_flag_compact(0, 16, 0, 0, 0, 0)

@@ -52,6 +52,15 @@ def test_argument_errors_are_codes_not_crashes():
         _lib.check(_lib.INSCY_ERR_ARGUMENT)
 
 
+def test_flag_compaction_hook_rejects_bad_arguments_before_any_device_call():
+    assert _lib.debug_flag_compact_temp(0) < 0
+    assert _lib.debug_flag_compact_temp(1 << 31) < 0
+    with pytest.raises(_lib.InscyError):
+        _lib.debug_flag_compact(0, 16, 0, 0, 0, 0)  # null flags
+    with pytest.raises(_lib.InscyError):
+        _lib.debug_flag_compact(16, 0, 16, 1 << 20, 16, 16)  # no flags
+
+
 def test_run_many_rejects_bad_arguments_before_any_device_call():
     L = _lib.lib()
     X = np.zeros((4, 2), dtype=np.float32)
