@@ -1,28 +1,39 @@
 """Benchmark: end-to-end INSCY on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode exact|batched] [--config 1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode exact|batched|learned] [--config 2]
 
 One step is one complete INSCY run (all subspaces, all clusters) over the
 synthetic data of the chosen workload, already resident in HBM; the timed
-region spans exactly K steps between device synchronisations.  The reported
-value is restricted (subspace, region) trees evaluated per second, the
-throughput unit of the reference's depth-first search; ms_per_step is the
-end-to-end time of one run (labels of every result subspace back in host
-memory).  The default path is the level-synchronous driver (`batched`, exact
-results); the depth-first driver (`exact`, one restricted tree at a time) is
-timed beside it outside the timed region and reported as depth_first_ms.
+region spans exactly K steps between device synchronisations.  The default
+workload is n=100,000 d=20 (20 Zipf-sized clusters, eps=0.02, c=8, mu=4,
+min_size=1000, rectangular).  The reported value is restricted (subspace,
+region) trees evaluated per second, the throughput unit of the reference's
+depth-first search; ms_per_step is the end-to-end time of one run (labels of
+every result subspace back in host memory).  The default path is the
+level-synchronous driver (`batched`, exact results); the depth-first driver
+(`exact`, one restricted tree at a time) is timed beside it outside the timed
+region and reported as depth_first_ms.
 
 Also reported:
+  parity        the last timed run equals the committed golden digests of the
+                oracle (tests/golden/config<N>.json), when present.
   roofline      the neighbour-count kernel (the dominant one; k_b_count in the
                 level-synchronous driver, k_grid_count in the depth-first one):
                 its algorithmic bytes (per counted point 4k+4, per neighbour
                 established 4+4k) over its device time measured with HIP events
                 on the engine's stream during the timed steps, against 8 TB/s
                 HBM; `traffic` comes from the committed rocprofv3 counter summary.
+                ms_per_step_without_events repeats a few runs without the
+                events, to show their overhead.
   cpu_baseline  the oracle (a single-threaded CPU restatement of the
                 reference's CPU INSCY) on the same data for a bounded time
-                budget, as trees per second.
-  parity        the result equals the committed golden digests of the oracle.
+                budget, as trees per second, with the host's CPU model.
+  dropin        the drop-in call GPU5() (host tensor in, Python lists out, as
+                the reference's op returns) once, in seconds.
+  learned       the learned-prune mode on the n=200,000 d=32 evaluation set
+                against the exact run of the same data: seconds, speed-up and
+                cluster F1.
+Progress lines go to stderr; the one JSON line to stdout.
 """
 
 from __future__ import annotations
@@ -47,10 +58,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", choices=["exact", "batched", "learned"], default="batched",
                     help="learned: the learned pruner (gpu_inscy_amd/pruner_weights.npz) against the exact run")
-    ap.add_argument("--depth-first-steps", type=int, default=2,
+    ap.add_argument("--depth-first-steps", type=int, default=1,
                     help="untimed runs of the depth-first driver for the comparison (0: skip)")
-    ap.add_argument("--config", type=int, default=1,
-                    help="workload index (1: n=25000 d=15; 2: n=100000 d=20; 3: 64 data sets)")
+    ap.add_argument("--config", type=int, default=2,
+                    help="workload index (2: n=100000 d=20, the default; 1: n=25000 d=15; 4: n=200000 d=32; "
+                         "3: 64 data sets)")
+    ap.add_argument("--no-dropin", dest="dropin", action="store_false",
+                    help="skip the drop-in GPU5() call (host tensor in, Python lists out)")
+    ap.add_argument("--learned-config", type=int, default=4,
+                    help="workload of the learned-prune leg (0: skip)")
     ap.add_argument("--real", choices=["glass", "vowel", "pendigits", "sky_0.5x0.5", "sky_1x1", "sky_1x2"],
                     help="one of the reference's real-data workloads instead of --config")
     ap.add_argument("--threshold", type=float, default=None,
@@ -84,6 +100,20 @@ def traffic_from_profile(kernel, cfg):
         return None
 
 
+def host_cpu():
+    """CPU model and hardware threads of this host (the CPU-baseline leg's machine)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -92,6 +122,7 @@ def main():
               "collectives", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
 
     from gpu_inscy_amd import _lib, datagen
@@ -111,30 +142,50 @@ def main():
     X = torch.from_numpy(w.X).to("cuda:0")
     torch.cuda.synchronize()
 
-    def step():
+    def step(m=mode):
         return INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, w.entropy_order,
-                         mode=mode)
+                         mode=m)
 
     res = None
     for _ in range(max(args.warmup, 0)):
+        res = None
         res = step()
-    if res is None:
-        res = step()
-    parity = golden_parity(args.config, res)
+    say(f"warm-up done ({args.warmup} runs)")
 
+    # timed region: K complete runs, the count kernel's launches bracketed by
+    # HIP events on the engine's stream (inscy_profile); only the last result
+    # is held, as a caller that consumes each run would
     _lib.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    runs = [step() for _ in range(args.steps)]
+    stats = []
+    for _ in range(args.steps):
+        res = None
+        res = step()
+        stats.append(res.stats)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     _lib.profile(False)
+    ms_step = elapsed / args.steps * 1e3
+    say(f"timed: {args.steps} runs, {ms_step:.3f} ms/run")
+    parity = golden_parity(args.config, res)  # the last timed run against the oracle's digests
+    say(f"parity of the last timed run: {parity}")
 
-    trees = sum(r.stats["trees"] for r in runs)
-    kernel_ms = sum(r.stats["count_kernel_ms"] for r in runs)
-    launches = sum(r.stats["count_launches"] for r in runs)
-    kbytes = sum(r.stats["count_bytes"] for r in runs)
-    ebytes = sum(r.stats["count_examined_bytes"] for r in runs)
+    # the same runs without the events, to show their overhead
+    unprof = max(1, min(3, args.steps))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(unprof):
+        r2 = step()
+        del r2
+    torch.cuda.synchronize()
+    ms_unprof = (time.perf_counter() - t1) / unprof * 1e3
+
+    trees = sum(st["trees"] for st in stats)
+    kernel_ms = sum(st["count_kernel_ms"] for st in stats)
+    launches = sum(st["count_launches"] for st in stats)
+    kbytes = sum(st["count_bytes"] for st in stats)
+    ebytes = sum(st["count_examined_bytes"] for st in stats)
     avg_launch_ms = kernel_ms / max(launches, 1)
     achieved = (kbytes / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
     kernel = "k_b_count" if args.mode == "batched" else "k_grid_count"
@@ -157,23 +208,35 @@ def main():
         "kernel_share_of_step": round(kernel_ms / (elapsed * 1e3), 4),
     }
 
-    other = None
+    # the depth-first driver (one restricted tree at a time) on the same data
+    other = same = None
     if args.depth_first_steps > 0:
         omode = _lib.MODE_EXACT if mode == _lib.MODE_BATCHED else _lib.MODE_BATCHED
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         for _ in range(args.depth_first_steps):
-            ores = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, w.entropy_order,
-                             mode=omode)
+            ores = None
+            ores = step(omode)
         torch.cuda.synchronize()
         other = (time.perf_counter() - t1) / args.depth_first_steps * 1e3
-        import numpy as np
         same = ores.subspaces == res.subspaces and np.array_equal(ores.labels, res.labels)
-    ms_step = elapsed / args.steps * 1e3
+        del ores
+        say(f"other driver: {other:.1f} ms/run, same result: {same}")
+
+    # the drop-in call (python/inscy.py GPU5: host tensor in, Python lists out)
+    dropin = None
+    if args.dropin:
+        Xh = torch.from_numpy(w.X)
+        t1 = time.perf_counter()
+        subs, cls = INSCY.GPU5(Xh, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, w.entropy_order)
+        dropin = round(time.perf_counter() - t1, 4)
+        del subs, cls
+        say(f"drop-in GPU5 with lists: {dropin} s")
 
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
+        model, threads = host_cpu()
         ctrees, csecs, finished = O.ref_rate(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular,
                                              max_seconds=args.cpu_seconds)
         cpu = {
@@ -184,7 +247,12 @@ def main():
             "sample": (f"first {ctrees} restricted trees of the depth-first run on the same {w.name} data "
                        f"({csecs:.1f} s budget{'; complete run' if finished else ''}); the full run evaluates "
                        f"{res.stats['trees']} trees"),
+            "cpu_model": model,
+            "host_threads": threads,
         }
+        say(f"cpu baseline: {cpu['value']} trees/s")
+
+    learned = bench_learned_leg(args) if args.learned_config else None
 
     out = {
         "metric": "end-to-end INSCY seconds + subspaces/s; exact vs batched vs learned-prune (F1)",
@@ -206,13 +274,16 @@ def main():
             "depth_first_ms" if args.mode == "batched" else "batched_ms": round(other, 3) if other else None,
             "speedup_vs_depth_first": (round(other / ms_step, 3) if args.mode == "batched" else
                                        round(ms_step / other, 3)) if other else None,
-            "modes_agree": same if other else None,
-            "phase_seconds": dict({k: round(runs[-1].stats[k + "_seconds"], 4) for k in ("expand", "replay", "emit")},
-                                  library=round(runs[-1].stats["seconds"], 4)),
-            "library_ms_per_step": round(sum(r.stats["seconds"] for r in runs) / args.steps * 1e3, 3),
+            "modes_agree": same,
+            "ms_per_step_without_events": round(ms_unprof, 3),
+            "dropin_gpu5_lists_seconds": dropin,
+            "phase_seconds": dict({k: round(stats[-1][k + "_seconds"], 4) for k in ("expand", "replay", "emit")},
+                                  library=round(stats[-1]["seconds"], 4)),
+            "library_ms_per_step": round(sum(st["seconds"] for st in stats) / args.steps * 1e3, 3),
             "trees_per_step": trees // max(args.steps, 1),
             "subspaces": res.stats["subspaces"],
             "clusters": res.stats["clusters"],
+            "learned": learned,
             "parallelism": "single GPU",
         },
         "parity": parity,
@@ -220,6 +291,43 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def say(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def bench_learned_leg(args):
+    """Learned-prune mode on its evaluation workload against the exact
+    level-synchronous run of the same data: seconds of each, speed-up and the
+    cluster F1 of the learned result (gpu_inscy_amd.pruner.cluster_f1)."""
+    import torch
+
+    from gpu_inscy_amd import _lib, datagen, pruner
+    from gpu_inscy_amd import inscy as INSCY
+
+    w = datagen.config(args.learned_config)
+    X = torch.from_numpy(w.X).to("cuda:0")
+    _, thr, min_depth, _ = pruner.load()
+    pruner.use(threshold=thr)
+
+    def run(m):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, w.entropy_order, mode=m)
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t
+
+    run(_lib.MODE_LEARNED)  # warm-up
+    exact, exact_s = run(_lib.MODE_BATCHED)
+    learned, learned_s = run(_lib.MODE_LEARNED)
+    f1 = pruner.cluster_f1(exact, learned)
+    out = {"workload": f"{w.name}: n={w.n} d={w.d} eps={w.eps} c={w.cells} min_size={w.min_size}",
+           "exact_seconds": round(exact_s, 4), "learned_seconds": round(learned_s, 4),
+           "speedup_vs_exact": round(exact_s / learned_s, 3), "cluster_f1": round(f1["f1"], 5),
+           "threshold": thr, "trees_exact": exact.stats["trees"], "trees_learned": learned.stats["trees"]}
+    say(f"learned leg: {out}")
+    return out
 
 
 # The reference's published end-to-end seconds on its real-data workloads

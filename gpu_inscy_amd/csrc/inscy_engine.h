@@ -32,6 +32,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -820,13 +821,18 @@ private:
                 const size_t total = (size_t) r * n;
                 const size_t per = (total + nthr - 1) / nthr;
                 std::vector<std::thread> th;
+                th.reserve(nthr);
                 for (unsigned t = 0; t < nthr; ++t) {
                     const size_t lo = std::min(total, t * per), hi = std::min(total, lo + per);
-                    if (lo < hi)
-                        th.emplace_back([=] {
-                            std::memcpy(dst + lo, reinterpret_cast<const int32_t *>(src) + lo,
-                                        (hi - lo) * sizeof(int32_t));
-                        });
+                    if (lo >= hi) continue;
+                    auto job = [=] {
+                        std::memcpy(dst + lo, reinterpret_cast<const int32_t *>(src) + lo, (hi - lo) * sizeof(int32_t));
+                    };
+                    try {
+                        th.emplace_back(job);
+                    } catch (const std::system_error &) {
+                        job();  // no thread available: this thread copies the slice
+                    }
                 }
                 for (auto &t : th) t.join();
                 const int *hc = reinterpret_cast<const int *>(src + (size_t) r * n * sizeof(int32_t));

@@ -19,6 +19,24 @@ static int fail(int code, const std::string &msg) {
     return code;
 }
 
+// The exception in flight as a return code and message: nothing thrown inside
+// the library crosses the C ABI.
+static int fail_current() {
+    try {
+        throw;
+    } catch (const Failure &f) {
+        return fail(f.code, f.what());
+    } catch (const std::bad_alloc &) {
+        return fail(INSCY_ERR_MEMORY, "host allocation failed");
+    } catch (const std::system_error &e) {
+        return fail(INSCY_ERR_DEVICE, std::string("host thread or system call failed: ") + e.what());
+    } catch (const std::exception &e) {
+        return fail(INSCY_ERR_DEVICE, std::string("internal error: ") + e.what());
+    } catch (...) {
+        return fail(INSCY_ERR_DEVICE, "unknown internal error");
+    }
+}
+
 static int check_params(int64_t n, int32_t d, const inscy_params *p, int32_t mode) {
     if (!p) return fail(INSCY_ERR_ARGUMENT, "params is NULL");
     if (n < 1 || n > (int64_t) INT_MAX / 2) return fail(INSCY_ERR_ARGUMENT, "n out of range");
@@ -116,12 +134,9 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
         *out = res.release();
         g_error.clear();
         return INSCY_OK;
-    } catch (const Failure &f) {
+    } catch (...) {
         if (own) (void) hipStreamDestroy(st);
-        return fail(f.code, f.what());
-    } catch (const std::bad_alloc &) {
-        if (own) (void) hipStreamDestroy(st);
-        return fail(INSCY_ERR_MEMORY, "host allocation failed");
+        return fail_current();
     }
 }
 
@@ -175,7 +190,14 @@ static int run_many_impl(const float *const *X, const int64_t *n, const int32_t 
         (void) hipStreamDestroy(st);
     };
     std::vector<std::thread> th;
-    for (int w = 1; w < workers; ++w) th.emplace_back(work);
+    th.reserve(workers);
+    for (int w = 1; w < workers; ++w) {
+        try {
+            th.emplace_back(work);
+        } catch (const std::system_error &) {
+            break;  // fewer workers: the queue is drained by those running
+        }
+    }
     work();
     for (auto &t : th) t.join();
     for (int32_t i = 0; i < count; ++i) {
@@ -438,8 +460,8 @@ int inscy_discretise(const float *d_X, int64_t n, int32_t d, int32_t cells, floa
         HIP_OK(hipFree(bad));
         if (h) return fail(INSCY_ERR_ARGUMENT, "X must be finite with every value in [0, 1]");
         return INSCY_OK;
-    } catch (const Failure &f) {
-        return fail(f.code, f.what());
+    } catch (...) {
+        return fail_current();
     }
 }
 
@@ -468,8 +490,8 @@ int inscy_neighbour_counts(const float *d_X, int64_t n, int32_t d, float eps, co
         HIP_OK(launch_gather(cnt, d_queries, (int) nq, d_out, nullptr));
         HIP_OK(hipDeviceSynchronize());
         return INSCY_OK;
-    } catch (const Failure &f) {
-        return fail(f.code, f.what());
+    } catch (...) {
+        return fail_current();
     }
 }
 
@@ -486,8 +508,8 @@ int inscy_dense_components(const float *d_X, int64_t n, int32_t d, const inscy_p
         HIP_OK(hipMemcpyAsync(d_comp, comp, (size_t) n * sizeof(int), hipMemcpyDeviceToDevice, nullptr));
         HIP_OK(hipDeviceSynchronize());
         return INSCY_OK;
-    } catch (const Failure &f) {
-        return fail(f.code, f.what());
+    } catch (...) {
+        return fail_current();
     }
 }
 

@@ -466,9 +466,11 @@ __device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double
         float y[U][K];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            // 32-bit byte offsets from the column bases (a subspace's columns hold n < 2^29 entries)
+            // 32-bit byte offsets into the float columns (a subspace's columns
+            // hold n < 2^30 entries, so 4 jj < 2^32); the 8-byte packed cells
+            // are indexed with the 64-bit position (8 jj reaches 2^33)
             const uint32_t at4 = (uint32_t) jj[u] << 2;
-            w[u] = packed ? *reinterpret_cast<const uint64_t *>(reinterpret_cast<const char *>(W.sigb) + 2 * at4) : 0ull;
+            w[u] = packed ? W.sigb[(uint32_t) jj[u]] : 0ull;
 #pragma unroll
             for (int t = 0; t < K; ++t) y[u][t] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(col[t]) + at4);
         }

@@ -129,6 +129,16 @@ def config(idx: int, repeat: int = 0) -> Workload:
     raise ValueError(f"no single-dataset workload {idx}")
 
 
+def config2_small() -> Workload:
+    """Config 2's generator and parameters (20 Zipf(1.2) clusters in 4-10 of
+    20 dims, sigma~U[0.02,0.05], 15% noise; eps=0.02 c=8 mu=4) at n=25,000,
+    min_size=0.01 n, seed 2500: small enough for the single-threaded
+    pointer-tree oracle."""
+    rng = np.random.default_rng(2500)
+    X, t, s = make_clusters(rng, 25000, 20, 20, (4, 10), (0.02, 0.05), 0.15, sizes="zipf")
+    return Workload("n25000_d20_20cl_cfg2params", X, t, s, eps=0.02, cells=8, F=1.0, num_obj=4, min_size=250)
+
+
 def multi_instance(seed: int = 3000, count: int = 64):
     """64 independent datasets: n log-uniform in [2k, 50k], d in 8..24, 3-12
     clusters (subspace dim 3-8, sigma 0.03, 10% noise), eps=0.015 c=6 mu=4,

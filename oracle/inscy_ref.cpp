@@ -391,7 +391,12 @@ struct NeighbourIndex {
         }
     }
 
-    std::vector<int> candidates(int p, const std::vector<int> &S) const {
+    // The candidate set is the points within one ε-cell of p in every matched
+    // dimension; it is drawn from the bucket of the matched dimension with the
+    // fewest points in p's three cells, then checked on the others.  ordered:
+    // the neighbourhood tree's leaf order (needed where the list's order
+    // matters: φ's float sum); otherwise the bucket order.
+    std::vector<int> candidates(int p, const std::vector<int> &S, bool ordered = true) const {
         std::vector<int> list;
         if (literal) {
             walk(list, D.row(p), 0, -1, 0, S);
@@ -399,26 +404,35 @@ struct NeighbourIndex {
         }
         std::vector<int> M = matched(S);
         const int d = D.d;
-        int m0 = M[0];
+        size_t best = 0, best_size = SIZE_MAX;
+        for (size_t t = 0; t < M.size(); ++t) {
+            int ct = ecell[(size_t) p * d + M[t]];
+            size_t sz = 0;
+            for (int c = std::max(0, ct - 1); c <= std::min(etree.cells - 1, ct + 1); ++c) sz += bucket[M[t]][c].size();
+            if (sz < best_size) best_size = sz, best = t;
+        }
+        int m0 = M[best];
         int c0 = ecell[(size_t) p * d + m0];
         for (int c = std::max(0, c0 - 1); c <= std::min(etree.cells - 1, c0 + 1); ++c) {
             for (int q : bucket[m0][c]) {
                 bool ok = true;
-                for (size_t t = 1; t < M.size() && ok; ++t) {
+                for (size_t t = 0; t < M.size() && ok; ++t) {
+                    if (t == best) continue;
                     int dq = ecell[(size_t) q * d + M[t]] - ecell[(size_t) p * d + M[t]];
                     ok = dq >= -1 && dq <= 1;
                 }
                 if (ok) list.push_back(q);
             }
         }
-        std::sort(list.begin(), list.end(), [&](int a, int b) { return rank[a] < rank[b]; });
+        if (ordered) std::sort(list.begin(), list.end(), [&](int a, int b) { return rank[a] < rank[b]; });
         return list;
     }
 
-    // neighborhood() (ClusteringCpu.cu:52-81): ordered ε-neighbours of p in S.
-    std::vector<int> neighbours(int p, const std::vector<int> &S, float eps) const {
+    // neighborhood() (ClusteringCpu.cu:52-81): ε-neighbours of p in S, in the
+    // neighbourhood tree's leaf order when ordered.
+    std::vector<int> neighbours(int p, const std::vector<int> &S, float eps, bool ordered = true) const {
         std::vector<int> out;
-        for (int q : candidates(p, S)) {
+        for (int q : candidates(p, S, ordered)) {
             if (q == p) continue;
             float dd = dist_s(D, p, q, S);
             if (eps >= dd) out.push_back(q);
@@ -786,7 +800,7 @@ struct SetEngine {
         SubspaceMemo &m = memo[mask_of(S)];
         if (m.cnt.empty()) {
             m.cnt.assign(D.n, -1);
-            m.phiv.assign(D.n, 0.f);
+            if (!rect) m.phiv.assign(D.n, 0.f);
         }
         return m;
     }
@@ -800,8 +814,8 @@ struct SetEngine {
 #pragma omp parallel for schedule(dynamic, 64)
         for (long i = 0; i < (long) todo.size(); ++i) {
             int p = todo[i];
-            std::vector<int> nb = nbr.neighbours(p, S, eps);
-            m.phiv[p] = rect ? 0.f : phi(D, p, nb, eps, S);
+            std::vector<int> nb = nbr.neighbours(p, S, eps, !rect);  // only φ's sum depends on the order
+            if (!rect) m.phiv[p] = phi(D, p, nb, eps, S);
             m.cnt[p] = (int) nb.size();
         }
     }
@@ -835,7 +849,7 @@ struct SetEngine {
         std::vector<std::vector<int>> lists(D.n);
 #pragma omp parallel for schedule(dynamic, 64)
         for (int p = 0; p < D.n; ++p)
-            if (dn[p]) lists[p] = nbr.neighbours(p, S, eps);
+            if (dn[p]) lists[p] = nbr.neighbours(p, S, eps, false);  // components: order-free
         std::vector<int> par(D.n);
         for (int i = 0; i < D.n; ++i) par[i] = i;
         for (int p = 0; p < D.n; ++p) {

@@ -456,18 +456,111 @@ def check_invariants(res, w):
         assert sizes.min() >= w.min_size
 
 
-def test_config2_full_size_deterministic_and_consistent():
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _golden(name):
+    import json
+    path = os.path.join(GOLDEN, name)
+    return json.load(open(path)) if os.path.exists(path) else None
+
+
+def assert_golden(res, g):
+    assert res.subspaces == [s for s, _, _, _ in g["subspaces"]]
+    assert [int(l.max()) + 1 for l in res.labels] == [x[1] for x in g["subspaces"]]
+    assert digests(res) == [x[3] for x in g["subspaces"]]
+    assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
+
+
+def test_config2_full_size_matches_oracle_and_is_deterministic():
+    # n=100,000 d=20, c=8, eps=0.02: both drivers against the oracle's digests
+    # (set restatement, tests/golden/config2.json), and run to run
     say("config2 full size")
     w = datagen.config(2)
+    g = _golden("config2.json")
+    if g is not None:
+        assert g["X_sha256"] == hashlib.sha256(w.X.tobytes()).hexdigest()
     X = torch.from_numpy(w.X).cuda()
-    a = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0)
-    b = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0)
-    assert a.subspaces == b.subspaces and np.array_equal(a.labels, b.labels)
-    assert a.stats["trees"] == b.stats["trees"] and a.stats["calls"] == b.stats["calls"]
-    check_invariants(a, w)
-    say("config2", a.stats)
-    # the level-synchronous driver gives the same result, trees and calls
     c = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0, mode=_lib.MODE_BATCHED)
+    say("config2 batched", c.stats)
+    check_invariants(c, w)
+    if g is not None:
+        assert_golden(c, g)
+    b = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0, mode=_lib.MODE_BATCHED)
+    assert c.subspaces == b.subspaces and np.array_equal(c.labels, b.labels)
+    del b
+    # the depth-first driver gives the same result, trees and calls
+    a = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0, mode=_lib.MODE_EXACT)
+    say("config2 depth-first", a.stats)
     assert a.subspaces == c.subspaces and np.array_equal(a.labels, c.labels)
     assert a.stats["trees"] == c.stats["trees"] and a.stats["calls"] == c.stats["calls"]
-    say("config2 batched", c.stats)
+    if g is None:
+        pytest.skip("no oracle digests for config 2 (tests/golden/make_golden.py 2): checked run to run only")
+
+
+def test_config2_parameters_at_25k_match_pointer_oracle():
+    # config 2's parameters (c=8, eps=0.02, d=20, Zipf sizes) at n=25,000,
+    # against the pointer-tree restatement (tests/golden/config2_small.json)
+    g = _golden("config2_small.json")
+    if g is None:
+        pytest.skip("no golden for config 2's parameters at n=25k")
+    w = datagen.config2_small()
+    assert g["X_sha256"] == hashlib.sha256(w.X.tobytes()).hexdigest()
+    say("config2 params n=25k")
+    X = torch.from_numpy(w.X).cuda()
+    for mode in (_lib.MODE_BATCHED, _lib.MODE_EXACT):
+        res = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0, mode=mode)
+        assert_golden(res, g)
+
+
+def test_run_many_matches_config3_oracle():
+    # the 64 independent data sets through one run_many call, each against
+    # the oracle's digests (tests/golden/config3.json)
+    g = _golden("config3.json")
+    sets = datagen.multi_instance()
+    if g is None or not g.get("complete"):
+        pytest.skip("no complete oracle digests for the 64 data sets")
+    say("run_many 64 sets vs oracle")
+    jobs = [(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0) for w in sets]
+    many = INSCY.run_many(jobs, mode=_lib.MODE_BATCHED)
+    assert len(many) == len(sets) == len(g["sets"])
+    for w, res, gs in zip(sets, many, g["sets"]):
+        assert gs["workload"] == w.name and gs["X_sha256"] == hashlib.sha256(w.X.tobytes()).hexdigest()
+        assert_golden(res, gs)
+
+
+def test_config4_full_size_deterministic_across_keep_budgets():
+    # n=200,000 d=32 (the learned-prune evaluation set): the level-synchronous
+    # run gives one result with its kept index sorts and columns, with none
+    # kept, and with the budget spent part way; canonical labels, sizes
+    # >= min_size and the result order hold
+    say("config4 full size")
+    w = datagen.config(4)
+    X = torch.from_numpy(w.X).cuda()
+    run = lambda: INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0, mode=_lib.MODE_BATCHED)
+    a = run()
+    say("config4", a.stats)
+    assert a.stats["sorts_kept"] > 0
+    masks = [sum(1 << k for k in s) for s in a.subspaces]
+    assert masks == sorted(masks) and len(set(masks)) == len(masks)
+    lab = a.labels
+    assert lab.min() >= -1
+    for i in range(0, len(lab), 97):  # a spread sample of rows for the per-cluster checks
+        row = lab[i]
+        k = int(row.max()) + 1
+        sizes = np.bincount(row[row >= 0], minlength=k)
+        assert k >= 1 and sizes.min() >= w.min_size
+        firsts = np.array([np.argmax(row == c) for c in range(k)])
+        assert np.all(np.diff(firsts) > 0)
+    for budget in (0, 1 << 30):
+        _lib.debug_keep_budget(budget)
+        try:
+            b = run()
+        finally:
+            _lib.debug_keep_budget(-1)
+        say("config4 keep budget", budget, b.stats["sorts_kept"])
+        if budget == 0:
+            assert b.stats["sorts_kept"] == 0
+        assert b.subspaces == a.subspaces and np.array_equal(b.labels, a.labels)
+        assert b.stats["trees"] == a.stats["trees"] and b.stats["calls"] == a.stats["calls"]
+        del b

@@ -39,6 +39,12 @@ def test_argument_errors_are_codes_not_crashes():
     assert L.inscy_run(X.ctypes.data, 4, 2, None, 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
     assert b"params" in L.inscy_last_error()
     assert L.inscy_run(X.ctypes.data, 0, 2, ctypes.byref(p), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
+    # n above 2^30 - 1 is refused before any device call (the walks' 32-bit
+    # column offsets hold 4 n bytes)
+    assert L.inscy_run(X.ctypes.data, 1 << 30, 2, ctypes.byref(p), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
+    assert b"n out of range" in L.inscy_last_error()
+    assert L.inscy_run_device(X.ctypes.data, 1 << 30, 2, ctypes.byref(p), 2, None, ctypes.byref(h)) == \
+        _lib.INSCY_ERR_ARGUMENT
     assert L.inscy_run(X.ctypes.data, 4, 65, ctypes.byref(p), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
     bad = _lib.make_params(0.0, 1.0, 2, 1, 1.0, 3, True, 0)
     assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(bad), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
