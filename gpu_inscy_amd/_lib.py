@@ -33,7 +33,8 @@ EXPORTED = [
     "inscy_result_labels_all", "inscy_run_many", "inscy_run_many_device", "inscy_release_memory",
     "inscy_run", "inscy_run_device", "inscy_result_num_subspaces", "inscy_result_subspace",
     "inscy_result_labels", "inscy_result_num_points", "inscy_result_stats", "inscy_result_free",
-    "inscy_last_error", "inscy_discretise", "inscy_neighbour_counts", "inscy_dense_components", "inscy_profile",
+    "inscy_last_error", "inscy_discretise", "inscy_neighbour_counts",
+    "inscy_dense_components", "inscy_profile",
     "inscy_set_pruner", "inscy_result_pairs", "inscy_mlp_forward", "inscy_mlp_gradient", "inscy_mlp_adam",
     "inscy_mlp_round",
 ]
@@ -154,6 +155,17 @@ def lib():
     L.inscy_debug_phi_exact.argtypes = [ctypes.c_int]
     L.inscy_debug_generic_walk.argtypes = [ctypes.c_int]
     L.inscy_debug_keep_budget.argtypes = [ctypes.c_int64]
+    L.inscy_debug_thresholds.argtypes = [ctypes.c_int32, ctypes.c_float, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]
+    L.inscy_debug_thresholds.restype = ctypes.c_int
+    L.inscy_debug_scan_temp.argtypes = [ctypes.c_int64]
+    L.inscy_debug_scan_temp.restype = ctypes.c_int64
+    L.inscy_debug_exclusive_sum.argtypes = [vp, vp, ctypes.c_int64, vp, ctypes.c_int64, vp]
+    L.inscy_debug_exclusive_sum.restype = ctypes.c_int
+    L.inscy_debug_sort_temp.argtypes = [ctypes.c_int64]
+    L.inscy_debug_sort_temp.restype = ctypes.c_int64
+    L.inscy_debug_sort_pairs.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int32, vp, ctypes.c_int64,
+                                         vp]
+    L.inscy_debug_sort_pairs.restype = ctypes.c_int
     L.inscy_debug_flag_compact_temp.argtypes = [ctypes.c_int64]
     L.inscy_debug_flag_compact_temp.restype = ctypes.c_int64
     L.inscy_debug_flag_compact.argtypes = [vp, ctypes.c_int64, vp, ctypes.c_int64, vp, vp, vp]

@@ -377,7 +377,7 @@ public:
             uint16_t *k1 = arena.take<uint16_t>(n);
             uint16_t *k2 = arena.take<uint16_t>(n);
             void *temp = arena.take<char>(tb);
-            HIP_OK(launch_rank_order(cellE, n, d, level.data(), rank_perm, tmp, k1, k2, temp, tb, st));
+            HIP_OK(launch_rank_order(cellE, n, d, ecells, level.data(), rank_perm, tmp, k1, k2, temp, tb, st));
             sync();
         }
         arena.release(0);

@@ -379,6 +379,50 @@ int inscy_profile(int on) {
     return INSCY_OK;
 }
 
+// Test hooks (not in the public header): the engine's exclusive prefix sum
+// and its stable (uint16 key, int value) sort on device arrays, with the
+// scratch sizes they need.
+int64_t inscy_debug_scan_temp(int64_t items) {
+    if (items <= 0 || items >= INT_MAX) return -1;
+    return (int64_t) scan_temp_bytes((int) items);
+}
+
+int inscy_debug_exclusive_sum(const int *in, int *out, int64_t items, void *temp, int64_t temp_bytes, void *stream) {
+    if (!in || !out || items <= 0 || items >= INT_MAX || temp_bytes < inscy_debug_scan_temp(items) ||
+        (!temp && temp_bytes > 0))
+        return fail(INSCY_ERR_ARGUMENT, "invalid argument");
+    hipError_t e = launch_exclusive_sum(temp, (size_t) temp_bytes, in, out, (int) items, (hipStream_t) stream);
+    return e == hipSuccess ? INSCY_OK : fail(INSCY_ERR_DEVICE, hipGetErrorString(e));
+}
+
+int64_t inscy_debug_sort_temp(int64_t n) {
+    if (n <= 0 || n >= INT_MAX) return -1;
+    return (int64_t) sort_temp_bytes((int) n);
+}
+
+int inscy_debug_sort_pairs(const uint16_t *keys, uint16_t *keys_tmp, uint16_t *keys_out, const int *vals,
+                           int *vals_tmp, int *vals_out, int64_t n, int32_t bits, void *temp, int64_t temp_bytes,
+                           void *stream) {
+    if (!keys || !keys_tmp || !keys_out || !vals || !vals_tmp || !vals_out || !temp || n <= 0 || n >= INT_MAX ||
+        bits < 1 || bits > 16 || temp_bytes < inscy_debug_sort_temp(n))
+        return fail(INSCY_ERR_ARGUMENT, "invalid argument");
+    hipError_t e = launch_sort_pairs_u16(keys, keys_tmp, keys_out, vals, vals_tmp, vals_out, (int) n, bits, temp,
+                                         (size_t) temp_bytes, (hipStream_t) stream);
+    return e == hipSuccess ? INSCY_OK : fail(INSCY_ERR_DEVICE, hipGetErrorString(e));
+}
+
+// Test hook (not in the public header): the density thresholds the engine
+// compares with, out = {c(k), alpha(k), expDen(k), omega(k)} (host arithmetic
+// only, no device call).
+int inscy_debug_thresholds(int32_t k, float eps, int32_t n, float *out) {
+    if (!out || k < 0 || k > kMaxDims || n < 1) return fail(INSCY_ERR_ARGUMENT, "invalid argument");
+    out[0] = ball(k);
+    out[1] = alpha_thr(k, eps, n);
+    out[2] = expden(k, eps, n);
+    out[3] = omega_thr(k);
+    return INSCY_OK;
+}
+
 // Test hook (not in the public header): record one trace row per evaluated
 // restricted tree — (mask, lo, hi, points, survivors, pass, redundant).
 int inscy_debug_trace(int on) {

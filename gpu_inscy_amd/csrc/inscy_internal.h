@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 
@@ -200,7 +201,15 @@ struct CopySeg {
 // Largest float s with sqrtf(s) <= eps: the distance test fl(sqrt(sum)) <= eps
 // of ClusteringCpu.cu:52-81 is then sum <= s_star without a device sqrt.
 float sqrt_threshold(float eps);
+// exclusive prefix sum of items ints (inscy_scan.hip); temp of scan_temp_bytes(items)
 size_t scan_temp_bytes(int items);
+hipError_t launch_exclusive_sum(void *temp, size_t temp_bytes, const int *in, int *out, int items, hipStream_t st);
+// stable sort of (uint16 key, int value) pairs by the low `bits` (<= 16) key
+// bits; two 8-bit passes above 8 bits go through keys_tmp / vals_tmp
+size_t sort_temp_bytes(int n);
+hipError_t launch_sort_pairs_u16(const uint16_t *keys, uint16_t *keys_tmp, uint16_t *keys_out, const int *vals,
+                                 int *vals_tmp, int *vals_out, int n, int bits, void *temp, size_t temp_bytes,
+                                 hipStream_t st);
 size_t select_temp_bytes(int items);
 // positions of the nonzero bytes of flags[0, nb) in order into out, their
 // number into *nsel; flags readable up to nb rounded up to 16 bytes, temp of
@@ -314,7 +323,7 @@ hipError_t launch_b_partition(const PartPair *pairs, const int *blk_off, int npa
                               int n, int thr, int wbit, int min_size, hipStream_t st);
 // points sorted by their eps cells over all dimensions in `level` order, ties
 // by id: the leaf order of the reference's neighbourhood tree (d stable passes)
-hipError_t launch_rank_order(const uint16_t *cellE, int n, int d, const int *level, int *perm, int *tmp,
+hipError_t launch_rank_order(const uint16_t *cellE, int n, int d, int ecells, const int *level, int *perm, int *tmp,
                              uint16_t *keys, uint16_t *keys2, void *temp, size_t temp_bytes, hipStream_t st);
 size_t rank_order_temp_bytes(int n);
 // path prune of many trees (blk_w over their points, blk_p over their paths)

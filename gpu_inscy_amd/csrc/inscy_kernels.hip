@@ -14,7 +14,6 @@
 #include <cmath>
 #include <cstring>
 
-#include <hipcub/hipcub.hpp>
 
 #include "inscy_device.h"
 #include "inscy_internal.h"
@@ -736,12 +735,6 @@ hipError_t launch_histc(const uint8_t *cellC, int n, int d, int cells, int *hist
     return hipGetLastError();
 }
 
-size_t scan_temp_bytes(int items) {
-    size_t bytes = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (int *) nullptr, (int *) nullptr, items) != hipSuccess)
-        return 1u << 20;
-    return bytes;
-}
 
 static hipError_t build_grid(const DevData &D, const SubspaceArg &S, const GridArg &G, const GridBuffers &B,
                              hipStream_t st) {
@@ -750,8 +743,7 @@ static hipError_t build_grid(const DevData &D, const SubspaceArg &S, const GridA
     for (int a = 0; a < G.j; ++a) ncell *= G.C;
     LAUNCH(k_fill<int>, dim3(blocks_for(ncell + 1)), dim3(kBlock), 0, st, B.hist, 0, (size_t) ncell + 1);
     LAUNCH(k_grid_keys, dim3(blocks_for(n)), dim3(kBlock), 0, st, D, S, G, B.keys, B.rank, B.hist);
-    size_t tb = B.scan_bytes;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(B.scan_temp, tb, B.hist, B.off, ncell + 1, st);
+    hipError_t e = launch_exclusive_sum(B.scan_temp, B.scan_bytes, B.hist, B.off, ncell + 1, st);
     if (e != hipSuccess) return e;
     LAUNCH(k_grid_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, st, D, S, B.keys, B.rank, B.off, B.order, B.Ys, B.Es);
     return hipGetLastError();
@@ -873,8 +865,7 @@ hipError_t launch_emit(const EmitRow *rows, int r, int n, int *flags, int *idx, 
     const size_t items = (size_t) r * n + 1;
     LAUNCH(k_fill<int>, dim3(1), dim3(kBlock), 0, st, flags + (size_t) r * n, 0, (size_t) 1);
     LAUNCH(k_emit_roots, dim3(blocks_for(n), r), dim3(kBlock), 0, st, rows, n, flags);
-    size_t tb = scan_bytes;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(scan_temp, tb, flags, idx, (int) items, st);
+    hipError_t e = launch_exclusive_sum(scan_temp, scan_bytes, flags, idx, (int) items, st);
     if (e != hipSuccess) return e;
     LAUNCH(k_emit_labels, dim3(blocks_for(n), r), dim3(kBlock), 0, st, rows, n, idx, labels);
     LAUNCH(k_emit_counts, dim3(blocks_for(r)), dim3(kBlock), 0, st, idx, n, r, counts);
@@ -911,29 +902,29 @@ hipError_t launch_iota(int *p, int n, hipStream_t st) {
     return hipGetLastError();
 }
 
-size_t rank_order_temp_bytes(int n) {
-    size_t bytes = 0;
-    if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (const uint16_t *) nullptr, (uint16_t *) nullptr,
-                                           (const int *) nullptr, (int *) nullptr, n, 0, 16) != hipSuccess)
-        return 1u << 20;
-    return bytes;
-}
+size_t rank_order_temp_bytes(int n) { return sort_temp_bytes(n); }
 
-// Least-significant dimension first, one stable radix pass per dimension,
-// starting from ascending ids: the result is sorted by the cells in `level`
-// order with ties by id.
-hipError_t launch_rank_order(const uint16_t *cellE, int n, int d, const int *level, int *perm, int *tmp,
+// Least-significant dimension first, one stable radix sort per dimension (one
+// or two 8-bit digit passes for cells below ecells), starting from ascending
+// ids: the result is sorted by the cells in `level` order with ties by id.
+hipError_t launch_rank_order(const uint16_t *cellE, int n, int d, int ecells, const int *level, int *perm, int *tmp,
                              uint16_t *keys, uint16_t *keys2, void *temp, size_t temp_bytes, hipStream_t st) {
     LAUNCH(k_iota, dim3(blocks_for(n)), dim3(kBlock), 0, st, perm, n);
+    int bits = 1;
+    while (bits < 16 && (1 << bits) < ecells) ++bits;
+    const bool two = bits > 8;
     int *a = perm, *b = tmp;
     for (int t = d - 1; t >= 0; --t) {
         LAUNCH(k_rank_keys, dim3(blocks_for(n)), dim3(kBlock), 0, st, cellE, n, level[t], a, keys);
-        size_t tb = temp_bytes;
-        hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, tb, keys, keys2, a, b, n, 0, 16, st);
+        // two passes: keys -> keys2 -> keys and a -> b -> a; one pass: keys -> keys2 and a -> b
+        hipError_t e = launch_sort_pairs_u16(keys, keys2, two ? keys : keys2, a, b, two ? a : b, n, bits, temp,
+                                             temp_bytes, st);
         if (e != hipSuccess) return e;
-        int *x = a;
-        a = b;
-        b = x;
+        if (!two) {
+            int *x = a;
+            a = b;
+            b = x;
+        }
     }
     if (a != perm) {
         hipError_t e = hipMemcpyAsync(perm, a, (size_t) n * sizeof(int), hipMemcpyDeviceToDevice, st);

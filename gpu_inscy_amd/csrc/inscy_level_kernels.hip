@@ -13,7 +13,6 @@
 #include <algorithm>
 #include <climits>
 
-#include <hipcub/hipcub.hpp>
 
 #include "inscy_device.h"
 #include "inscy_internal.h"
@@ -224,31 +223,41 @@ __global__ void k_b_qstart(const int *__restrict__ qlist, const int *__restrict_
 // holds the query, so it is not empty)
 __device__ __forceinline__ void build_runs(const GridArg &G, const int *__restrict__ off, const int *qe, int stride,
                                            int base0, int *q) {
+    // rows r = 3 xi + yi over the cells within one of the query's in the first
+    // (and second) index dimension, each a run of the last one's cells within
+    // one; every row's two offsets are loaded before any is used, so the
+    // set-up costs one memory round trip
     const int C = G.C;
     const int c0 = qe[G.tI[0] * stride];
-    int ca, ce;
-    if (G.j == 1) {
-        ca = off[max(c0 - 1, 0)];
-        ce = off[min(c0 + 1, C - 1) + 1];
-    } else {
-        const int cl = qe[G.tI[G.j - 1] * stride];
-        const int lo = max(cl - 1, 0), hi = min(cl + 1, C - 1);
-        const int row = G.j == 2 ? c0 * C : (c0 * C + qe[G.tI[1] * stride]) * C;
-        ca = off[row + lo];
-        ce = off[row + hi + 1];
+    const int c1 = G.j == 3 ? qe[G.tI[1] * stride] : 0;
+    const int cl = qe[G.tI[G.j - 1] * stride];
+    const int lo = max(cl - 1, 0), hi = min(cl + 1, C - 1);
+    const int x0 = max(c0 - 1, 0), nx = G.j == 1 ? 1 : min(c0 + 1, C - 1) - x0 + 1;
+    const int y0 = max(c1 - 1, 0), ny = G.j == 3 ? min(c1 + 1, C - 1) - y0 + 1 : 1;
+    const int own = G.j == 1 ? 0 : 3 * (c0 - x0) + (G.j == 3 ? c1 - y0 : 0);
+    int a[9], e[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        // rows past the edge of the grid load a row inside it (no branch
+        // around the loads) and are dropped below
+        const int xi = min(r / 3, nx - 1), yi = min(r % 3, ny - 1);
+        const int row = G.j == 1 ? 0 : G.j == 2 ? (x0 + xi) * C : ((x0 + xi) * C + y0 + yi) * C;
+        a[r] = off[row + lo];
+        e[r] = off[row + hi + 1];
     }
     int nr = 0, acc = 0;
     q[9] = 0;
     q[20] = 0;
-    for_ranges(G, off, qe, stride, [&](int a, int e) {
-        if (e > a) {
-            if (a == ca && e == ce) q[20] = nr;
-            q[nr] = a - base0;
-            acc += e - a;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        if (r / 3 < nx && r % 3 < ny && e[r] > a[r]) {
+            if (r == own) q[20] = nr;
+            q[nr] = a[r] - base0;
+            acc += e[r] - a[r];
             q[10 + nr] = acc;
             ++nr;
         }
-    });
+    }
     q[19] = nr;
 }
 
@@ -325,10 +334,32 @@ __device__ __forceinline__ void lds_layout(char *smem, int k, float *&sy, int *&
     qr = reinterpret_cast<int *>(smem + 8 * (size_t) QPB * k + 8 * QPB);
 }
 
-// the query's coordinates (sorted position i) and cells (point p) into LDS
+// the query's coordinates (sorted position i) and cells (point p) into LDS:
+// the first 8 dimensions' loads are all issued before any store (a store to
+// LDS through a generic pointer would otherwise order each load after the
+// previous store, one memory round trip per dimension)
 __device__ __forceinline__ void stage_query_b(const BatchSub &B, int n, const float *__restrict__ Ys, int p, int i,
                                               float *sy, int *se, int stride) {
-    for (int t = 0; t < B.S.k; ++t) {
+    constexpr int kStage = 8;
+    const int k = B.S.k;
+    float y[kStage];
+    int c[kStage];
+#pragma unroll
+    for (int t = 0; t < kStage; ++t) {
+        y[t] = 0.0f;
+        c[t] = 0;
+        if (t < k) {
+            y[t] = Ys[col_of(B, t, n) + i];
+            c[t] = B.cellE[(size_t) B.S.dims[t] * n + p];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kStage; ++t)
+        if (t < k) {
+            sy[t * stride] = y[t];
+            se[t * stride] = c[t];
+        }
+    for (int t = kStage; t < k; ++t) {
         sy[t * stride] = Ys[col_of(B, t, n) + i];
         se[t * stride] = B.cellE[(size_t) B.S.dims[t] * n + p];
     }
@@ -686,8 +717,9 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
     const int base0 = (int) B.obase;
     int *q = qr + lq * kRangeWords;
     if (sub == 0 && active) {
+        const uint64_t qsg = sigb[i];
         stage_query_b(B, n, Ys, p, i, sy + lq, se + lq, QPB);
-        qs[lq] = sigb[i];
+        qs[lq] = qsg;
         build_runs(B.G, off + B.hoff, se + lq, QPB, base0, q);
     }
     __syncthreads();
@@ -865,8 +897,9 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
     if (active && pr[p] == giant[blockIdx.y]) active = false;
     int *q = qr + lq * kRangeWords;
     if (sub == 0 && active) {
+        const uint64_t qsg = sigb[i];
         stage_query_b(B, n, Ys, p, i, sy + lq, se + lq, QPB);
-        qs[lq] = sigb[i];
+        qs[lq] = qsg;
         build_runs(B.G, off + B.hoff, se + lq, QPB, (int) B.obase, q);
     }
     __syncthreads();
@@ -1579,8 +1612,7 @@ hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, cons
     if (nmiss > 0)
         LAUNCH(k_b_keys, dim3(blocks_for(D.n), nmiss), dim3(kBlock), 0, st, D, subs, miss_reps, X.keys, X.rank,
                X.hist);
-    size_t tb = X.scan_bytes;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(X.scan_temp, tb, X.hist, X.off, nscan, st);
+    hipError_t e = launch_exclusive_sum(X.scan_temp, X.scan_bytes, X.hist, X.off, nscan, st);
     if (e != hipSuccess) return e;
     if (nmiss > 0)
         LAUNCH(k_b_scatter, dim3(blocks_for(D.n), nmiss), dim3(kBlock), 0, st, D, subs, miss_reps, X.keys, X.rank,
@@ -1695,9 +1727,8 @@ hipError_t launch_flag_compact(const uint8_t *flags, size_t nb, void *temp, size
     char *base = static_cast<char *>(temp);
     int *tcnt = reinterpret_cast<int *>(base);
     int *toff = reinterpret_cast<int *>(base + cnt_bytes);
-    size_t tb = temp_bytes - 2 * cnt_bytes;
     LAUNCH(k_b_flag_count, dim3(nt), dim3(kBlock), 0, st, flags, nb, nt, tcnt);
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(base + 2 * cnt_bytes, tb, tcnt, toff, nt + 1, st);
+    hipError_t e = launch_exclusive_sum(base + 2 * cnt_bytes, temp_bytes - 2 * cnt_bytes, tcnt, toff, nt + 1, st);
     if (e != hipSuccess) return e;
     LAUNCH(k_b_flag_write, dim3(nt), dim3(kBlock), 0, st, flags, nb, toff, nt, out, nsel);
     return hipGetLastError();

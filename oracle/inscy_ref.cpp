@@ -1243,6 +1243,21 @@ int oracle_neighbours(const float *X, int n, int d, float eps, const int *S, int
     return (int) nb.size();
 }
 
+// φ_S(p) (ClusteringCpu.cu:116-124) of the points ps[0..np) over their
+// ε-neighbours in the neighbourhood tree's leaf order (literal: by walking the
+// tree; otherwise the index), as the density test of the clustering uses it.
+void oracle_phi_many(const float *X, int n, int d, float eps, const int *S, int k, const int *ps, int np,
+                     int literal, float *out) {
+    Data D{X, n, d};
+    std::vector<int> order(d);
+    for (int i = 0; i < d; ++i) order[i] = i;
+    int ecells = (int) std::ceil(1.0 / (double) eps);
+    Tree etree = build_tree(D, order, ecells, eps);
+    NeighbourIndex nbr(D, etree, order, literal != 0);
+    std::vector<int> SS(S, S + k);
+    for (int i = 0; i < np; ++i) out[i] = phi(D, ps[i], nbr.neighbours(ps[i], SS, eps), eps, SS);
+}
+
 // Dimension order used by the SCY-tree levels (identity unless entropy_order != 0).
 void oracle_dimension_order(const float *X, int n, int d, int cells, int entropy_order, int *out) {
     Data D{X, n, d};

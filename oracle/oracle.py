@@ -129,6 +129,22 @@ def neighbours(X, eps, S, p, literal=False):
     return out[:m].copy()
 
 
+def phi_many(X, eps, S, ps, literal=False):
+    """φ_S(p) of the points ps over their ε-neighbours in the neighbourhood
+    tree's leaf order (ClusteringCpu.cu:116-124)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, d = X.shape
+    S = np.ascontiguousarray(S, dtype=np.int32)
+    ps = np.ascontiguousarray(ps, dtype=np.int32)
+    out = np.zeros(len(ps), dtype=np.float32)
+    L = lib()
+    L.oracle_phi_many.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                  ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                  ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+    L.oracle_phi_many(_fp(X), n, d, eps, _ip(S), len(S), _ip(ps), len(ps), int(literal), _fp(out))
+    return out
+
+
 def dimension_order(X, cells, entropy_order):
     X = np.ascontiguousarray(X, dtype=np.float32)
     n, d = X.shape

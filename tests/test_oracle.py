@@ -31,10 +31,103 @@ def ball_closed_form(k):
     return 3.14 ** ((k - 1) // 2) * 2 ** ((k + 1) // 2) / double_fact
 
 
+# The reference's rounding sequence (ClusteringCpu.cu:36-38,127-164), written
+# from that source with explicit float32/float64 steps: gamma() returns a float
+# of a double product, c() divides two floats, alpha()/expDen() round double
+# products of C pow() to float, omega() rounds a double quotient.
+def ref_gamma(m):
+    if m == 2:
+        return np.float32(1.0)
+    if m == 1:
+        return np.float32(math.sqrt(3.14))
+    return np.float32((m / 2.0 - 1.0) * float(ref_gamma(m - 2)))
+
+
+def ref_c(k):
+    return np.float32(np.float32(math.pow(3.14, k / 2.0)) / ref_gamma(k + 2))
+
+
+def ref_alpha(k, eps, n):
+    e = float(np.float32(eps))
+    r = np.float32(float(2 * n) * math.pow(e, k) * float(ref_c(k)))
+    return np.float32(float(r) / (math.pow(1.0, k) * (k + 2)))
+
+
+def ref_expden(k, eps, n):
+    nc = np.float32(np.float32(n) * ref_c(k))
+    r = np.float32(float(nc) * math.pow(float(np.float32(eps)), k))
+    return np.float32(float(r) / math.pow(1.0, k))
+
+
+def ref_omega(k):
+    return np.float32(2.0 / (k + 2.0))
+
+
+def exact_ball(k):
+    """c(k) = 3.14^(k/2) / Gamma(k/2 + 1) with Gamma(1/2) = sqrt(3.14), to 50 digits."""
+    from decimal import Decimal, getcontext
+    getcontext().prec = 50
+    pi = Decimal("3.14")
+    if k % 2 == 0:
+        return float(pi ** (k // 2) / math.factorial(k // 2))
+    # odd k: sqrt(pi) cancels; c(k) = pi^((k-1)/2) 2^((k+1)/2) / k!!
+    return float(pi ** ((k - 1) // 2) * 2 ** ((k + 1) // 2) / math.prod(range(k, 0, -2)))
+
+
+def ulps(a, b):
+    a, b = np.float32(a), np.float32(b)
+    return abs(int(a.view(np.int32)) - int(b.view(np.int32)))
+
+
+THRESHOLD_CASES = [(k, eps, n) for k in range(0, 33) for eps, n in
+                   [(0.01, 1500), (0.015, 25000), (0.02, 100000), (0.02, 200000), (0.003, 7), (0.25, (1 << 30) - 1)]]  # n <= 2^30 - 1: 2 n fits an int
+
+
 @pytest.mark.parametrize("k", list(range(1, 33)))
-def test_unit_ball_volume_closed_form(k):
-    got = O.thresholds(k, 0.01, 1000)["c"]
-    assert got == pytest.approx(ball_closed_form(k), rel=2e-6)
+def test_unit_ball_volume_bit_exact_and_within_the_closed_form(k):
+    got = np.float32(O.thresholds(k, 0.01, 1000)["c"])
+    # bit-exact: the reference's float/double sequence
+    assert got.view(np.int32) == ref_c(k).view(np.int32)
+    # and within the rounding of that sequence of the exact value: one
+    # rounding per gamma step, the pow, the division
+    assert ulps(got, exact_ball(k)) <= 3 + k // 2
+
+
+def test_thresholds_bit_exact_for_every_k_eps_n():
+    for k, eps, n in THRESHOLD_CASES:
+        t = O.thresholds(k, eps, n)
+        for name, want in (("c", ref_c(k)), ("alpha", ref_alpha(k, eps, n)), ("expden", ref_expden(k, eps, n)),
+                           ("omega", ref_omega(k))):
+            got = np.float32(t[name])
+            assert got.view(np.int32) == want.view(np.int32), (name, k, eps, n, got, want)
+
+
+def test_threshold_check_catches_a_wrong_float_double_order():
+    # the bit-exact check is discriminating: computing c(k) wholly in double,
+    # or Gamma in float arithmetic, changes the float result for some k
+    def c_double(k):
+        return np.float32(math.pow(3.14, k / 2.0) / float(math.gamma(k / 2.0 + 1) / math.sqrt(math.pi)
+                                                           * (math.sqrt(3.14) if k % 2 else 1.0)))
+
+    # (Gamma's float recursion itself is no such mistake: each product of a
+    # half-integer and a float is exact in double, so it rounds once either way)
+    def gamma_true_pi(m):
+        if m == 2:
+            return np.float32(1.0)
+        if m == 1:
+            return np.float32(math.sqrt(math.pi))
+        return np.float32((m / 2.0 - 1.0) * float(gamma_true_pi(m - 2)))
+
+    def c_true_pi_gamma(k):
+        return np.float32(np.float32(math.pow(3.14, k / 2.0)) / gamma_true_pi(k + 2))
+
+    assert any(c_double(k) != ref_c(k) for k in range(1, 33))
+    assert any(c_true_pi_gamma(k) != ref_c(k) for k in range(1, 33))
+
+    def expden_all_float(k, eps, n):
+        return np.float32(np.float32(n) * ref_c(k) * np.float32(eps) ** np.float32(k))
+
+    assert any(expden_all_float(k, e, n) != ref_expden(k, e, n) for k, e, n in THRESHOLD_CASES if k > 0)
 
 
 def test_unit_ball_uses_pi_314_not_math_pi():
@@ -47,13 +140,28 @@ def test_unit_ball_uses_pi_314_not_math_pi():
 
 @pytest.mark.parametrize("k,eps,n", [(1, 0.01, 1500), (2, 0.015, 25000), (5, 0.02, 100000), (15, 0.015, 25000),
                                      (20, 0.02, 100000)])
-def test_density_thresholds_closed_form(k, eps, n):
+def test_density_thresholds_within_rounding_of_the_closed_form(k, eps, n):
+    # expDen = n c(k) eps^k and alpha = 2 n eps^k c(k) / (k + 2), eps taken as
+    # its float32 value; a few float roundings beyond those of c(k)
     t = O.thresholds(k, eps, n)
-    c = ball_closed_form(k)
     e32 = float(np.float32(eps))
-    assert t["expden"] == pytest.approx(n * c * e32 ** k, rel=1e-5)
-    assert t["alpha"] == pytest.approx(2 * n * e32 ** k * c / (k + 2), rel=1e-5)
-    assert t["omega"] == pytest.approx(2.0 / (k + 2), rel=1e-7)
+    assert ulps(t["expden"], n * exact_ball(k) * e32 ** k) <= 6 + k // 2
+    assert ulps(t["alpha"], 2 * n * e32 ** k * exact_ball(k) / (k + 2)) <= 6 + k // 2
+    assert t["omega"] == np.float32(2.0 / (k + 2))
+
+
+def test_engine_thresholds_equal_the_oracle_bit_for_bit():
+    # the engine's host copies (no device call)
+    import ctypes
+    from gpu_inscy_amd import _lib
+    L = _lib.lib()
+    out = (ctypes.c_float * 4)()
+    for k, eps, n in THRESHOLD_CASES:
+        assert L.inscy_debug_thresholds(k, eps, n, out) == 0
+        t = O.thresholds(k, eps, n)
+        got = np.array(out[:], dtype=np.float32)
+        want = np.array([t["c"], t["alpha"], t["expden"], t["omega"]], dtype=np.float32)
+        assert np.array_equal(got.view(np.int32), want.view(np.int32)), (k, eps, n, got, want)
 
 
 # ------------------------------------------------------------------ neighbourhood
@@ -240,6 +348,108 @@ def test_literal_walk_oracle_matches_index_oracle_on_config0():
     a = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0, literal=False, trace=True)
     b = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0, literal=True, trace=True)
     assert O.same_result(a, b)[0] and np.array_equal(a.trace, b.trace)
+
+
+# ------------------------------------------------------------------ phi density
+def test_phi_hand_computed_on_dyadic_points():
+    # eps = 0.5 (two eps-cells, all adjacent); every distance, ratio and square
+    # below is a short dyadic fraction, so phi is exact:
+    #   p = 0 in S = {0}: q at 0.25 -> r 0.5 -> 0.75; q at 0.125 -> 0.9375;
+    #   q at 0.5 -> r 1 -> 0; q at 0.875 is beyond eps        => 1.6875
+    #   p = 0 in S = {0, 1}: q = (0.1875, 0.25) at 0.3125 -> r 0.625 -> 0.609375;
+    #   q = (0.125, 0) -> 0.9375; q = (0.25, 0.5) at 0.5590.. -> out   => 1.546875
+    X = np.array([[0.0, 0.0], [0.25, 0.75], [0.125, 0.0], [0.5, 0.875], [0.875, 0.0], [0.1875, 0.25],
+                  [0.25, 0.5]], dtype=np.float32)
+    # S = {0}: neighbours of 0 are 1 (0.25), 2 (0.125), 3 (0.5), 5 (0.1875), 6 (0.25)
+    want0 = 0.75 + 0.9375 + 0.0 + (1 - 0.375 ** 2) + 0.75
+    assert O.phi_many(X, 0.5, [0], [0])[0] == np.float32(want0)
+    assert O.phi_many(X, 0.5, [0, 1], [0])[0] == np.float32(0.609375 + 0.9375)
+    for lit in (False, True):
+        assert O.phi_many(X, 0.5, [1], [4], literal=lit)[0] == np.float32(1.0 + 1.0 + 0.75)  # 0, 2 at 0; 5 at 0.25
+
+
+def ecells_of(X, eps):
+    ec = int(math.ceil(1.0 / float(np.float32(eps))))
+    ecs = np.float32(1.0) / np.float32(ec)
+    return np.minimum((X / ecs).astype(np.int64), ec - 1)
+
+
+def phi_float32_leaf_order(X, eps, S, p):
+    """ClusteringCpu.cu:116-124 in numpy: terms 1 - (dist/eps)^2 with the
+    distance, ratio and square in float, added to a float sum through double,
+    over the neighbours in the neighbourhood tree's leaf order: ascending
+    eps-cells over all dimensions (level 0 first), then insertion (id) order."""
+    X = np.asarray(X, dtype=np.float32)
+    nb = sorted(brute_neighbours(X, eps, S, p))
+    cells = ecells_of(X, eps)
+    nb.sort(key=lambda q: (tuple(cells[q].tolist()), q))
+    e = np.float32(eps)
+    acc = np.float32(0.0)
+    for q in nb:
+        s2 = np.float32(0.0)
+        for k in S:
+            diff = np.float32(X[p, k] - X[q, k])
+            s2 = np.float32(s2 + np.float32(diff * diff))
+        r = np.float32(np.float32(np.sqrt(s2)) / e)
+        sq = np.float32(r * r)
+        acc = np.float32(float(acc) + (1.0 - float(sq)))
+    return acc, nb
+
+
+@pytest.mark.parametrize("seed,S,eps", [(0, [0], 0.05), (1, [0, 2], 0.08), (2, [1, 2, 3], 0.12), (3, [3], 0.03)])
+def test_phi_bit_exact_against_float32_leaf_order_emulation(seed, S, eps):
+    w = datagen.small(seed, n=500, d=4, n_clusters=2, eps=eps, sigma=0.03)
+    X = w.X
+    ps = np.arange(0, len(X), 7, dtype=np.int32)
+    got = O.phi_many(X, eps, S, ps)
+    lit = O.phi_many(X, eps, S, ps[:20], literal=True)
+    assert np.array_equal(lit.view(np.int32), got[:20].view(np.int32))
+    order_matters = 0
+    for p, g in zip(ps, got):
+        want, nb = phi_float32_leaf_order(X, eps, S, int(p))
+        assert np.float32(g).view(np.int32) == want.view(np.int32), (p, g, want)
+        # the same terms summed in reverse give another float for some points:
+        # the check pins the order, not only the terms
+        rev = np.float32(0.0)
+        for q in reversed(nb):
+            d2 = np.float32(0.0)
+            for k in S:
+                diff = np.float32(X[p, k] - X[q, k])
+                d2 = np.float32(d2 + np.float32(diff * diff))
+            r = np.float32(np.float32(np.sqrt(d2)) / np.float32(eps))
+            rev = np.float32(float(rev) + (1.0 - float(np.float32(r * r))))
+        order_matters += int(rev != want)
+    assert order_matters > 0 or len(S) == 0
+
+
+@pytest.mark.parametrize("seed,S,eps", [(4, [0, 1], 0.1), (5, [2], 0.04), (6, [0, 1, 2, 3], 0.2)])
+def test_phi_within_the_double_sum_band(seed, S, eps):
+    # float64 brute force: the float phi is within a relative N 2^-22 of the
+    # double sum of the same float terms in any order (the band the engine's
+    # double-sum scan relies on to decide thresholds), and within N 2^-20 of
+    # the phi of float64 distances (a dropped term or a wrong sign is not)
+    w = datagen.small(seed, n=800, d=4, n_clusters=3, eps=eps, sigma=0.04)
+    X = w.X
+    ps = np.arange(0, len(X), 5, dtype=np.int32)
+    got = O.phi_many(X, eps, S, ps).astype(np.float64)
+    X64 = X.astype(np.float64)
+    for p, g in zip(ps, got):
+        nb = np.array(sorted(brute_neighbours(X, eps, S, int(p))), dtype=np.int64)
+        N = len(nb)
+        if N == 0:
+            assert g == 0.0
+            continue
+        diff = (X[p, S][None, :] - X[nb][:, S]).astype(np.float32)
+        s2 = np.zeros(N, dtype=np.float32)
+        for t in range(len(S)):
+            s2 = (s2 + (diff[:, t] * diff[:, t]).astype(np.float32)).astype(np.float32)
+        r = (np.sqrt(s2).astype(np.float32) / np.float32(eps)).astype(np.float32)
+        terms = 1.0 - (r * r).astype(np.float32).astype(np.float64)
+        dsum = float(np.sum(terms[np.random.default_rng(int(p)).permutation(N)]))
+        assert abs(g - dsum) <= N * 2.0 ** -22 * dsum + 1e-12
+        d64 = np.sqrt(((X64[p, S][None, :] - X64[nb][:, S]) ** 2).sum(axis=1)) / float(np.float32(eps))
+        phi64 = float(np.sum(1.0 - d64 ** 2))
+        assert abs(g - phi64) <= N * 2.0 ** -20 * max(phi64, 1.0)
 
 
 def test_non_rectangular_mode_oracles_agree_on_config0_shape():
