@@ -33,7 +33,7 @@ EXPORTED = [
     "inscy_result_labels_all", "inscy_run_many", "inscy_run_many_device", "inscy_release_memory",
     "inscy_run", "inscy_run_device", "inscy_result_num_subspaces", "inscy_result_subspace",
     "inscy_result_labels", "inscy_result_num_points", "inscy_result_stats", "inscy_result_free",
-    "inscy_last_error", "inscy_discretise", "inscy_neighbour_counts",
+    "inscy_last_error", "inscy_discretise", "inscy_neighbour_counts", "inscy_neighbour_counts_noalloc",
     "inscy_dense_components", "inscy_profile",
     "inscy_set_pruner", "inscy_result_pairs", "inscy_mlp_forward", "inscy_mlp_gradient", "inscy_mlp_adam",
     "inscy_mlp_round",
@@ -130,6 +130,9 @@ def lib():
     L.inscy_neighbour_counts.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_float, i32p, ctypes.c_int32,
                                          vp, ctypes.c_int64, vp]
     L.inscy_neighbour_counts.restype = ctypes.c_int
+    L.inscy_neighbour_counts_noalloc.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_float, i32p,
+                                                 ctypes.c_int32, vp, ctypes.c_int64, vp, vp]
+    L.inscy_neighbour_counts_noalloc.restype = ctypes.c_int
     L.inscy_dense_components.argtypes = [vp, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(Params), i32p,
                                          ctypes.c_int32, vp]
     L.inscy_dense_components.restype = ctypes.c_int

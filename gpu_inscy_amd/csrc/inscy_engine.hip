@@ -539,6 +539,32 @@ int inscy_neighbour_counts(const float *d_X, int64_t n, int32_t d, float eps, co
     }
 }
 
+int inscy_neighbour_counts_noalloc(const float *d_X, int64_t n, int32_t d, float eps, const int32_t *dims, int32_t k,
+                                   const int32_t *d_queries, int64_t nq, int32_t *d_out, void *stream) {
+    if (!d_X || !dims || k < 1 || k > d || (nq > 0 && (!d_queries || !d_out)) || nq < 0 || nq > INT_MAX || n < 1 ||
+        n > (int64_t) INT_MAX / 2 || d < 1 || d > kMaxDims || !(eps > 0.0f) || !std::isfinite(eps))
+        return fail(INSCY_ERR_ARGUMENT, "invalid argument");
+    const double ec = std::ceil(1.0 / (double) eps);
+    if (ec > 65535.0) return fail(INSCY_ERR_UNSUPPORTED, "neighborhood_size below 1/65535 is not supported");
+    try {
+        (void) mask_from(dims, k, d);
+        const size_t need = noalloc_scratch_need((int) n, k, (int) ec);
+        if (need > kNoAllocScratchBytes)
+            return fail(INSCY_ERR_ARGUMENT, "n and k need " + std::to_string(need) + " bytes of scratch; the allocation-free "
+                        "entry point holds " + std::to_string(kNoAllocScratchBytes) + " (use inscy_neighbour_counts)");
+        SubspaceArg S{};
+        S.k = k;
+        for (int t = 0; t < k; ++t) S.dims[t] = dims[t];
+        unsigned char *base = nullptr;
+        HIP_OK(noalloc_scratch(&base));
+        HIP_OK(launch_neighbour_counts_noalloc(base, d_X, (int) n, d, eps, sqrt_threshold(eps), S, d_queries, (int) nq,
+                                               d_out, (hipStream_t) stream));
+        return INSCY_OK;
+    } catch (...) {
+        return fail_current();
+    }
+}
+
 int inscy_dense_components(const float *d_X, int64_t n, int32_t d, const inscy_params *p, const int32_t *dims,
                            int32_t k, int32_t *d_comp) {
     if (!d_X || !p || !dims || k < 1 || k > d || !d_comp || n < 1 || d < 1 || d > kMaxDims ||

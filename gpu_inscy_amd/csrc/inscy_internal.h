@@ -416,4 +416,17 @@ constexpr int kBlockThreads = 256;   // threads per block of every pass
 constexpr int kQueriesPerBlock = 64;  // queries per block of the batched neighbour passes
 constexpr int kSampleEdges = 2;       // neighbours recorded per point by the count walk (Afforest's sample rounds)
 
+// ------------------------------------------------------------ allocation-free neighbour counts
+// One file-scope device array per device (inscy_noalloc.hip) holds every
+// buffer of the pass: bytes needed for n points, k dimensions and ecells
+// cells per dimension; the array's address on the current device (looked up
+// once per device); the pass itself on `st` (counts of the query ids,
+// -1 for an id outside [0, n)).
+constexpr size_t kNoAllocScratchBytes = (size_t) 64 << 20;
+size_t noalloc_scratch_need(int n, int k, int ecells);
+hipError_t noalloc_scratch(unsigned char **out);
+hipError_t launch_neighbour_counts_noalloc(unsigned char *base, const float *X, int n, int d, float eps,
+                                           float s_star, const SubspaceArg &S, const int *queries, int nq, int *out,
+                                           hipStream_t st);
+
 }  // namespace inscy

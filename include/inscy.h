@@ -221,6 +221,22 @@ int inscy_discretise(const float *d_X, int64_t n, int32_t d, int32_t cells, floa
 int inscy_neighbour_counts(const float *d_X, int64_t n, int32_t d, float eps, const int32_t *dims, int32_t k,
                            const int32_t *d_queries, int64_t nq, int32_t *d_out);
 
+/* inscy_neighbour_counts without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same counts, for k ascending dimension ids; out[i] = -1 for
+ * a query id outside [0, n).  Every buffer of the pass lives in one 64 MiB
+ * device array per device, held by the library until the process ends; inputs
+ * whose pass needs more (about (12 k + 16) n bytes plus 8 bytes per grid cell)
+ * are refused with INSCY_ERR_ARGUMENT before any device call.  The work is
+ * queued on `stream` (a hipStream_t, NULL for the null stream) and the
+ * function returns without waiting: d_out holds the counts once the stream's
+ * work is done.  X values outside [0, 1] are clamped, not rejected.  Calls on
+ * one device share the array: calls that overlap on one device (different
+ * streams or threads) race and must be serialised by the caller. */
+int inscy_neighbour_counts_noalloc(const float *d_X, int64_t n, int32_t d, float eps, const int32_t *dims, int32_t k,
+                                   const int32_t *d_queries, int64_t nq, int32_t *d_out, void *stream);
+
 /* Dense components of subspace S over all points (reference:
  * ClusteringCpu.cu:317-389 BFS expansion): comp[p] = smallest point id of p's
  * component in the dense-dense epsilon graph, or -1 when p is not dense with

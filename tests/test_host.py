@@ -85,6 +85,23 @@ def test_run_many_rejects_bad_arguments_before_any_device_call():
     assert L.inscy_run_many(ptrs, n, d, ent, 1, 2, out) == _lib.INSCY_ERR_UNSUPPORTED
 
 
+def test_noalloc_counts_refuse_what_the_scratch_cannot_hold_before_any_device_call():
+    L = _lib.lib()
+    dims = (ctypes.c_int32 * 8)(*range(8))
+    X = np.zeros((4, 8), dtype=np.float32)
+    # (12 k + 16) n bytes at k = 8, n = 2^24: far beyond the 64 MiB array
+    rc = L.inscy_neighbour_counts_noalloc(X.ctypes.data, 1 << 24, 8, 0.01, dims, 8, X.ctypes.data, 1, X.ctypes.data,
+                                          None)
+    assert rc == _lib.INSCY_ERR_ARGUMENT and b"scratch" in L.inscy_last_error()
+    assert L.inscy_neighbour_counts_noalloc(X.ctypes.data, 4, 8, 0.0, dims, 8, X.ctypes.data, 1, X.ctypes.data,
+                                            None) == _lib.INSCY_ERR_ARGUMENT
+    assert L.inscy_neighbour_counts_noalloc(X.ctypes.data, 4, 8, 0.01, dims, 9, X.ctypes.data, 1, X.ctypes.data,
+                                            None) == _lib.INSCY_ERR_ARGUMENT
+    bad = (ctypes.c_int32 * 2)(3, 1)  # not ascending
+    assert L.inscy_neighbour_counts_noalloc(X.ctypes.data, 4, 8, 0.01, bad, 2, X.ctypes.data, 1, X.ctypes.data,
+                                            None) == _lib.INSCY_ERR_ARGUMENT
+
+
 def test_result_accessors_tolerate_null():
     L = _lib.lib()
     assert L.inscy_result_num_subspaces(None) == 0
