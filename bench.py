@@ -71,6 +71,8 @@ def parse():
                     help="one of the reference's real-data workloads instead of --config")
     ap.add_argument("--threshold", type=float, default=None,
                     help="learned mode: pruner threshold instead of the trained network's own")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer 3 times (calibrates the HBM byte counters)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
@@ -131,6 +133,17 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe > 0:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of its dispatches
+        # give the scale from the counters to HBM bytes
+        nb = args.copy_probe << 20
+        a = torch.ones(nb // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": nb, "copies": 3}), flush=True)
+        return
     if args.mode == "learned":
         return bench_learned(args)
     mode = _lib.MODE_EXACT if args.mode == "exact" else _lib.MODE_BATCHED

@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libinscy_hip.so")
+# INSCY_LIBRARY: another build of the same library (same C ABI) to load instead
+LIB_PATH = os.environ.get("INSCY_LIBRARY") or os.path.join(HERE, "libinscy_hip.so")
 
 INSCY_OK = 0
 INSCY_ERR_ARGUMENT = 1

@@ -40,16 +40,17 @@ def flags() -> list:
     ]
 
 
-def build(verbose: bool = False) -> str:
-    """Compiles every HIP source and links the shared library; returns its path."""
+def build(verbose: bool = False, csrc: str = CSRC, out: str = LIB) -> str:
+    """Compiles every HIP source (from csrc) and links the shared library
+    (out); returns its path."""
     cc = hipcc()
     objs = []
-    build_dir = os.path.join(HERE, "_build")
+    build_dir = os.path.join(os.path.dirname(out), "_build")
     os.makedirs(build_dir, exist_ok=True)
     procs = []
     for src in SOURCES:  # the translation units compile concurrently
         obj = os.path.join(build_dir, src.replace(".hip", ".o"))
-        cmd = [cc, *flags(), "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc, *flags(), "-c", os.path.join(csrc, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((cmd, subprocess.Popen(cmd)))
@@ -57,13 +58,13 @@ def build(verbose: bool = False) -> str:
     for cmd, proc in procs:
         if proc.wait() != 0:
             raise subprocess.CalledProcessError(proc.returncode, cmd)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

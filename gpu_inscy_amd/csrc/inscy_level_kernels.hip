@@ -450,7 +450,9 @@ __device__ __forceinline__ bool walk_more(const QueryWalk &W, bool phi, int foun
 // is masked), so one memory round trip serves a step, and the full sum
 // compared once equals the early-exit test (the partial sums never
 // decrease).  The run bounds stay in registers while the lanes move through
-// a run.
+// a run.  The rotated candidate order is walked as two segments, [rot,
+// total) from the query's own run, then [0, rot), so no candidate needs a
+// wrap-around test.
 template <int K, bool PHI>
 __device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double &dsum, bool &more, long long &seen) {
     constexpr int LPQ = kBLanes, U = 4, STEP = LPQ * U;
@@ -467,77 +469,73 @@ __device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double
         qv[t] = W.qy[t * QPB];
     }
     const bool packed = B.nchk4 > 0;
-    int r = 0, rend = q[10], roff = q[0] - q[9];
-    bool wrapped = false;
-    for (int c0 = 0; c0 < total && more; c0 += STEP) {
-        int jj[U];
-        bool valid[U];
+    for (int segm = 0; segm < 2; ++segm) {
+        // segment 0 starts in the own run (which holds rot), segment 1 in run 0
+        const int cbeg = segm == 0 ? rot : 0, cend = segm == 0 ? total : rot;
+        int r = segm == 0 ? q[20] : 0;
+        int rend = q[10 + r], roff = q[r] - q[9 + r];
+        for (int c0 = cbeg; c0 < cend && more; c0 += STEP) {
+            int jj[U];
+            bool valid[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int c = c0 + u * LPQ + sub;
-            valid[u] = c < total;
-            int cr = min(c, total - 1) + rot;
-            if (cr >= total) {
-                cr -= total;
-                if (!wrapped) {
-                    wrapped = true;
-                    r = 0;
-                    rend = q[10];
-                    roff = q[0] - q[9];
+            for (int u = 0; u < U; ++u) {
+                const int c = c0 + u * LPQ + sub;
+                valid[u] = c < cend;
+                const int cr = min(c, cend - 1);
+                while (cr >= rend) {
+                    ++r;
+                    rend = q[10 + r];
+                    roff = q[r] - q[9 + r];
                 }
+                jj[u] = cr + roff;
             }
-            while (cr >= rend) {
-                ++r;
-                rend = q[10 + r];
-                roff = q[r] - q[9 + r];
+            uint64_t w[U];
+            float y[U][K];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                // 32-bit byte offsets into the float columns (a subspace's columns
+                // hold n < 2^30 entries, so 4 jj < 2^32); the 8-byte packed cells
+                // are indexed with the 64-bit position (8 jj reaches 2^33)
+                const uint32_t at4 = (uint32_t) jj[u] << 2;
+                w[u] = packed ? W.sigb[(uint32_t) jj[u]] : 0ull;
+#pragma unroll
+                for (int t = 0; t < K; ++t)
+                    y[u][t] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(col[t]) + at4);
             }
-            jj[u] = cr + roff;
-        }
-        uint64_t w[U];
-        float y[U][K];
+            bool hit[U];
+            float accs[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            // 32-bit byte offsets into the float columns (a subspace's columns
-            // hold n < 2^30 entries, so 4 jj < 2^32); the 8-byte packed cells
-            // are indexed with the 64-bit position (8 jj reaches 2^33)
-            const uint32_t at4 = (uint32_t) jj[u] << 2;
-            w[u] = packed ? W.sigb[(uint32_t) jj[u]] : 0ull;
+            for (int u = 0; u < U; ++u) {
+                float acc = 0.0f;
 #pragma unroll
-            for (int t = 0; t < K; ++t) y[u][t] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(col[t]) + at4);
-        }
-        bool hit[U];
-        float accs[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            float acc = 0.0f;
-#pragma unroll
-            for (int t = 0; t < K; ++t) {
-                const float diff = qv[t] - y[u][t];
-                const float sq = diff * diff;
-                acc = acc + sq;
+                for (int t = 0; t < K; ++t) {
+                    const float diff = qv[t] - y[u][t];
+                    const float sq = diff * diff;
+                    acc = acc + sq;
+                }
+                // every term evaluated (no short cut): the test compiles without branches
+                hit[u] = valid[u] & (jj[u] != W.i) & cells_near_packed(w[u], W.qsig) & (acc <= W.s_star);
+                accs[u] = acc;
             }
-            // every term evaluated (no short cut): the test compiles without branches
-            hit[u] = valid[u] & (jj[u] != W.i) & cells_near_packed(w[u], W.qsig) & (acc <= W.s_star);
-            accs[u] = acc;
-        }
-        if (PHI) {
-            walk_tally<U, PHI>(W, jj, hit, accs, found, dsum);
-        } else {
-            // the query's hits of the step summed over its 4 lanes (one quad) with
-            // two DPP moves; the ballots only while the two sample edges are open
-            int h = 0;
+            if (PHI) {
+                walk_tally<U, PHI>(W, jj, hit, accs, found, dsum);
+            } else {
+                // the query's hits of the step summed over its 4 lanes (one quad) with
+                // two DPP moves; the ballots only while the two sample edges are open
+                int h = 0;
 #pragma unroll
-            for (int u = 0; u < U; ++u) h += hit[u] ? 1 : 0;
-            h += __builtin_amdgcn_update_dpp(0, h, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-            h += __builtin_amdgcn_update_dpp(0, h, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-            if (found < kSampleEdges && h > 0) {
-                int f = found;
-                walk_tally<U, false>(W, jj, hit, accs, f, dsum);
+                for (int u = 0; u < U; ++u) h += hit[u] ? 1 : 0;
+                h += __builtin_amdgcn_update_dpp(0, h, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+                h += __builtin_amdgcn_update_dpp(0, h, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+                if (found < kSampleEdges && h > 0) {
+                    int f = found;
+                    walk_tally<U, false>(W, jj, hit, accs, f, dsum);
+                }
+                found += h;
             }
-            found += h;
+            seen += min(STEP, cend - c0);
+            more = walk_more(W, PHI, found, dsum, total);
         }
-        seen += min(STEP, total - c0);
-        more = walk_more(W, PHI, found, dsum, total);
     }
 }
 
