@@ -281,11 +281,10 @@ __global__ void k_dense_init(const int *__restrict__ order, int n, const int *__
                              uint8_t *__restrict__ dsorted, int *__restrict__ par, int *__restrict__ csize) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int p = order[i];
-    bool dense = cnt[p] >= thr_dense;
-    dsorted[i] = dense ? 1 : 0;
-    par[p] = dense ? p : -1;
-    csize[p] = 0;
+    // i: a sorted position for the dense flag, a point id for the rest
+    dsorted[i] = cnt[order[i]] >= thr_dense ? 1 : 0;
+    par[i] = cnt[i] >= thr_dense ? i : -1;
+    csize[i] = 0;
 }
 
 // ------------------------------------------------------------ dense components

@@ -796,11 +796,12 @@ __global__ void k_b_dense_init(const BatchSub *__restrict__ subs, int n, const i
     const BatchSub &B = subs[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    // i is a sorted position for the dense flag and a point id for the
+    // parents and sizes, so every write is coalesced
     const int p = order[B.obase + i];
-    const bool dense = meets(B.cnt[p], B.thr_dense, B.dbit);
-    dsorted[B.base + i] = dense ? 1 : 0;
-    par[B.base + p] = dense ? p : -1;
-    B.csize[p] = 0;
+    dsorted[B.base + i] = meets(B.cnt[p], B.thr_dense, B.dbit) ? 1 : 0;
+    par[B.base + i] = meets(B.cnt[i], B.thr_dense, B.dbit) ? i : -1;
+    B.csize[i] = 0;
 }
 
 // Sample phase of the components: every dense point links to the dense ones
