@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libinscy_hip.so")
 SOURCES = ["inscy_kernels.hip", "inscy_level_kernels.hip", "inscy_pruner.hip", "inscy_mlp.hip", "inscy_level.hip",
            "inscy_engine.hip", "inscy_noalloc.hip",
-           "inscy_scan.hip"]
+           "inscy_scan.hip", "inscy_perm.hip"]
 ARCH = os.environ.get("INSCY_OFFLOAD_ARCH", "gfx950")
 
 

@@ -388,6 +388,9 @@ public:
     // level-synchronous driver (INSCY_MODE_BATCHED), same result
     void run_levels(inscy_result &out);
     bool levels_mode = false;
+    // permuted SCY-tree level order (entropy_order != 0): the depth-first
+    // driver over s-connection chains (inscy_perm.hip)
+    void run_perm(inscy_result &out);
     // learned pruner (INSCY_MODE_LEARNED): device bf16 parameters, the score
     // below which a (tree, dimension) pair is skipped, and the smallest |S| of
     // a tree whose pairs may be skipped; pair collection (INSCY_MODE_COLLECT)
@@ -637,6 +640,8 @@ private:
     struct Group {
         int lo, hi;
     };
+
+    void recurse_perm(const TreeRef &T, int first);
 
     void recurse(const TreeRef &T, int first) {
         stats.calls++;

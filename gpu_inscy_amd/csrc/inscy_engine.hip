@@ -45,12 +45,16 @@ static int check_params(int64_t n, int32_t d, const inscy_params *p, int32_t mod
         return fail(INSCY_ERR_ARGUMENT, "neighborhood_size must be > 0");
     if (p->number_of_cells < 1 || p->number_of_cells > 255)
         return fail(INSCY_ERR_ARGUMENT, "number_of_cells must be in [1, 255]");
-    // A permuted SCY-tree level order changes which s-connections restrict()
-    // copies (ScyTreeNode.cu:286-289 keeps every node whose dimension id
-    // exceeds the restricted one); the engine's path model covers the
-    // identity order only.
-    if (p->entropy_order != 0) return fail(INSCY_ERR_UNSUPPORTED, "entropy_order != 0 is not implemented");
     if (mode < INSCY_MODE_EXACT || mode > INSCY_MODE_COLLECT) return fail(INSCY_ERR_ARGUMENT, "unknown mode");
+    // A permuted SCY-tree level order runs the depth-first driver over
+    // s-connection chains (inscy_perm.hip), with neighbour-count density.
+    if (p->entropy_order != 0) {
+        if (p->rectangular == 0)
+            return fail(INSCY_ERR_UNSUPPORTED, "entropy_order != 0 with rectangular = false is not implemented");
+        if (mode == INSCY_MODE_LEARNED || mode == INSCY_MODE_COLLECT)
+            return fail(INSCY_ERR_UNSUPPORTED, "entropy_order != 0 runs the exact drivers only");
+        if (d > 255) return fail(INSCY_ERR_ARGUMENT, "d must be at most 255");
+    }
     return INSCY_OK;
 }
 
@@ -115,14 +119,16 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
             eng.profile_on = g_profile_on;
             // phi density (rectangular = false) is implemented in the
             // level-synchronous driver only; it serves every mode
-            eng.levels_mode = mode != INSCY_MODE_EXACT || p->rectangular == 0;
+            eng.levels_mode = (mode != INSCY_MODE_EXACT || p->rectangular == 0) && p->entropy_order == 0;
             eng.collect = mode == INSCY_MODE_COLLECT;
             if (mode == INSCY_MODE_LEARNED) {
                 eng.pruner = pruner_on_device(eng.prune_thr, eng.prune_min_depth);
                 if (!eng.pruner) throw Failure(INSCY_ERR_ARGUMENT, "INSCY_MODE_LEARNED needs a network (inscy_set_pruner)");
             }
             eng.setup(d_X);
-            if (eng.levels_mode)
+            if (p->entropy_order != 0)
+                eng.run_perm(*res);
+            else if (eng.levels_mode)
                 eng.run_levels(*res);
             else
                 eng.run(*res);

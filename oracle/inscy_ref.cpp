@@ -747,6 +747,16 @@ struct STree {
     std::vector<int> S;
     std::vector<int> pts;
     std::vector<SPath> sp;
+    std::vector<struct Chain> ch;  // permuted level order: the s-connection chains
+};
+
+// An s-connection chain of a permuted-level-order tree: it starts at the level
+// of dimension b under point x's regular path and runs through x's cells down
+// to level `end` (SCY-tree level numbering; restricted levels are skipped).
+struct Chain {
+    int b;
+    int x;
+    int end;
 };
 
 struct U128Hash {
@@ -937,6 +947,182 @@ struct SetEngine {
         return out;
     }
 
+    // ---- permuted level order (entropy_order != 0)
+    // lvl[k]: SCY-tree level of dimension k; ord[t]: dimension at level t.
+    std::vector<int> lvl, ord;
+
+    // the unrestricted dimensions of a tree with restricted set S, in level order
+    std::vector<int> levels_of(const std::vector<int> &S) const {
+        std::vector<char> in(D.d, 0);
+        for (int k : S) in[k] = 1;
+        std::vector<int> out;
+        for (int t = 0; t < D.d; ++t)
+            if (!in[ord[t]]) out.push_back(ord[t]);
+        return out;
+    }
+
+    // restrict_node (ScyTreeNode.cu:227-320) on one chain of the tree T
+    // (restricted set S_T) for dimension m and cell group [lo, hi]: whether
+    // the chain is in the restricted tree, and its new last level.  A chain
+    // node off the restricted level is kept when its dimension id exceeds m
+    // or a node below it is kept; the chain node on the restricted level
+    // passes (and keeps the chain above it) when its cell is in the group.
+    bool restrict_chain(const Chain &c, int m, int lo, int hi, const std::vector<char> &restricted, Chain &out) const {
+        const int Lb = lvl[c.b], Lm = lvl[m];
+        const int v = cc(c.x, m);
+        const bool cell_in = v >= lo && v <= hi;
+        auto deepest_above_m = [&](int from, int to) {  // deepest unrestricted level in [from, to] with dim > m
+            int best = -1;
+            for (int t = from; t <= to; ++t)
+                if (t != Lm && !restricted[ord[t]] && ord[t] > m) best = t;
+            return best;
+        };
+        out = c;
+        if (Lb > Lm) {
+            if (!cell_in) return false;  // hangs below the restricted level, under x's cell
+            const int e = deepest_above_m(Lb, c.end);
+            if (e < 0) return false;
+            out.end = e;
+            return true;
+        }
+        const bool passes = c.end >= Lm && cell_in;
+        if (passes) {
+            int e = deepest_above_m(Lm + 1, c.end);
+            if (e < 0) {
+                e = Lm - 1;  // the chain node above the restricted level
+                while (e > Lb && restricted[ord[e]]) --e;
+            }
+            out.end = e;
+            return true;
+        }
+        const int e = deepest_above_m(Lb, std::min(c.end, Lm - 1));
+        if (e < 0) return false;
+        out.end = e;
+        return true;
+    }
+
+    // pruneRecursionAndRemove2 + propergate_count2 (ScyTreeNode.cu:446-467,
+    // 582-667) on the chains of a restricted tree R (from T by dimension m):
+    // a chain stays when its regular parent keeps a weak-dense point, unless
+    // its same-cell regular sibling existed before the prune and lost its
+    // points.  Regular nodes before the prune: prefixes of R's points, the
+    // nodes of T above the restricted level kept because a level between
+    // them and the restricted one has a dimension id above m, and the
+    // ancestors of R's chains.
+    std::vector<Chain> prune_chains(const STree &R, const std::vector<int> &W, const std::vector<int> &Tpts, int m) const {
+        const std::vector<int> levels = levels_of(R.S);
+        const int L = (int) levels.size();
+        std::vector<int> pos(D.d, -1);
+        for (int t = 0; t < L; ++t) pos[levels[t]] = t;
+        const int Lm = lvl[m];
+        // level_cond[t]: the node at child level index t lies above level Lm
+        // and some unrestricted level between it and Lm has a dimension > m
+        std::vector<char> restricted(D.d, 0);
+        for (int k : R.S) restricted[k] = 1;
+        std::vector<char> level_cond(L, 0);
+        for (int t = 0; t < L; ++t) {
+            const int Lt = lvl[levels[t]];
+            if (Lt >= Lm) continue;
+            for (int u = Lt; u < Lm; ++u)
+                if (!restricted[ord[u]] && ord[u] > m) level_cond[t] = 1;
+        }
+        PrefixSet pw, pp, pt, pa;
+        for (int w : W) insert_prefixes(pw, w, levels, L);
+        for (int p : R.pts) insert_prefixes(pp, p, levels, L);
+        for (int p : Tpts) insert_prefixes(pt, p, levels, L);
+        for (const Chain &c : R.ch) insert_prefixes(pa, c.x, levels, pos[c.b]);
+        std::vector<Chain> out;
+        for (const Chain &c : R.ch) {
+            const int t = pos[c.b];
+            const bool parent = t == 0 || pw.count({prefix_key(c.x, levels, t), t});
+            if (!parent) continue;
+            const auto q = std::make_pair(prefix_key(c.x, levels, t + 1), t + 1);
+            const bool existed = pp.count(q) || pa.count(q) || (level_cond[t] && pt.count(q));
+            if (pw.count(q) || !existed) out.push_back(c);
+        }
+        return out;
+    }
+
+    // mergeWithNeighbors merges the trees of cells lo+1.. into the tree of
+    // cell lo, and mergeNodes (ScyTreeNode.cu:803-833) adds a node's count and
+    // points only into a node whose count is positive.  When m's level is the
+    // deepest unrestricted one, the points sit one level up, where the tree of
+    // cell lo keeps an empty node for every prefix of T when that level's
+    // dimension id exceeds m: the points of such a prefix are then kept only
+    // if one of them lies in cell lo.  With no level left above, the root
+    // plays that node.
+    std::vector<int> merged_points(const STree &T, const std::vector<int> &pts, int m, int lo,
+                                   const std::vector<char> &restricted) const {
+        const int Lm = lvl[m];
+        std::vector<int> above;
+        for (int t = 0; t < D.d; ++t) {
+            if (restricted[ord[t]] || t == Lm) continue;
+            if (t > Lm) return pts;  // m's level is not the deepest: nothing is lost
+            above.push_back(ord[t]);
+        }
+        if (!above.empty() && above.back() < m) return pts;  // no empty node at the point level
+        std::set<unsigned __int128> with_lo;
+        const int A = (int) above.size();
+        for (int p : pts)
+            if (cc(p, m) == lo) with_lo.insert(prefix_key(p, above, A));
+        std::vector<int> out;
+        for (int p : pts)
+            if (with_lo.count(prefix_key(p, above, A))) out.push_back(p);
+        return out;
+    }
+
+    void rec_perm(const STree &T, int first) {
+        calls++;
+        std::vector<char> restricted(D.d, 0);
+        for (int k : T.S) restricted[k] = 1;
+        for (int m = first; m < D.d; ++m) {
+            std::vector<int> S = T.S;
+            S.push_back(m);
+            std::vector<char> flag(cells, 0);
+            for (const Chain &c : T.ch)
+                if (c.b == m) flag[cc(c.x, m)] = 1;
+            std::set<int> labelled;
+            for (int c = 0; c < cells; ++c) {
+                int lo = c;
+                while (flag[c] && c < cells - 1) ++c;
+                int hi = c;
+                STree R;
+                R.S = S;
+                for (int p : T.pts) {
+                    int v = cc(p, m);
+                    if (v >= lo && v <= hi) R.pts.push_back(p);
+                }
+                for (const Chain &ch : T.ch) {
+                    if (ch.b == m) continue;
+                    Chain o;
+                    if (restrict_chain(ch, m, lo, hi, restricted, o)) R.ch.push_back(o);
+                }
+                if (hi > lo) R.pts = merged_points(T, R.pts, m, lo, restricted);
+                ensure(S, R.pts);
+                SubspaceMemo &ms = memo_of(S);
+                std::vector<int> W;
+                for (int p : R.pts)
+                    if (weak(ms, p)) W.push_back(p);
+                trees++;
+                bool pass = (int) W.size() >= min_size;
+                size_t at = trace.size();
+                if (trace_on)
+                    trace.push_back(TraceRow{mask_of(S), lo, hi, (int) R.pts.size(), (int) W.size(), pass ? 1 : 0, -1});
+                if (!pass) continue;
+                R.ch = prune_chains(R, W, T.pts, m);
+                R.pts = W;
+                rec_perm(R, m + 1);
+                bool keep = prune_redundancy(R, W);
+                if (trace_on) trace[at].redundant = keep ? 0 : 1;
+                if (!keep) continue;
+                SubspaceMemo &mc = components(S);
+                for (int w : W)
+                    if (mc.comp[w] >= 0) labelled.insert(mc.comp[w]);
+            }
+            join(S, labelled);
+        }
+    }
+
     bool prune_redundancy(const STree &R, const std::vector<int> &W) {
         int max_min = 0;
         for (auto &entry : result) {
@@ -1062,6 +1248,51 @@ void *oracle_ref_run(const float *X, int n, int d, float eps, float F, int num_o
     return run;
 }
 
+
+// Set-based restatement for any SCY-tree level order (entropy_order != 0
+// included): s-connection chains with their last level instead of paths.
+void *oracle_sets_run_order(const float *X, int n, int d, float eps, float F, int num_obj, int min_size, float r,
+                            int cells, int rectangular, int entropy_order, int threads, int trace) {
+    Data D{X, n, d};
+    std::vector<int> order = dimension_order(D, cells, entropy_order);
+    int ecells = (int) std::ceil(1.0 / (double) eps);
+    Tree etree = build_tree(D, order, ecells, eps);
+    NeighbourIndex nbr(D, etree, order, false);
+    if (threads > 0) omp_set_num_threads(threads);
+    SetEngine E(D, nbr, eps, F, r, num_obj, min_size, cells, rectangular != 0);
+    E.trace_on = trace != 0;
+    E.ord = order;
+    E.lvl.assign(d, 0);
+    for (int t = 0; t < d; ++t) E.lvl[order[t]] = t;
+    STree root;
+    root.pts.resize(n);
+    for (int i = 0; i < n; ++i) root.pts[i] = i;
+    float cs = 1.0f / (float) cells;
+    for (int x = 0; x < n; ++x)
+        for (int b = 0; b < d; ++b) {
+            float xb = D.row(x)[b];
+            int cb = std::min((int) (xb / cs), cells - 1);
+            float upper = (float) (cb + 1) * cs;
+            float bound = upper - eps;
+            if (xb >= bound) root.ch.push_back(Chain{b, x, d - 1});
+        }
+    E.rec_perm(root, 0);
+    Run *run = new Run;
+    for (auto &kv : E.result) {
+        run->subspaces.push_back(kv.first);
+        SubspaceMemo &m = E.components(kv.first);
+        std::vector<int> lab(n, -1);
+        for (int i = 0; i < n; ++i)
+            if (m.comp[i] >= 0 && kv.second.count(m.comp[i])) lab[i] = m.comp[i];
+        run->labels.push_back(lab);
+    }
+    run->trace = std::move(E.trace);
+    run->calls = E.calls;
+    run->trees = E.trees;
+    run->queries = E.queries;
+    run->nbr_total = 0;
+    return run;
+}
 
 // Set-based restatement; same outputs as oracle_ref_run (labels are component
 // root ids, i.e. equal to the reference's up to canonical relabelling).

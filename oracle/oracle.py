@@ -208,6 +208,46 @@ def run_sets(X, eps, F, num_obj, min_size, r=1.0, cells=3, rectangular=False, th
         L.oracle_ref_free(h)
 
 
+def run_sets_order(X, eps, F, num_obj, min_size, r=1.0, cells=3, rectangular=False, entropy_order=0, threads=0,
+                   trace=False) -> OracleRun:
+    """Set-based restatement for any SCY-tree level order (s-connection
+    chains with their last level)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    n, d = X.shape
+    L = lib()
+    L.oracle_sets_run_order.restype = ctypes.c_void_p
+    L.oracle_sets_run_order.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                        ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    h = L.oracle_sets_run_order(_fp(X), n, d, eps, F, num_obj, min_size, r, cells, int(rectangular), entropy_order,
+                                threads, int(trace))
+    return _collect(h, n, d, trace)
+
+
+def _collect(h, n, d, trace) -> OracleRun:
+    L = lib()
+    try:
+        cnt = L.oracle_ref_count(h)
+        subs, labs = [], []
+        dims = np.zeros(max(d, 1), dtype=np.int32)
+        for i in range(cnt):
+            lab = np.zeros(n, dtype=np.int32)
+            k = L.oracle_ref_get(h, i, _ip(dims), _ip(lab))
+            subs.append([int(x) for x in dims[:k]])
+            labs.append(lab)
+        st = np.zeros(4, dtype=np.int64)
+        L.oracle_ref_stats(h, st.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+        tr = None
+        if trace:
+            m = L.oracle_ref_trace_len(h)
+            tr = np.zeros((m, 7), dtype=np.int64)
+            if m:
+                L.oracle_ref_trace(h, tr.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)))
+        return OracleRun(subs, labs, [int(x) for x in st], tr)
+    finally:
+        L.oracle_ref_free(h)
+
+
 def same_result(a: OracleRun, b: OracleRun) -> tuple:
     """(equal, message): subspace lists and canonical labels identical."""
     if a.subspaces != b.subspaces:

@@ -1,0 +1,87 @@
+"""entropy_order != 0: the SCY-tree levels in the entropy order of the
+dimensions (get_subspace_order, inscy_map.cpp:61-121).  The engine's
+depth-first driver over s-connection chains against the oracle's pointer-tree
+restatement (restrict_node's handling of a permuted level order,
+ScyTreeNode.cu:227-320, mergeNodes :803-833, propergate_count2 :446-467):
+subspaces, canonical labels, counters and the per-tree decisions."""
+
+import sys
+
+import numpy as np
+import pytest
+
+from gpu_inscy_amd import _lib, datagen
+from gpu_inscy_amd import inscy as INSCY
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    _lib.lib()
+
+
+def case(seed):
+    rng = np.random.default_rng(20_000 + seed)
+    d = int(rng.integers(2, 7))
+    n = int(rng.integers(80, 420))
+    cells = int(rng.integers(2, 7))
+    eps = float(rng.choice([0.02, 0.04, 0.06, 0.1, 0.15]))
+    return datagen.small(20_000 + seed, n=n, d=d, n_clusters=int(rng.integers(1, 5)), eps=eps, cells=cells,
+                         num_obj=int(rng.integers(2, 6)), min_size=int(rng.integers(3, 20)),
+                         sigma=float(rng.choice([0.02, 0.04, 0.07, 0.12])), sub_dims=(1, min(d, 4)))
+
+
+def check(w, order, mode):
+    ref = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, order, trace=True)
+    _lib.debug_trace(True)
+    try:
+        res = INSCY.run(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, order,
+                        mode=mode)
+        rows = _lib.debug_trace_rows()
+    finally:
+        _lib.debug_trace(False)
+    assert res.subspaces == ref.subspaces
+    for s, a, b in zip(ref.subspaces, ref.labels, res.labels):
+        assert np.array_equal(O.canonical(a), b), f"labels differ in {s}"
+    assert res.stats["trees"] == ref.trees and res.stats["calls"] == ref.calls
+    assert np.array_equal(rows, ref.trace)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_permuted_level_order_matches_oracle(seed):
+    w = case(seed)
+    for order in (1, -1):
+        say("entropy", seed, order, w.n, w.d, w.cells)
+        check(w, order, _lib.MODE_EXACT)
+
+
+@pytest.mark.parametrize("order", [1, -1])
+def test_config0_with_entropy_order(order):
+    w = datagen.config(0)
+    say("config0 entropy", order)
+    assert list(O.dimension_order(w.X, w.cells, order)) != list(range(w.d))
+    check(w, order, _lib.MODE_EXACT)
+    # the level-synchronous mode names the same exact result
+    check(w, order, _lib.MODE_BATCHED)
+
+
+def test_entropy_order_limits():
+    w = datagen.config(0)
+    X = torch.from_numpy(w.X).cuda()
+    with pytest.raises(_lib.InscyError) as e:
+        INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, False, 1)
+    assert e.value.code == _lib.INSCY_ERR_UNSUPPORTED
+    with pytest.raises(_lib.InscyError) as e:
+        INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 1, mode=_lib.MODE_COLLECT)
+    assert e.value.code == _lib.INSCY_ERR_UNSUPPORTED

@@ -504,3 +504,25 @@ def test_redundancy_hand_built():
     res2 = res + [([0, 2], [1, 1, 1, 1, 1, -1])]              # second superspace, min 5
     assert not O.redundancy_test(res2, 6, [0, 1, 2, 3], [0], r=1.0)   # max of mins = 5 >= 4
     assert O.redundancy_test(res2, 6, [0, 1, 2, 3, 4, 5], [0], r=1.0)  # 6 > 5
+
+
+# ------------------------------------------------------------------ permuted level order
+@pytest.mark.parametrize("seed", range(40))
+def test_set_restatement_of_a_permuted_level_order_matches_the_pointer_tree(seed):
+    # the chain model (truncated s-connection chains, empty nodes above the
+    # restricted level, mergeNodes' positive-count rule) against the literal
+    # pointer-tree restatement, both density modes, ascending and descending
+    # entropy order; per-tree traces included
+    rng = np.random.default_rng(10_000 + seed)
+    d = int(rng.integers(2, 8)); n = int(rng.integers(80, 400)); cells = int(rng.integers(2, 7))
+    eps = float(rng.choice([0.02, 0.04, 0.06, 0.1, 0.15]))
+    rect = bool(rng.integers(0, 2))
+    w = datagen.small(10_000 + seed, n=n, d=d, n_clusters=int(rng.integers(1, 5)), eps=eps, cells=cells,
+                      num_obj=int(rng.integers(2, 6)), min_size=int(rng.integers(3, 20)),
+                      sigma=float(rng.choice([0.02, 0.04, 0.07, 0.12])), sub_dims=(1, min(d, 4)))
+    for order in (1, -1, 0):
+        a = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, rect, order, trace=True)
+        b = O.run_sets_order(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, rect, order, threads=1, trace=True)
+        assert O.same_result(a, b)[0], (order, O.same_result(a, b)[1])
+        assert np.array_equal(a.trace, b.trace)
+        assert (a.calls, a.trees) == (b.calls, b.trees)
