@@ -311,7 +311,7 @@ public:
         // the level-synchronous driver keeps its lists in pools; its arena holds
         // the replay's label flags (one per open dimension) and the emit buffers
         if (levels_mode) want = (size_t) n * (d + 8) + pass + (64u << 20);
-        if (const size_t free_b = device_available_bytes()) want = std::min(want, free_b / 3);
+        if (const size_t free_b = device_available_bytes() / (size_t) mem_share) want = std::min(want, free_b / 3);
         arena.init(want);
 
         int *hist = arena.take<int>((size_t) d * ecells);
@@ -401,6 +401,7 @@ public:
     bool phi_all_exact = false;  // test hook: every phi by the leaf-order recomputation
     bool generic_walk = false;   // test hook: no neighbour walk compiled for its k
     long long keep_budget_override = -1;  // test hook: bytes kept across batches (< 0: the batch budget)
+    int mem_share = 1;  // engines running at once on the device (each budgets 1/mem_share of the free bytes)
 
     void run(inscy_result &out) {
         int *P = arena.take<int>(n);

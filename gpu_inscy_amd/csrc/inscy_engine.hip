@@ -100,7 +100,7 @@ static const uint16_t *pruner_on_device(float &thr, int &min_depth) {
 }
 
 static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *p, int32_t mode, hipStream_t user,
-                    inscy_result **out) {
+                    inscy_result **out, int mem_share = 1) {
     auto t0 = std::chrono::steady_clock::now();
     hipStream_t st = user;
     bool own = false;
@@ -117,6 +117,7 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
             eng.generic_walk = g_generic_walk;
             eng.keep_budget_override = g_keep_budget;
             eng.profile_on = g_profile_on;
+            eng.mem_share = mem_share;
             // phi density (rectangular = false) is implemented in the
             // level-synchronous driver only; it serves every mode
             eng.levels_mode = (mode != INSCY_MODE_EXACT || p->rectangular == 0) && p->entropy_order == 0;
@@ -149,7 +150,8 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
 // Many independent data sets: worker threads, each with its own stream and
 // engine, take the sets from a queue ordered by estimated cost (n^2 d,
 // largest first) so that the longest runs start first and the streams overlap
-// on the device.
+// on the device.  Each engine budgets a 1/workers share of the free device
+// memory, so the engines running at once do not claim the same bytes.
 static int run_many_impl(const float *const *X, const int64_t *n, const int32_t *d, const inscy_params *params,
                          int32_t count, int32_t mode, bool host_input, inscy_result **out) {
     for (int32_t i = 0; i < count; ++i) out[i] = nullptr;
@@ -184,12 +186,12 @@ static int run_many_impl(const float *const *X, const int64_t *n, const int32_t 
                     rc[i] = INSCY_ERR_DEVICE;
                     msg[i] = "input copy failed";
                 } else {
-                    rc[i] = run_impl(dX, n[i], d[i], &params[i], mode, st, &out[i]);
+                    rc[i] = run_impl(dX, n[i], d[i], &params[i], mode, st, &out[i], workers);
                 }
                 (void) hipStreamSynchronize(st);
                 dev_free(dX);
             } else {
-                rc[i] = run_impl(X[i], n[i], d[i], &params[i], mode, st, &out[i]);
+                rc[i] = run_impl(X[i], n[i], d[i], &params[i], mode, st, &out[i], workers);
             }
             if (rc[i] != INSCY_OK) msg[i] = g_error;
         }

@@ -411,7 +411,7 @@ struct TreeHit {
 };
 hipError_t launch_tree_hits(const TreeHit *trees, const int *blk_off, int ntrees, int nblk, int *hit, hipStream_t st);
 
-constexpr int kRangeWords = 21;      // per query in LDS: 9 range starts, 10 prefix sums, count, own run
+constexpr int kRangeWords = 31;      // per query in LDS: 9 range starts, 10 prefix sums, count, own run, 10 group sums
 constexpr int kBlockThreads = 256;   // threads per block of every pass
 constexpr int kQueriesPerBlock = 64;  // queries per block of the batched neighbour passes
 constexpr int kSampleEdges = 2;       // neighbours recorded per point by the count walk (Afforest's sample rounds)

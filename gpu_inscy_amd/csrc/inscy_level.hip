@@ -55,7 +55,7 @@ void Engine::run_levels(inscy_result &out) {
     batch.chunk = (size_t) 256 << 20;
     up.get((size_t) 4 << 20);
     batch_budget = (size_t) 2 << 30;
-    if (const size_t free_b = device_available_bytes())
+    if (const size_t free_b = device_available_bytes() / (size_t) mem_share)
         batch_budget = std::min<size_t>(std::max<size_t>(free_b / 8, (size_t) 256 << 20), (size_t) 8 << 30);
     kept.clear();
     keptp.reset();
@@ -255,7 +255,11 @@ void Engine::expand_level(const std::vector<int> &frontier, const int *counts) {
             GridArg G = grid_of(S);
             long long ncell = 1;
             for (int q = 0; q < G.j; ++q) ncell *= G.C;
-            size_t est = (size_t) n * (26 + 6 * (size_t) S.k) + (size_t) ncell * 8;
+            // process_batch's takes per subspace: counts, keys, rank, order,
+            // query flags and list, two sample edges, packed cells (own and
+            // shared), dense flags and parents (50 bytes a point), k coordinate
+            // columns and cells (6k), the phi pass's queue (8), grid offsets
+            size_t est = (size_t) n * (50 + 6 * (size_t) S.k + (phi_mode ? 8 : 0)) + (size_t) ncell * 8;
             for (int a = c.t0; a < c.t1; ++a) {
                 if (!pair_on(a, c.m)) continue;
                 const LTree &t = lt[lv_act[a]];
@@ -446,7 +450,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.order = batch.take<int>((size_t) B * n);
     X.hist = batch.take<int>(X.nhist);
     X.off = batch.take<int>(X.nhist);
-    X.Ys = batch.take<float>(coord);
+    X.Ys = batch.take<float>(coord + 4);  // the grouped walks read up to 3 past a column's end
     X.Es = batch.take<uint16_t>(coord);
     X.qflag = batch.take<uint8_t>(((size_t) B * n + 15) & ~(size_t) 15);  // read 16 bytes a lane
     X.qlist = batch.take<int>(std::max<long long>(qoff, 1));
@@ -455,7 +459,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.select_bytes = select_temp_bytes(B * n);
     X.select_temp = batch.take<char>(X.select_bytes);
     X.nb = batch.take<int>((size_t) kSampleEdges * B * n);
-    X.sig = batch.take<uint64_t>((size_t) B * n);
+    X.sig = batch.take<uint64_t>((size_t) B * n + 4);
     X.sigS = batch.take<uint64_t>((size_t) reps.size() * n);
     X.reps = upload(batch, reps);
     X.nrep = (int) reps.size();

@@ -220,7 +220,9 @@ __global__ void k_b_qstart(const int *__restrict__ qlist, const int *__restrict_
 // the run lengths, q[19] the number of runs.  A lane maps its candidate
 // number to a position with a run pointer that only moves forward.
 // q[20]: the run of the query's own cells in the first index dimensions (it
-// holds the query, so it is not empty)
+// holds the query, so it is not empty).  q[21..30]: prefix sums of the runs'
+// lengths in groups of four (ceil(length / 4)), for the walks that take four
+// consecutive candidates of one run per lane.
 __device__ __forceinline__ void build_runs(const GridArg &G, const int *__restrict__ off, const int *qe, int stride,
                                            int base0, int *q) {
     // rows r = 3 xi + yi over the cells within one of the query's in the first
@@ -245,16 +247,19 @@ __device__ __forceinline__ void build_runs(const GridArg &G, const int *__restri
         a[r] = off[row + lo];
         e[r] = off[row + hi + 1];
     }
-    int nr = 0, acc = 0;
+    int nr = 0, acc = 0, gacc = 0;
     q[9] = 0;
     q[20] = 0;
+    q[21] = 0;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
         if (r / 3 < nx && r % 3 < ny && e[r] > a[r]) {
             if (r == own) q[20] = nr;
             q[nr] = a[r] - base0;
             acc += e[r] - a[r];
+            gacc += (e[r] - a[r] + 3) >> 2;
             q[10 + nr] = acc;
+            q[22 + nr] = gacc;
             ++nr;
         }
     }
@@ -489,18 +494,47 @@ __device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double
                 }
                 jj[u] = cr + roff;
             }
-            uint64_t w[U];
             float y[U][K];
+            bool near[U];
+            if (packed) {
+                // the packed cells first; the coordinates only of candidates
+                // within one cell on every checked dimension (the vector memory
+                // pipeline, not the ALUs, bounds this walk, and most candidates
+                // of a subspace with checked dimensions fail the cell test)
+                uint64_t w[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                // 32-bit byte offsets into the float columns (a subspace's columns
-                // hold n < 2^30 entries, so 4 jj < 2^32); the 8-byte packed cells
-                // are indexed with the 64-bit position (8 jj reaches 2^33)
-                const uint32_t at4 = (uint32_t) jj[u] << 2;
-                w[u] = packed ? W.sigb[(uint32_t) jj[u]] : 0ull;
+                for (int u = 0; u < U; ++u) w[u] = W.sigb[(uint32_t) jj[u]];
+                // every cell test before the first coordinate load: the mask is
+                // pinned in a register here, so one wait serves the four words
+                // and the loads of the four candidates are all in flight at once
+                int nm = 0;
 #pragma unroll
-                for (int t = 0; t < K; ++t)
-                    y[u][t] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(col[t]) + at4);
+                for (int u = 0; u < U; ++u) nm |= (valid[u] & cells_near_packed(w[u], W.qsig)) ? 1 << u : 0;
+                asm volatile("" : "+v"(nm));
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    near[u] = (nm >> u) & 1;
+                    // 32-bit byte offsets into the float columns (a subspace's
+                    // columns hold n < 2^30 entries, so 4 jj < 2^32)
+                    const uint32_t at4 = (uint32_t) jj[u] << 2;
+                    if (near[u]) {
+#pragma unroll
+                        for (int t = 0; t < K; ++t)
+                            y[u][t] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(col[t]) + at4);
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < K; ++t) y[u][t] = qv[t];
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t at4 = (uint32_t) jj[u] << 2;
+                    near[u] = valid[u];
+#pragma unroll
+                    for (int t = 0; t < K; ++t)
+                        y[u][t] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(col[t]) + at4);
+                }
             }
             bool hit[U];
             float accs[U];
@@ -514,7 +548,7 @@ __device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double
                     acc = acc + sq;
                 }
                 // every term evaluated (no short cut): the test compiles without branches
-                hit[u] = valid[u] & (jj[u] != W.i) & cells_near_packed(w[u], W.qsig) & (acc <= W.s_star);
+                hit[u] = near[u] & (jj[u] != W.i) & (acc <= W.s_star);
                 accs[u] = acc;
             }
             if (PHI) {
@@ -535,6 +569,99 @@ __device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double
             }
             seen += min(STEP, cend - c0);
             more = walk_more(W, PHI, found, dsum, total);
+        }
+    }
+}
+
+// Counts (not phi) with k = K: the walk of walk_fast, with each lane taking
+// four consecutive candidates of one run per step, so a step loads each
+// coordinate column with one 16-byte load per lane (and the packed cells
+// with two) instead of four 4-byte loads.  The vector memory pipeline bounds
+// these walks (its address and data units are busy nearly all the time),
+// and its cost grows with the number of load instructions.  The runs are
+// concatenated in groups of four (q[21..30]); the last group of a run masks
+// the positions past the run's end, which are read (up to three past the
+// run: the batch's coordinate and packed-cell buffers end with padding) but
+// never counted.  A lane's first group position is 4-byte aligned only;
+// dword-aligned multi-dword global loads are supported on gfx950.
+template <int K>
+__device__ __forceinline__ void walk_vec(const QueryWalk &W, int &found, bool &more, long long &seen) {
+    typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+    typedef unsigned long long u2u __attribute__((ext_vector_type(2), aligned(8)));
+    constexpr int LPQ = kBLanes, U = 4;
+    const BatchSub &B = W.B;
+    const int *q = W.q;
+    const int sub = __lane_id() % LPQ;
+    const int nr = q[19];
+    const int gtotal = q[21 + nr];
+    // start at a pseudo-random group of the query's own run (spreads the
+    // sample edges), walk to the end, then from the start to there
+    const int own = q[20];
+    const int og = q[21 + own], olen = q[22 + own] - og;
+    const int grot = gtotal > 0 ? og + (int) (((uint32_t) W.p * 2654435761u) % (uint32_t) max(olen, 1)) : 0;
+    const float *col[K];
+    float qv[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        col[t] = W.Ys + (t == B.pos_m ? B.mcoord : B.coord + (long long) t * W.n);
+        qv[t] = W.qy[t * QPB];
+    }
+    for (int segm = 0; segm < 2; ++segm) {
+        const int gbeg = segm == 0 ? grot : 0, gend = segm == 0 ? gtotal : grot;
+        int r = segm == 0 ? own : 0;
+        int rgend = q[22 + r];
+        for (int g0 = gbeg; g0 < gend && more; g0 += LPQ) {
+            const int g = min(g0 + sub, gend - 1);
+            while (g >= rgend) {
+                ++r;
+                rgend = q[22 + r];
+            }
+            const int e0 = (g - q[21 + r]) << 2;  // the group's first element in run r
+            const int len = q[10 + r] - q[9 + r];
+            const int nv = g0 + sub < gend ? min(U, len - e0) : 0;  // its elements inside the run
+            const int jj0 = q[r] + e0;
+            const u2u wa = *reinterpret_cast<const u2u *>(W.sigb + (uint32_t) jj0);
+            const u2u wb = *reinterpret_cast<const u2u *>(W.sigb + (uint32_t) jj0 + 2);
+            f4u y[K];
+#pragma unroll
+            for (int t = 0; t < K; ++t)
+                y[t] = *reinterpret_cast<const f4u *>(reinterpret_cast<const char *>(col[t]) +
+                                                      ((uint32_t) jj0 << 2));
+            const uint64_t w[U] = {wa.x, wa.y, wb.x, wb.y};
+            int jj[U];
+            bool hit[U];
+            float accs[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                float acc = 0.0f;
+#pragma unroll
+                for (int t = 0; t < K; ++t) {
+                    const float diff = qv[t] - y[t][u];
+                    const float sq = diff * diff;
+                    acc = acc + sq;
+                }
+                jj[u] = jj0 + u;
+                hit[u] = (u < nv) & (jj[u] != W.i) & cells_near_packed(w[u], W.qsig) & (acc <= W.s_star);
+                accs[u] = acc;
+            }
+            // the query's hits and examined candidates of the step, summed over
+            // its quad with two DPP moves each
+            int h = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) h += hit[u] ? 1 : 0;
+            int ex = nv;
+            h += __builtin_amdgcn_update_dpp(0, h, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+            h += __builtin_amdgcn_update_dpp(0, h, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+            ex += __builtin_amdgcn_update_dpp(0, ex, 0xB1, 0xF, 0xF, false);
+            ex += __builtin_amdgcn_update_dpp(0, ex, 0x4E, 0xF, 0xF, false);
+            if (found < kSampleEdges && h > 0) {
+                double unused = 0.0;
+                int f = found;
+                walk_tally<U, false>(W, jj, hit, accs, f, unused);
+            }
+            found += h;
+            seen += ex;
+            more = found < B.cap;
         }
     }
 }
@@ -730,7 +857,9 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
                           nb + kSampleEdges * (B.base + p)};
         // K > 0: every subspace of the launch has k = K with its checked
         // dimensions packed (the host checks), so the walk is compiled for it
-        if constexpr (K > 0)
+        if constexpr (K > 0 && !PHI)
+            walk_vec<K>(W, found, more, seen);
+        else if constexpr (K > 0)
             walk_fast<K, PHI>(W, found, dsum, more, seen);
         else if (k <= kFastDims && B.restmask == 0)
             walk_fast_rt<PHI>(W, found, dsum, more, seen);
@@ -934,18 +1063,37 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
                 }
                 jj[u] = cr + roff;
             }
+            // dense flags and packed cells first; ids and coordinates only of
+            // dense candidates within one cell on every checked dimension
             uint64_t w[U];
             uint8_t dn[U];
-            int qq[U];
-            float y[U][K];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const uint32_t at = (uint32_t) jj[u];
                 dn[u] = ds[at];
-                qq[u] = ord[at];
                 w[u] = packed ? sigb[at] : 0ull;
+            }
+            // every test before the first id or coordinate load (one wait)
+            int nm = 0;
 #pragma unroll
-                for (int t = 0; t < K; ++t) y[u][t] = col[t][at];
+            for (int u = 0; u < U; ++u) nm |= (valid[u] & (dn[u] != 0) & cells_near_packed(w[u], qsig)) ? 1 << u : 0;
+            asm volatile("" : "+v"(nm));
+            bool near[U];
+            int qq[U];
+            float y[U][K];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                near[u] = (nm >> u) & 1;
+                const uint32_t at = (uint32_t) jj[u];
+                if (near[u]) {
+                    qq[u] = ord[at];
+#pragma unroll
+                    for (int t = 0; t < K; ++t) y[u][t] = col[t][at];
+                } else {
+                    qq[u] = p;
+#pragma unroll
+                    for (int t = 0; t < K; ++t) y[u][t] = qv[t];
+                }
             }
             bool hit[U];
 #pragma unroll
@@ -957,7 +1105,7 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
                     const float sq = diff * diff;
                     acc = acc + sq;
                 }
-                hit[u] = valid[u] & (jj[u] != i) & (dn[u] != 0) & cells_near_packed(w[u], qsig) & (acc <= s_star);
+                hit[u] = near[u] & (jj[u] != i) & (acc <= s_star);
             }
             // parents only move up to roots: a parent equal to p's root is p's set
             int pq[U];

@@ -61,17 +61,17 @@ def run(X, neighborhood_size, F, num_obj, min_size, r=1.0, number_of_cells=3, re
                               entropy_order)
     try:
         import torch
-        if isinstance(X, torch.Tensor):
-            if X.dim() != 2:
-                raise ValueError("X must be [n, d]")
-            if X.is_cuda:
-                Xc = X.detach().to(torch.float32).contiguous()
-                with torch.cuda.device(Xc.device):
-                    stream = torch.cuda.current_stream(Xc.device).cuda_stream
-                    return _lib.run_device(Xc.data_ptr(), Xc.shape[0], Xc.shape[1], params, mode, stream)
-            X = X.detach().to(torch.float32).contiguous().numpy()
     except ImportError:
-        pass
+        torch = None
+    if torch is not None and isinstance(X, torch.Tensor):
+        if X.dim() != 2:
+            raise ValueError("X must be [n, d]")
+        if X.is_cuda:
+            Xc = X.detach().to(torch.float32).contiguous()
+            with torch.cuda.device(Xc.device):
+                stream = torch.cuda.current_stream(Xc.device).cuda_stream
+                return _lib.run_device(Xc.data_ptr(), Xc.shape[0], Xc.shape[1], params, mode, stream)
+        X = X.detach().to(torch.float32).contiguous().numpy()
     return _lib.run_host(np.asarray(X, dtype=np.float32), params, mode)
 
 

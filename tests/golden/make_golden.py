@@ -7,6 +7,8 @@
     python tests/golden/make_golden.py 2 --threads 6   # config 2 (n=100k, d=20), set restatement
     python tests/golden/make_golden.py 3 --threads 6   # config 3: the 64 data sets, set restatement
     python tests/golden/make_golden.py 2small     # config 2's parameters at n=25,000, pointer-tree oracle
+    python tests/golden/make_golden.py 1ent --threads 4   # config 1, ascending entropy level order
+    python tests/golden/make_golden.py 1ent- --threads 4  # config 1, descending entropy level order
     python tests/golden/make_golden.py check1 --threads 8  # set restatement on config 1 vs config1.json
 
 The oracle is the reference's CPU INSCY restated (oracle/inscy_ref.cpp): the
@@ -108,6 +110,19 @@ def main() -> None:
         else:
             meta["subspaces"] = rows(ref)
             write_json(f"config1{tag}.json", meta)
+    elif a.config in ("1ent", "1ent-"):
+        # config 1 with the SCY-tree levels in ascending (1ent) or descending
+        # (1ent-) entropy order, from the set restatement of a permuted level
+        # order (tests/test_oracle.py pins it against the pointer tree)
+        order = 1 if a.config == "1ent" else -1
+        import dataclasses
+        w = dataclasses.replace(datagen.config(1), entropy_order=order)
+        run = O.run_sets_order(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, order, threads=a.threads)
+        meta = meta_of(w, run)
+        meta["entropy_order"] = order
+        meta["oracle"] = "set restatement, permuted level order"
+        meta["subspaces"] = rows(run)
+        write_json(f"config1_entropy{'' if order == 1 else '_desc'}.json", meta)
     elif a.config == "check1":
         w = datagen.config(1)
         run = run_workload(w, "sets", a.threads)
