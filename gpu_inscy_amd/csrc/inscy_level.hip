@@ -447,7 +447,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.nhist = (int) (hoff + 1);
     X.keys = batch.take<int>((size_t) B * n);
     X.rank = batch.take<int>((size_t) B * n);
-    X.order = batch.take<int>((size_t) B * n);
+    X.order = batch.take<int>((size_t) B * n + 4);  // the grouped link walk reads up to 3 ids past a run
     X.hist = batch.take<int>(X.nhist);
     X.off = batch.take<int>(X.nhist);
     X.Ys = batch.take<float>(coord + 4);  // the grouped walks read up to 3 past a column's end
@@ -467,7 +467,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     X.fbcap = phi_mode ? B * n : 1;  // each point of each subspace is sent at most once per pass
     X.fb = batch.take<int2>(X.fbcap);
     HIP_OK(launch_fill_i32(X.nb, -1, (size_t) kSampleEdges * B * n, st));
-    X.dsorted = batch.take<uint8_t>((size_t) B * n);
+    X.dsorted = batch.take<uint8_t>((size_t) B * n + 8);  // the grouped link walk reads aligned words past the end
     X.par = batch.take<int>((size_t) B * n);
     X.giant = batch.take<int>(B);
     X.scan_bytes = scan_temp_bytes(X.nhist);

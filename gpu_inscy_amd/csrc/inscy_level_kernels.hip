@@ -944,18 +944,22 @@ __global__ void k_b_link_sample(const BatchSub *__restrict__ subs, int n, const 
     const BatchSub &B = subs[blockIdx.y];
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const int *cnt = B.cnt;
-    const int c = cnt[p];
-    if (!meets(c, B.thr_dense, B.dbit)) return;
-    // the recorded neighbours: min(count, kSampleEdges) of them (phi mode records them too)
-    const int q = nb[kSampleEdges * (B.base + p) + round];
-    if (B.dbit == 0 && c <= round) return;
-    if (q < 0 || q >= n || !meets(cnt[q], B.thr_dense, B.dbit)) return;
+    // a point is dense exactly when its parent is set (k_b_dense_init gives
+    // dense points themselves, the others -1, and unions only move roots to
+    // smaller roots), so the parents answer both density tests without the
+    // counts
     int *pr = par + B.base;
+    const int pp = pr[p];
+    if (pp < 0) return;
+    // the recorded neighbours: min(count, kSampleEdges) of them, -1 after
+    // (phi mode records them too)
+    const int q = nb[kSampleEdges * (B.base + p) + round];
+    if (q < 0 || q >= n) return;
     // after the first round's compression every point's parent is its root:
     // equal parents are one set, decided without reading the root's own entry
     // (which every point of a large component would read at once)
-    if (pr[p] == pr[q]) return;
+    const int pq = pr[q];
+    if (pq < 0 || pq == pp) return;
     uf_union(pr, p, q);
 }
 
@@ -1038,7 +1042,14 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
     const int total = q[9 + q[19]];
     int rp = uf_find(pr, p);
     if constexpr (K > 0) {
-        constexpr int U = 4, STEP = LPQ * U;
+        // each lane four consecutive candidates of one run per step (the
+        // groups of walk_vec): per step one 16-byte load for the ids and each
+        // coordinate column, two for the packed cells and two aligned words
+        // for the four dense flags
+        typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+        typedef int i4u __attribute__((ext_vector_type(4), aligned(4)));
+        typedef unsigned long long u2u __attribute__((ext_vector_type(2), aligned(8)));
+        constexpr int U = 4;
         const float *col[K];
         float qv[K];
 #pragma unroll
@@ -1046,66 +1057,43 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
             col[t] = Ys + (t == B.pos_m ? B.mcoord : B.coord + (long long) t * n);
             qv[t] = qy[t * QPB];
         }
-        const bool packed = B.nchk4 > 0;
-        int r = 0, rend = q[10], roff = q[0] - q[9];
-        for (int c0 = 0; c0 < total; c0 += STEP) {
-            int jj[U];
-            bool valid[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int c = c0 + u * LPQ + sub;
-                valid[u] = c < total;
-                const int cr = min(c, total - 1);
-                while (cr >= rend) {
-                    ++r;
-                    rend = q[10 + r];
-                    roff = q[r] - q[9 + r];
-                }
-                jj[u] = cr + roff;
+        const uint32_t *dw = reinterpret_cast<const uint32_t *>(dsorted);  // 256-byte aligned
+        const int nr = q[19];
+        const int gtotal = q[21 + nr];
+        int r = 0, rgend = q[22];
+        for (int g0 = 0; g0 < gtotal; g0 += LPQ) {
+            const int g = min(g0 + sub, gtotal - 1);
+            while (g >= rgend) {
+                ++r;
+                rgend = q[22 + r];
             }
-            // dense flags and packed cells first; ids and coordinates only of
-            // dense candidates within one cell on every checked dimension
-            uint64_t w[U];
-            uint8_t dn[U];
+            const int e0 = (g - q[21 + r]) << 2;
+            const int len = q[10 + r] - q[9 + r];
+            const int nv = g0 + sub < gtotal ? min(U, len - e0) : 0;
+            const int jj0 = q[r] + e0;
+            const size_t at = (size_t) B.base + (size_t) jj0;  // byte index of the first dense flag
+            const uint32_t d0 = dw[at >> 2], d1 = dw[(at >> 2) + 1];
+            const u2u wa = *reinterpret_cast<const u2u *>(sigb + (uint32_t) jj0);
+            const u2u wb = *reinterpret_cast<const u2u *>(sigb + (uint32_t) jj0 + 2);
+            const i4u qv4 = *reinterpret_cast<const i4u *>(ord + (uint32_t) jj0);
+            f4u y[K];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const uint32_t at = (uint32_t) jj[u];
-                dn[u] = ds[at];
-                w[u] = packed ? sigb[at] : 0ull;
-            }
-            // every test before the first id or coordinate load (one wait)
-            int nm = 0;
-#pragma unroll
-            for (int u = 0; u < U; ++u) nm |= (valid[u] & (dn[u] != 0) & cells_near_packed(w[u], qsig)) ? 1 << u : 0;
-            asm volatile("" : "+v"(nm));
-            bool near[U];
-            int qq[U];
-            float y[U][K];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                near[u] = (nm >> u) & 1;
-                const uint32_t at = (uint32_t) jj[u];
-                if (near[u]) {
-                    qq[u] = ord[at];
-#pragma unroll
-                    for (int t = 0; t < K; ++t) y[u][t] = col[t][at];
-                } else {
-                    qq[u] = p;
-#pragma unroll
-                    for (int t = 0; t < K; ++t) y[u][t] = qv[t];
-                }
-            }
+            for (int t = 0; t < K; ++t) y[t] = *reinterpret_cast<const f4u *>(col[t] + (uint32_t) jj0);
+            const uint32_t dn4 = __builtin_amdgcn_alignbyte(d1, d0, (uint32_t) (at & 3));
+            const uint64_t w[U] = {wa.x, wa.y, wb.x, wb.y};
+            const int qq[U] = {qv4.x, qv4.y, qv4.z, qv4.w};
             bool hit[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 float acc = 0.0f;
 #pragma unroll
                 for (int t = 0; t < K; ++t) {
-                    const float diff = qv[t] - y[u][t];
+                    const float diff = qv[t] - y[t][u];
                     const float sq = diff * diff;
                     acc = acc + sq;
                 }
-                hit[u] = near[u] & (jj[u] != i) & (acc <= s_star);
+                hit[u] = (u < nv) & (jj0 + u != i) & (((dn4 >> (8 * u)) & 0xFFu) != 0) &
+                         cells_near_packed(w[u], qsig) & (acc <= s_star);
             }
             // parents only move up to roots: a parent equal to p's root is p's set
             int pq[U];

@@ -85,3 +85,25 @@ def test_entropy_order_limits():
     with pytest.raises(_lib.InscyError) as e:
         INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 1, mode=_lib.MODE_COLLECT)
     assert e.value.code == _lib.INSCY_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("order,name", [(1, "config1_entropy.json"), (-1, "config1_entropy_desc.json")])
+def test_config1_with_entropy_order_matches_golden(order, name):
+    # n=25,000 d=15 with the levels in ascending / descending entropy order:
+    # both modes against the oracle's digests (set restatement of a permuted
+    # level order, tests/golden/make_golden.py 1ent / 1ent-)
+    import hashlib
+    import json
+    import os
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", name)))
+    w = datagen.config(1)
+    assert g["X_sha256"] == hashlib.sha256(w.X.tobytes()).hexdigest() and g["entropy_order"] == order
+    X = torch.from_numpy(w.X).cuda()
+    for mode in (_lib.MODE_EXACT, _lib.MODE_BATCHED):
+        say("config1 entropy", order, mode)
+        res = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, order, mode=mode)
+        assert res.subspaces == [s for s, _, _, _ in g["subspaces"]]
+        assert [int(l.max()) + 1 for l in res.labels] == [x[1] for x in g["subspaces"]]
+        digests = [hashlib.sha256(np.ascontiguousarray(l, dtype=np.int32).tobytes()).hexdigest() for l in res.labels]
+        assert digests == [x[3] for x in g["subspaces"]]
+        assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
