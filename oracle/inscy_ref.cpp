@@ -1037,7 +1037,12 @@ struct SetEngine {
             const bool parent = t == 0 || pw.count({prefix_key(c.x, levels, t), t});
             if (!parent) continue;
             const auto q = std::make_pair(prefix_key(c.x, levels, t + 1), t + 1);
-            const bool existed = pp.count(q) || pa.count(q) || (level_cond[t] && pt.count(q));
+            bool existed = pp.count(q) || pa.count(q) || (level_cond[t] && pt.count(q));
+            // mutation knob (see prune above): 7 ignores the sibling rule, 8
+            // ignores the empty nodes kept above m's level
+            static const int mutate = getenv("ORACLE_SETS_MUTATE") ? atoi(getenv("ORACLE_SETS_MUTATE")) : 0;
+            if (mutate == 7) existed = false;
+            if (mutate == 8) existed = pp.count(q) || pa.count(q);
             if (pw.count(q) || !existed) out.push_back(c);
         }
         return out;
@@ -1053,6 +1058,9 @@ struct SetEngine {
     // plays that node.
     std::vector<int> merged_points(const STree &T, const std::vector<int> &pts, int m, int lo,
                                    const std::vector<char> &restricted) const {
+        // mutation knob (see prune above): 6 ignores mergeNodes' positive-count rule
+        static const int mutate = getenv("ORACLE_SETS_MUTATE") ? atoi(getenv("ORACLE_SETS_MUTATE")) : 0;
+        if (mutate == 6) return pts;
         const int Lm = lvl[m];
         std::vector<int> above;
         for (int t = 0; t < D.d; ++t) {

@@ -3,6 +3,7 @@ CPU INSCY) — exact for every integer output: cells, neighbour counts,
 components, subspace sets, canonical labels, per-tree decisions."""
 
 import ctypes
+import hashlib
 import os
 import sys
 

@@ -526,3 +526,35 @@ def test_set_restatement_of_a_permuted_level_order_matches_the_pointer_tree(seed
         assert O.same_result(a, b)[0], (order, O.same_result(a, b)[1])
         assert np.array_equal(a.trace, b.trace)
         assert (a.calls, a.trees) == (b.calls, b.trees)
+
+
+def permuted_case(seed):
+    rng = np.random.default_rng(10_000 + seed)
+    d = int(rng.integers(2, 8)); n = int(rng.integers(80, 400)); cells = int(rng.integers(2, 7))
+    eps = float(rng.choice([0.02, 0.04, 0.06, 0.1, 0.15]))
+    w = datagen.small(10_000 + seed, n=n, d=d, n_clusters=int(rng.integers(1, 5)), eps=eps, cells=cells,
+                      num_obj=int(rng.integers(2, 6)), min_size=int(rng.integers(3, 20)),
+                      sigma=float(rng.choice([0.02, 0.04, 0.07, 0.12])), sub_dims=(1, min(d, 4)))
+    return w
+
+
+def permuted_agree(seed, order):
+    w = permuted_case(seed)
+    a = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, order, trace=True)
+    b = O.run_sets_order(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, order, threads=1, trace=True)
+    return O.same_result(a, b)[0] and np.array_equal(a.trace, b.trace)
+
+
+@pytest.mark.parametrize("mutation", [6, 7, 8])
+def test_permuted_cross_check_catches_planted_mistakes(mutation):
+    # the set restatement of a permuted level order run with a planted mistake
+    # (6: mergeNodes' positive-count rule ignored, 7: the sibling rule ignored,
+    # 8: the empty nodes kept above the restricted level not counted as
+    # existing siblings) must disagree with the pointer tree on some cases
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from tests.test_oracle import permuted_agree\n"
+            "print(sum(1 for s in range(120) for o in (1, -1) if not permuted_agree(s, o)))\n") % ROOT
+    env = dict(os.environ, ORACLE_SETS_MUTATE=str(mutation))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr
+    assert int(out.stdout.strip().splitlines()[-1]) > 0
