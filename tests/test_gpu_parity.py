@@ -517,17 +517,28 @@ def test_config2_parameters_at_25k_match_pointer_oracle():
 def test_run_many_matches_config3_oracle():
     # the 64 independent data sets through one run_many call, each against
     # the oracle's digests (tests/golden/config3.json)
+    # (the digests are written set by set; while the oracle's run is
+    # incomplete the sets it has finished are compared)
     g = _golden("config3.json")
     sets = datagen.multi_instance()
-    if g is None or not g.get("complete"):
-        pytest.skip("no complete oracle digests for the 64 data sets")
-    say("run_many 64 sets vs oracle")
+    if g is None or not g.get("sets"):
+        pytest.skip("no oracle digests for the 64 data sets")
+    say("run_many 64 sets vs oracle", len(g["sets"]), "with digests")
     jobs = [(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 0) for w in sets]
     many = INSCY.run_many(jobs, mode=_lib.MODE_BATCHED)
-    assert len(many) == len(sets) == len(g["sets"])
-    for w, res, gs in zip(sets, many, g["sets"]):
-        assert gs["workload"] == w.name and gs["X_sha256"] == hashlib.sha256(w.X.tobytes()).hexdigest()
+    assert len(many) == len(sets) == 64
+    by_name = {gs["workload"]: gs for gs in g["sets"]}
+    if g.get("complete"):
+        assert len(by_name) == len(sets)
+    checked = 0
+    for w, res in zip(sets, many):
+        gs = by_name.get(w.name)
+        if gs is None:
+            continue
+        assert gs["X_sha256"] == hashlib.sha256(w.X.tobytes()).hexdigest()
         assert_golden(res, gs)
+        checked += 1
+    assert checked == len(by_name) > 0
 
 
 def test_config4_full_size_deterministic_across_keep_budgets():
