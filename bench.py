@@ -74,12 +74,14 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 3 times (calibrates the HBM byte counters)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--phi", action="store_true",
+                    help="rectangular=False (the phi density, the wrappers' default) on the same workload")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     return ap.parse_args()
 
 
-def golden_parity(cfg, res):
-    path = os.path.join(ROOT, "tests", "golden", f"config{cfg}.json")
+def golden_parity(cfg, res, phi=False):
+    path = os.path.join(ROOT, "tests", "golden", f"config{cfg}{'_phi' if phi else ''}.json")
     if not os.path.exists(path):
         return None
     import numpy as np
@@ -152,6 +154,9 @@ def main():
     if args.real:
         return bench_real(args, mode)
     w = datagen.config(args.config)
+    if args.phi:
+        import dataclasses
+        w = dataclasses.replace(w, rectangular=False)
     X = torch.from_numpy(w.X).to("cuda:0")
     torch.cuda.synchronize()
 
@@ -181,7 +186,7 @@ def main():
     _lib.profile(False)
     ms_step = elapsed / args.steps * 1e3
     say(f"timed: {args.steps} runs, {ms_step:.3f} ms/run")
-    parity = golden_parity(args.config, res)  # the last timed run against the oracle's digests
+    parity = golden_parity(args.config, res, args.phi)  # the last timed run against the oracle's digests
     say(f"parity of the last timed run: {parity}")
 
     # the same runs without the events, to show their overhead
@@ -209,7 +214,7 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 6),
-        "traffic": traffic_from_profile(kernel, args.config),
+        "traffic": None if args.phi else traffic_from_profile(kernel, args.config),
         "algorithmic_bytes_per_launch": round(kbytes / max(launches, 1), 1),
         "avg_launch_us": round(avg_launch_ms * 1e3, 3),
         "launches_per_step": launches // max(args.steps, 1),

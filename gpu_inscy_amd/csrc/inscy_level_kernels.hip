@@ -449,134 +449,10 @@ __device__ __forceinline__ bool walk_more(const QueryWalk &W, bool phi, int foun
     return found < W.B.cap;
 }
 
-// k = K dimensions, every checked one packed.  The walk has no branches
-// around its loads: every lane loads the packed cells and the K coordinates
-// of its candidates at once (a lane past the end re-reads the last one and
-// is masked), so one memory round trip serves a step, and the full sum
-// compared once equals the early-exit test (the partial sums never
-// decrease).  The run bounds stay in registers while the lanes move through
-// a run.  The rotated candidate order is walked as two segments, [rot,
-// total) from the query's own run, then [0, rot), so no candidate needs a
-// wrap-around test.
-template <int K, bool PHI>
-__device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double &dsum, bool &more, long long &seen) {
-    constexpr int LPQ = kBLanes, U = 4, STEP = LPQ * U;
-    const BatchSub &B = W.B;
-    const int *q = W.q;
-    const int sub = __lane_id() % LPQ;
-    const int total = q[9 + q[19]];
-    const int rot = walk_rot(W, total);
-    const float *col[K];
-    float qv[K];
-#pragma unroll
-    for (int t = 0; t < K; ++t) {
-        col[t] = W.Ys + (t == B.pos_m ? B.mcoord : B.coord + (long long) t * W.n);
-        qv[t] = W.qy[t * QPB];
-    }
-    const bool packed = B.nchk4 > 0;
-    for (int segm = 0; segm < 2; ++segm) {
-        // segment 0 starts in the own run (which holds rot), segment 1 in run 0
-        const int cbeg = segm == 0 ? rot : 0, cend = segm == 0 ? total : rot;
-        int r = segm == 0 ? q[20] : 0;
-        int rend = q[10 + r], roff = q[r] - q[9 + r];
-        for (int c0 = cbeg; c0 < cend && more; c0 += STEP) {
-            int jj[U];
-            bool valid[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int c = c0 + u * LPQ + sub;
-                valid[u] = c < cend;
-                const int cr = min(c, cend - 1);
-                while (cr >= rend) {
-                    ++r;
-                    rend = q[10 + r];
-                    roff = q[r] - q[9 + r];
-                }
-                jj[u] = cr + roff;
-            }
-            float y[U][K];
-            bool near[U];
-            if (packed) {
-                // the packed cells first; the coordinates only of candidates
-                // within one cell on every checked dimension (the vector memory
-                // pipeline, not the ALUs, bounds this walk, and most candidates
-                // of a subspace with checked dimensions fail the cell test)
-                uint64_t w[U];
-#pragma unroll
-                for (int u = 0; u < U; ++u) w[u] = W.sigb[(uint32_t) jj[u]];
-                // every cell test before the first coordinate load: the mask is
-                // pinned in a register here, so one wait serves the four words
-                // and the loads of the four candidates are all in flight at once
-                int nm = 0;
-#pragma unroll
-                for (int u = 0; u < U; ++u) nm |= (valid[u] & cells_near_packed(w[u], W.qsig)) ? 1 << u : 0;
-                asm volatile("" : "+v"(nm));
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    near[u] = (nm >> u) & 1;
-                    // 32-bit byte offsets into the float columns (a subspace's
-                    // columns hold n < 2^30 entries, so 4 jj < 2^32)
-                    const uint32_t at4 = (uint32_t) jj[u] << 2;
-                    if (near[u]) {
-#pragma unroll
-                        for (int t = 0; t < K; ++t)
-                            y[u][t] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(col[t]) + at4);
-                    } else {
-#pragma unroll
-                        for (int t = 0; t < K; ++t) y[u][t] = qv[t];
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint32_t at4 = (uint32_t) jj[u] << 2;
-                    near[u] = valid[u];
-#pragma unroll
-                    for (int t = 0; t < K; ++t)
-                        y[u][t] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(col[t]) + at4);
-                }
-            }
-            bool hit[U];
-            float accs[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                float acc = 0.0f;
-#pragma unroll
-                for (int t = 0; t < K; ++t) {
-                    const float diff = qv[t] - y[u][t];
-                    const float sq = diff * diff;
-                    acc = acc + sq;
-                }
-                // every term evaluated (no short cut): the test compiles without branches
-                hit[u] = near[u] & (jj[u] != W.i) & (acc <= W.s_star);
-                accs[u] = acc;
-            }
-            if (PHI) {
-                walk_tally<U, PHI>(W, jj, hit, accs, found, dsum);
-            } else {
-                // the query's hits of the step summed over its 4 lanes (one quad) with
-                // two DPP moves; the ballots only while the two sample edges are open
-                int h = 0;
-#pragma unroll
-                for (int u = 0; u < U; ++u) h += hit[u] ? 1 : 0;
-                h += __builtin_amdgcn_update_dpp(0, h, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-                h += __builtin_amdgcn_update_dpp(0, h, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-                if (found < kSampleEdges && h > 0) {
-                    int f = found;
-                    walk_tally<U, false>(W, jj, hit, accs, f, dsum);
-                }
-                found += h;
-            }
-            seen += min(STEP, cend - c0);
-            more = walk_more(W, PHI, found, dsum, total);
-        }
-    }
-}
-
-// Counts (not phi) with k = K: the walk of walk_fast, with each lane taking
-// four consecutive candidates of one run per step, so a step loads each
-// coordinate column with one 16-byte load per lane (and the packed cells
-// with two) instead of four 4-byte loads.  The vector memory pipeline bounds
+// k = K dimensions, every checked one packed (the host checks), counts or
+// phi.  Each lane takes four consecutive candidates of one run per step, so a
+// step loads each coordinate column with one 16-byte load per lane (and the
+// packed cells with two) instead of four 4-byte loads.  The vector memory pipeline bounds
 // these walks (its address and data units are busy nearly all the time),
 // and its cost grows with the number of load instructions.  The runs are
 // concatenated in groups of four (q[21..30]); the last group of a run masks
@@ -584,8 +460,8 @@ __device__ __forceinline__ void walk_fast(const QueryWalk &W, int &found, double
 // run: the batch's coordinate and packed-cell buffers end with padding) but
 // never counted.  A lane's first group position is 4-byte aligned only;
 // dword-aligned multi-dword global loads are supported on gfx950.
-template <int K>
-__device__ __forceinline__ void walk_vec(const QueryWalk &W, int &found, bool &more, long long &seen) {
+template <int K, bool PHI>
+__device__ __forceinline__ void walk_vec(const QueryWalk &W, int &found, double &dsum, bool &more, long long &seen) {
     typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
     typedef unsigned long long u2u __attribute__((ext_vector_type(2), aligned(8)));
     constexpr int LPQ = kBLanes, U = 4;
@@ -593,7 +469,7 @@ __device__ __forceinline__ void walk_vec(const QueryWalk &W, int &found, bool &m
     const int *q = W.q;
     const int sub = __lane_id() % LPQ;
     const int nr = q[19];
-    const int gtotal = q[21 + nr];
+    const int gtotal = q[21 + nr], total = q[9 + nr];
     // start at a pseudo-random group of the query's own run (spreads the
     // sample edges), walk to the end, then from the start to there
     const int own = q[20];
@@ -644,16 +520,28 @@ __device__ __forceinline__ void walk_vec(const QueryWalk &W, int &found, bool &m
                 hit[u] = (u < nv) & (jj[u] != W.i) & cells_near_packed(w[u], W.qsig) & (acc <= W.s_star);
                 accs[u] = acc;
             }
-            // the query's hits and examined candidates of the step, summed over
-            // its quad with two DPP moves each
+            // the query's examined candidates of the step, summed over its
+            // quad with two DPP moves
+            int ex = nv;
+            ex += __builtin_amdgcn_update_dpp(0, ex, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+            ex += __builtin_amdgcn_update_dpp(0, ex, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+            // the query's hits, summed over the quad with two DPP moves (phi:
+            // and the step's phi terms); the ballots of walk_tally only while
+            // the two sample edges are open
             int h = 0;
 #pragma unroll
             for (int u = 0; u < U; ++u) h += hit[u] ? 1 : 0;
-            int ex = nv;
-            h += __builtin_amdgcn_update_dpp(0, h, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-            h += __builtin_amdgcn_update_dpp(0, h, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-            ex += __builtin_amdgcn_update_dpp(0, ex, 0xB1, 0xF, 0xF, false);
-            ex += __builtin_amdgcn_update_dpp(0, ex, 0x4E, 0xF, 0xF, false);
+            h += __builtin_amdgcn_update_dpp(0, h, 0xB1, 0xF, 0xF, false);
+            h += __builtin_amdgcn_update_dpp(0, h, 0x4E, 0xF, 0xF, false);
+            if (PHI) {
+                double part = 0.0;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (hit[u]) part += phi_term(accs[u], W.eps);
+                part += __shfl_xor(part, 1);
+                part += __shfl_xor(part, 2);
+                dsum += part;
+            }
             if (found < kSampleEdges && h > 0) {
                 double unused = 0.0;
                 int f = found;
@@ -661,7 +549,7 @@ __device__ __forceinline__ void walk_vec(const QueryWalk &W, int &found, bool &m
             }
             found += h;
             seen += ex;
-            more = found < B.cap;
+            more = walk_more(W, PHI, found, dsum, total);
         }
     }
 }
@@ -857,10 +745,8 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
                           nb + kSampleEdges * (B.base + p)};
         // K > 0: every subspace of the launch has k = K with its checked
         // dimensions packed (the host checks), so the walk is compiled for it
-        if constexpr (K > 0 && !PHI)
-            walk_vec<K>(W, found, more, seen);
-        else if constexpr (K > 0)
-            walk_fast<K, PHI>(W, found, dsum, more, seen);
+        if constexpr (K > 0)
+            walk_vec<K, PHI>(W, found, dsum, more, seen);
         else if (k <= kFastDims && B.restmask == 0)
             walk_fast_rt<PHI>(W, found, dsum, more, seen);
         else
