@@ -825,13 +825,14 @@ void Engine::set_bounds(uint64_t s, BatchSub &x) {
     int lo = std::min(weak_thr, dense_thr[k]);
     x.nbound = 0;
     x.lo = lo;
-    if (phi_mode) return;  // decision bits are no counts
     for (int t = 0; t < k && x.nbound < kMaxBounds; ++t) {
         const uint64_t sub = s & ~(1ull << x.S.dims[t]);
         auto it = subs.find(sub);
         if (sub && it != subs.end() && it->second.entry >= 0) x.bound[x.nbound++] = it->second.cnt;
     }
-    if (x.nbound > 0) lo = std::min(lo, cap_of(k - 1));
+    // counts are exact below the cap of |S| - 1 (phi: the stored values carry
+    // an upper bound, used in the kernel)
+    if (x.nbound > 0 && !phi_mode) lo = std::min(lo, cap_of(k - 1));
     x.lo = lo;
 }
 

@@ -330,6 +330,21 @@ constexpr double kPhiRelDefault = 1.0 / 4194304.0;
 // density tests on a stored value: counts (bit == 0) or phi decision bits
 __device__ __forceinline__ bool meets(int v, int thr, int bit) { return bit ? (v >= 0 && (v & bit) != 0) : v >= thr; }
 
+// Stored value of phi mode: (q << 2) | bits, bits the decisions (1
+// weak-dense, 2 dense) and q = ceil(4 U) for an upper bound U on the
+// reference's phi of the point in S and in every superset of S (q =
+// kPhiNoBound: none known).  phi is monotone like the counts: a superset of S
+// has fewer neighbours, each term no larger (the float distance sums over
+// more dimensions, every rounding monotone), so T, the exact sum of the
+// float terms, only falls; the float leaf-order sum of N terms lies within a
+// relative N 2^-22 of T (the band of the passes), hence U = T_upper (1 + N
+// 2^-22) with N the neighbours in S bounds phi in every superset.
+constexpr int kPhiNoBound = (1 << 29) - 1;
+__device__ __forceinline__ int phi_code(int bits, double ub) {
+    const int q = ub >= 0.0 && ub < (double) (kPhiNoBound - 1) * 0.25 ? (int) ceil(ub * 4.0) : kPhiNoBound;
+    return (q << 2) | bits;
+}
+
 // LDS of the neighbour passes: per query k coordinates, k cells, its packed
 // cells (2 words) and its runs
 __device__ __forceinline__ void lds_layout(char *smem, int k, float *&sy, int *&se, uint64_t *&qs, int *&qr) {
@@ -715,15 +730,16 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
     active = active && cnt[p] < 0;
     // N_S(p) is a subset of N_S'(p) for S' a subset of S: the smallest
     // complete count of the subspaces S minus one dimension, when below both
-    // thresholds of S, decides p without a scan (and stays an upper bound)
+    // thresholds of S, decides p without a scan (and stays an upper bound);
+    // phi: the smallest stored upper bound, below both phi thresholds
     if (active && B.nbound > 0) {
         int bb = INT_MAX;
         for (int t = 0; t < B.nbound; ++t) {
             const int v = B.bound[t][p];
-            if (v >= 0) bb = min(bb, v);
+            if (v >= 0) bb = min(bb, PHI ? v >> 2 : v);
         }
-        if (bb < B.lo) {
-            if (sub == 0) cnt[p] = bb;
+        if (PHI ? bb < kPhiNoBound && (double) bb * 0.25 < (double) fminf(B.phi_w, B.phi_d) : bb < B.lo) {
+            if (sub == 0) cnt[p] = PHI ? bb << 2 : bb;
             active = false;
         }
     }
@@ -758,6 +774,7 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
             const double hi = dsum * (1.0 + (double) found * phirel);
             int code = 0;
             bool unsure = false;
+            double ub = -1.0;  // stopped early: no bound
             if (!more) {
                 code = 3;
             } else {
@@ -766,9 +783,12 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
                 else if (!(hi < tw)) unsure = true;
                 if (lo >= td) code |= 2;
                 else if (!(hi < td)) unsure = true;
+                // hi bounds T, the exact sum of the terms (the double scan-order
+                // sum is within N 2^-52 of it)
+                ub = hi * (1.0 + (double) found * phirel);
             }
             if (sub == 0) {
-                cnt[p] = code;
+                cnt[p] = phi_code(code, ub);
                 if (unsure) {
                     const int at = atomicAdd(nfb, 1);
                     if (at < fbcap) fb[at] = make_int2(b, p);
@@ -1241,6 +1261,7 @@ __global__ __launch_bounds__(kBlock) void k_b_phi_exact(DevData D, const BatchSu
         }
         __syncthreads();
         float sum = 0.0f;
+        long long nterm = 0;
         for (int base = 0; base < n; base += kBlock) {
             const int rk = base + (int) threadIdx.x;
             bool hit = false;
@@ -1278,9 +1299,15 @@ __global__ __launch_bounds__(kBlock) void k_b_phi_exact(DevData D, const BatchSu
             __syncthreads();
             if (threadIdx.x == 0)
                 for (int t = 0; t < tot; ++t) sum = (float) ((double) sum + terms[t]);
+            nterm += tot;
             __syncthreads();
         }
-        if (threadIdx.x == 0) B.cnt[p] = (sum >= B.phi_w ? 1 : 0) | (sum >= B.phi_d ? 2 : 0);
+        // the float sum bounds T within N 2^-22 (so T <= sum (1 + 2 N 2^-22)
+        // while N 2^-22 <= 1/2), and the stored bound adds N 2^-22 once more
+        const double rel = (double) nterm * (1.0 / 4194304.0);
+        if (threadIdx.x == 0)
+            B.cnt[p] = phi_code((sum >= B.phi_w ? 1 : 0) | (sum >= B.phi_d ? 2 : 0),
+                                rel <= 0.25 ? (double) sum * (1.0 + 4.0 * rel) : -1.0);
         __syncthreads();
     }
 }
