@@ -120,6 +120,32 @@ __device__ __forceinline__ void stage_query(const SubspaceArg &S, int n, const f
     }
 }
 
+// p[0, count) = v with 16-byte stores over the aligned middle (grid-stride;
+// the ends one element at a time)
+template <class T>
+__device__ __forceinline__ void fill_span(T *p, T v, size_t count) {
+    static_assert(16 % sizeof(T) == 0, "element size divides 16");
+    constexpr size_t per = 16 / sizeof(T);
+    size_t head = (per - ((size_t) reinterpret_cast<uintptr_t>(p) / sizeof(T)) % per) % per;
+    if (head > count) head = count;
+    const size_t nvec = (count - head) / per, tail = head + nvec * per;
+    union {
+        T e[per];
+        uint4 u;
+    } w;
+#pragma unroll
+    for (size_t k = 0; k < per; ++k) w.e[k] = v;
+    const size_t t0 = (size_t) blockIdx.x * blockDim.x + threadIdx.x, step = (size_t) gridDim.x * blockDim.x;
+    uint4 *q = reinterpret_cast<uint4 *>(p + head);
+    for (size_t j = t0; j < nvec; j += step) q[j] = w.u;
+    if (t0 < head) p[t0] = v;
+    if (tail + t0 < count) p[tail + t0] = v;
+}
+inline unsigned fill_blocks(size_t count, size_t elem) {
+    const size_t vec = (count * elem + 15) / 16 + 2;
+    return (unsigned) std::max<size_t>(1, std::min<size_t>((vec + 255) / 256, 4096));
+}
+
 // Union-find over parents that only move towards smaller ids (par[x] <= x,
 // x a root iff par[x] == x; unions link the larger root under the smaller
 // with a device-scope compare-and-swap).  Every value ever held by par[x] is

@@ -466,8 +466,18 @@ private:
     int *pin_i(size_t off = 0) { return reinterpret_cast<int *>(pin) + off; }
 
     void sync() {
+        flush_uploads();
         HIP_OK(hipStreamSynchronize(st));
         up_top = 0;  // every staged upload has been consumed
+    }
+    // uploads between defer_uploads() and flush_uploads() whose staging and
+    // device places are contiguous (consecutive takes of one pool) go to the
+    // device in one copy; flush before the launch that reads them
+    void defer_uploads() { up_defer = true; }
+    void flush_uploads() {
+        if (pend_bytes) HIP_OK(hipMemcpyAsync(pend_dst, up.p + pend_src, pend_bytes, hipMemcpyHostToDevice, st));
+        pend_bytes = 0;
+        up_defer = false;
     }
 
     // next event pair when profiling, else an empty timer
@@ -958,6 +968,9 @@ private:
     HostBuf up, down;
     HostBuf emit_stage[2];
     size_t up_top = 0;
+    bool up_defer = false;                          // defer_uploads()
+    char *pend_dst = nullptr;                       // the deferred copy: device place,
+    size_t pend_src = 0, pend_bytes = 0;            // staging offset and bytes
     size_t batch_budget = 0;
     bool phi_mode = false;          // rectangular = false
     float phi_weak = 0.f;

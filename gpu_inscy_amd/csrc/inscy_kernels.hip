@@ -652,8 +652,7 @@ __global__ void k_emit_counts(const int *__restrict__ idx, int n, int r, int *__
 
 template <class T>
 __global__ void k_fill(T *p, T v, size_t count) {
-    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) p[i] = v;
+    fill_span(p, v, count);
 }
 
 __global__ void k_gather(const int *__restrict__ src, const int *__restrict__ idx, int count, int *__restrict__ out) {
@@ -722,14 +721,14 @@ hipError_t launch_discretise(const float *X, int n, int d, int cells, float eps,
 
 hipError_t launch_hist1(const uint16_t *cellE, int n, int d, int ecells, int *hist, hipStream_t st) {
     size_t nd = (size_t) n * d;
-    LAUNCH(k_fill<int>, dim3(blocks_for((size_t) d * ecells)), dim3(kBlock), 0, st, hist, 0, (size_t) d * ecells);
+    LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) d * ecells, sizeof(int))), dim3(kBlock), 0, st, hist, 0, (size_t) d * ecells);
     LAUNCH(k_hist1, dim3(blocks_for(nd)), dim3(kBlock), 0, st, cellE, n, d, ecells, hist);
     return hipGetLastError();
 }
 
 hipError_t launch_histc(const uint8_t *cellC, int n, int d, int cells, int *hist, hipStream_t st) {
     size_t nd = (size_t) n * d;
-    LAUNCH(k_fill<int>, dim3(blocks_for((size_t) d * cells)), dim3(kBlock), 0, st, hist, 0, (size_t) d * cells);
+    LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) d * cells, sizeof(int))), dim3(kBlock), 0, st, hist, 0, (size_t) d * cells);
     LAUNCH(k_histc, dim3(blocks_for(nd)), dim3(kBlock), 0, st, cellC, n, d, cells, hist);
     return hipGetLastError();
 }
@@ -740,7 +739,7 @@ static hipError_t build_grid(const DevData &D, const SubspaceArg &S, const GridA
     const int n = D.n;
     int ncell = 1;
     for (int a = 0; a < G.j; ++a) ncell *= G.C;
-    LAUNCH(k_fill<int>, dim3(blocks_for(ncell + 1)), dim3(kBlock), 0, st, B.hist, 0, (size_t) ncell + 1);
+    LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) ncell + 1, sizeof(int))), dim3(kBlock), 0, st, B.hist, 0, (size_t) ncell + 1);
     LAUNCH(k_grid_keys, dim3(blocks_for(n)), dim3(kBlock), 0, st, D, S, G, B.keys, B.rank, B.hist);
     hipError_t e = launch_exclusive_sum(B.scan_temp, B.scan_bytes, B.hist, B.off, ncell + 1, st);
     if (e != hipSuccess) return e;
@@ -806,7 +805,7 @@ hipError_t launch_partition(const int *P, const int *nP, int nP_max, const int *
                             const int *weakcnt, int thr, int m, int *cnt, int *outP, int *outW, int *outx,
                             uint64_t *outm, hipStream_t st) {
     int bp = (int) blocks_for(nP_max), bs = (int) blocks_for(nsp_max);
-    LAUNCH(k_fill<int>, dim3(blocks_for((size_t) 6 * G)), dim3(kBlock), 0, st, cnt, 0, (size_t) 6 * G);
+    LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) 6 * G, sizeof(int))), dim3(kBlock), 0, st, cnt, 0, (size_t) 6 * G);
     if (bp + bs == 0) return hipGetLastError();
     PartArg A{P, nP, spx, spm, nsp, cellC_m, gid, weakcnt, thr, m, G, bp};
     LAUNCH(k_part_count, dim3(bp + bs), dim3(kBlock), 0, st, A, cnt);
@@ -818,10 +817,10 @@ hipError_t launch_path_prune(const DevData &D, uint64_t smask, const int *W, con
                              const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, int *stamp, int tid,
                              unsigned long long *bits, uint64_t *table, uint64_t tmask, int *outx, uint64_t *outm,
                              int *nout, hipStream_t st) {
-    LAUNCH(k_fill<int>, dim3(1), dim3(kBlock), 0, st, nout, 0, (size_t) 1);
+    LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) 1, sizeof(int))), dim3(kBlock), 0, st, nout, 0, (size_t) 1);
     if (nsp_max <= 0) return hipGetLastError();
-    LAUNCH(k_fill<uint64_t>, dim3(blocks_for(tmask + 1)), dim3(kBlock), 0, st, table, kEmpty, (size_t) (tmask + 1));
-    LAUNCH(k_fill<unsigned long long>, dim3(1), dim3(kBlock), 0, st, bits, 0ull, (size_t) 1);
+    LAUNCH(k_fill<uint64_t>, dim3(fill_blocks((size_t) (tmask + 1), sizeof(uint64_t))), dim3(kBlock), 0, st, table, kEmpty, (size_t) (tmask + 1));
+    LAUNCH(k_fill<unsigned long long>, dim3(fill_blocks((size_t) 1, sizeof(unsigned long long))), dim3(kBlock), 0, st, bits, 0ull, (size_t) 1);
     if (nW_max > 0) LAUNCH(k_stamp, dim3(blocks_for(nW_max)), dim3(kBlock), 0, st, W, nW, stamp, tid);
     LAUNCH(k_path_bits, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, spx, spm, nsp, stamp, tid, bits);
     if (nW_max > 0)
@@ -862,7 +861,7 @@ hipError_t launch_emit(const EmitRow *rows, int r, int n, int *flags, int *idx, 
                        void *scan_temp, size_t scan_bytes, hipStream_t st) {
     if (r <= 0) return hipSuccess;
     const size_t items = (size_t) r * n + 1;
-    LAUNCH(k_fill<int>, dim3(1), dim3(kBlock), 0, st, flags + (size_t) r * n, 0, (size_t) 1);
+    LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) 1, sizeof(int))), dim3(kBlock), 0, st, flags + (size_t) r * n, 0, (size_t) 1);
     LAUNCH(k_emit_roots, dim3(blocks_for(n), r), dim3(kBlock), 0, st, rows, n, flags);
     hipError_t e = launch_exclusive_sum(scan_temp, scan_bytes, flags, idx, (int) items, st);
     if (e != hipSuccess) return e;
@@ -873,19 +872,19 @@ hipError_t launch_emit(const EmitRow *rows, int r, int n, int *flags, int *idx, 
 
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t count, hipStream_t st) {
     if (!count) return hipSuccess;
-    LAUNCH(k_fill<uint8_t>, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, v, count);
+    LAUNCH(k_fill<uint8_t>, dim3(fill_blocks(count, sizeof(uint8_t))), dim3(kBlock), 0, st, p, v, count);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t count, hipStream_t st) {
     if (!count) return hipSuccess;
-    LAUNCH(k_fill<uint64_t>, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, v, count);
+    LAUNCH(k_fill<uint64_t>, dim3(fill_blocks(count, sizeof(uint64_t))), dim3(kBlock), 0, st, p, v, count);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_i32(int *p, int v, size_t count, hipStream_t st) {
     if (!count) return hipSuccess;
-    LAUNCH(k_fill<int>, dim3(blocks_for(count)), dim3(kBlock), 0, st, p, v, count);
+    LAUNCH(k_fill<int>, dim3(fill_blocks(count, sizeof(int))), dim3(kBlock), 0, st, p, v, count);
     return hipGetLastError();
 }
 
@@ -933,7 +932,7 @@ hipError_t launch_rank_order(const uint16_t *cellE, int n, int d, int ecells, co
 }
 
 hipError_t launch_root_paths(const uint64_t *nearm, int n, int *outx, uint64_t *outm, int *nout, hipStream_t st) {
-    LAUNCH(k_fill<int>, dim3(1), dim3(kBlock), 0, st, nout, 0, (size_t) 1);
+    LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) 1, sizeof(int))), dim3(kBlock), 0, st, nout, 0, (size_t) 1);
     LAUNCH(k_root_paths, dim3(blocks_for(n)), dim3(kBlock), 0, st, nearm, n, outx, outm, nout);
     return hipGetLastError();
 }

@@ -33,12 +33,28 @@ void Engine::put(void *dst, const void *src, size_t bytes) {
     if (!bytes) return;
     const size_t need = (bytes + 255) & ~(size_t) 255;
     if (up_top + need > up.cap) {
-        if (up_top > 0) sync();  // staged copies still read the buffer
+        const bool defer = up_defer;
+        if (up_top > 0) sync();  // staged copies still read the buffer (sync issues a deferred one)
+        up_defer = defer;
         up_top = 0;
         if (need > up.cap) up.get(std::max(need, 2 * up.cap));
     }
     std::memcpy(up.p + up_top, src, bytes);
-    HIP_OK(hipMemcpyAsync(dst, up.p + up_top, bytes, hipMemcpyHostToDevice, st));
+    if (up_defer) {
+        // pool takes and staging places are both rounded to 256 bytes, so a
+        // contiguous pair extends the pending copy (padding bytes included:
+        // they lie inside the take)
+        if (pend_bytes && static_cast<char *>(dst) == pend_dst + pend_bytes && up_top == pend_src + pend_bytes) {
+            pend_bytes += need;
+        } else {
+            if (pend_bytes) HIP_OK(hipMemcpyAsync(pend_dst, up.p + pend_src, pend_bytes, hipMemcpyHostToDevice, st));
+            pend_dst = static_cast<char *>(dst);
+            pend_src = up_top;
+            pend_bytes = need;
+        }
+    } else {
+        HIP_OK(hipMemcpyAsync(dst, up.p + up_top, bytes, hipMemcpyHostToDevice, st));
+    }
     up_top += need;
 }
 
@@ -183,8 +199,10 @@ void Engine::expand_level(const std::vector<int> &frontier, const int *counts) {
     }
     int *flags = cur.take<int>(ftot);
     HIP_OK(launch_fill_i32(flags, 0, ftot, st));
+    defer_uploads();
     FlagTree *dft = upload(cur, ft);
     int *dfb = upload(cur, fblk);
+    flush_uploads();
     HIP_OK(launch_b_flags(dft, dfb, na, fblk[na], cellC, n, cells, flags, st));
     const int nf = (int) frontier.size();
     char *h = down.get((size_t) (ftot + nf) * sizeof(int));
@@ -541,8 +559,10 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
             qblk.push_back(qblk.back() + nblocks(t.nW));
         }
     }
+    defer_uploads();
     QuerySeg *dqs = upload(batch, qs);
     int *dqb = upload(batch, qblk);
+    flush_uploads();
     HIP_OK(launch_b_queries(dsubs, B, n, dqs, dqb, (int) qs.size(), qblk.back(), X, st));
     std::vector<int> cblk(B + 1, 0);
     for (int b = 0; b < B; ++b) cblk[b + 1] = cblk[b] + (hs[b].qmax + QPB - 1) / QPB;
@@ -603,8 +623,10 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         pp.bp = nblocks(t.nW);
         pblk[j + 1] = pblk[j] + pp.bp + nblocks(t.nsp);
     }
+    defer_uploads();
     PartPair *dpairs = upload(batch, pairs);
     int *dpb = upload(batch, pblk);
+    flush_uploads();
     HIP_OK(launch_b_partition(dpairs, dpb, np, pblk[np], cellC, n, weak_thr, phi_mode ? 1 : 0, prm.min_size, st));
     char *h = down.get((size_t) std::max(pc_total, 1) * sizeof(int));
     HIP_OK(hipMemcpyAsync(h, pcnt, (size_t) std::max(pc_total, 1) * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -716,9 +738,11 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
             bw[c + 1] = bw[c] + nblocks(prune[c].nW);
             bp[c + 1] = bp[c] + nblocks(prune[c].nsp);
         }
+        defer_uploads();
         PruneChild *dpc = upload(batch, prune);
         int *dbw = upload(batch, bw);
         int *dbp = upload(batch, bp);
+        flush_uploads();
         HIP_OK(launch_b_prune(D, dpc, nc, dbw, bw[nc], dbp, bp[nc], st));
     }
 
@@ -765,15 +789,19 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     if (!copies.empty()) {
         std::vector<int> cb(copies.size() + 1, 0);
         for (size_t c = 0; c < copies.size(); ++c) cb[c + 1] = cb[c] + nblocks(copies[c].len);
+        defer_uploads();
         CopySeg *dcs = upload(batch, copies);
         int *dcb2 = upload(batch, cb);
+        flush_uploads();
         HIP_OK(launch_b_copy(dcs, dcb2, (int) copies.size(), cb.back(), st));
     }
     if (!stamps.empty()) {
         std::vector<int> sbk(stamps.size() + 1, 0);
         for (size_t c = 0; c < stamps.size(); ++c) sbk[c + 1] = sbk[c] + nblocks(stamps[c].len);
+        defer_uploads();
         CopySeg *dss = upload(batch, stamps);
         int *dsb = upload(batch, sbk);
+        flush_uploads();
         HIP_OK(launch_b_stamp(dss, dsb, (int) stamps.size(), sbk.back(), st));
     }
     if (ncomp > 0) {
@@ -967,8 +995,10 @@ void Engine::replay_staged() {
                 bm.push_back(bm.back() + (x.kind != 1 ? rep_mem_blocks(n) : 0));
             }
             const int no = (int) ops.size();
+            defer_uploads();
             RepOp *dops = upload(batch, ops);
             int *dbr = upload(batch, br), *dbc = upload(batch, bc), *dbk = upload(batch, bk), *dbm = upload(batch, bm);
+            flush_uploads();
             HIP_OK(launch_rep_stage(dops, no, dbr, br.back(), dbc, bc.back(), dbk, bk.back(), dbm, bm.back(), entries,
                                     prm.r, prm.min_size, n, keepv, st));
             sync();  // the chunk's scratch and descriptors are reused by the next one
@@ -1021,8 +1051,10 @@ void Engine::score_pairs(int na) {
     float *feat = cur.take<float>((size_t) np * kMlpIn);
     HIP_OK(launch_fill_i32(acc, 0, (size_t) np * stride, st));
     HIP_OK(launch_fill_u64(reinterpret_cast<uint64_t *>(accs), 0ull, np, st));
+    defer_uploads();
     FeatPair *dfp = upload(cur, fp);
     int *dblk = upload(cur, blk);
+    flush_uploads();
     HIP_OK(launch_pair_features(dfp, dblk, np, blk[np], D.Xt, cellC, n, d, cells, prm.min_size, weak_thr, acc, accs,
                                 feat, st));
     if (collect) {
@@ -1080,8 +1112,10 @@ void Engine::finish_collect(inscy_result &out) {
     int *hit = store.take<int>(std::max(nt, 1));
     HIP_OK(launch_fill_i32(hit, 0, std::max(nt, 1), st));
     if (!th.empty()) {
+        defer_uploads();
         TreeHit *dth = upload(store, th);
         int *dblk = upload(store, blk);
+        flush_uploads();
         HIP_OK(launch_tree_hits(dth, dblk, (int) th.size(), blk.back(), hit, st));
     }
     std::vector<int> hh(nt), keep(nt);

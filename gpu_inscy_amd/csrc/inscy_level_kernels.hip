@@ -1626,8 +1626,7 @@ __global__ void k_b_entries(Entry *__restrict__ E, int at, const Entry *__restri
 
 template <class T>
 __global__ void k_b_fill(T *p, T v, size_t count) {
-    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < count) p[i] = v;
+    fill_span(p, v, count);
 }
 
 }  // namespace
@@ -1656,7 +1655,7 @@ hipError_t launch_b_index(const DevData &D, const BatchSub *subs, int nsub, cons
     // the scan over the sorted indexes' cells ends with their total, the
     // first position of the kept ones
     const int nscan = hmiss + 1;
-    LAUNCH(k_b_fill<int>, dim3(blocks_for(nscan)), dim3(kBlock), 0, st, X.hist, 0, (size_t) nscan);
+    LAUNCH(k_b_fill<int>, dim3(fill_blocks((size_t) nscan, sizeof(int))), dim3(kBlock), 0, st, X.hist, 0, (size_t) nscan);
     if (nmiss > 0)
         LAUNCH(k_b_keys, dim3(blocks_for(D.n), nmiss), dim3(kBlock), 0, st, D, subs, miss_reps, X.keys, X.rank,
                X.hist);
@@ -1786,7 +1785,7 @@ hipError_t launch_b_queries(const BatchSub *subs, int nsub, int n, const QuerySe
                             int nseg, int nblk, const BatchIndex &X, hipStream_t st) {
     if (nsub <= 0) return hipSuccess;
     const size_t nb = (size_t) nsub * n;
-    LAUNCH(k_b_fill<uint8_t>, dim3(blocks_for(nb)), dim3(kBlock), 0, st, X.qflag, (uint8_t) 0, nb);
+    LAUNCH(k_b_fill<uint8_t>, dim3(fill_blocks(nb, 1)), dim3(kBlock), 0, st, X.qflag, (uint8_t) 0, nb);
     if (nblk > 0)
         LAUNCH(k_b_qmark, dim3(nblk), dim3(kBlock), 0, st, subs, segs, blk_off, nseg, X.keys, X.qflag);
     hipError_t e = launch_flag_compact(X.qflag, nb, X.select_temp, X.select_bytes, X.qlist, X.nsel, st);
