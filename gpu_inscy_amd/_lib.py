@@ -35,7 +35,7 @@ EXPORTED = [
     "inscy_run", "inscy_run_device", "inscy_result_num_subspaces", "inscy_result_subspace",
     "inscy_result_labels", "inscy_result_num_points", "inscy_result_stats", "inscy_result_free",
     "inscy_last_error", "inscy_discretise", "inscy_neighbour_counts", "inscy_neighbour_counts_noalloc",
-    "inscy_dense_components", "inscy_profile",
+    "inscy_dense_components", "inscy_profile", "inscy_set_tuning",
     "inscy_set_pruner", "inscy_result_pairs", "inscy_mlp_forward", "inscy_mlp_gradient", "inscy_mlp_adam",
     "inscy_mlp_round",
 ]
@@ -141,6 +141,8 @@ def lib():
     L.inscy_release_memory.restype = ctypes.c_int
     L.inscy_profile.argtypes = [ctypes.c_int]
     L.inscy_profile.restype = ctypes.c_int
+    L.inscy_set_tuning.argtypes = [ctypes.c_int32, ctypes.c_int32]
+    L.inscy_set_tuning.restype = ctypes.c_int32
     fp = ctypes.POINTER(ctypes.c_float)
     L.inscy_set_pruner.argtypes = [fp, ctypes.c_float, ctypes.c_int32]
     L.inscy_set_pruner.restype = ctypes.c_int
@@ -311,6 +313,12 @@ def run_many(ptrs, shapes, params, mode: int = MODE_BATCHED, device: bool = True
 
 def profile(on: bool) -> None:
     lib().inscy_profile(int(on))
+
+
+def set_tuning(max_batch: int = 0, workers: int = 0) -> None:
+    """Child subspaces per level-synchronous batch and run_many worker streams
+    for later calls (0: the defaults).  Results do not depend on them."""
+    check(lib().inscy_set_tuning(int(max_batch), int(workers)))
 
 
 def debug_trace(on: bool) -> None:

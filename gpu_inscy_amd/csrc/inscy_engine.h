@@ -402,6 +402,7 @@ public:
     bool generic_walk = false;   // test hook: no neighbour walk compiled for its k
     long long keep_budget_override = -1;  // test hook: bytes kept across batches (< 0: the batch budget)
     int mem_share = 1;  // engines running at once on the device (each budgets 1/mem_share of the free bytes)
+    int max_batch_override = 0;  // inscy_set_tuning: child subspaces per batch (0: the default)
 
     void run(inscy_result &out) {
         int *P = arena.take<int>(n);
@@ -972,6 +973,7 @@ private:
     char *pend_dst = nullptr;                       // the deferred copy: device place,
     size_t pend_src = 0, pend_bytes = 0;            // staging offset and bytes
     size_t batch_budget = 0;
+    size_t max_batch = 0;  // child subspaces per batch
     bool phi_mode = false;          // rectangular = false
     float phi_weak = 0.f;
     std::vector<float> phi_dense;   // [k]

@@ -13,6 +13,8 @@ static bool g_phi_all_exact = false;
 static bool g_generic_walk = false;
 static long long g_keep_budget = -1;
 static bool g_profile_on = false;
+static int g_max_batch = 0;  // inscy_set_tuning: child subspaces per batch (0: the default)
+static int g_workers = 0;    // inscy_set_tuning: run_many worker streams (0: the default)
 
 static int fail(int code, const std::string &msg) {
     g_error = msg;
@@ -116,6 +118,7 @@ static int run_impl(const float *d_X, int64_t n, int32_t d, const inscy_params *
             eng.phi_all_exact = g_phi_all_exact;
             eng.generic_walk = g_generic_walk;
             eng.keep_budget_override = g_keep_budget;
+            eng.max_batch_override = g_max_batch;
             eng.profile_on = g_profile_on;
             eng.mem_share = mem_share;
             // phi density (rectangular = false) is implemented in the
@@ -162,8 +165,9 @@ static int run_many_impl(const float *const *X, const int64_t *n, const int32_t 
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
         return (double) n[a] * (double) n[a] * d[a] > (double) n[b] * (double) n[b] * d[b];
     });
-    int workers = 4;
+    int workers = 8;
     if (const char *w = std::getenv("INSCY_WORKERS")) workers = std::max(1, std::atoi(w));
+    if (g_workers > 0) workers = g_workers;
     workers = std::min<int>(workers, count);
     std::atomic<int> next{0};
     std::vector<int> rc(count, INSCY_OK);
@@ -384,6 +388,15 @@ int inscy_release_memory(void) {
 
 int inscy_profile(int on) {
     g_profile_on = on != 0;
+    return INSCY_OK;
+}
+
+int inscy_set_tuning(int32_t max_batch, int32_t workers) {
+    if (max_batch < 0 || max_batch > 65536 || workers < 0 || workers > 64)
+        return fail(INSCY_ERR_ARGUMENT, "max_batch must be in [0, 65536] and workers in [0, 64]");
+    g_max_batch = max_batch;
+    g_workers = workers;
+    g_error.clear();
     return INSCY_OK;
 }
 

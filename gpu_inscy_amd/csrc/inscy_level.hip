@@ -24,7 +24,7 @@
 namespace inscy {
 
 namespace {
-constexpr size_t kMaxBatch = 512;  // child subspaces per batch
+constexpr size_t kMaxBatch = 1024;  // child subspaces per batch (inscy_set_tuning, INSCY_MAX_BATCH override)
 constexpr uint64_t kEmptySlot = ~0ull;  // free slot of a path-prune table
 inline int nblocks(long long count) { return (int) ((count + kBlockThreads - 1) / kBlockThreads); }
 }  // namespace
@@ -70,6 +70,9 @@ void Engine::run_levels(inscy_result &out) {
     store.chunk = (size_t) 256 << 20;
     batch.chunk = (size_t) 256 << 20;
     up.get((size_t) 4 << 20);
+    max_batch = kMaxBatch;
+    if (const char *e = std::getenv("INSCY_MAX_BATCH")) max_batch = (size_t) std::max(1, std::atoi(e));
+    if (max_batch_override > 0) max_batch = (size_t) max_batch_override;
     batch_budget = (size_t) 2 << 30;
     if (const size_t free_b = device_available_bytes() / (size_t) mem_share)
         batch_budget = std::min<size_t>(std::max<size_t>(free_b / 8, (size_t) 256 << 20), (size_t) 8 << 30);
@@ -267,7 +270,7 @@ void Engine::expand_level(const std::vector<int> &frontier, const int *counts) {
         size_t j = i;
         size_t bytes = 0;
         long long hsum = 1;
-        while (j < cs.size() && j - i < kMaxBatch) {
+        while (j < cs.size() && j - i < max_batch) {
             const ChildSub &c = cs[j];
             SubspaceArg S = arg_of(c.s);
             GridArg G = grid_of(S);

@@ -144,6 +144,14 @@ void inscy_result_free(inscy_result *res);
  * reference; used by the benchmark's roofline figures). */
 int inscy_profile(int on);
 
+/* Tuning of later calls of this process (no counterpart in the reference;
+ * gpu_inscy_amd/autotune.py measures and applies it): max_batch child
+ * subspaces per level-synchronous batch (default 1024, still bounded by
+ * memory), workers streams of inscy_run_many (default 8); 0 restores a
+ * default.  INSCY_ERR_ARGUMENT outside [0, 65536] and [0, 64].  Results do
+ * not depend on either. */
+int32_t inscy_set_tuning(int32_t max_batch, int32_t workers);
+
 /* Device and pinned host blocks of finished runs are cached for reuse by
  * later runs (freeing them would synchronise the whole device); this returns
  * them to the runtime.  Call it only while no run is in progress.  No

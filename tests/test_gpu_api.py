@@ -59,3 +59,19 @@ def test_gpu_learned_wrapper():
     assert len(subs) == len(labels) > 0 and all(len(l) == w.n for l in labels)
     _lib.set_pruner(None, 0.0)
     INSCY.GPU_learned._loaded = False
+
+
+def test_autotune_settings_do_not_change_results():
+    # the tuner checks every setting's result against the first one; one
+    # child subspace per batch against the default bounds exercise the
+    # batching, and the best setting is applied
+    from gpu_inscy_amd import autotune
+    say("autotune config 0")
+    try:
+        rec = autotune.tune(0, runs=1, batches=(1, 3, 1024), workers=(0,))
+        assert [r["max_batch"] for r in rec["results"]] == [1, 3, 1024]
+        assert rec["best"]["max_batch"] in (1, 3, 1024)
+        rec3 = autotune.tune(0, phi=True, runs=1, batches=(2, 1024), workers=(0,))
+        assert len(rec3["results"]) == 2
+    finally:
+        _lib.set_tuning(0, 0)

@@ -184,3 +184,19 @@ def test_experiment_grids_follow_the_reference_harness():
     assert experiments.run_file("GPU_INSCY", 1500, 15, 4, 0.01, 1.0, 1.0, 8, 0.05, 0, 4) == \
         "GPU_INSCYn1500d15c4N_size0.01F1.0r1.0num_obj8min_size0.05order0.npz"
     assert experiments.run_file("INSCY", 1500, 15, 4, 0.01, 1.0, 1.0, 8, 0.05, 0, 10).endswith("order0cl10.npz")
+
+
+def test_tuning_rejects_bad_arguments_and_applies_saved_files(tmp_path):
+    # inscy_set_tuning touches no device: arguments are checked, 0 restores the defaults
+    import json
+    from gpu_inscy_amd import autotune
+    for mb, wk in ((-1, 0), (0, -1), (65537, 0), (0, 65)):
+        with pytest.raises(_lib.InscyError) as e:
+            _lib.set_tuning(mb, wk)
+        assert e.value.code == _lib.INSCY_ERR_ARGUMENT
+    path = tmp_path / "tuning.json"
+    path.write_text(json.dumps({"best": {"max_batch": 512, "workers": 2}, "results": []}))
+    try:
+        assert autotune.apply(str(path)) == {"max_batch": 512, "workers": 2}
+    finally:
+        _lib.set_tuning(0, 0)
