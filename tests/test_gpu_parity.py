@@ -576,3 +576,35 @@ def test_config4_full_size_deterministic_across_keep_budgets():
         assert b.subspaces == a.subspaces and np.array_equal(b.labels, a.labels)
         assert b.stats["trees"] == a.stats["trees"] and b.stats["calls"] == a.stats["calls"]
         del b
+
+
+def degenerate_cases():
+    # inputs at the edges of the algorithm: one point, one dimension,
+    # duplicates only, a neighbourhood as wide as the data, one c-grid cell,
+    # min_size above n, thresholds no point meets, r = 0
+    rng = np.random.default_rng(7)
+    X = rng.random((80, 3)).astype(np.float32)
+    X[:30] = 0.2 + 0.01 * rng.random((30, 3)).astype(np.float32)
+    X1 = np.concatenate([0.3 + 0.02 * rng.random((40, 1)), rng.random((20, 1))]).astype(np.float32)
+    return {
+        "n1_d1": (np.array([[0.5]], np.float32), 0.05, 1.0, 1, 1, 1.0, 3),
+        "n2_same_d3": (np.full((2, 3), 0.25, np.float32), 0.05, 1.0, 1, 1, 1.0, 3),
+        "dups50_d2": (np.full((50, 2), 0.5, np.float32), 0.05, 1.0, 2, 5, 1.0, 4),
+        "eps1_d3": (X, 1.0, 1.0, 2, 5, 1.0, 3),
+        "cells1_d3": (X, 0.05, 1.0, 2, 5, 1.0, 1),
+        "min_size_above_n": (X, 0.05, 1.0, 2, 500, 1.0, 3),
+        "F100": (X, 0.05, 100.0, 2, 5, 1.0, 3),
+        "r0": (X, 0.05, 1.0, 2, 5, 0.0, 3),
+        "d1_cluster": (X1, 0.02, 1.0, 2, 5, 1.0, 3),
+    }
+
+
+@pytest.mark.parametrize("name", sorted(degenerate_cases()))
+@pytest.mark.parametrize("rect", [True, False])
+def test_degenerate_inputs_match_oracle(name, rect):
+    X, eps, F, num_obj, min_size, r, c = degenerate_cases()[name]
+    w = datagen.Workload(name, X, np.full(len(X), -1, np.int32), [], eps=eps, cells=c, F=F, num_obj=num_obj,
+                         min_size=min_size, r=r, rectangular=rect)
+    say("degenerate", name, rect)
+    for mode in (_lib.MODE_EXACT, _lib.MODE_BATCHED):
+        compare_with_oracle(w, mode)
