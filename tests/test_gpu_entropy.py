@@ -107,3 +107,15 @@ def test_config1_with_entropy_order_matches_golden(order, name):
         digests = [hashlib.sha256(np.ascontiguousarray(l, dtype=np.int32).tobytes()).hexdigest() for l in res.labels]
         assert digests == [x[3] for x in g["subspaces"]]
         assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
+
+
+@pytest.mark.parametrize("order", [1, -1])
+def test_degenerate_inputs_with_entropy_order(order):
+    # the inputs of test_gpu_parity's degenerate cases with the permuted level order
+    from tests.test_gpu_parity import degenerate_cases
+    for name, (X, eps, F, num_obj, min_size, r, c) in sorted(degenerate_cases().items()):
+        w = datagen.Workload(name, X, np.full(len(X), -1, np.int32), [], eps=eps, cells=c, F=F, num_obj=num_obj,
+                             min_size=min_size, r=r, rectangular=True)
+        say("degenerate entropy", name, order)
+        check(w, order, _lib.MODE_EXACT)
+        check(w, order, _lib.MODE_BATCHED)
