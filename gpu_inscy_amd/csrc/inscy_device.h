@@ -120,6 +120,12 @@ __device__ __forceinline__ void stage_query(const SubspaceArg &S, int n, const f
     }
 }
 
+// the ceil(1/eps)-grid cell of a coordinate, exactly as k_discretise computes
+// it (correctly rounded 1/ecells and quotient): cells recomputed from a
+// gathered coordinate equal the stored ones
+__device__ __forceinline__ float ecell_width(int ecells) { return __fdiv_rn(1.0f, (float) ecells); }
+__device__ __forceinline__ int ecell_of(float x, float ecs, int ecells) { return min((int) __fdiv_rn(x, ecs), ecells - 1); }
+
 // p[0, count) = v with 16-byte stores over the aligned middle (grid-stride;
 // the ends one element at a time)
 template <class T>

@@ -97,6 +97,8 @@ __global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const i
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int p = order[B.obase + i];
+    const float ecs = ecell_width(D.ecells);
+    uint64_t w = 0;
     for (int t = 0; t < B.S.k; ++t) {
         if (t == B.pos_m) continue;
         const int s = B.S.dims[t];
@@ -110,11 +112,17 @@ __global__ void k_b_gather(DevData D, const BatchSub *__restrict__ subs, const i
             if (dst) dst[i] = v;
         }
         Ys[B.coord + (long long) t * n + i] = v;
-        if (need_es) Es[B.coord + (long long) t * n + i] = D.cellE[(size_t) s * n + p];
+        // the cells from the coordinate itself (k_discretise's formula on the
+        // same value): no scattered read of the cell array
+        int f = -1;
+        for (int u = 0; u < B.nchk4; ++u)
+            if (B.chk[u] == t) f = u;
+        if (need_es || f >= 0) {
+            const int e = ecell_of(v, ecs, D.ecells);
+            if (need_es) Es[B.coord + (long long) t * n + i] = (uint16_t) e;
+            if (f >= 0) w |= (uint64_t) e << (16 * f);
+        }
     }
-    uint64_t w = 0;
-    for (int f = 0; f < B.nchk4; ++f)
-        if (B.chk[f] != B.pos_m) w |= (uint64_t) D.cellE[(size_t) B.S.dims[B.chk[f]] * n + p] << (16 * f);
     sigS[B.obase + i] = w;
 }
 
@@ -143,7 +151,7 @@ __global__ void k_b_gather_own(DevData D, const BatchSub *__restrict__ subs, con
         bool own_checked = false;
         for (int f = 0; f < B.nchk4; ++f) own_checked = own_checked || B.chk[f] == B.pos_m;
         if (need_es || own_checked) {
-            const uint16_t e = D.cellE[(size_t) s * n + p];
+            const uint16_t e = (uint16_t) ecell_of(v, ecell_width(D.ecells), D.ecells);
             if (need_es) Es[B.mcoord + i] = e;
             for (int f = 0; f < B.nchk4; ++f)
                 if (B.chk[f] == B.pos_m) w |= (uint64_t) e << (16 * f);
