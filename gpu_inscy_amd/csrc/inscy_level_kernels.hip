@@ -366,30 +366,27 @@ __device__ __forceinline__ void lds_layout(char *smem, int k, float *&sy, int *&
 // the first 8 dimensions' loads are all issued before any store (a store to
 // LDS through a generic pointer would otherwise order each load after the
 // previous store, one memory round trip per dimension)
+// (the cells are computed from the coordinates with the discretisation's
+// formula, G.C being the number of eps cells: no scattered read by point id)
 __device__ __forceinline__ void stage_query_b(const BatchSub &B, int n, const float *__restrict__ Ys, int p, int i,
                                               float *sy, int *se, int stride) {
     constexpr int kStage = 8;
     const int k = B.S.k;
+    const float ecs = ecell_width(B.G.C);
+    (void) p;
     float y[kStage];
-    int c[kStage];
 #pragma unroll
-    for (int t = 0; t < kStage; ++t) {
-        y[t] = 0.0f;
-        c[t] = 0;
-        if (t < k) {
-            y[t] = Ys[col_of(B, t, n) + i];
-            c[t] = B.cellE[(size_t) B.S.dims[t] * n + p];
-        }
-    }
+    for (int t = 0; t < kStage; ++t) y[t] = t < k ? Ys[col_of(B, t, n) + i] : 0.0f;
 #pragma unroll
     for (int t = 0; t < kStage; ++t)
         if (t < k) {
             sy[t * stride] = y[t];
-            se[t * stride] = c[t];
+            se[t * stride] = ecell_of(y[t], ecs, B.G.C);
         }
     for (int t = kStage; t < k; ++t) {
-        sy[t * stride] = Ys[col_of(B, t, n) + i];
-        se[t * stride] = B.cellE[(size_t) B.S.dims[t] * n + p];
+        const float v = Ys[col_of(B, t, n) + i];
+        sy[t * stride] = v;
+        se[t * stride] = ecell_of(v, ecs, B.G.C);
     }
 }
 
