@@ -21,10 +21,10 @@ Also reported:
                 level-synchronous driver, k_grid_count in the depth-first one):
                 its algorithmic bytes (per counted point 4k+4, per neighbour
                 established 4+4k) over its device time measured with HIP events
-                on the engine's stream during the timed steps, against 8 TB/s
-                HBM; `traffic` comes from the committed rocprofv3 counter summary.
-                ms_per_step_without_events repeats a few runs without the
-                events, to show their overhead.
+                on the engine's stream in a separate pass of up to 3 runs after
+                the timed steps (ms_per_step_with_events shows their overhead),
+                against 8 TB/s HBM; `traffic` comes from the committed rocprofv3
+                counter summary.
   cpu_baseline  the oracle (a single-threaded CPU restatement of the
                 reference's CPU INSCY) on the same data for a bounded time
                 budget, as trees per second, with the host's CPU model.
@@ -170,10 +170,9 @@ def main():
         res = step()
     say(f"warm-up done ({args.warmup} runs)")
 
-    # timed region: K complete runs, the count kernel's launches bracketed by
-    # HIP events on the engine's stream (inscy_profile); only the last result
-    # is held, as a caller that consumes each run would
-    _lib.profile(True)
+    # timed region: K complete runs; only the last result is held, as a
+    # caller that consumes each run would
+    _lib.profile(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats = []
@@ -183,27 +182,31 @@ def main():
         stats.append(res.stats)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    _lib.profile(False)
     ms_step = elapsed / args.steps * 1e3
     say(f"timed: {args.steps} runs, {ms_step:.3f} ms/run")
     parity = golden_parity(args.config, res, args.phi)  # the last timed run against the oracle's digests
     say(f"parity of the last timed run: {parity}")
 
-    # the same runs without the events, to show their overhead
-    unprof = max(1, min(3, args.steps))
+    # the roofline pass: the same runs with the count kernel's launches
+    # bracketed by HIP events on the engine's stream (inscy_profile)
+    nprof = max(1, min(3, args.steps))
+    pstats = []
+    _lib.profile(True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for _ in range(unprof):
+    for _ in range(nprof):
         r2 = step()
+        pstats.append(r2.stats)
         del r2
     torch.cuda.synchronize()
-    ms_unprof = (time.perf_counter() - t1) / unprof * 1e3
+    ms_prof = (time.perf_counter() - t1) / nprof * 1e3
+    _lib.profile(False)
 
     trees = sum(st["trees"] for st in stats)
-    kernel_ms = sum(st["count_kernel_ms"] for st in stats)
-    launches = sum(st["count_launches"] for st in stats)
-    kbytes = sum(st["count_bytes"] for st in stats)
-    ebytes = sum(st["count_examined_bytes"] for st in stats)
+    kernel_ms = sum(st["count_kernel_ms"] for st in pstats)
+    launches = sum(st["count_launches"] for st in pstats)
+    kbytes = sum(st["count_bytes"] for st in pstats)
+    ebytes = sum(st["count_examined_bytes"] for st in pstats)
     avg_launch_ms = kernel_ms / max(launches, 1)
     achieved = (kbytes / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
     kernel = "k_b_count" if args.mode == "batched" else "k_grid_count"
@@ -217,13 +220,13 @@ def main():
         "traffic": None if args.phi else traffic_from_profile(kernel, args.config),
         "algorithmic_bytes_per_launch": round(kbytes / max(launches, 1), 1),
         "avg_launch_us": round(avg_launch_ms * 1e3, 3),
-        "launches_per_step": launches // max(args.steps, 1),
+        "launches_per_step": launches // nprof,
         # the candidates' packed cells and coordinates the walks read (mostly
         # L2 hits): what the kernel moves through the caches, beside the
         # algorithmic bytes of the neighbours it establishes
         "examined_bytes_per_launch": round(ebytes / max(launches, 1), 1),
         "examined_GBs": round((ebytes / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e9, 2) if avg_launch_ms > 0 else 0.0,
-        "kernel_share_of_step": round(kernel_ms / (elapsed * 1e3), 4),
+        "kernel_share_of_step": round(kernel_ms / nprof / ms_step, 4),
     }
 
     # the depth-first driver (one restricted tree at a time) on the same data
@@ -293,7 +296,7 @@ def main():
             "speedup_vs_depth_first": (round(other / ms_step, 3) if args.mode == "batched" else
                                        round(ms_step / other, 3)) if other else None,
             "modes_agree": same,
-            "ms_per_step_without_events": round(ms_unprof, 3),
+            "ms_per_step_with_events": round(ms_prof, 3),
             "dropin_gpu5_lists_seconds": dropin,
             "phase_seconds": dict({k: round(stats[-1][k + "_seconds"], 4) for k in ("expand", "replay", "emit")},
                                   library=round(stats[-1]["seconds"], 4)),
@@ -515,18 +518,40 @@ def bench_many(args, mode):
     torch.cuda.synchronize()
     for _ in range(max(args.warmup, 1)):
         first = INSCY.run_many(jobs, mode=mode)
-    _lib.profile(True)
+    _lib.profile(False)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    runs = [INSCY.run_many(jobs, mode=mode) for _ in range(args.steps)]
+    last = None
+    trees = 0
+    for _ in range(args.steps):
+        last = None
+        last = INSCY.run_many(jobs, mode=mode)
+        trees += sum(r.stats["trees"] for r in last)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    runs = [last]
+    # the last timed call against the oracle's digests of the sets it has finished
+    parity, checked = None, 0
+    gpath = os.path.join(ROOT, "tests", "golden", "config3.json")
+    if os.path.exists(gpath):
+        import numpy as np
+        gold = {g["workload"]: g for g in json.load(open(gpath))["sets"]}
+        parity = True
+        for w, r in zip(sets, last):
+            g = gold.get(w.name)
+            if g is None:
+                continue
+            checked += 1
+            digs = [hashlib.sha256(np.ascontiguousarray(l, dtype=np.int32).tobytes()).hexdigest() for l in r.labels]
+            parity = parity and r.subspaces == [x[0] for x in g["subspaces"]] and digs == [x[3] for x in g["subspaces"]]
+    # the roofline pass: one call with the count kernel's launches bracketed by HIP events
+    _lib.profile(True)
+    prof = INSCY.run_many(jobs, mode=mode)
+    torch.cuda.synchronize()
     _lib.profile(False)
-    flat = [r for step in runs for r in step]
-    trees = sum(r.stats["trees"] for r in flat)
-    kernel_ms = sum(r.stats["count_kernel_ms"] for r in flat)
-    launches = sum(r.stats["count_launches"] for r in flat)
-    kbytes = sum(r.stats["count_bytes"] for r in flat)
+    kernel_ms = sum(r.stats["count_kernel_ms"] for r in prof)
+    launches = sum(r.stats["count_launches"] for r in prof)
+    kbytes = sum(r.stats["count_bytes"] for r in prof)
     avg_launch_ms = kernel_ms / max(launches, 1)
     achieved = (kbytes / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
     same = all(a.subspaces == b.subspaces for a, b in zip(first, runs[-1]))
@@ -551,9 +576,10 @@ def bench_many(args, mode):
             "trees_per_step": trees // max(args.steps, 1),
             "subspaces": sum(r.stats["subspaces"] for r in runs[-1]),
             "deterministic": same,
+            "parity_sets": checked,
             "parallelism": "single GPU, concurrent streams",
         },
-        "parity": None,
+        "parity": parity,
         "roofline": {"bound": "hbm", "kernel": "k_b_count" if args.mode == "batched" else "k_grid_count",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None},
