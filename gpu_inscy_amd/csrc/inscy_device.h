@@ -187,6 +187,20 @@ __device__ __forceinline__ void uf_union(int *par, int a, int b) {
     }
 }
 
+// OR of every lane's m into *dst (all lanes of the wave take part)
+__device__ __forceinline__ void wave_or_mask(dmask_t m, dmask_t *dst) {
+    uint64_t lo = dlo(m), hi = dhi(m);
+    for (int off = 32; off > 0; off >>= 1) {
+        lo |= __shfl_xor(lo, off);
+        hi |= __shfl_xor(hi, off);
+    }
+    if (__lane_id() == 0) {
+        unsigned long long *w = reinterpret_cast<unsigned long long *>(dst);
+        if (lo) atomicOr(&w[0], (unsigned long long) lo);
+        if (hi) atomicOr(&w[1], (unsigned long long) hi);
+    }
+}
+
 constexpr uint64_t kEmpty = ~0ull;
 
 __device__ __forceinline__ uint64_t mix64(uint64_t h) {
@@ -200,11 +214,11 @@ __device__ __forceinline__ uint64_t mix64(uint64_t h) {
 
 // a and b share their c-grid cells on every unrestricted dimension at SCY-tree
 // levels 0..upto (the regular-node prefix of the tree)
-__device__ __forceinline__ bool same_prefix(const DevData &D, uint64_t smask, int a, int b, int upto) {
+__device__ __forceinline__ bool same_prefix(const DevData &D, dmask_t smask, int a, int b, int upto) {
     if (b < 0 || b >= D.n) return false;
     for (int t = 0; t <= upto; ++t) {
         const int k = D.level[t];
-        if ((smask >> k) & 1ull) continue;
+        if (dhas(smask, k)) continue;
         if (D.cellC[(size_t) k * D.n + a] != D.cellC[(size_t) k * D.n + b]) return false;
     }
     return true;

@@ -229,14 +229,15 @@ struct HostBuf {
 };
 
 struct TreeRef {
-    uint64_t smask;
+    dmask_t smask;
     const int *P;      // device point ids
     const int *nP;     // device count
     int nP_max;        // host bound (exact where the host knows it)
     const int *spx;    // device path origins
-    const uint64_t *spm;
+    const uint64_t *spm;  // their masks, mw words each
     const int *nsp;
     int nsp_max;
+    const uint64_t *chain = nullptr;  // permuted-order driver: chain words instead of spm
 };
 
 // What the algorithm needs about one subspace S, computed on first use:
@@ -288,11 +289,12 @@ public:
             phi_dense.push_back(
                 std::max(prm.F * alpha_thr(k, prm.neighborhood_size, n), (float) prm.num_obj * omega_thr(k)));
 
+        mw = d <= 64 ? 1 : 2;
         size_t nd = (size_t) n * d;
         HIP_OK(dev_alloc(&Xt, nd * sizeof(float)));
         HIP_OK(dev_alloc(&cellC, nd));
         HIP_OK(dev_alloc(&cellE, nd * sizeof(uint16_t)));
-        HIP_OK(dev_alloc(&nearm, (size_t) n * sizeof(uint64_t)));
+        HIP_OK(dev_alloc(&nearm, (size_t) n * sizeof(dmask_t)));
         HIP_OK(dev_alloc(&small, kSmallWords * sizeof(unsigned long long)));
         HIP_OK(dev_alloc(&stamp, (size_t) n * sizeof(int)));
         HIP_OK(dev_alloc(&qmark, (size_t) n * sizeof(int)));
@@ -316,8 +318,8 @@ public:
 
         int *hist = arena.take<int>((size_t) d * ecells);
         HIP_OK(launch_fill_i32(reinterpret_cast<int *>(small), 0, 2 * kSmallWords, st));
-        HIP_OK(launch_discretise(d_X, n, d, cells, prm.neighborhood_size, Xt, cellC, cellE, nearm,
-                                 reinterpret_cast<int *>(small), st));
+        HIP_OK(launch_discretise(d_X, n, d, cells, prm.neighborhood_size, Xt, cellC, cellE,
+                                 reinterpret_cast<uint64_t *>(nearm), 2, reinterpret_cast<int *>(small), st));
         HIP_OK(launch_hist1(cellE, n, d, ecells, hist, st));
         // c-grid occupancy: the input of the entropy order only
         int *histc = arena.take<int>((size_t) d * cells);
@@ -408,12 +410,12 @@ public:
         int *P = arena.take<int>(n);
         int *nP = arena.take<int>(1);
         int *spx = arena.take<int>(n);
-        uint64_t *spm = arena.take<uint64_t>(n);
+        uint64_t *spm = arena.take<uint64_t>((size_t) n * mw);
         int *nsp = arena.take<int>(1);
         HIP_OK(launch_iota(P, n, st));
         HIP_OK(launch_fill_i32(nP, n, 1, st));
-        HIP_OK(launch_root_paths(nearm, n, spx, spm, nsp, st));
-        TreeRef root{0ull, P, nP, n, spx, spm, nsp, n};
+        HIP_OK(launch_root_paths(nearm, n, spx, spm, mw, nsp, st));
+        TreeRef root{(dmask_t) 0, P, nP, n, spx, spm, nsp, n};
         auto t0 = std::chrono::steady_clock::now();
         recurse(root, 0);
         sync();
@@ -424,10 +426,10 @@ public:
     }
 
     // dense components of one subspace (kernel-level entry point)
-    const Subspace &subspace(uint64_t smask) { return ensure_components(smask); }
+    const Subspace &subspace(dmask_t smask) { return ensure_components(smask); }
 
     // exact |N_S(p)| of every point, without the threshold cap (kernel-level entry point)
-    const int *exact_counts(uint64_t smask) {
+    const int *exact_counts(dmask_t smask) {
         Subspace &s = sub_of(smask);
         SubspaceArg S = arg_of(smask);
         GridArg G = grid_of(S);
@@ -521,11 +523,13 @@ private:
         HIP_OK(launch_fill_i32(d_nE, nentries, 1, st));
     }
 
-    SubspaceArg arg_of(uint64_t smask) const {
+    SubspaceArg arg_of(dmask_t smask) const {
+        if (dcount(smask) > kMaxSubDims)
+            throw Failure(INSCY_ERR_UNSUPPORTED, "a subspace of more than 64 dimensions is not supported");
         SubspaceArg S{};
         S.k = 0;
         for (int k = 0; k < d; ++k)
-            if ((smask >> k) & 1ull) S.dims[S.k++] = k;
+            if (dhas(smask, k)) S.dims[S.k++] = k;
         return S;
     }
 
@@ -588,7 +592,7 @@ private:
     // threshold of |S| (clustering), so they are needed up to the larger one
     int cap_of(int k) const { return std::max(std::max(weak_thr, dense_thr[k]), 1); }
 
-    Subspace &sub_of(uint64_t smask) {
+    Subspace &sub_of(dmask_t smask) {
         Subspace &s = subs[smask];
         if (!s.cnt) {
             s.cnt = slabs.take<int>(n);
@@ -598,7 +602,7 @@ private:
     }
 
     // |N_S(p)| of the points stamped with markid that are not counted yet.
-    Subspace &ensure_counts(uint64_t smask, int markid) {
+    Subspace &ensure_counts(dmask_t smask, int markid) {
         Subspace &s = sub_of(smask);
         if (s.entry >= 0) return s;  // all points counted by the components pass
         SubspaceArg S = arg_of(smask);
@@ -613,7 +617,7 @@ private:
 
     // Dense components of S (counting every point not yet counted) and its
     // result-table row, once per subspace.
-    Subspace &ensure_components(uint64_t smask) {
+    Subspace &ensure_components(dmask_t smask) {
         Subspace &s = sub_of(smask);
         if (s.entry >= 0) return s;
         SubspaceArg S = arg_of(smask);
@@ -627,7 +631,7 @@ private:
         const int *bound = nullptr;
         int lo = std::min(weak_thr, dense_thr[S.k]);
         for (int t = 0; t < S.k && !bound; ++t) {
-            uint64_t sub = smask & ~(1ull << S.dims[t]);
+            const dmask_t sub = smask & ~dbit(S.dims[t]);
             auto it = subs.find(sub);
             if (sub && it != subs.end() && it->second.entry >= 0) {
                 bound = it->second.cnt;
@@ -665,12 +669,12 @@ private:
         // s-connected cells, then the cell groups (ScyTreeNode.cu:837-851)
         int *flags = arena.take<int>((size_t) nd * cells);
         HIP_OK(launch_fill_i32(flags, 0, (size_t) nd * cells, st));
-        HIP_OK(launch_flags_all(T.spx, T.spm, T.nsp, T.nsp_max, cellC, n, first, cells, flags, st));
+        HIP_OK(launch_flags_all(T.spx, T.spm, mw, T.nsp, T.nsp_max, cellC, n, first, cells, flags, st));
         // the subspaces of the children are counted on this tree's points while
         // the flags travel
         const int markid = ++mark_id;
         HIP_OK(launch_mark(T.P, T.nP, T.nP_max, qmark, markid, st));
-        for (int m = first; m < d; ++m) ensure_counts(T.smask | (1ull << m), markid);
+        for (int m = first; m < d; ++m) ensure_counts(T.smask | dbit(m), markid);
         HIP_OK(hipMemcpyAsync(pin, flags, (size_t) nd * cells * sizeof(int), hipMemcpyDeviceToHost, st));
         sync();
         std::vector<std::vector<Group>> groups(nd);
@@ -702,9 +706,9 @@ private:
             pt.P = arena.take<int>(std::max(T.nP_max, 1));
             pt.W = arena.take<int>(std::max(T.nP_max, 1));
             pt.x = arena.take<int>(std::max(T.nsp_max, 1));
-            pt.mk = arena.take<uint64_t>(std::max(T.nsp_max, 1));
-            const Subspace &sub = subs[T.smask | (1ull << m)];
-            HIP_OK(launch_partition(T.P, T.nP, T.nP_max, T.spx, T.spm, T.nsp, T.nsp_max, cellC + (size_t) m * n,
+            pt.mk = arena.take<uint64_t>((size_t) std::max(T.nsp_max, 1) * mw);
+            const Subspace &sub = subs[T.smask | dbit(m)];
+            HIP_OK(launch_partition(T.P, T.nP, T.nP_max, T.spx, T.spm, mw, T.nsp, T.nsp_max, cellC + (size_t) m * n,
                                     d_gid + (size_t) (m - first) * cells, G, sub.cnt, weak_thr, m, pt.cnt, pt.P, pt.W,
                                     pt.x, pt.mk, st));
         }
@@ -722,7 +726,7 @@ private:
         std::vector<int> counts(hostc, hostc + tot);
 
         for (int m = first; m < d; ++m) {
-            const uint64_t smask = T.smask | (1ull << m);
+            const dmask_t smask = T.smask | dbit(m);
             const auto &gs = groups[m - first];
             const int G = (int) gs.size();
             const int *c3 = counts.data() + cnt_at[m - first];
@@ -734,27 +738,27 @@ private:
                 const int nPr = c3[g], nW = c3[G + g], nSP = c3[2 * G + g];
                 const int *W = pt.W + offW;
                 const int *spx = pt.x + offS;
-                const uint64_t *spm = pt.mk + offS;
+                const uint64_t *spm = pt.mk + (size_t) offS * mw;
                 offW += nW;
                 offS += nSP;
                 stats.trees++;
                 const bool pass = nW >= prm.min_size;
                 size_t at = trace.size();
-                if (trace_on) trace.push_back({(long long) smask, gs[g].lo, gs[g].hi, nPr, nW, pass ? 1 : 0, -1});
+                if (trace_on) trace.push_back({(long long) dlo(smask), gs[g].lo, gs[g].hi, nPr, nW, pass ? 1 : 0, -1});
                 if (!pass) continue;
                 const size_t child_mark = arena.mark();
                 const int tid = ++tree_id;
                 // s-connection paths surviving the prune
                 int *spx2 = arena.take<int>(std::max(nSP, 1));
-                uint64_t *spm2 = arena.take<uint64_t>(std::max(nSP, 1));
+                uint64_t *spm2 = arena.take<uint64_t>((size_t) std::max(nSP, 1) * mw);
                 int *nsp2 = arena.take<int>(1);
                 {
                     size_t tmark = arena.mark();
                     size_t slots = table_slots(nW, d - 1 - m);
                     uint64_t *table = arena.take<uint64_t>(nSP > 0 ? slots : 1);
-                    HIP_OK(launch_path_prune(D, smask, W, pt.cnt + G + g, nW, spx, spm, pt.cnt + 2 * G + g, nSP,
-                                             stamp, tid, small + 16, table, (uint64_t) (slots - 1), spx2, spm2, nsp2,
-                                             st));
+                    HIP_OK(launch_path_prune(D, smask, W, pt.cnt + G + g, nW, spx, spm, mw, pt.cnt + 2 * G + g, nSP,
+                                             stamp, tid, reinterpret_cast<dmask_t *>(small + 16), table,
+                                             (uint64_t) (slots - 1), spx2, spm2, nsp2, st));
                     arena.release(tmark);  // stream order keeps the table alive for the queued kernels
                 }
                 // components of S now: the children's counts are bounded by them
@@ -787,7 +791,7 @@ private:
     }
 
     // join (util.cu:655-743), on the device
-    void join(uint64_t smask, const uint8_t *lab) {
+    void join(dmask_t smask, const uint8_t *lab) {
         Subspace &s = ensure_components(smask);
         HIP_OK(launch_join(lab, n, s.comp, s.csize, prm.min_size, prm.r, entries, d_nE, smask, s.entry, s.inres,
                            d_list, d_nlist, keepf, dropf, freshf, st));
@@ -809,7 +813,7 @@ private:
         for (int e : rows) {
             std::vector<int32_t> dims;
             for (int k = 0; k < d; ++k)
-                if ((tab[e].mask >> k) & 1ull) dims.push_back(k);
+                if (dhas(tab[e].mask, k)) dims.push_back(k);
             out.subspaces.push_back(std::move(dims));
         }
         out.labels.reset(alloc_labels((size_t) std::max(m, 1) * n));
@@ -910,7 +914,8 @@ private:
     float *Xt = nullptr;
     uint8_t *cellC = nullptr;
     uint16_t *cellE = nullptr;
-    uint64_t *nearm = nullptr;
+    dmask_t *nearm = nullptr;
+    int mw = 1;  // words per path mask: 1 for d <= 64, else 2
     unsigned long long *small = nullptr;
     int *stamp = nullptr;
     int *qmark = nullptr;  // [n] stamp of the tree whose points are being counted
@@ -922,7 +927,7 @@ private:
     DevData D{};
     Arena arena;
     Slabs slabs;
-    std::unordered_map<uint64_t, Subspace> subs;
+    std::unordered_map<dmask_t, Subspace, DMaskHash> subs;
     // device result table
     Entry *entries = nullptr;
     int *scratch = nullptr;
@@ -933,19 +938,19 @@ private:
 
     // ---------------------------------------------------------- level-synchronous driver state
     struct LTree {
-        uint64_t smask = 0;
+        dmask_t smask = 0;
         int first = 0, m = -1, lo = 0, hi = 0, pre = 0, nW = 0;
         bool pass = false;
         const int *W = nullptr;      // device: its points (weak-dense points of its group)
         int *spx = nullptr;          // device: its s-connection paths (while its level is open)
-        uint64_t *spm = nullptr;
+        uint64_t *spm = nullptr;     // their masks, mw words each
         int nsp_max = 0, nsp = 0;
         int slot = -1;               // its path counter in the level's counter array
         std::vector<int> kids;       // children in (dimension, cell group) order
     };
     // one child subspace S | {m} of the frontier trees act[t0..t1) (all of mask S)
     struct ChildSub {
-        uint64_t s;
+        dmask_t s;
         int m, t0, t1;
     };
     std::vector<LTree> lt;
@@ -1001,12 +1006,12 @@ private:
     std::vector<int> sup_flat;  // superset rows of every subspace (host copy of d_sup)
     static unsigned blocks_for_host(long long count) { return (unsigned) ((count + kBlockThreads - 1) / kBlockThreads); }
     // replay state: superset rows per subspace, decisions per tree
-    std::vector<uint64_t> emask;                 // mask of every result-table row
-    std::unordered_map<uint64_t, std::pair<int, int>> sup_at;  // subspace -> (offset, count) in d_sup
+    std::vector<dmask_t> emask;                  // mask of every result-table row
+    std::unordered_map<dmask_t, std::pair<int, int>, DMaskHash> sup_at;  // subspace -> (offset, count) in d_sup
     int *d_sup = nullptr;
     int *keepv = nullptr;
     int rep_join = 0;  // number of the last join
-    void set_bounds(uint64_t s, BatchSub &x);
+    void set_bounds(dmask_t s, BatchSub &x);
     void trace_tree(int f, const std::vector<int> &keep);
     std::vector<KernelTimer> timers;
     // pairs of the level: first pair of each active tree, pruner decisions

@@ -42,7 +42,7 @@ static int fail_current() {
 static int check_params(int64_t n, int32_t d, const inscy_params *p, int32_t mode) {
     if (!p) return fail(INSCY_ERR_ARGUMENT, "params is NULL");
     if (n < 1 || n > (int64_t) INT_MAX / 2) return fail(INSCY_ERR_ARGUMENT, "n out of range");
-    if (d < 1 || d > kMaxDims) return fail(INSCY_ERR_ARGUMENT, "d must be in [1, 64]");
+    if (d < 1 || d > kMaxDims) return fail(INSCY_ERR_ARGUMENT, "d must be in [1, 128]");
     if (!(p->neighborhood_size > 0.0f) || !std::isfinite(p->neighborhood_size))
         return fail(INSCY_ERR_ARGUMENT, "neighborhood_size must be > 0");
     if (p->number_of_cells < 1 || p->number_of_cells > 255)
@@ -519,7 +519,8 @@ int inscy_discretise(const float *d_X, int64_t n, int32_t d, int32_t cells, floa
         int *bad = nullptr;
         HIP_OK(hipMalloc(&bad, sizeof(int)));
         HIP_OK(hipMemset(bad, 0, sizeof(int)));
-        HIP_OK(launch_discretise(d_X, (int) n, d, cells, eps, nullptr, d_cell_c, d_cell_e, d_near_mask, bad, 0));
+        HIP_OK(launch_discretise(d_X, (int) n, d, cells, eps, nullptr, d_cell_c, d_cell_e, d_near_mask, (d + 63) / 64,
+                                 bad, 0));
         int h = 0;
         HIP_OK(hipMemcpy(&h, bad, sizeof(int), hipMemcpyDeviceToHost));
         HIP_OK(hipFree(bad));
@@ -532,12 +533,13 @@ int inscy_discretise(const float *d_X, int64_t n, int32_t d, int32_t cells, floa
 
 // Kernel-level entry points: the production engine's subspace pass on its own.
 namespace {
-uint64_t mask_from(const int32_t *dims, int32_t k, int d) {
-    uint64_t m = 0;
+dmask_t mask_from(const int32_t *dims, int32_t k, int d) {
+    if (k > kMaxSubDims) throw Failure(INSCY_ERR_ARGUMENT, "a subspace has at most 64 dimensions");
+    dmask_t m = 0;
     for (int i = 0; i < k; ++i) {
         if (dims[i] < 0 || dims[i] >= d || (i && dims[i] <= dims[i - 1]))
             throw Failure(INSCY_ERR_ARGUMENT, "dims must be ascending ids in [0, d)");
-        m |= 1ull << dims[i];
+        m |= dbit(dims[i]);
     }
     return m;
 }

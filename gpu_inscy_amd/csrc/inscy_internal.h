@@ -10,12 +10,54 @@
 
 namespace inscy {
 
-constexpr int kMaxDims = 64;  // subspace bitmasks are uint64
+// Sets of dimensions (subspaces, s-connection paths, the near-edge marks of a
+// point) are 128-bit masks, bit j for dimension j; their numeric order is the
+// order of the reference's vec_cmp (util.cu:633-653: the larger highest
+// differing dimension is the larger set).
+typedef unsigned __int128 dmask_t;
+constexpr int kMaxDims = 128;     // d
+constexpr int kMaxSubDims = 64;   // dimensions of one subspace (positions in S are uint64 masks)
+
+__host__ __device__ __forceinline__ dmask_t dbit(int j) { return (dmask_t) 1 << j; }
+__host__ __device__ __forceinline__ bool dhas(dmask_t m, int j) { return (bool) ((m >> j) & 1u); }
+// dimensions above j (j in [-1, kMaxDims))
+__host__ __device__ __forceinline__ dmask_t dabove(int j) {
+    return j + 1 >= kMaxDims ? (dmask_t) 0 : ~(dmask_t) 0 << (j + 1);
+}
+__host__ __device__ __forceinline__ uint64_t dlo(dmask_t m) { return (uint64_t) m; }
+__host__ __device__ __forceinline__ uint64_t dhi(dmask_t m) { return (uint64_t) (m >> 64); }
+__host__ __device__ __forceinline__ dmask_t dmake(uint64_t lo, uint64_t hi) { return (dmask_t) hi << 64 | lo; }
+__host__ __device__ __forceinline__ int dcount(dmask_t m) {
+    return __builtin_popcountll(dlo(m)) + __builtin_popcountll(dhi(m));
+}
+// lowest dimension of a non-empty mask
+__host__ __device__ __forceinline__ int dfirst(dmask_t m) {
+    return dlo(m) ? __builtin_ctzll(dlo(m)) : 64 + __builtin_ctzll(dhi(m));
+}
+static_assert(sizeof(dmask_t) == 16 && alignof(dmask_t) == 16, "one layout on host and device");
+// Arrays of s-connection path masks hold mw 64-bit words per entry: 1 when
+// d <= 64 (half the bytes of the partitions and path lists), 2 above.
+__host__ __device__ __forceinline__ dmask_t mload(const uint64_t *a, long long i, int mw) {
+    return mw == 1 ? (dmask_t) a[i] : reinterpret_cast<const dmask_t *>(a)[i];
+}
+__host__ __device__ __forceinline__ void mstore(uint64_t *a, long long i, dmask_t v, int mw) {
+    if (mw == 1)
+        a[i] = dlo(v);
+    else
+        reinterpret_cast<dmask_t *>(a)[i] = v;
+}
+
+struct DMaskHash {
+    size_t operator()(dmask_t m) const {
+        uint64_t h = dlo(m) * 0x9E3779B97F4A7C15ull ^ (dhi(m) + 0x632BE59BD9B4E019ull);
+        return (size_t) (h ^ (h >> 29));
+    }
+};
 
 // Subspace argument passed by value to kernels: k ascending dimension ids.
 struct SubspaceArg {
     int k;
-    int dims[kMaxDims];
+    int dims[kMaxSubDims];
 };
 
 // Device-resident discretised data set.  All per-dimension arrays are
@@ -59,7 +101,7 @@ struct GridBuffers {
 // One subspace of the device-resident result table: its dense components and
 // which of them the result holds (nkept > 0 <=> the subspace is in the result).
 struct Entry {
-    uint64_t mask;
+    dmask_t mask;
     const int *comp;   // [n] component root of each point, -1 if not dense
     const int *csize;  // [n] component sizes indexed by root
     uint8_t *inres;    // [n] component kept in the result
@@ -152,9 +194,9 @@ struct BatchIndex {
 // Tree whose s-connection paths set the merge flags of its dimensions >= first.
 struct FlagTree {
     const int *spx;
-    const uint64_t *spm;
+    const uint64_t *spm;  // mw words per path
     const int *nsp;  // device count
-    int first, flag_off;
+    int first, flag_off, mw;
 };
 
 // Points of a tree that are queries of subspace b.
@@ -167,29 +209,29 @@ struct QuerySeg {
 struct PartPair {
     const int *P;
     const int *spx;
-    const uint64_t *spm;
+    const uint64_t *spm;  // mw words per path
     const int *gid;      // [cells]
     const int *weakcnt;  // counts of S ∪ {m}
     int *cnt;            // [6G]: points, weak-dense points, paths per group; then cursors
     int *outW;
     int *outx;
     uint64_t *outm;
-    int nP, nsp, m, G, bp, pad;
+    int nP, nsp, m, G, bp, mw;
 };
 
 // Path prune of one surviving tree.
 struct PruneChild {
-    uint64_t smask;
+    dmask_t smask;
     uint64_t tmask;
     const int *W;
     const int *spx;
-    const uint64_t *spm;
+    const uint64_t *spm;  // mw words per path
     uint64_t *table;
-    unsigned long long *bits;
+    dmask_t *bits;
     int *outx;
     uint64_t *outm;
     int *nout;
-    int nW, nsp, bw, pad;
+    int nW, nsp, bw, mw;
 };
 
 struct CopySeg {
@@ -218,8 +260,10 @@ hipError_t launch_flag_compact(const uint8_t *flags, size_t nb, void *temp, size
                                hipStream_t st);
 
 // Launchers (all asynchronous on `st`; return the hipError_t of the launches).
+// discretise: the near-edge marks of point p are nw 64-bit words at nearw[p * nw]
+// (nw = 2 is dmask_t[n]; 1 suffices for d <= 64)
 hipError_t launch_discretise(const float *X, int n, int d, int cells, float eps, float *Xt, uint8_t *cellC,
-                             uint16_t *cellE, uint64_t *nearm, int *bad, hipStream_t st);
+                             uint16_t *cellE, uint64_t *nearw, int nw, int *bad, hipStream_t st);
 hipError_t launch_hist1(const uint16_t *cellE, int n, int d, int ecells, int *hist, hipStream_t st);
 // occupancy of the c-grid per dimension ([d][cells]), the input of the entropy order
 hipError_t launch_histc(const uint8_t *cellC, int n, int d, int cells, int *hist, hipStream_t st);
@@ -235,21 +279,21 @@ hipError_t launch_subspace_components(const DevData &D, const SubspaceArg &S, co
                                       int *cnt, int *comp, int *csize, unsigned long long *stat,
                                       const KernelTimer &tm, hipStream_t st);
 hipError_t launch_mark(const int *P, const int *nP, int nP_max, int *mark, int markid, hipStream_t st);
-hipError_t launch_flags_all(const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, const uint8_t *cellC,
-                            int n, int first, int cells, int *flags, hipStream_t st);
-hipError_t launch_partition(const int *P, const int *nP, int nP_max, const int *spx, const uint64_t *spm,
+hipError_t launch_flags_all(const int *spx, const uint64_t *spm, int mw, const int *nsp, int nsp_max,
+                            const uint8_t *cellC, int n, int first, int cells, int *flags, hipStream_t st);
+hipError_t launch_partition(const int *P, const int *nP, int nP_max, const int *spx, const uint64_t *spm, int mw,
                             const int *nsp, int nsp_max, const uint8_t *cellC_m, const int *gid, int G,
                             const int *weakcnt, int thr, int m, int *cnt, int *outP, int *outW, int *outx,
                             uint64_t *outm, hipStream_t st);
-hipError_t launch_path_prune(const DevData &D, uint64_t smask, const int *W, const int *nW, int nW_max,
-                             const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, int *stamp, int tid,
-                             unsigned long long *bits, uint64_t *table, uint64_t tmask, int *outx, uint64_t *outm,
+hipError_t launch_path_prune(const DevData &D, dmask_t smask, const int *W, const int *nW, int nW_max,
+                             const int *spx, const uint64_t *spm, int mw, const int *nsp, int nsp_max, int *stamp,
+                             int tid, dmask_t *bits, uint64_t *table, uint64_t tmask, int *outx, uint64_t *outm,
                              int *nout, hipStream_t st);
 hipError_t launch_redundancy_seed(const int *W, const int *nW_dev, int nW, int *stamp, int tid, const Entry *E,
-                                  const int *nE, uint64_t smask, const int *comp, int *list, int *nlist, int *mn,
+                                  const int *nE, dmask_t smask, const int *comp, int *list, int *nlist, int *mn,
                                   float r, int *keep, uint8_t *lab, hipStream_t st);
 hipError_t launch_join(const uint8_t *lab, int n, const int *compS, const int *csizeS, int min_size, float r,
-                       Entry *E, const int *nE, uint64_t smask, int eS, uint8_t *inresS, int *list, int *nlist,
+                       Entry *E, const int *nE, dmask_t smask, int eS, uint8_t *inresS, int *list, int *nlist,
                        uint8_t *keepf, uint8_t *drop, uint8_t *fresh, hipStream_t st);
 hipError_t launch_set_entry(Entry *E, int idx, const Entry &e, int *nE, int count, hipStream_t st);
 // canonical labels of r result subspaces: flags and idx hold r*n + 1 ints,
@@ -264,7 +308,7 @@ hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t count, hipStream_t st);
 hipError_t launch_fill_i32(int *p, int v, size_t count, hipStream_t st);
 hipError_t launch_gather(const int *src, const int *idx, int count, int *out, hipStream_t st);
 hipError_t launch_iota(int *p, int n, hipStream_t st);
-hipError_t launch_root_paths(const uint64_t *nearm, int n, int *outx, uint64_t *outm, int *nout, hipStream_t st);
+hipError_t launch_root_paths(const dmask_t *nearm, int n, int *outx, uint64_t *outm, int mw, int *nout, hipStream_t st);
 
 // level-synchronous passes (blk_off: per-segment block offsets, nblk = blk_off[nseg])
 hipError_t launch_b_flags(const FlagTree *T, const int *blk_off, int nt, int nblk, const uint8_t *cellC, int n,
@@ -383,7 +427,7 @@ struct FeatPair {
     const int *W;
     const int *cntS;  // counts of S, capped at cap
     const int *gid;   // [cells]
-    uint64_t mask;    // S | {m}
+    dmask_t mask;     // S | {m}
     int nW, pre, m, k, first, ngroups, bp, cap;
 };
 // acc: per pair kHistBins + cells + 2 ints (value histogram, cell histogram,

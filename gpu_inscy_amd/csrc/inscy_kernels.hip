@@ -30,10 +30,11 @@ namespace {
 // s-connection test of construct_s_connection (:34-49): x >= (c+1)*(1/c) - eps.
 __global__ void k_discretise(const float *__restrict__ X, int n, int d, int cells, float cs, int ecells,
                              float ecs, float eps, float *__restrict__ Xt, uint8_t *__restrict__ cellC,
-                             uint16_t *__restrict__ cellE, uint64_t *__restrict__ nearm, int *__restrict__ bad) {
+                             uint16_t *__restrict__ cellE, uint64_t *__restrict__ nearw, int nw,
+                             int *__restrict__ bad) {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    uint64_t nm = 0;
+    dmask_t nm = 0;
     bool out_of_range = false;
     for (int k = 0; k < d; ++k) {
         float x = X[(size_t) p * d + k];
@@ -46,9 +47,11 @@ __global__ void k_discretise(const float *__restrict__ X, int n, int d, int cell
         cellE[(size_t) k * n + p] = (uint16_t) e;
         float upper = __fmul_rn((float) (c + 1), cs);
         float bound = __fsub_rn(upper, eps);
-        if (x >= bound) nm |= 1ull << k;
+        if (x >= bound) nm |= dbit(k);
     }
-    nearm[p] = nm;
+    // nw 64-bit words per point, dimension k at bit k % 64 of word k / 64
+    nearw[(size_t) p * nw] = dlo(nm);
+    if (nw > 1) nearw[(size_t) p * nw + 1] = dhi(nm);
     if (out_of_range) atomicOr(bad, 1);
 }
 
@@ -73,17 +76,17 @@ __global__ void k_histc(const uint8_t *__restrict__ cellC, int n, int d, int cel
 // restrict_node :262-267): cell c of dimension b is s-connected iff a path
 // branching at b starts from a point in cell c.  One pass sets the flags of
 // every dimension b >= first.
-__global__ void k_flags_all(const int *__restrict__ spx, const uint64_t *__restrict__ spm, const int *__restrict__ nsp,
-                            const uint8_t *__restrict__ cellC, int n, int first, int cells,
-                            int *__restrict__ flags) {
+__global__ void k_flags_all(const int *__restrict__ spx, const uint64_t *__restrict__ spm, int mw,
+                            const int *__restrict__ nsp, const uint8_t *__restrict__ cellC, int n, int first,
+                            int cells, int *__restrict__ flags) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *nsp) return;
     int x = spx[i];
-    uint64_t m = first >= 64 ? 0ull : (spm[i] >> first);
+    dmask_t m = mload(spm, i, mw) & dabove(first - 1);
     while (m) {
-        int b = __ffsll((long long) m) - 1;
+        const int b = dfirst(m);
         m &= m - 1;
-        flags[b * cells + cellC[(size_t) (b + first) * n + x]] = 1;
+        flags[(b - first) * cells + cellC[(size_t) b * n + x]] = 1;
     }
 }
 
@@ -95,13 +98,13 @@ struct PartArg {
     const int *P;
     const int *nP;
     const int *spx;
-    const uint64_t *spm;
+    const uint64_t *spm;  // mw words per path
     const int *nsp;
     const uint8_t *cellC_m;
     const int *gid;      // [cells] group of each cell of m
     const int *weakcnt;  // neighbour counts of the subspace S ∪ {m}
     int thr, m, G;
-    int nP_blocks;
+    int nP_blocks, mw;
 };
 
 __global__ void k_part_count(PartArg A, int *__restrict__ cnt) {
@@ -119,7 +122,7 @@ __global__ void k_part_count(PartArg A, int *__restrict__ cnt) {
     } else {
         int i = (blockIdx.x - A.nP_blocks) * blockDim.x + threadIdx.x;
         if (i < *A.nsp) {
-            uint64_t mk = A.m >= 63 ? 0ull : (A.spm[i] & (~0ull << (A.m + 1)));
+            const dmask_t mk = mload(A.spm, i, A.mw) & dabove(A.m);
             if (mk) atomicAdd(&h[2 * A.G + A.gid[A.cellC_m[A.spx[i]]]], 1);
         }
     }
@@ -152,10 +155,10 @@ __global__ void k_part_scatter(PartArg A, int *__restrict__ cnt, int *__restrict
         if (weak) outW[base[A.G + g] + sw] = p;
     } else {
         int i = (blockIdx.x - A.nP_blocks) * blockDim.x + threadIdx.x;
-        uint64_t mk = 0;
+        dmask_t mk = 0;
         int x = 0, g = 0;
         if (i < *A.nsp) {
-            mk = A.m >= 63 ? 0ull : (A.spm[i] & (~0ull << (A.m + 1)));
+            mk = mload(A.spm, i, A.mw) & dabove(A.m);
             x = A.spx[i];
             g = A.gid[A.cellC_m[x]];
         }
@@ -163,7 +166,7 @@ __global__ void k_part_scatter(PartArg A, int *__restrict__ cnt, int *__restrict
         if (mk) {
             int pos = base[2 * A.G + g] + sx;
             outx[pos] = x;
-            outm[pos] = mk;
+            mstore(outm, pos, mk, A.mw);
         }
     }
 }
@@ -417,29 +420,28 @@ __global__ void k_stamp(const int *__restrict__ W, const int *__restrict__ nW, i
     if (i < *nW) stamp[W[i]] = tid;
 }
 
-__global__ void k_path_bits(const int *__restrict__ spx, const uint64_t *__restrict__ spm, const int *__restrict__ nsp,
-                            const int *__restrict__ stamp, int tid, unsigned long long *__restrict__ bits) {
+__global__ void k_path_bits(const int *__restrict__ spx, const uint64_t *__restrict__ spm, int mw,
+                            const int *__restrict__ nsp, const int *__restrict__ stamp, int tid,
+                            dmask_t *__restrict__ bits) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t m = 0;
-    if (i < *nsp && stamp[spx[i]] != tid) m = spm[i];
-    for (int off = 32; off > 0; off >>= 1) m |= __shfl_xor(m, off);
-    if (__lane_id() == 0 && m) atomicOr(bits, (unsigned long long) m);
+    dmask_t m = 0;
+    if (i < *nsp && stamp[spx[i]] != tid) m = mload(spm, i, mw);
+    wave_or_mask(m, bits);
 }
 
-__global__ void k_path_insert(DevData D, uint64_t smask, const int *__restrict__ W, const int *__restrict__ nW,
-                              const unsigned long long *__restrict__ bits, uint64_t *__restrict__ table,
-                              uint64_t tmask) {
+__global__ void k_path_insert(DevData D, dmask_t smask, const int *__restrict__ W, const int *__restrict__ nW,
+                              const dmask_t *__restrict__ bits, uint64_t *__restrict__ table, uint64_t tmask) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *nW) return;
-    uint64_t need = *bits;
+    const dmask_t need = *bits;
     if (!need) return;
     int w = W[i];
     uint64_t h = 0x243F6A8885A308D3ull;
     for (int t = 0; t < D.d; ++t) {
         const int k = D.level[t];
-        if ((smask >> k) & 1ull) continue;
+        if (dhas(smask, k)) continue;
         h = mix64(h + D.cellC[(size_t) k * D.n + w] + 1);
-        if (!((need >> k) & 1ull)) continue;
+        if (!dhas(need, k)) continue;
         uint64_t e = mix64(h ^ (0x9E3779B97F4A7C15ull * (uint64_t) (k + 1)));
         uint64_t entry = (e & 0xFFFFFFFF00000000ull) | (uint32_t) w;
         uint64_t slot = e & tmask;
@@ -458,31 +460,32 @@ __global__ void k_path_insert(DevData D, uint64_t smask, const int *__restrict__
     }
 }
 
-__global__ void k_path_check(DevData D, uint64_t smask, const int *__restrict__ spx, const uint64_t *__restrict__ spm,
+__global__ void k_path_check(DevData D, dmask_t smask, const int *__restrict__ spx, const uint64_t *__restrict__ spm,
+                             int mw,
                              const int *__restrict__ nsp, const int *__restrict__ stamp, int tid,
                              const uint64_t *__restrict__ table, uint64_t tmask, int *__restrict__ outx,
                              uint64_t *__restrict__ outm, int *__restrict__ nout) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int x = 0;
-    uint64_t m = 0;
+    dmask_t m = 0;
     if (i < *nsp) {
         x = spx[i];
-        m = spm[i];
+        m = mload(spm, i, mw);
         if (stamp[x] != tid) {
             uint64_t h = 0x243F6A8885A308D3ull;
-            uint64_t keep = 0;
+            dmask_t keep = 0;
             for (int t = 0; t < D.d; ++t) {
                 const int k = D.level[t];
-                if ((smask >> k) & 1ull) continue;
+                if (dhas(smask, k)) continue;
                 h = mix64(h + D.cellC[(size_t) k * D.n + x] + 1);
-                if (!((m >> k) & 1ull)) continue;
+                if (!dhas(m, k)) continue;
                 uint64_t e = mix64(h ^ (0x9E3779B97F4A7C15ull * (uint64_t) (k + 1)));
                 uint64_t slot = e & tmask;
                 while (true) {
                     uint64_t cur = table[slot];  // complete: the inserts ran in an earlier launch
                     if (cur == kEmpty) break;
                     if ((cur >> 32) == (e >> 32) && same_prefix(D, smask, x, (int) (uint32_t) cur, t)) {
-                        keep |= 1ull << k;
+                        keep |= dbit(k);
                         break;
                     }
                     slot = (slot + 1) & tmask;
@@ -494,7 +497,7 @@ __global__ void k_path_check(DevData D, uint64_t smask, const int *__restrict__ 
     int slot = wave_slot(m != 0, nout);
     if (m != 0) {
         outx[slot] = x;
-        outm[slot] = m;
+        mstore(outm, slot, m, mw);
     }
 }
 
@@ -505,7 +508,7 @@ __global__ void k_path_check(DevData D, uint64_t smask, const int *__restrict__ 
 
 // entries e whose mask contains smask (subspace_of, util.cu:619-631) and that
 // are in the result now
-__global__ void k_select(const Entry *__restrict__ E, const int *__restrict__ nE, uint64_t smask, int *__restrict__ list,
+__global__ void k_select(const Entry *__restrict__ E, const int *__restrict__ nE, dmask_t smask, int *__restrict__ list,
                          int *__restrict__ nlist) {
     __shared__ int cnt;
     if (threadIdx.x == 0) cnt = 0;
@@ -672,14 +675,14 @@ __global__ void k_rank_keys(const uint16_t *__restrict__ cellE, int n, int dim, 
 }
 
 // Root s-connection paths: every point with at least one near-edge dimension.
-__global__ void k_root_paths(const uint64_t *__restrict__ nearm, int n, int *__restrict__ outx,
-                             uint64_t *__restrict__ outm, int *__restrict__ nout) {
+__global__ void k_root_paths(const dmask_t *__restrict__ nearm, int n, int *__restrict__ outx,
+                             uint64_t *__restrict__ outm, int mw, int *__restrict__ nout) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t m = i < n ? nearm[i] : 0ull;
+    const dmask_t m = i < n ? nearm[i] : (dmask_t) 0;
     int slot = wave_slot(m != 0, nout);
     if (m != 0) {
         outx[slot] = i;
-        outm[slot] = m;
+        mstore(outm, slot, m, mw);
     }
 }
 
@@ -710,12 +713,12 @@ float sqrt_threshold(float eps) {
 #define LAUNCH(kernel, grid, block, lds, st, ...) hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__)
 
 hipError_t launch_discretise(const float *X, int n, int d, int cells, float eps, float *Xt, uint8_t *cellC,
-                             uint16_t *cellE, uint64_t *nearm, int *bad, hipStream_t st) {
+                             uint16_t *cellE, uint64_t *nearw, int nw, int *bad, hipStream_t st) {
     int ecells = (int) ceil(1.0 / (double) eps);
     float cs = 1.0f / (float) cells;
     float ecs = 1.0f / (float) ecells;
     LAUNCH(k_discretise, dim3(blocks_for(n)), dim3(kBlock), 0, st, X, n, d, cells, cs, ecells, ecs, eps, Xt, cellC,
-           cellE, nearm, bad);
+           cellE, nearw, nw, bad);
     return hipGetLastError();
 }
 
@@ -793,45 +796,47 @@ hipError_t launch_mark(const int *P, const int *nP, int nP_max, int *mark, int m
     return hipGetLastError();
 }
 
-hipError_t launch_flags_all(const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, const uint8_t *cellC,
-                            int n, int first, int cells, int *flags, hipStream_t st) {
+hipError_t launch_flags_all(const int *spx, const uint64_t *spm, int mw, const int *nsp, int nsp_max,
+                            const uint8_t *cellC, int n, int first, int cells, int *flags, hipStream_t st) {
     if (nsp_max <= 0) return hipSuccess;
-    LAUNCH(k_flags_all, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, spx, spm, nsp, cellC, n, first, cells, flags);
+    LAUNCH(k_flags_all, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, spx, spm, mw, nsp, cellC, n, first, cells,
+           flags);
     return hipGetLastError();
 }
 
-hipError_t launch_partition(const int *P, const int *nP, int nP_max, const int *spx, const uint64_t *spm,
+hipError_t launch_partition(const int *P, const int *nP, int nP_max, const int *spx, const uint64_t *spm, int mw,
                             const int *nsp, int nsp_max, const uint8_t *cellC_m, const int *gid, int G,
                             const int *weakcnt, int thr, int m, int *cnt, int *outP, int *outW, int *outx,
                             uint64_t *outm, hipStream_t st) {
     int bp = (int) blocks_for(nP_max), bs = (int) blocks_for(nsp_max);
     LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) 6 * G, sizeof(int))), dim3(kBlock), 0, st, cnt, 0, (size_t) 6 * G);
     if (bp + bs == 0) return hipGetLastError();
-    PartArg A{P, nP, spx, spm, nsp, cellC_m, gid, weakcnt, thr, m, G, bp};
+    PartArg A{P, nP, spx, spm, nsp, cellC_m, gid, weakcnt, thr, m, G, bp, mw};
     LAUNCH(k_part_count, dim3(bp + bs), dim3(kBlock), 0, st, A, cnt);
     LAUNCH(k_part_scatter, dim3(bp + bs), dim3(kBlock), 0, st, A, cnt, outP, outW, outx, outm);
     return hipGetLastError();
 }
 
-hipError_t launch_path_prune(const DevData &D, uint64_t smask, const int *W, const int *nW, int nW_max,
-                             const int *spx, const uint64_t *spm, const int *nsp, int nsp_max, int *stamp, int tid,
-                             unsigned long long *bits, uint64_t *table, uint64_t tmask, int *outx, uint64_t *outm,
+hipError_t launch_path_prune(const DevData &D, dmask_t smask, const int *W, const int *nW, int nW_max,
+                             const int *spx, const uint64_t *spm, int mw, const int *nsp, int nsp_max, int *stamp,
+                             int tid, dmask_t *bits, uint64_t *table, uint64_t tmask, int *outx, uint64_t *outm,
                              int *nout, hipStream_t st) {
     LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) 1, sizeof(int))), dim3(kBlock), 0, st, nout, 0, (size_t) 1);
     if (nsp_max <= 0) return hipGetLastError();
     LAUNCH(k_fill<uint64_t>, dim3(fill_blocks((size_t) (tmask + 1), sizeof(uint64_t))), dim3(kBlock), 0, st, table, kEmpty, (size_t) (tmask + 1));
-    LAUNCH(k_fill<unsigned long long>, dim3(fill_blocks((size_t) 1, sizeof(unsigned long long))), dim3(kBlock), 0, st, bits, 0ull, (size_t) 1);
+    LAUNCH(k_fill<uint64_t>, dim3(fill_blocks((size_t) 2, sizeof(uint64_t))), dim3(kBlock), 0, st,
+           reinterpret_cast<uint64_t *>(bits), (uint64_t) 0, (size_t) 2);
     if (nW_max > 0) LAUNCH(k_stamp, dim3(blocks_for(nW_max)), dim3(kBlock), 0, st, W, nW, stamp, tid);
-    LAUNCH(k_path_bits, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, spx, spm, nsp, stamp, tid, bits);
+    LAUNCH(k_path_bits, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, spx, spm, mw, nsp, stamp, tid, bits);
     if (nW_max > 0)
         LAUNCH(k_path_insert, dim3(blocks_for(nW_max)), dim3(kBlock), 0, st, D, smask, W, nW, bits, table, tmask);
-    LAUNCH(k_path_check, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, D, smask, spx, spm, nsp, stamp, tid, table,
-           tmask, outx, outm, nout);
+    LAUNCH(k_path_check, dim3(blocks_for(nsp_max)), dim3(kBlock), 0, st, D, smask, spx, spm, mw, nsp, stamp, tid,
+           table, tmask, outx, outm, nout);
     return hipGetLastError();
 }
 
 hipError_t launch_redundancy_seed(const int *W, const int *nW_dev, int nW, int *stamp, int tid, const Entry *E,
-                                  const int *nE, uint64_t smask, const int *comp, int *list, int *nlist, int *mn,
+                                  const int *nE, dmask_t smask, const int *comp, int *list, int *nlist, int *mn,
                                   float r, int *keep, uint8_t *lab, hipStream_t st) {
     LAUNCH(k_select, dim3(1), dim3(kBlock), 0, st, E, nE, smask, list, nlist);
     if (nW > 0) LAUNCH(k_stamp, dim3(blocks_for(nW)), dim3(kBlock), 0, st, W, nW_dev, stamp, tid);
@@ -842,7 +847,7 @@ hipError_t launch_redundancy_seed(const int *W, const int *nW_dev, int nW, int *
 }
 
 hipError_t launch_join(const uint8_t *lab, int n, const int *compS, const int *csizeS, int min_size, float r,
-                       Entry *E, const int *nE, uint64_t smask, int eS, uint8_t *inresS, int *list, int *nlist,
+                       Entry *E, const int *nE, dmask_t smask, int eS, uint8_t *inresS, int *list, int *nlist,
                        uint8_t *keepf, uint8_t *drop, uint8_t *fresh, hipStream_t st) {
     LAUNCH(k_join_keep, dim3(blocks_for(n)), dim3(kBlock), 0, st, lab, n, csizeS, min_size, keepf, drop);
     LAUNCH(k_select, dim3(1), dim3(kBlock), 0, st, E, nE, smask, list, nlist);
@@ -931,9 +936,9 @@ hipError_t launch_rank_order(const uint16_t *cellE, int n, int d, int ecells, co
     return hipGetLastError();
 }
 
-hipError_t launch_root_paths(const uint64_t *nearm, int n, int *outx, uint64_t *outm, int *nout, hipStream_t st) {
+hipError_t launch_root_paths(const dmask_t *nearm, int n, int *outx, uint64_t *outm, int mw, int *nout, hipStream_t st) {
     LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) 1, sizeof(int))), dim3(kBlock), 0, st, nout, 0, (size_t) 1);
-    LAUNCH(k_root_paths, dim3(blocks_for(n)), dim3(kBlock), 0, st, nearm, n, outx, outm, nout);
+    LAUNCH(k_root_paths, dim3(blocks_for(n)), dim3(kBlock), 0, st, nearm, n, outx, outm, mw, nout);
     return hipGetLastError();
 }
 

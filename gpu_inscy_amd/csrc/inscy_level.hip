@@ -98,10 +98,10 @@ void Engine::run_levels(inscy_result &out) {
     root.nW = n;
     root.pre = n;
     root.spx = L0.take<int>(n);
-    root.spm = L0.take<uint64_t>(n);
+    root.spm = L0.take<uint64_t>((size_t) n * mw);
     root.nsp_max = n;
     root.slot = 0;
-    HIP_OK(launch_root_paths(nearm, n, root.spx, root.spm, cnt0, st));
+    HIP_OK(launch_root_paths(nearm, n, root.spx, root.spm, mw, cnt0, st));
     lt.push_back(std::move(root));
     stats.calls = 1;
 
@@ -130,15 +130,15 @@ void Engine::run_levels(inscy_result &out) {
         std::vector<std::vector<uint64_t>> has(d, std::vector<uint64_t>(words, 0));
         for (int e = 0; e < ne; ++e)
             for (int k = 0; k < d; ++k)
-                if ((emask[e] >> k) & 1ull) has[k][e >> 6] |= 1ull << (e & 63);
+                if (dhas(emask[e], k)) has[k][e >> 6] |= 1ull << (e & 63);
         std::vector<int> flat;
         std::vector<uint64_t> acc(words);
         sup_at.clear();
         for (int e = 0; e < ne; ++e) {
-            const uint64_t sm = emask[e];
+            const dmask_t sm = emask[e];
             std::fill(acc.begin(), acc.end(), ~0ull);
             for (int k = 0; k < d; ++k)
-                if ((sm >> k) & 1ull)
+                if (dhas(sm, k))
                     for (int w = 0; w < words; ++w) acc[w] &= has[k][w];
             const int at = (int) flat.size();
             for (int w = 0; w < words; ++w)
@@ -195,7 +195,7 @@ void Engine::expand_level(const std::vector<int> &frontier, const int *counts) {
     int ftot = 0;
     for (int a = 0; a < na; ++a) {
         const LTree &t = lt[lv_act[a]];
-        ft[a] = FlagTree{t.spx, t.spm, counts + t.slot, t.first, ftot};
+        ft[a] = FlagTree{t.spx, t.spm, counts + t.slot, t.first, ftot, mw};
         lv_gidoff[a] = ftot;
         ftot += (d - t.first) * cells;
         fblk[a + 1] = fblk[a] + nblocks(t.nsp_max);
@@ -248,7 +248,7 @@ void Engine::expand_level(const std::vector<int> &frontier, const int *counts) {
 
     // child subspaces: per run of frontier trees with one mask, S | {m} for m >= first
     std::vector<ChildSub> cs;
-    std::set<uint64_t> seen;
+    std::set<dmask_t> seen;
     for (int a = 0; a < na;) {
         int b = a;
         while (b < na && lt[lv_act[b]].smask == lt[lv_act[a]].smask) ++b;
@@ -259,7 +259,7 @@ void Engine::expand_level(const std::vector<int> &frontier, const int *counts) {
         for (int m = t.first; m < d; ++m) {
             bool any = false;  // some pair of this child subspace is not skipped
             for (int x = a; x < b && !any; ++x) any = pair_on(x, m);
-            if (any) cs.push_back({t.smask | (1ull << m), m, a, b});
+            if (any) cs.push_back({t.smask | dbit(m), m, a, b});
         }
         a = b;
     }
@@ -620,7 +620,8 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         pp.cnt = pcnt + pi.cnt_off;
         pp.outW = batch.take<int>(std::max(t.nW, 1));
         pp.outx = batch.take<int>(std::max(t.nsp, 1));
-        pp.outm = batch.take<uint64_t>(std::max(t.nsp, 1));
+        pp.outm = batch.take<uint64_t>((size_t) std::max(t.nsp, 1) * mw);
+        pp.mw = mw;
         pp.m = c.m;
         pp.G = pi.G;
         pp.bp = nblocks(t.nW);
@@ -696,7 +697,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
                     ch.slot = lv_nslots++;
                     ch.nsp_max = nSP;
                     ch.spx = nxt.take<int>(std::max(nSP, 1));
-                    ch.spm = nxt.take<uint64_t>(std::max(nSP, 1));
+                    ch.spm = nxt.take<uint64_t>((size_t) std::max(nSP, 1) * mw);
                     if (nSP > 0) {
                         PruneChild pc;
                         std::memset(&pc, 0, sizeof(pc));
@@ -705,7 +706,8 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
                         pc.tmask = (uint64_t) (slots - 1);
                         pc.W = pairs[j].outW + offW;
                         pc.spx = pairs[j].outx + offS;
-                        pc.spm = pairs[j].outm + offS;
+                        pc.spm = pairs[j].outm + (size_t) offS * mw;
+                        pc.mw = mw;
                         pc.outx = ch.spx;
                         pc.outm = ch.spm;
                         pc.nout = lv_next_counts + ch.slot;
@@ -731,9 +733,9 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     if (!prune.empty()) {
         const int nc = (int) prune.size();
         uint64_t *tables = batch.take<uint64_t>(table_total);
-        unsigned long long *bits = batch.take<unsigned long long>(nc);
+        dmask_t *bits = batch.take<dmask_t>(nc);
         HIP_OK(launch_fill_u64(tables, kEmptySlot, table_total, st));
-        HIP_OK(launch_fill_u64(reinterpret_cast<uint64_t *>(bits), 0ull, nc, st));
+        HIP_OK(launch_fill_u64(reinterpret_cast<uint64_t *>(bits), 0ull, (size_t) 2 * nc, st));
         std::vector<int> bw(nc + 1, 0), bp(nc + 1, 0);
         for (int c = 0; c < nc; ++c) {
             prune[c].table = tables + prune_slots[c];
@@ -768,7 +770,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         int ci = 0;
         for (int b = 0; b < B; ++b) {
             if (!passing[b]) continue;
-            const uint64_t s = cs[i0 + b].s;
+            const dmask_t s = cs[i0 + b].s;
             BatchSub x = hs[b];
             x.cnt = cntP + (size_t) ci * n;
             x.comp = compP + (size_t) ci * n;
@@ -851,13 +853,13 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
 // dense components were built) bound the counts of S from above; a bound
 // decides a point only below both thresholds of S and below the cap of the
 // smaller subspaces, under which their counts are exact.
-void Engine::set_bounds(uint64_t s, BatchSub &x) {
+void Engine::set_bounds(dmask_t s, BatchSub &x) {
     const int k = x.S.k;
     int lo = std::min(weak_thr, dense_thr[k]);
     x.nbound = 0;
     x.lo = lo;
     for (int t = 0; t < k && x.nbound < kMaxBounds; ++t) {
-        const uint64_t sub = s & ~(1ull << x.S.dims[t]);
+        const dmask_t sub = s & ~dbit(x.S.dims[t]);
         auto it = subs.find(sub);
         if (sub && it != subs.end() && it->second.entry >= 0) x.bound[x.nbound++] = it->second.cnt;
     }
@@ -876,7 +878,7 @@ void Engine::set_bounds(uint64_t s, BatchSub &x) {
 // steps of one stage run together, a handful of launches per stage.
 void Engine::replay_staged() {
     struct Step {
-        uint64_t s;
+        dmask_t s;
         int kind, tree, t_lo, t_hi;
     };
     std::vector<Step> steps;
@@ -895,7 +897,7 @@ void Engine::replay_staged() {
                 if (lt[kids[j]].pass) pk.push_back(kids[j]);
                 ++j;
             }
-            const uint64_t s = lt[kids[i]].smask;
+            const dmask_t s = lt[kids[i]].smask;
             if (pk.size() == 1) {
                 walk(pk[0]);
                 steps.push_back({s, 0, pk[0], pk[0], pk[0] + 1});
@@ -1025,14 +1027,14 @@ void Engine::score_pairs(int na) {
         // counts of S are complete for every subspace holding a surviving
         // tree; in phi mode they are decision bits and are left out
         const int *cntS = t.smask && !phi_mode ? subs.at(t.smask).cnt : nullptr;
-        const int k = __builtin_popcountll(t.smask);
+        const int k = dcount(t.smask);
         for (int m = t.first; m < d; ++m) {
             FeatPair f;
             std::memset(&f, 0, sizeof(f));
             f.W = t.W;
             f.cntS = cntS;
             f.gid = lv_gid + lv_gidoff[a] + (m - t.first) * cells;
-            f.mask = t.smask | (1ull << m);
+            f.mask = t.smask | dbit(m);
             f.nW = t.nW;
             f.pre = t.pre;
             f.m = m;
@@ -1085,7 +1087,7 @@ void Engine::score_pairs(int na) {
         lv_skip.assign(np, 0);
         for (int a = 0; a < na; ++a) {
             const LTree &t = lt[lv_act[a]];
-            if (__builtin_popcountll(t.smask) < prune_min_depth) continue;
+            if (dcount(t.smask) < prune_min_depth) continue;
             for (int m = t.first; m < d; ++m) {
                 const int j = lv_pair0[a] + m - t.first;
                 if (hp[j] < prune_thr) {
@@ -1153,7 +1155,7 @@ void Engine::finish_collect(inscy_result &out) {
             out.pair_label[j] = pos ? 1 : 0;
             out.pair_cost[2 * j] = t.nW;
             out.pair_cost[2 * j + 1] = cost;
-            out.pair_info[4 * j] = __builtin_popcountll(t.smask);
+            out.pair_info[4 * j] = dcount(t.smask);
             out.pair_info[4 * j + 1] = col_m[j];
             out.pair_info[4 * j + 2] = t.nW;
             out.pair_info[4 * j + 3] = i;
@@ -1170,7 +1172,7 @@ void Engine::finish_collect(inscy_result &out) {
 void Engine::trace_tree(int f, const std::vector<int> &keep) {
     for (int c : lt[f].kids) {
         const LTree &t = lt[c];
-        trace.push_back({(long long) t.smask, t.lo, t.hi, t.pre, t.nW, t.pass ? 1 : 0,
+        trace.push_back({(long long) dlo(t.smask), t.lo, t.hi, t.pre, t.nW, t.pass ? 1 : 0,
                          t.pass ? (keep[c] ? 0 : 1) : -1});
         if (t.pass) trace_tree(c, keep);
     }

@@ -39,12 +39,12 @@ __global__ void k_b_flags(const FlagTree *__restrict__ T, const int *__restrict_
     const int i = (blockIdx.x - blk_off[t]) * blockDim.x + threadIdx.x;
     if (i >= *F.nsp) return;
     const int x = origin(F.spx[i]);
-    uint64_t m = F.first >= 64 ? 0ull : (F.spm[i] >> F.first);
+    dmask_t m = mload(F.spm, i, F.mw) & dabove(F.first - 1);
     int *fl = flags + F.flag_off;
     while (m) {
-        int b = __ffsll((long long) m) - 1;
+        const int b = dfirst(m);
         m &= m - 1;
-        fl[b * cells + cellC[(size_t) (b + F.first) * n + x]] = 1;
+        fl[(b - F.first) * cells + cellC[(size_t) b * n + x]] = 1;
     }
 }
 
@@ -1067,7 +1067,7 @@ __global__ void k_b_part_count(const PartPair *__restrict__ pairs, const int *__
     } else {
         const int i = (blk - A.bp) * blockDim.x + threadIdx.x;
         if (i < A.nsp) {
-            const uint64_t mk = A.m >= 63 ? 0ull : (A.spm[i] & (~0ull << (A.m + 1)));
+            const dmask_t mk = mload(A.spm, i, A.mw) & dabove(A.m);
             if (mk) atomicAdd(&h[2 * G + A.gid[cm[origin(A.spx[i])]]], 1);
         }
     }
@@ -1108,7 +1108,7 @@ __global__ void k_b_part_scatter(const PartPair *__restrict__ pairs, const int *
     int *cur = A.cnt + 3 * G;
     const bool pts = blk < A.bp;
     int key = -1, rnk = 0, val = 0;
-    uint64_t mk = 0;
+    dmask_t mk = 0;
     if (pts) {
         const int i = blk * blockDim.x + threadIdx.x;
         if (i < A.nP) {
@@ -1122,7 +1122,7 @@ __global__ void k_b_part_scatter(const PartPair *__restrict__ pairs, const int *
     } else {
         const int i = (blk - A.bp) * blockDim.x + threadIdx.x;
         if (i < A.nsp) {
-            mk = A.m >= 63 ? 0ull : (A.spm[i] & (~0ull << (A.m + 1)));
+            mk = mload(A.spm, i, A.mw) & dabove(A.m);
             const int xr = A.spx[i];
             const int xo = origin(xr);
             const int g = A.gid[cm[xo]];
@@ -1143,7 +1143,7 @@ __global__ void k_b_part_scatter(const PartPair *__restrict__ pairs, const int *
             A.outW[pos] = val;
         } else {
             A.outx[pos] = val;
-            A.outm[pos] = mk;
+            mstore(A.outm, pos, mk, A.mw);
         }
     }
 }
@@ -1155,10 +1155,9 @@ __global__ void k_b_path_bits(const PruneChild *__restrict__ C, const int *__res
     const int c = seg_of(blk_p, nc, blockIdx.x);
     const PruneChild &P = C[c];
     const int i = (blockIdx.x - blk_p[c]) * blockDim.x + threadIdx.x;
-    uint64_t m = 0;
-    if (i < P.nsp && P.spx[i] < 0) m = P.spm[i];
-    for (int o = 32; o > 0; o >>= 1) m |= __shfl_xor(m, o);
-    if (__lane_id() == 0 && m) atomicOr(P.bits, (unsigned long long) m);
+    dmask_t m = 0;
+    if (i < P.nsp && P.spx[i] < 0) m = mload(P.spm, i, P.mw);
+    wave_or_mask(m, P.bits);
 }
 
 __global__ void k_b_path_insert(DevData D, const PruneChild *__restrict__ C, const int *__restrict__ blk_w, int nc) {
@@ -1166,16 +1165,16 @@ __global__ void k_b_path_insert(DevData D, const PruneChild *__restrict__ C, con
     const PruneChild &P = C[c];
     const int i = (blockIdx.x - blk_w[c]) * blockDim.x + threadIdx.x;
     if (i >= P.nW) return;
-    const uint64_t need = *P.bits;
+    const dmask_t need = *P.bits;
     if (!need) return;
     const int w = P.W[i];
-    const uint64_t smask = P.smask;
+    const dmask_t smask = P.smask;
     uint64_t h = 0x243F6A8885A308D3ull;
     for (int t = 0; t < D.d; ++t) {
         const int k = D.level[t];
-        if ((smask >> k) & 1ull) continue;
+        if (dhas(smask, k)) continue;
         h = mix64(h + D.cellC[(size_t) k * D.n + w] + 1);
-        if (!((need >> k) & 1ull)) continue;
+        if (!dhas(need, k)) continue;
         const uint64_t e = mix64(h ^ (0x9E3779B97F4A7C15ull * (uint64_t) (k + 1)));
         const uint64_t entry = (e & 0xFFFFFFFF00000000ull) | (uint32_t) w;
         uint64_t slot = e & P.tmask;
@@ -1199,27 +1198,27 @@ __global__ void k_b_path_check(DevData D, const PruneChild *__restrict__ C, cons
     const PruneChild &P = C[c];
     const int i = (blockIdx.x - blk_p[c]) * blockDim.x + threadIdx.x;
     int x = 0;
-    uint64_t m = 0;
+    dmask_t m = 0;
     if (i < P.nsp) {
         x = P.spx[i];
-        m = P.spm[i];
+        m = mload(P.spm, i, P.mw);
         if (x < 0) {
             const int xo = ~x;
-            const uint64_t smask = P.smask;
+            const dmask_t smask = P.smask;
             uint64_t h = 0x243F6A8885A308D3ull;
-            uint64_t keep = 0;
+            dmask_t keep = 0;
             for (int t = 0; t < D.d; ++t) {
                 const int k = D.level[t];
-                if ((smask >> k) & 1ull) continue;
+                if (dhas(smask, k)) continue;
                 h = mix64(h + D.cellC[(size_t) k * D.n + xo] + 1);
-                if (!((m >> k) & 1ull)) continue;
+                if (!dhas(m, k)) continue;
                 const uint64_t e = mix64(h ^ (0x9E3779B97F4A7C15ull * (uint64_t) (k + 1)));
                 uint64_t slot = e & P.tmask;
                 while (true) {
                     const uint64_t cur = P.table[slot];  // complete: the inserts ran in an earlier launch
                     if (cur == kEmpty) break;
                     if ((cur >> 32) == (e >> 32) && same_prefix(D, smask, xo, (int) (uint32_t) cur, t)) {
-                        keep |= 1ull << k;
+                        keep |= dbit(k);
                         break;
                     }
                     slot = (slot + 1) & P.tmask;
@@ -1231,7 +1230,7 @@ __global__ void k_b_path_check(DevData D, const PruneChild *__restrict__ C, cons
     const int slot = wave_slot(m != 0, P.nout);
     if (m != 0) {
         P.outx[slot] = x;
-        P.outm[slot] = m;
+        mstore(P.outm, slot, m, P.mw);
     }
 }
 
@@ -1249,8 +1248,8 @@ __global__ __launch_bounds__(kBlock) void k_b_phi_exact(DevData D, const BatchSu
                                                         float s_star) {
     __shared__ double terms[kBlock];
     __shared__ int wcnt[kBlock / 64];
-    __shared__ float qv[kMaxDims];
-    __shared__ int qc[kMaxDims];
+    __shared__ float qv[kMaxSubDims];
+    __shared__ int qc[kMaxSubDims];
     const int count = min(*nfb, fbcap);
     const int n = D.n;
     const int lane = __lane_id(), wave = threadIdx.x / 64;

@@ -45,17 +45,17 @@ constexpr uint64_t kSeed = 0x243F6A8885A308D3ull;
 __device__ __forceinline__ uint64_t level_salt(int k) { return 0x9E3779B97F4A7C15ull * (uint64_t) (k + 1); }
 
 // prefix of z through level index `upto` (unrestricted levels of smask)
-__device__ __forceinline__ uint64_t prefix_hash(const DevData &D, uint64_t smask, int z, int upto) {
+__device__ __forceinline__ uint64_t prefix_hash(const DevData &D, dmask_t smask, int z, int upto) {
     uint64_t h = kSeed;
     for (int t = 0; t <= upto; ++t) {
         const int k = D.level[t];
-        if ((smask >> k) & 1ull) continue;
+        if (dhas(smask, k)) continue;
         h = mix64(h + D.cellC[(size_t) k * D.n + z] + 1);
     }
     return mix64(h ^ level_salt(D.level[upto]));
 }
 
-__device__ void table_insert(const DevData &D, uint64_t smask, uint64_t *table, uint64_t tmask, uint64_t e, int z,
+__device__ void table_insert(const DevData &D, dmask_t smask, uint64_t *table, uint64_t tmask, uint64_t e, int z,
                              int t) {
     const uint64_t entry = (e & 0xFFFFFFFF00000000ull) | (uint32_t) z;
     uint64_t slot = e & tmask;
@@ -71,7 +71,7 @@ __device__ void table_insert(const DevData &D, uint64_t smask, uint64_t *table, 
     }
 }
 
-__device__ bool table_has(const DevData &D, uint64_t smask, const uint64_t *table, uint64_t tmask, uint64_t e, int z,
+__device__ bool table_has(const DevData &D, dmask_t smask, const uint64_t *table, uint64_t tmask, uint64_t e, int z,
                           int t) {
     uint64_t slot = e & tmask;
     while (true) {
@@ -82,15 +82,15 @@ __device__ bool table_has(const DevData &D, uint64_t smask, const uint64_t *tabl
     }
 }
 
-__global__ void k_perm_root(const uint64_t *__restrict__ nearm, int n, int d, int *__restrict__ cx,
+__global__ void k_perm_root(const dmask_t *__restrict__ nearm, int n, int d, int *__restrict__ cx,
                             uint64_t *__restrict__ cm, int *__restrict__ nout) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    uint64_t m = nearm[p];
+    dmask_t m = nearm[p];
     if (!m) return;
-    int at = atomicAdd(nout, __popcll(m));
+    int at = atomicAdd(nout, dcount(m));
     while (m) {
-        const int b = __ffsll((long long) m) - 1;
+        const int b = dfirst(m);
         m &= m - 1;
         cx[at] = p;
         cm[at] = chain_word(b, d - 1);
@@ -108,7 +108,7 @@ __global__ void k_perm_flags(const int *__restrict__ cx, const uint64_t *__restr
 }
 
 // the chains of group g after the restriction of T (smask) on m
-__global__ void k_perm_route(DevData D, uint64_t smask, int m, const int *__restrict__ gid, int g,
+__global__ void k_perm_route(DevData D, dmask_t smask, int m, const int *__restrict__ gid, int g,
                              const int *__restrict__ cx, const uint64_t *__restrict__ cm, const int *__restrict__ nch,
                              int *__restrict__ ox, uint64_t *__restrict__ om, int *__restrict__ nout) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -123,7 +123,7 @@ __global__ void k_perm_route(DevData D, uint64_t smask, int m, const int *__rest
         int best = -1;
         for (int t = from; t <= to; ++t) {
             const int k = D.level[t];
-            if (t != Lm && !((smask >> k) & 1ull) && k > m) best = t;
+            if (t != Lm && !dhas(smask, k) && k > m) best = t;
         }
         return best;
     };
@@ -135,7 +135,7 @@ __global__ void k_perm_route(DevData D, uint64_t smask, int m, const int *__rest
         e = deepest(Lm + 1, end);
         if (e < 0) {
             e = Lm - 1;  // the chain node above the restricted level
-            while (e > Lb && ((smask >> D.level[e]) & 1ull)) --e;
+            while (e > Lb && dhas(smask, D.level[e])) --e;
         }
     } else {
         e = deepest(Lb, min(end, Lm - 1));
@@ -147,22 +147,22 @@ __global__ void k_perm_route(DevData D, uint64_t smask, int m, const int *__rest
 }
 
 // prefixes of the points at every unrestricted level whose dimension is in `need`
-__global__ void k_perm_insert_pts(DevData D, uint64_t smask, const int *__restrict__ pts, const int *__restrict__ npts,
-                                  uint64_t need, uint64_t *__restrict__ table, uint64_t tmask) {
+__global__ void k_perm_insert_pts(DevData D, dmask_t smask, const int *__restrict__ pts, const int *__restrict__ npts,
+                                  dmask_t need, uint64_t *__restrict__ table, uint64_t tmask) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= *npts || !need) return;
     const int z = pts[i];
     uint64_t h = kSeed;
     for (int t = 0; t < D.d; ++t) {
         const int k = D.level[t];
-        if ((smask >> k) & 1ull) continue;
+        if (dhas(smask, k)) continue;
         h = mix64(h + D.cellC[(size_t) k * D.n + z] + 1);
-        if ((need >> k) & 1ull) table_insert(D, smask, table, tmask, mix64(h ^ level_salt(k)), z, t);
+        if (dhas(need, k)) table_insert(D, smask, table, tmask, mix64(h ^ level_salt(k)), z, t);
     }
 }
 
 // regular ancestors of the chains' first nodes: x's prefixes at the levels above b's
-__global__ void k_perm_insert_anchors(DevData D, uint64_t smask, const int *__restrict__ cx,
+__global__ void k_perm_insert_anchors(DevData D, dmask_t smask, const int *__restrict__ cx,
                                       const uint64_t *__restrict__ cm, const int *__restrict__ nch,
                                       uint64_t *__restrict__ table, uint64_t tmask) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,7 +171,7 @@ __global__ void k_perm_insert_anchors(DevData D, uint64_t smask, const int *__re
     uint64_t h = kSeed;
     for (int t = 0; t < Lb; ++t) {
         const int k = D.level[t];
-        if ((smask >> k) & 1ull) continue;
+        if (dhas(smask, k)) continue;
         h = mix64(h + D.cellC[(size_t) k * D.n + z] + 1);
         table_insert(D, smask, table, tmask, mix64(h ^ level_salt(k)), z, t);
     }
@@ -180,10 +180,10 @@ __global__ void k_perm_insert_anchors(DevData D, uint64_t smask, const int *__re
 struct PermTables {
     const uint64_t *W, *P, *T, *A;
     uint64_t wmask, pmask, tmask, amask;
-    uint64_t cond;  // dimensions whose nodes above m's level a larger dimension id below keeps
+    dmask_t cond;  // dimensions whose nodes above m's level a larger dimension id below keeps
 };
 
-__global__ void k_perm_prune(DevData D, uint64_t smask, PermTables Tb, const int *__restrict__ cx,
+__global__ void k_perm_prune(DevData D, dmask_t smask, PermTables Tb, const int *__restrict__ cx,
                              const uint64_t *__restrict__ cm, const int *__restrict__ nch, int *__restrict__ ox,
                              uint64_t *__restrict__ om, int *__restrict__ nout) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -193,13 +193,13 @@ __global__ void k_perm_prune(DevData D, uint64_t smask, PermTables Tb, const int
     const int b = chain_b(w), Lb = D.lvl_of[b];
     // the regular parent: x's prefix through the last unrestricted level above b's
     int tp = Lb - 1;
-    while (tp >= 0 && ((smask >> D.level[tp]) & 1ull)) --tp;
+    while (tp >= 0 && dhas(smask, D.level[tp])) --tp;
     if (tp >= 0 && !table_has(D, smask, Tb.W, Tb.wmask, prefix_hash(D, smask, z, tp), z, tp)) return;
     const uint64_t e = prefix_hash(D, smask, z, Lb);
     if (!table_has(D, smask, Tb.W, Tb.wmask, e, z, Lb)) {
         const bool existed = table_has(D, smask, Tb.P, Tb.pmask, e, z, Lb) ||
                              table_has(D, smask, Tb.A, Tb.amask, e, z, Lb) ||
-                             (((Tb.cond >> b) & 1ull) && table_has(D, smask, Tb.T, Tb.tmask, e, z, Lb));
+                             (dhas(Tb.cond, b) && table_has(D, smask, Tb.T, Tb.tmask, e, z, Lb));
         if (existed) return;  // the sibling lost its points: propergate_count2 erases the chain
     }
     const int at = atomicAdd(nout, 1);
@@ -209,7 +209,7 @@ __global__ void k_perm_prune(DevData D, uint64_t smask, PermTables Tb, const int
 
 // the points of G's first cell mark their prefixes through level index tkey
 // (tkey < 0: the root, flag[0])
-__global__ void k_perm_lo_mark(DevData D, uint64_t smask, const int *__restrict__ pts, const int *__restrict__ npts,
+__global__ void k_perm_lo_mark(DevData D, dmask_t smask, const int *__restrict__ pts, const int *__restrict__ npts,
                                int m, int lo, int tkey, uint64_t *__restrict__ table, uint64_t tmask,
                                int *__restrict__ flag) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -222,7 +222,7 @@ __global__ void k_perm_lo_mark(DevData D, uint64_t smask, const int *__restrict_
         table_insert(D, smask, table, tmask, prefix_hash(D, smask, z, tkey), z, tkey);
 }
 
-__global__ void k_perm_lo_filter(DevData D, uint64_t smask, const int *__restrict__ in, const int *__restrict__ nin,
+__global__ void k_perm_lo_filter(DevData D, dmask_t smask, const int *__restrict__ in, const int *__restrict__ nin,
                                  int tkey, const uint64_t *__restrict__ table, uint64_t tmask,
                                  const int *__restrict__ flag, int *__restrict__ out, int *__restrict__ nout) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -257,7 +257,7 @@ void Engine::run_perm(inscy_result &out) {
     HIP_OK(hipMemcpyAsync(pin, nch, sizeof(int), hipMemcpyDeviceToHost, st));
     sync();
     const int nch_h = pin_i()[0];
-    TreeRef root{0ull, P, nP, n, cx, cm, nch, nch_h};
+    TreeRef root{(dmask_t) 0, P, nP, n, cx, nullptr, nch, nch_h, cm};
     auto t0 = std::chrono::steady_clock::now();
     recurse_perm(root, 0);
     sync();
@@ -275,12 +275,12 @@ void Engine::recurse_perm(const TreeRef &T, int first) {
     int *flags = arena.take<int>((size_t) nd * cells);
     HIP_OK(launch_fill_i32(flags, 0, (size_t) nd * cells, st));
     if (T.nsp_max > 0)
-        LAUNCH(k_perm_flags, dim3(blocks_for(T.nsp_max)), dim3(kBlock), 0, st, T.spx, T.spm, T.nsp, cellC, n, first,
+        LAUNCH(k_perm_flags, dim3(blocks_for(T.nsp_max)), dim3(kBlock), 0, st, T.spx, T.chain, T.nsp, cellC, n, first,
                cells, flags);
     HIP_OK(hipGetLastError());
     const int markid = ++mark_id;
     HIP_OK(launch_mark(T.P, T.nP, T.nP_max, qmark, markid, st));
-    for (int m = first; m < d; ++m) ensure_counts(T.smask | (1ull << m), markid);
+    for (int m = first; m < d; ++m) ensure_counts(T.smask | dbit(m), markid);
     HIP_OK(hipMemcpyAsync(pin, flags, (size_t) nd * cells * sizeof(int), hipMemcpyDeviceToHost, st));
     sync();
     std::vector<std::vector<Group>> groups(nd);
@@ -300,7 +300,7 @@ void Engine::recurse_perm(const TreeRef &T, int first) {
     HIP_OK(hipMemcpyAsync(d_gid, pin, gid.size() * sizeof(int), hipMemcpyHostToDevice, st));
 
     for (int m = first; m < d; ++m) {
-        const uint64_t smask = T.smask | (1ull << m);
+        const dmask_t smask = T.smask | dbit(m);
         const auto &gs = groups[m - first];
         const int G = (int) gs.size();
         const size_t dim_mark = arena.mark();
@@ -308,7 +308,7 @@ void Engine::recurse_perm(const TreeRef &T, int first) {
         int *PP = arena.take<int>(std::max(T.nP_max, 1));
         int *PW = arena.take<int>(std::max(T.nP_max, 1));
         const Subspace &sub = subs[smask];
-        HIP_OK(launch_partition(T.P, T.nP, T.nP_max, nullptr, nullptr, nullptr, 0, cellC + (size_t) m * n,
+        HIP_OK(launch_partition(T.P, T.nP, T.nP_max, nullptr, nullptr, mw, nullptr, 0, cellC + (size_t) m * n,
                                 d_gid + (size_t) (m - first) * cells, G, sub.cnt, weak_thr, m, cnt, PP, PW, nullptr,
                                 nullptr, st));
         HIP_OK(hipMemcpyAsync(pin, cnt, (size_t) 2 * G * sizeof(int), hipMemcpyDeviceToHost, st));
@@ -320,18 +320,18 @@ void Engine::recurse_perm(const TreeRef &T, int first) {
         int deeper = 0, tabove = -1;
         for (int t = 0; t < d; ++t) {
             const int k = level[t];
-            if (((T.smask >> k) & 1ull) || k == m) continue;
+            if (dhas(T.smask, k) || k == m) continue;
             if (t > Lm) deeper = 1;
             else tabove = t;
         }
         const bool filt = !deeper && (tabove < 0 || level[tabove] > m);
         // dimensions of nodes above m's level kept empty by a larger dimension id below
-        uint64_t cond = 0;
+        dmask_t cond = 0;
         for (int t = 0; t < Lm; ++t) {
             const int k = level[t];
-            if ((T.smask >> k) & 1ull) continue;
+            if (dhas(T.smask, k)) continue;
             for (int u = t; u < Lm; ++u)
-                if (!((T.smask >> level[u]) & 1ull) && level[u] > m) cond |= 1ull << k;
+                if (!dhas(T.smask, level[u]) && level[u] > m) cond |= dbit(k);
         }
         // the dimension's cluster marks, below every group's buffers (a
         // group's buffers are released while its kernels may still be queued:
@@ -377,7 +377,7 @@ void Engine::recurse_perm(const TreeRef &T, int first) {
             stats.trees++;
             const bool pass = nW >= prm.min_size;
             const size_t at = trace.size();
-            if (trace_on) trace.push_back({(long long) smask, gs[g].lo, gs[g].hi, nPr, nW, pass ? 1 : 0, -1});
+            if (trace_on) trace.push_back({(long long) dlo(smask), gs[g].lo, gs[g].hi, nPr, nW, pass ? 1 : 0, -1});
             if (!pass) {
                 arena.release(group_mark);
                 continue;
@@ -394,13 +394,13 @@ void Engine::recurse_perm(const TreeRef &T, int first) {
             HIP_OK(launch_fill_i32(nc2, 0, 1, st));
             if (T.nsp_max > 0) {
                 LAUNCH(k_perm_route, dim3(blocks_for(T.nsp_max)), dim3(kBlock), 0, st, D, T.smask, m,
-                       d_gid + (size_t) (m - first) * cells, g, T.spx, T.spm, T.nsp, rx, rm, nr);
+                       d_gid + (size_t) (m - first) * cells, g, T.spx, T.chain, T.nsp, rx, rm, nr);
                 int free_dims = 0;
-                uint64_t need = 0;
+                dmask_t need = 0;
                 for (int k = 0; k < d; ++k)
-                    if (!((smask >> k) & 1ull)) {
+                    if (!dhas(smask, k)) {
                         ++free_dims;
-                        need |= 1ull << k;
+                        need |= dbit(k);
                     }
                 const size_t ws = slots_for((long long) nW * free_dims), ps = slots_for((long long) nPr * free_dims),
                              ts = slots_for((long long) T.nP_max * free_dims),
@@ -428,7 +428,7 @@ void Engine::recurse_perm(const TreeRef &T, int first) {
                 HIP_OK(hipGetLastError());
             }
             Subspace &s2 = ensure_components(smask);
-            TreeRef child{smask, Wg, nWg, nW, cx2, cm2, nc2, T.nsp_max};
+            TreeRef child{smask, Wg, nWg, nW, cx2, nullptr, nc2, T.nsp_max, cm2};
             recurse_perm(child, m + 1);
             // pruneRedundancy then cluster, on the device in stream order
             if (!lab) {

@@ -125,7 +125,7 @@ __global__ void k_pair_finalize(const FeatPair *__restrict__ pairs, int np, int 
     f[30] = hasS ? (float) ((double) accs[j] / ((double) nW * (double) max(P.cap, 1))) : 0.0f;
     f[31] = hasS ? 1.0f : 0.0f;
     for (int b = 0; b < 32; ++b) f[32 + b] = 0.0f;
-    for (uint64_t mk = P.mask; mk; mk &= mk - 1) f[32 + (__ffsll((long long) mk) - 1) % 32] = 1.0f;
+    for (dmask_t mk = P.mask; mk; mk &= mk - 1) f[32 + dfirst(mk) % 32] = 1.0f;
 }
 
 __global__ void k_tree_hits(const TreeHit *__restrict__ trees, const int *__restrict__ blk_off, int nt,

@@ -139,6 +139,20 @@ def config2_small() -> Workload:
     return Workload("n25000_d20_20cl_cfg2params", X, t, s, eps=0.02, cells=8, F=1.0, num_obj=4, min_size=250)
 
 
+def wide(n: int, d: int, eps: float = 0.01, cells: int = 4, num_obj: int = 8, sigma: float = 0.005,
+         rectangular: bool = True, seed: int = 11) -> Workload:
+    """n points in d dimensions (up to 128) with 3 clusters in 2-4 dimensional
+    subspaces drawn from all d dimensions (5% noise), min_size = 0.05 n:
+    dimension ids beyond 64, as in the reference's large-d sweep
+    (experiments/run.py:62-66: d up to 100, eps 0.01, mu 8), at sizes the
+    oracle finishes."""
+    rng = np.random.default_rng(seed)
+    X, t, s = make_clusters(rng, n, d, 3, (2, 4), sigma, 0.05)
+    name = f"wide_n{n}_d{d}_c{cells}{'' if rectangular else '_phi'}"
+    return Workload(name, X, t, s, eps=eps, cells=cells, F=1.0, num_obj=num_obj, min_size=int(0.05 * n),
+                    rectangular=rectangular)
+
+
 def multi_instance(seed: int = 3000, count: int = 64):
     """64 independent datasets: n log-uniform in [2k, 50k], d in 8..24, 3-12
     clusters (subspace dim 3-8, sigma 0.03, 10% noise), eps=0.015 c=6 mu=4,

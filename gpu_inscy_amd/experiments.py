@@ -19,7 +19,7 @@ dimension masks) and `clusterings_i` (int32 [#subspaces, n] canonical
 labels).  run.py stores the same lists as pickled object arrays; plain
 arrays load without unpickling.  Grid points with a file are skipped, as in
 run.py.  Grid points the engine does not run are reported and skipped:
-d > 64 (the large-d sweep's d = 65..100), and entropy_order != 0 with the
+d > 128 (the reference's grids stop at d = 100), and entropy_order != 0 with the
 learned pruner (the permuted level order runs in the exact and level-batched
 modes).
 """
@@ -131,8 +131,8 @@ def main(argv=None):
                                             if os.path.exists(path):
                                                 print("Experiment already performed!", tag, flush=True)
                                                 continue
-                                            if d > 64:
-                                                print("Skipped (the engine runs d <= 64):", tag, flush=True)
+                                            if d > 128:
+                                                print("Skipped (the engine runs d <= 128):", tag, flush=True)
                                                 continue
                                             if order != 0 and mode == _lib.MODE_LEARNED:
                                                 print("Skipped (entropy_order != 0 is not run by the learned "

@@ -216,8 +216,9 @@ const char *inscy_last_error(void);
 /* Discretisation of X into the c-grid and the ceil(1/eps)-grid plus the
  * s-connection marks (reference: ScyTreeNode.cu:100-102 get_cell_no and
  * :34-49 construct_s_connection).  Outputs, all dimension-major [d][n]:
- * cell_c (uint8), cell_e (uint16), near_mask[n] (bit k set when point p lies
- * within eps of the upper edge of its c-grid cell in dimension k, d <= 64). */
+ * cell_c (uint8), cell_e (uint16), near_mask[n][(d + 63) / 64] (bit k % 64 of
+ * word k / 64 set when point p lies within eps of the upper edge of its c-grid
+ * cell in dimension k; one word per point for d <= 64). */
 int inscy_discretise(const float *d_X, int64_t n, int32_t d, int32_t cells, float eps, uint8_t *d_cell_c,
                      uint16_t *d_cell_e, uint64_t *d_near_mask);
 

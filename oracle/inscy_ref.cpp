@@ -554,9 +554,10 @@ struct Ctx {
 
 struct Deadline {};
 
-uint64_t mask_of(const std::vector<int> &S) {
-    uint64_t m = 0;
-    for (int k : S) m |= (uint64_t) 1 << k;
+// the set S as a bitmask (d <= 128); trace rows keep its low 64 bits
+unsigned __int128 mask_of(const std::vector<int> &S) {
+    unsigned __int128 m = 0;
+    for (int k : S) m |= (unsigned __int128) 1 << k;
     return m;
 }
 
@@ -672,7 +673,7 @@ void inscy(const Tree &T, Ctx &C, int first) {
             if (C.timed && std::chrono::steady_clock::now() > C.deadline) throw Deadline{};
             bool pass = prune(R, C);
             C.trees++;
-            TraceRow row{mask_of(S), lo, cell, pre, R.npoints, pass ? 1 : 0, -1};
+            TraceRow row{(uint64_t) mask_of(S), lo, cell, pre, R.npoints, pass ? 1 : 0, -1};
             size_t trace_at = C.trace.size();
             if (C.trace_on) C.trace.push_back(row);
             if (pass) {
@@ -785,7 +786,7 @@ struct SetEngine {
     int num_obj, min_size, cells;
     bool rect;
     std::vector<uint8_t> cellc;  // [p*d+k]
-    std::map<uint64_t, SubspaceMemo> memo;
+    std::map<unsigned __int128, SubspaceMemo> memo;
     std::map<std::vector<int>, std::set<int>, VecCmp> result;
     long long calls = 0, trees = 0, queries = 0;
     bool trace_on = false;
@@ -1115,7 +1116,7 @@ struct SetEngine {
                 bool pass = (int) W.size() >= min_size;
                 size_t at = trace.size();
                 if (trace_on)
-                    trace.push_back(TraceRow{mask_of(S), lo, hi, (int) R.pts.size(), (int) W.size(), pass ? 1 : 0, -1});
+                    trace.push_back(TraceRow{(uint64_t) mask_of(S), lo, hi, (int) R.pts.size(), (int) W.size(), pass ? 1 : 0, -1});
                 if (!pass) continue;
                 R.ch = prune_chains(R, W, T.pts, m);
                 R.pts = W;
@@ -1207,7 +1208,7 @@ struct SetEngine {
                 bool pass = (int) W.size() >= min_size;
                 size_t at = trace.size();
                 if (trace_on)
-                    trace.push_back(TraceRow{mask_of(S), lo, hi, (int) R.pts.size(), (int) W.size(), pass ? 1 : 0, -1});
+                    trace.push_back(TraceRow{(uint64_t) mask_of(S), lo, hi, (int) R.pts.size(), (int) W.size(), pass ? 1 : 0, -1});
                 if (!pass) continue;
                 R.sp = prune_paths(R, W);
                 R.pts = W;

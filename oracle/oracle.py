@@ -174,11 +174,18 @@ def canonical(labels: np.ndarray) -> np.ndarray:
     return out
 
 
+def _check_prefix_keys(cells, d):
+    # the set restatement keys a cell prefix as a base-`cells` number in 128 bits
+    if cells ** d >= 2 ** 128:
+        raise ValueError(f"set restatement: cells**d must be below 2**128 (cells={cells}, d={d}); use run()")
+
+
 def run_sets(X, eps, F, num_obj, min_size, r=1.0, cells=3, rectangular=False, threads=0,
              trace=False) -> OracleRun:
     """Set-based restatement of the same algorithm (identity dimension order)."""
     X = np.ascontiguousarray(X, dtype=np.float32)
     n, d = X.shape
+    _check_prefix_keys(cells, d)
     L = lib()
     L.oracle_sets_run.restype = ctypes.c_void_p
     L.oracle_sets_run.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_float,
@@ -214,6 +221,7 @@ def run_sets_order(X, eps, F, num_obj, min_size, r=1.0, cells=3, rectangular=Fal
     chains with their last level)."""
     X = np.ascontiguousarray(X, dtype=np.float32)
     n, d = X.shape
+    _check_prefix_keys(cells, d)
     L = lib()
     L.oracle_sets_run_order.restype = ctypes.c_void_p
     L.oracle_sets_run_order.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int, ctypes.c_float,

@@ -10,6 +10,8 @@
     python tests/golden/make_golden.py 1ent --threads 4   # config 1, ascending entropy level order
     python tests/golden/make_golden.py 1ent- --threads 4  # config 1, descending entropy level order
     python tests/golden/make_golden.py check1 --threads 8  # set restatement on config 1 vs config1.json
+    python tests/golden/make_golden.py wide --threads 4    # d = 100 and 127 (dimension ids beyond 64)
+    python tests/golden/make_golden.py wide_ent --threads 2  # d = 127 in both entropy level orders
 
 The oracle is the reference's CPU INSCY restated (oracle/inscy_ref.cpp): the
 pointer-tree restatement for configs 0, 1 and 2small, and the set restatement
@@ -80,6 +82,10 @@ def write_json(name: str, obj) -> None:
     with open(tmp, "w") as f:
         json.dump(obj, f)
     os.replace(tmp, os.path.join(HERE, name))
+
+
+WIDE_CASES = [("ref", datagen.wide(300, 100, cells=4)), ("sets", datagen.wide(600, 127, cells=2)),
+              ("sets", datagen.wide(2000, 100, cells=2))]
 
 
 def main() -> None:
@@ -166,6 +172,37 @@ def main() -> None:
             print(f"set {i} {w.name}: {len(run.subspaces)} subspaces, {run.trees} trees, "
                   f"{time.time() - t1:.1f} s", flush=True)
         write_json("config3.json", dict(oracle="set restatement", sets=out, complete=True))
+    elif a.config == "wide":
+        # dimension sets beyond 64 bits: the pointer tree at the large-d
+        # sweep's parameters (c=4, eps=0.01, mu=8) and the set restatement at
+        # c=2 (its prefix keys hold 2^127 cells)
+        out = []
+        for engine, w in WIDE_CASES:
+            t1 = time.time()
+            run = run_workload(w, engine, a.threads)
+            m = meta_of(w, run)
+            m["generator"] = f"datagen.wide({w.n}, {w.d}, cells={w.cells}, rectangular={w.rectangular})"
+            m["oracle"] = "pointer-tree restatement" if engine == "ref" else "set restatement"
+            m["subspaces"] = rows(run)
+            out.append(m)
+            print(f"{w.name}: {len(run.subspaces)} subspaces, {run.trees} trees, {time.time() - t1:.1f} s", flush=True)
+        write_json("wide.json", dict(sets=out))
+    elif a.config == "wide_ent":
+        # d = 127 in ascending and descending entropy level order (set
+        # restatement of a permuted level order)
+        import dataclasses
+        out = []
+        for order in (1, -1):
+            w = dataclasses.replace(datagen.wide(600, 127, cells=2), entropy_order=order)
+            run = O.run_sets_order(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, order,
+                                   threads=a.threads)
+            m = meta_of(w, run)
+            m["generator"] = "datagen.wide(600, 127, cells=2)"
+            m["entropy_order"] = order
+            m["oracle"] = "set restatement, permuted level order"
+            m["subspaces"] = rows(run)
+            out.append(m)
+        write_json("wide_entropy.json", dict(sets=out))
     else:
         raise SystemExit(f"unknown config {a.config!r}")
     print(f"done in {time.time() - t0:.1f} s", flush=True)

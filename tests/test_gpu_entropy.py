@@ -109,6 +109,30 @@ def test_config1_with_entropy_order_matches_golden(order, name):
         assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
 
 
+@pytest.mark.parametrize("case", [0, 1])
+def test_wide_dimensions_with_entropy_order_match_golden(case):
+    # d = 127 with the levels in entropy order: the permuted-order driver's
+    # dimension sets beyond 64 (tests/golden/make_golden.py wide_ent)
+    import hashlib
+    import json
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "wide_entropy.json")
+    if not os.path.exists(path):
+        pytest.skip("no oracle digests for the wide entropy-order cases")
+    g = json.load(open(path))["sets"][case]
+    order = g["entropy_order"]
+    w = datagen.wide(g["n"], g["d"], cells=g["cells"])
+    assert g["X_sha256"] == hashlib.sha256(w.X.tobytes()).hexdigest()
+    X = torch.from_numpy(w.X).cuda()
+    for mode in (_lib.MODE_EXACT, _lib.MODE_BATCHED):
+        say("wide entropy", order, mode)
+        res = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, order, mode=mode)
+        assert res.subspaces == [s for s, _, _, _ in g["subspaces"]]
+        digests = [hashlib.sha256(np.ascontiguousarray(l, dtype=np.int32).tobytes()).hexdigest() for l in res.labels]
+        assert digests == [x[3] for x in g["subspaces"]]
+        assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
+
+
 @pytest.mark.parametrize("order", [1, -1])
 def test_degenerate_inputs_with_entropy_order(order):
     # the inputs of test_gpu_parity's degenerate cases with the permuted level order

@@ -47,11 +47,14 @@ def host_cells(X, cells, eps):
     return c, e, near
 
 
-@pytest.mark.parametrize("seed,cells,eps", [(0, 4, 0.01), (1, 6, 0.015), (2, 8, 0.02), (3, 3, 0.1), (4, 255, 0.003)])
-def test_discretise_bit_exact(seed, cells, eps):
-    say("discretise", seed, cells, eps)
+@pytest.mark.parametrize("seed,cells,eps,d", [(0, 4, 0.01, 7), (1, 6, 0.015, 7), (2, 8, 0.02, 7), (3, 3, 0.1, 7),
+                                              (4, 255, 0.003, 7), (5, 4, 0.05, 64), (6, 4, 0.05, 100),
+                                              (7, 6, 0.02, 128)])
+def test_discretise_bit_exact(seed, cells, eps, d):
+    # near marks: (d + 63) // 64 words per point, dimension k at bit k % 64 of word k // 64
+    say("discretise", seed, cells, eps, d)
     rng = np.random.default_rng(seed)
-    X = rng.random((3000, 7), dtype=np.float32)
+    X = rng.random((3000, d), dtype=np.float32)
     X[:5] = 0.0
     X[5:10] = 1.0
     # values on exact cell edges
@@ -61,7 +64,8 @@ def test_discretise_bit_exact(seed, cells, eps):
     dX = dev(X)
     cc = torch.zeros((d, n), dtype=torch.uint8, device="cuda")
     ce = torch.zeros((d, n), dtype=torch.int16, device="cuda")
-    nm = torch.zeros(n, dtype=torch.int64, device="cuda")
+    nw = (d + 63) // 64
+    nm = torch.full((n, nw), -7, dtype=torch.int64, device="cuda")
     _lib.check(_lib.lib().inscy_discretise(dX.data_ptr(), n, d, cells, eps, cc.data_ptr(), ce.data_ptr(),
                                            nm.data_ptr()))
     torch.cuda.synchronize()
@@ -69,9 +73,10 @@ def test_discretise_bit_exact(seed, cells, eps):
     assert np.array_equal(cc.cpu().numpy().T.astype(np.int32), c)
     assert np.array_equal(ce.cpu().numpy().T.astype(np.int32) & 0xFFFF, e)
     bits = nm.cpu().numpy().view(np.uint64)
-    want = np.zeros(n, dtype=np.uint64)
+    want = np.zeros((n, nw), dtype=np.uint64)
     for k in range(d):
-        want |= near[:, k].astype(np.uint64) << np.uint64(k)
+        want[:, k // 64] |= near[:, k].astype(np.uint64) << np.uint64(k % 64)
+    assert near.any(axis=0).all()
     assert np.array_equal(bits, want)
 
 
@@ -98,12 +103,42 @@ def test_neighbour_counts_match_oracle(seed, S, eps):
     dims = (ctypes.c_int32 * len(S))(*S)
     out = torch.zeros(len(qs), dtype=torch.int32, device="cuda")
     dq = dev(qs)
-    _lib.check(_lib.lib().inscy_neighbour_counts(dev(X).data_ptr(), n, d, eps, dims, len(S), dq.data_ptr(),
+    dX = dev(X)  # held: a freed temporary's block could be handed to a later allocation
+    _lib.check(_lib.lib().inscy_neighbour_counts(dX.data_ptr(), n, d, eps, dims, len(S), dq.data_ptr(),
                                                  len(qs), out.data_ptr()))
     torch.cuda.synchronize()
     got = out.cpu().numpy()
     want = np.array([len(O.neighbours(X, eps, S, int(q))) for q in qs])
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("S", [[70], [3, 64, 99], [63, 64], [1, 2, 65, 90, 127]])
+def test_neighbour_counts_wide_dimensions(S):
+    # dimension ids beyond 64 (d = 128).  The neighbour rule reads only the
+    # dimensions of S, so the counts equal those of the columns X[:, S] as a
+    # k-dimensional set, which the oracle checks on a sample of the queries
+    say("counts wide", S)
+    rng = np.random.default_rng(len(S))
+    n, d, eps = 2000, 128, 0.08
+    X = rng.random((n, d), dtype=np.float32)
+    X[: n // 2, S] = (0.5 + 0.02 * rng.standard_normal((n // 2, len(S)))).clip(0, 1).astype(np.float32)
+    qs = rng.choice(n, size=300, replace=False).astype(np.int32)
+
+    def counts(Xc, dims_list):
+        dims = (ctypes.c_int32 * len(dims_list))(*dims_list)
+        out = torch.zeros(len(qs), dtype=torch.int32, device="cuda")
+        dX, dq = dev(Xc), dev(qs)  # held until the call returns
+        _lib.check(_lib.lib().inscy_neighbour_counts(dX.data_ptr(), n, Xc.shape[1], eps, dims, len(dims_list),
+                                                     dq.data_ptr(), len(qs), out.data_ptr()))
+        torch.cuda.synchronize()
+        return out.cpu().numpy()
+
+    got = counts(X, S)
+    Xs = np.ascontiguousarray(X[:, S])
+    assert got.sum() > 0
+    assert np.array_equal(got, counts(Xs, list(range(len(S)))))
+    want = np.array([len(O.neighbours(Xs, eps, list(range(len(S))), int(q))) for q in qs[:40]])
+    assert np.array_equal(got[:40], want)
 
 
 @pytest.mark.parametrize("seed,S", [(0, [0, 1]), (1, [2]), (2, [0, 2, 3])])
@@ -115,7 +150,8 @@ def test_dense_components_match_bruteforce(seed, S):
     p = _lib.make_params(w.eps, 1.0, w.num_obj, 0, 1.0, w.cells, True, 0)
     dims = (ctypes.c_int32 * len(S))(*S)
     comp = torch.zeros(n, dtype=torch.int32, device="cuda")
-    _lib.check(_lib.lib().inscy_dense_components(dev(X).data_ptr(), n, d, ctypes.byref(p), dims, len(S),
+    dX = dev(X)
+    _lib.check(_lib.lib().inscy_dense_components(dX.data_ptr(), n, d, ctypes.byref(p), dims, len(S),
                                                  comp.data_ptr()))
     torch.cuda.synchronize()
     got = comp.cpu().numpy()
@@ -471,6 +507,25 @@ def assert_golden(res, g):
     assert [int(l.max()) + 1 for l in res.labels] == [x[1] for x in g["subspaces"]]
     assert digests(res) == [x[3] for x in g["subspaces"]]
     assert res.stats["trees"] == g["trees"] and res.stats["calls"] == g["calls"]
+
+
+@pytest.mark.parametrize("case", [0, 1, 2])
+def test_wide_dimensions_match_oracle(case):
+    # d = 100 and 127: subspaces, s-connection paths, near-edge marks and the
+    # result order with dimension ids beyond 64, both drivers against the
+    # oracle's digests (tests/golden/wide.json)
+    g = _golden("wide.json")
+    if g is None:
+        pytest.skip("no oracle digests for the wide cases (tests/golden/make_golden.py wide)")
+    m = g["sets"][case]
+    w = datagen.wide(m["n"], m["d"], cells=m["cells"], rectangular=m["rectangular"])
+    say("wide", w.name)
+    assert m["X_sha256"] == hashlib.sha256(w.X.tobytes()).hexdigest()
+    assert any(len(x[0]) > 1 and max(x[0]) >= 64 for x in m["subspaces"])
+    X = torch.from_numpy(w.X).cuda()
+    for mode in (_lib.MODE_BATCHED, _lib.MODE_EXACT):
+        res = INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, w.rectangular, 0, mode=mode)
+        assert_golden(res, m)
 
 
 def test_config2_full_size_matches_oracle_and_is_deterministic():

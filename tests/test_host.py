@@ -45,7 +45,9 @@ def test_argument_errors_are_codes_not_crashes():
     assert b"n out of range" in L.inscy_last_error()
     assert L.inscy_run_device(X.ctypes.data, 1 << 30, 2, ctypes.byref(p), 2, None, ctypes.byref(h)) == \
         _lib.INSCY_ERR_ARGUMENT
-    assert L.inscy_run(X.ctypes.data, 4, 65, ctypes.byref(p), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
+    # d up to 128 (dimension sets are 128-bit masks)
+    assert L.inscy_run(X.ctypes.data, 4, 129, ctypes.byref(p), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
+    assert b"[1, 128]" in L.inscy_last_error()
     bad = _lib.make_params(0.0, 1.0, 2, 1, 1.0, 3, True, 0)
     assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(bad), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
     cells = _lib.make_params(0.1, 1.0, 2, 1, 1.0, 0, True, 0)
@@ -81,7 +83,7 @@ def test_run_many_rejects_bad_arguments_before_any_device_call():
     out = (ctypes.c_void_p * 1)()
     assert L.inscy_run_many(ptrs, n, d, ps, 0, 2, out) == _lib.INSCY_ERR_ARGUMENT
     assert L.inscy_run_many(None, n, d, ps, 1, 2, out) == _lib.INSCY_ERR_ARGUMENT
-    d0 = (ctypes.c_int32 * 1)(65)
+    d0 = (ctypes.c_int32 * 1)(129)
     assert L.inscy_run_many(ptrs, n, d0, ps, 1, 2, out) == _lib.INSCY_ERR_ARGUMENT
     assert b"data set 0" in L.inscy_last_error()
     ent = (_lib.Params * 1)(_lib.make_params(0.1, 1.0, 2, 1, 1.0, 3, False, 1))

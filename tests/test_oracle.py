@@ -274,6 +274,25 @@ def test_pointer_tree_and_set_restatements_agree_small():
     assert not bad, f"seeds {bad[:10]}"
 
 
+def test_pointer_tree_and_set_restatements_agree_beyond_64_dimensions():
+    # d = 100: subspaces, s-connection paths and near-edge marks with
+    # dimension ids beyond 64 (both restatements hold them in 128 bits)
+    rect = True
+    w = datagen.wide(200, 100, cells=2)
+    a = O.run_ref(w.X, w.eps, 1.0, w.num_obj, w.min_size, 1.0, w.cells, rect, 0, trace=True)
+    b = O.run_sets(w.X, w.eps, 1.0, w.num_obj, w.min_size, 1.0, w.cells, rect, threads=2, trace=True)
+    ok, msg = O.same_result(a, b)
+    assert ok, msg
+    assert np.array_equal(a.trace, b.trace) and a.calls == b.calls
+    assert any(len(s) > 1 and max(s) >= 64 for s in a.subspaces)
+
+
+def test_set_restatement_refuses_prefix_keys_beyond_128_bits():
+    X = np.full((4, 65), 0.5, dtype=np.float32)
+    with pytest.raises(ValueError):
+        O.run_sets(X, 0.1, 1.0, 1, 1, 1.0, 4, True)
+
+
 @pytest.mark.parametrize("mutation", [1, 3])
 def test_cross_check_catches_planted_mistakes(mutation):
     # the set restatement run with a planted mistake in its s-connection prune
@@ -492,6 +511,10 @@ def test_join_result_order_is_ascending_bitmask():
     res = [([0, 1], [0, -1, -1]), ([2], [0, -1, -1]), ([1], [0, -1, -1])]
     out = O.join_test(res, [0, 0, -1], [0, 2], 1, r=10.0)
     assert [s for s, _ in out] == [[1], [0, 1], [2], [0, 2]]
+    # beyond 64 dimensions the largest differing dimension decides (vec_cmp)
+    res = [([70], [0, -1, -1]), ([0, 69], [0, -1, -1]), ([5, 100], [0, -1, -1])]
+    out = O.join_test(res, [0, 0, -1], [64], 1, r=10.0)
+    assert [s for s, _ in out] == [[64], [0, 69], [70], [5, 100]]
 
 
 def test_redundancy_hand_built():
