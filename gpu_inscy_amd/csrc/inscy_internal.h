@@ -459,6 +459,7 @@ constexpr int kRangeWords = 31;      // per query in LDS: 9 range starts, 10 pre
 constexpr int kBlockThreads = 256;   // threads per block of every pass
 constexpr int kQueriesPerBlock = 64;  // queries per block of the batched neighbour passes
 constexpr int kSampleEdges = 2;       // neighbours recorded per point by the count walk (Afforest's sample rounds)
+constexpr int kPartPerThread = 8;     // elements per thread of the batched partition passes
 
 // ------------------------------------------------------------ allocation-free neighbour counts
 // One file-scope device array per device (inscy_noalloc.hip) holds every

@@ -624,8 +624,8 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         pp.mw = mw;
         pp.m = c.m;
         pp.G = pi.G;
-        pp.bp = nblocks(t.nW);
-        pblk[j + 1] = pblk[j] + pp.bp + nblocks(t.nsp);
+        pp.bp = nblocks((t.nW + kPartPerThread - 1) / kPartPerThread);
+        pblk[j + 1] = pblk[j] + pp.bp + nblocks((t.nsp + kPartPerThread - 1) / kPartPerThread);
     }
     defer_uploads();
     PartPair *dpairs = upload(batch, pairs);

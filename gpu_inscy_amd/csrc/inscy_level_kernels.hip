@@ -1057,18 +1057,22 @@ __global__ void k_b_part_count(const PartPair *__restrict__ pairs, const int *__
     for (int t = threadIdx.x; t < 3 * G; t += blockDim.x) h[t] = 0;
     __syncthreads();
     if (blk < A.bp) {
-        const int i = blk * blockDim.x + threadIdx.x;
-        if (i < A.nP) {
-            const int p = A.P[i];
-            const int g = A.gid[cm[p]];
-            atomicAdd(&h[g], 1);
-            if (meets(A.weakcnt[p], thr, wbit)) atomicAdd(&h[G + g], 1);
+        for (int u = 0; u < kPartPerThread; ++u) {
+            const int i = (blk * kPartPerThread + u) * blockDim.x + threadIdx.x;
+            if (i < A.nP) {
+                const int p = A.P[i];
+                const int g = A.gid[cm[p]];
+                atomicAdd(&h[g], 1);
+                if (meets(A.weakcnt[p], thr, wbit)) atomicAdd(&h[G + g], 1);
+            }
         }
     } else {
-        const int i = (blk - A.bp) * blockDim.x + threadIdx.x;
-        if (i < A.nsp) {
-            const dmask_t mk = mload(A.spm, i, A.mw) & dabove(A.m);
-            if (mk) atomicAdd(&h[2 * G + A.gid[cm[origin(A.spx[i])]]], 1);
+        for (int u = 0; u < kPartPerThread; ++u) {
+            const int i = ((blk - A.bp) * kPartPerThread + u) * blockDim.x + threadIdx.x;
+            if (i < A.nsp) {
+                const dmask_t mk = mload(A.spm, i, A.mw) & dabove(A.m);
+                if (mk) atomicAdd(&h[2 * G + A.gid[cm[origin(A.spx[i])]]], 1);
+            }
         }
     }
     __syncthreads();
@@ -1079,7 +1083,8 @@ __global__ void k_b_part_count(const PartPair *__restrict__ pairs, const int *__
 // Only the groups whose weak-dense points reach min_size become surviving
 // trees; the lists of the others are never read, so they are not written (a
 // pair without such a group does nothing).  Slots: LDS ranks within the block,
-// then one global reservation per (block, group).
+// then one global reservation per (block, group).  Each thread takes
+// kPartPerThread elements, one block-width apart.
 __global__ void k_b_part_scatter(const PartPair *__restrict__ pairs, const int *__restrict__ blk_off, int np,
                                  const uint8_t *__restrict__ cellC, int n, int thr, int wbit, int min_size) {
     __shared__ int base[3 * 256];
@@ -1107,43 +1112,50 @@ __global__ void k_b_part_scatter(const PartPair *__restrict__ pairs, const int *
     if (!any_pass) return;  // uniform over the block
     int *cur = A.cnt + 3 * G;
     const bool pts = blk < A.bp;
-    int key = -1, rnk = 0, val = 0;
-    dmask_t mk = 0;
-    if (pts) {
-        const int i = blk * blockDim.x + threadIdx.x;
-        if (i < A.nP) {
-            const int p = A.P[i];
-            const int g = A.gid[cm[p]];
-            if (A.cnt[G + g] >= min_size && meets(A.weakcnt[p], thr, wbit)) {
-                key = G + g;
-                val = p;
+    int key[kPartPerThread], rnk[kPartPerThread], val[kPartPerThread];
+    dmask_t mk[kPartPerThread];
+    for (int u = 0; u < kPartPerThread; ++u) {
+        key[u] = -1;
+        rnk[u] = 0;
+        val[u] = 0;
+        mk[u] = 0;
+        if (pts) {
+            const int i = (blk * kPartPerThread + u) * blockDim.x + threadIdx.x;
+            if (i < A.nP) {
+                const int p = A.P[i];
+                const int g = A.gid[cm[p]];
+                if (A.cnt[G + g] >= min_size && meets(A.weakcnt[p], thr, wbit)) {
+                    key[u] = G + g;
+                    val[u] = p;
+                }
+            }
+        } else {
+            const int i = ((blk - A.bp) * kPartPerThread + u) * blockDim.x + threadIdx.x;
+            if (i < A.nsp) {
+                mk[u] = mload(A.spm, i, A.mw) & dabove(A.m);
+                const int xr = A.spx[i];
+                const int xo = origin(xr);
+                const int g = A.gid[cm[xo]];
+                if (mk[u] && A.cnt[G + g] >= min_size) {
+                    key[u] = 2 * G + g;
+                    val[u] = (xr >= 0 && meets(A.weakcnt[xo], thr, wbit)) ? xo : ~xo;
+                }
             }
         }
-    } else {
-        const int i = (blk - A.bp) * blockDim.x + threadIdx.x;
-        if (i < A.nsp) {
-            mk = mload(A.spm, i, A.mw) & dabove(A.m);
-            const int xr = A.spx[i];
-            const int xo = origin(xr);
-            const int g = A.gid[cm[xo]];
-            if (mk && A.cnt[G + g] >= min_size) {
-                key = 2 * G + g;
-                val = (xr >= 0 && meets(A.weakcnt[xo], thr, wbit)) ? xo : ~xo;
-            }
-        }
+        if (key[u] >= 0) rnk[u] = atomicAdd(&lcnt[key[u]], 1);
     }
-    if (key >= 0) rnk = atomicAdd(&lcnt[key], 1);
     __syncthreads();
     for (int t = threadIdx.x; t < 3 * G; t += blockDim.x)
         if (lcnt[t]) lres[t] = atomicAdd(&cur[t], lcnt[t]);
     __syncthreads();
-    if (key >= 0) {
-        const int pos = base[key] + lres[key] + rnk;
+    for (int u = 0; u < kPartPerThread; ++u) {
+        if (key[u] < 0) continue;
+        const int pos = base[key[u]] + lres[key[u]] + rnk[u];
         if (pts) {
-            A.outW[pos] = val;
+            A.outW[pos] = val[u];
         } else {
-            A.outx[pos] = val;
-            mstore(A.outm, pos, mk, A.mw);
+            A.outx[pos] = val[u];
+            mstore(A.outm, pos, mk[u], A.mw);
         }
     }
 }
