@@ -819,7 +819,12 @@ private:
         out.labels.reset(alloc_labels((size_t) std::max(m, 1) * n));
         // chunks of rows: labels on the device, read back into one of two
         // pinned buffers while the host copies the previous chunk out with
-        // several threads
+        // several threads.  A row's labels run to its kept clusters - 1, so
+        // when every row has at most 255 (65,535) clusters they travel as
+        // label + 1 in one (two) bytes and are widened on the host
+        int maxk = 0;
+        for (int e : rows) maxk = std::max(maxk, tab[e].nkept);
+        const int width = maxk <= 255 ? 1 : (maxk <= 65535 ? 2 : 4);
         const size_t chunk_bytes = (size_t) 128 << 20;
         const int R = (int) std::max<size_t>(1, std::min<size_t>((size_t) m, chunk_bytes / ((size_t) n * 4)));
         if (m > 0) {
@@ -827,11 +832,13 @@ private:
             const size_t items = (size_t) R * n + 1;
             int *flags = batch.take<int>(items);
             int *idx = batch.take<int>(items);
-            int *lab = batch.take<int>((size_t) R * n);
+            char *lab = batch.take<char>((size_t) R * n * width);
             int *cnts = batch.take<int>(R);
             const size_t sb = scan_temp_bytes((int) items);
             void *tmp = batch.take<char>(sb);
-            const size_t stage_bytes = (size_t) R * n * sizeof(int32_t) + (size_t) R * sizeof(int) + 256;
+            // the row counts follow the labels at a 256-byte boundary
+            auto cnt_at = [&](int r) { return ((size_t) r * n * width + 255) & ~(size_t) 255; };
+            const size_t stage_bytes = cnt_at(R) + (size_t) R * sizeof(int);
             char *stage[2] = {emit_stage[0].get(stage_bytes), emit_stage[1].get(stage_bytes)};
             hipEvent_t done[2];
             HIP_OK(event_pool().get(&done[0]));
@@ -847,7 +854,16 @@ private:
                     const size_t lo = std::min(total, t * per), hi = std::min(total, lo + per);
                     if (lo >= hi) continue;
                     auto job = [=] {
-                        std::memcpy(dst + lo, reinterpret_cast<const int32_t *>(src) + lo, (hi - lo) * sizeof(int32_t));
+                        if (width == 4) {
+                            std::memcpy(dst + lo, reinterpret_cast<const int32_t *>(src) + lo,
+                                        (hi - lo) * sizeof(int32_t));
+                        } else if (width == 1) {
+                            const uint8_t *s8 = reinterpret_cast<const uint8_t *>(src);
+                            for (size_t i = lo; i < hi; ++i) dst[i] = (int32_t) s8[i] - 1;
+                        } else {
+                            const uint16_t *s16 = reinterpret_cast<const uint16_t *>(src);
+                            for (size_t i = lo; i < hi; ++i) dst[i] = (int32_t) s16[i] - 1;
+                        }
                     };
                     try {
                         th.emplace_back(job);
@@ -856,8 +872,12 @@ private:
                     }
                 }
                 for (auto &t : th) t.join();
-                const int *hc = reinterpret_cast<const int *>(src + (size_t) r * n * sizeof(int32_t));
-                for (int t = 0; t < r; ++t) stats.clusters += hc[t];
+                const int *hc = reinterpret_cast<const int *>(src + cnt_at(r));
+                for (int t = 0; t < r; ++t) {
+                    if (width < 4 && hc[t] > (width == 1 ? 255 : 65535))
+                        throw Failure(INSCY_ERR_DEVICE, "internal: a result row has more clusters than its table entry");
+                    stats.clusters += hc[t];
+                }
             };
             int prev_c0 = -1, prev_r = 0, prev_buf = 0;
             int buf = 0;
@@ -866,9 +886,9 @@ private:
                 std::vector<EmitRow> er(r);
                 for (int t = 0; t < r; ++t) er[t] = EmitRow{tab[rows[c0 + t]].comp, tab[rows[c0 + t]].inres};
                 EmitRow *der = upload(batch, er);
-                HIP_OK(launch_emit(der, r, n, flags, idx, lab, cnts, tmp, sb, st));
-                HIP_OK(hipMemcpyAsync(stage[buf], lab, (size_t) r * n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-                HIP_OK(hipMemcpyAsync(stage[buf] + (size_t) r * n * sizeof(int32_t), cnts, (size_t) r * sizeof(int),
+                HIP_OK(launch_emit(der, r, n, flags, idx, lab, width, cnts, tmp, sb, st));
+                HIP_OK(hipMemcpyAsync(stage[buf], lab, (size_t) r * n * width, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipMemcpyAsync(stage[buf] + cnt_at(r), cnts, (size_t) r * sizeof(int),
                                       hipMemcpyDeviceToHost, st));
                 HIP_OK(hipEventRecord(done[buf], st));
                 if (prev_c0 >= 0) {

@@ -302,7 +302,9 @@ struct EmitRow {
     const int *comp;
     const uint8_t *inres;
 };
-hipError_t launch_emit(const EmitRow *rows, int r, int n, int *flags, int *idx, int *labels, int *counts,
+// width: bytes per label written (4: int32 labels; 1 / 2: label + 1 as
+// uint8 / uint16, for rows with at most 255 / 65535 clusters)
+hipError_t launch_emit(const EmitRow *rows, int r, int n, int *flags, int *idx, void *labels, int width, int *counts,
                        void *scan_temp, size_t scan_bytes, hipStream_t st);
 hipError_t launch_fill_u8(uint8_t *p, uint8_t v, size_t count, hipStream_t st);
 hipError_t launch_fill_i32(int *p, int v, size_t count, hipStream_t st);

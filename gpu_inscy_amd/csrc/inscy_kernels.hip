@@ -638,14 +638,18 @@ __global__ void k_emit_roots(const EmitRow *__restrict__ rows, int n, int *__res
     flags[(size_t) blockIdx.y * n + i] = (R.comp[i] == i && R.inres[i]) ? 1 : 0;
 }
 
+// T = int: the label; T = uint8_t / uint16_t: the label + 1 (0 for noise), for
+// rows whose labels fit (the host reads them back at a quarter or half the bytes)
+template <class T>
 __global__ void k_emit_labels(const EmitRow *__restrict__ rows, int n, const int *__restrict__ idx,
-                              int *__restrict__ labels) {
+                              T *__restrict__ labels) {
     const EmitRow R = rows[blockIdx.y];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const size_t row = (size_t) blockIdx.y * n;
     const int c = R.comp[i];
-    labels[row + i] = (c >= 0 && R.inres[c]) ? idx[row + c] - idx[row] : -1;
+    const int v = (c >= 0 && R.inres[c]) ? idx[row + c] - idx[row] : -1;
+    labels[row + i] = (T) (sizeof(T) == sizeof(int) ? v : v + 1);
 }
 
 __global__ void k_emit_counts(const int *__restrict__ idx, int n, int r, int *__restrict__ counts) {
@@ -862,7 +866,7 @@ hipError_t launch_set_entry(Entry *E, int idx, const Entry &e, int *nE, int coun
     return hipGetLastError();
 }
 
-hipError_t launch_emit(const EmitRow *rows, int r, int n, int *flags, int *idx, int *labels, int *counts,
+hipError_t launch_emit(const EmitRow *rows, int r, int n, int *flags, int *idx, void *labels, int width, int *counts,
                        void *scan_temp, size_t scan_bytes, hipStream_t st) {
     if (r <= 0) return hipSuccess;
     const size_t items = (size_t) r * n + 1;
@@ -870,7 +874,15 @@ hipError_t launch_emit(const EmitRow *rows, int r, int n, int *flags, int *idx, 
     LAUNCH(k_emit_roots, dim3(blocks_for(n), r), dim3(kBlock), 0, st, rows, n, flags);
     hipError_t e = launch_exclusive_sum(scan_temp, scan_bytes, flags, idx, (int) items, st);
     if (e != hipSuccess) return e;
-    LAUNCH(k_emit_labels, dim3(blocks_for(n), r), dim3(kBlock), 0, st, rows, n, idx, labels);
+    if (width == 1)
+        LAUNCH(k_emit_labels<uint8_t>, dim3(blocks_for(n), r), dim3(kBlock), 0, st, rows, n, idx,
+               static_cast<uint8_t *>(labels));
+    else if (width == 2)
+        LAUNCH(k_emit_labels<uint16_t>, dim3(blocks_for(n), r), dim3(kBlock), 0, st, rows, n, idx,
+               static_cast<uint16_t *>(labels));
+    else
+        LAUNCH(k_emit_labels<int>, dim3(blocks_for(n), r), dim3(kBlock), 0, st, rows, n, idx,
+               static_cast<int *>(labels));
     LAUNCH(k_emit_counts, dim3(blocks_for(r)), dim3(kBlock), 0, st, idx, n, r, counts);
     return hipGetLastError();
 }

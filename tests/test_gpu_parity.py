@@ -528,6 +528,38 @@ def test_wide_dimensions_match_oracle(case):
         assert_golden(res, m)
 
 
+def _many_clusters(case):
+    rng = np.random.default_rng(5 + case)
+    if case == 0:
+        # 300 clusters of 10 points along dimension 0: labels read back in two bytes
+        k, per = 300, 10
+        X = np.empty((k * per, 2), dtype=np.float32)
+        X[:, 0] = np.repeat((np.arange(k) + 0.5) / k, per) + rng.uniform(-1e-4, 1e-4, k * per)
+        X[:, 1] = rng.random(k * per)
+        return X.clip(0, 1).astype(np.float32), 0.001, 3, 5
+    # a 257 x 257 grid of two-point clusters: 66,049 clusters in {0, 1}, labels in four bytes
+    g = 257
+    a = (np.arange(g) + 0.5) / g
+    cx, cy = np.meshgrid(a, a, indexing="ij")
+    X = np.repeat(np.stack([cx.ravel(), cy.ravel()], 1), 2, axis=0) + rng.uniform(-2e-5, 2e-5, (2 * g * g, 2))
+    return X.clip(0, 1).astype(np.float32), 0.0005, 1, 1
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_rows_with_many_clusters_match_oracle(case):
+    # the labels of rows with more than 255 (65,535) clusters leave the device
+    # in two (four) bytes; fewer clusters, as in every other test, in one
+    X, eps, mu, ms = _many_clusters(case)
+    say("many clusters", case, X.shape)
+    ref = O.run_ref(X, eps, 1.0, mu, ms, 1.0, 4, True)
+    assert max(int(l.max()) + 1 for l in ref.labels) > (255 if case == 0 else 65535)
+    for mode in (_lib.MODE_BATCHED, _lib.MODE_EXACT):
+        res = INSCY.run(torch.from_numpy(X).cuda(), eps, 1.0, mu, ms, 1.0, 4, True, 0, mode=mode)
+        assert res.subspaces == ref.subspaces
+        for got, want in zip(res.labels, ref.labels):
+            assert np.array_equal(got, O.canonical(want))
+
+
 def test_config2_full_size_matches_oracle_and_is_deterministic():
     # n=100,000 d=20, c=8, eps=0.02: both drivers against the oracle's digests
     # (set restatement, tests/golden/config2.json), and run to run
