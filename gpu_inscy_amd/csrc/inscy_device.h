@@ -201,6 +201,10 @@ __device__ __forceinline__ void wave_or_mask(dmask_t m, dmask_t *dst) {
     }
 }
 
+// density tests on a stored value: counts (bit == 0) or phi decision bits
+// (bit 1 weak-dense, bit 2 dense)
+__device__ __forceinline__ bool meets(int v, int thr, int bit) { return bit ? (v >= 0 && (v & bit) != 0) : v >= thr; }
+
 constexpr uint64_t kEmpty = ~0ull;
 
 __device__ __forceinline__ uint64_t mix64(uint64_t h) {

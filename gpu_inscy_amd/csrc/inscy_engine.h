@@ -601,10 +601,37 @@ private:
         return s;
     }
 
-    // |N_S(p)| of the points stamped with markid that are not counted yet.
+    // phi mode (rectangular = false): the decision bits of the points of S
+    // without a value, stamped markid when mark is given, each summed in the
+    // neighbourhood tree's leaf order (the permuted-order driver's density)
+    void phi_points(dmask_t smask, Subspace &s, const int *mark, int markid) {
+        BatchSub x;
+        std::memset(&x, 0, sizeof(x));
+        x.S = arg_of(smask);
+        x.matchmask = match_of(x.S);
+        x.phi_w = phi_weak;
+        x.phi_d = phi_dense[x.S.k];
+        x.dbit = 2;
+        x.cnt = s.cnt;
+        size_t mk = arena.mark();
+        BatchSub *dx = arena.take<BatchSub>(1);
+        put(dx, &x, sizeof(x));
+        int2 *fb = arena.take<int2>(n);
+        int *nfb = arena.take<int>(1);
+        flush_uploads();
+        HIP_OK(launch_phi_points(D, dx, s.cnt, mark, markid, fb, nfb, rank_perm, prm.neighborhood_size, s_star, st));
+        arena.release(mk);  // stream order keeps the buffers for the queued kernels
+    }
+
+    // |N_S(p)| of the points stamped with markid that are not counted yet
+    // (phi mode: their decision bits).
     Subspace &ensure_counts(dmask_t smask, int markid) {
         Subspace &s = sub_of(smask);
         if (s.entry >= 0) return s;  // all points counted by the components pass
+        if (phi_mode) {
+            phi_points(smask, s, qmark, markid);
+            return s;
+        }
         SubspaceArg S = arg_of(smask);
         GridArg G = grid_of(S);
         size_t mk = arena.mark();
@@ -627,10 +654,12 @@ private:
         s.inres = slabs.take<uint8_t>(n);
         int *mem = slabs.take<int>(n);
         // a subspace S' = S minus one dimension whose counts are complete bounds
-        // the counts of S from above (exact where below its own cap)
+        // the counts of S from above (exact where below its own cap); phi
+        // mode decides every remaining point in leaf order first
         const int *bound = nullptr;
         int lo = std::min(weak_thr, dense_thr[S.k]);
-        for (int t = 0; t < S.k && !bound; ++t) {
+        if (phi_mode) phi_points(smask, s, nullptr, 0);
+        for (int t = 0; t < S.k && !bound && !phi_mode; ++t) {
             const dmask_t sub = smask & ~dbit(S.dims[t]);
             auto it = subs.find(sub);
             if (sub && it != subs.end() && it->second.entry >= 0) {
@@ -640,8 +669,8 @@ private:
         }
         size_t mk = arena.mark();
         GridBuffers B = grid_buffers(S, G);
-        HIP_OK(launch_subspace_components(D, S, G, s_star, cap_of(S.k), lo, dense_thr[S.k], B, bound, s.cnt, s.comp,
-                                          s.csize, small + kStatAt, timer(), st));
+        HIP_OK(launch_subspace_components(D, S, G, s_star, cap_of(S.k), lo, dense_thr[S.k], phi_mode ? 2 : 0, B, bound,
+                                          s.cnt, s.comp, s.csize, small + kStatAt, timer(), st));
         arena.release(mk);
         HIP_OK(launch_fill_u8(s.inres, 0, n, st));
         grow_entries(nentries + 1);
@@ -709,8 +738,8 @@ private:
             pt.mk = arena.take<uint64_t>((size_t) std::max(T.nsp_max, 1) * mw);
             const Subspace &sub = subs[T.smask | dbit(m)];
             HIP_OK(launch_partition(T.P, T.nP, T.nP_max, T.spx, T.spm, mw, T.nsp, T.nsp_max, cellC + (size_t) m * n,
-                                    d_gid + (size_t) (m - first) * cells, G, sub.cnt, weak_thr, m, pt.cnt, pt.P, pt.W,
-                                    pt.x, pt.mk, st));
+                                    d_gid + (size_t) (m - first) * cells, G, sub.cnt, weak_thr, 0, m, pt.cnt, pt.P,
+                                    pt.W, pt.x, pt.mk, st));
         }
         std::vector<size_t> cnt_at(nd);
         size_t tot = 0;

@@ -49,10 +49,9 @@ static int check_params(int64_t n, int32_t d, const inscy_params *p, int32_t mod
         return fail(INSCY_ERR_ARGUMENT, "number_of_cells must be in [1, 255]");
     if (mode < INSCY_MODE_EXACT || mode > INSCY_MODE_COLLECT) return fail(INSCY_ERR_ARGUMENT, "unknown mode");
     // A permuted SCY-tree level order runs the depth-first driver over
-    // s-connection chains (inscy_perm.hip), with neighbour-count density.
+    // s-connection chains (inscy_perm.hip), with neighbour-count density or,
+    // for rectangular = false, phi summed in the neighbourhood tree's leaf order.
     if (p->entropy_order != 0) {
-        if (p->rectangular == 0)
-            return fail(INSCY_ERR_UNSUPPORTED, "entropy_order != 0 with rectangular = false is not implemented");
         if (mode == INSCY_MODE_LEARNED || mode == INSCY_MODE_COLLECT)
             return fail(INSCY_ERR_UNSUPPORTED, "entropy_order != 0 runs the exact drivers only");
         if (d > 255) return fail(INSCY_ERR_ARGUMENT, "d must be at most 255");

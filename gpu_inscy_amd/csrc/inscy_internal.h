@@ -274,8 +274,9 @@ hipError_t launch_subspace_counts(const DevData &D, const SubspaceArg &S, const 
                                   unsigned long long *stat, const KernelTimer &tm, hipStream_t st);
 // remaining counts, then the dense components of S (comp = smallest member id or -1)
 // (bound: complete counts of a subspace of S, or null; lo: the smaller threshold of S)
+// dbit != 0: cnt holds phi decision bits and bit dbit is the dense test
 hipError_t launch_subspace_components(const DevData &D, const SubspaceArg &S, const GridArg &G, float s_star,
-                                      int cap, int lo, int thr_dense, const GridBuffers &B, const int *bound,
+                                      int cap, int lo, int thr_dense, int dbit, const GridBuffers &B, const int *bound,
                                       int *cnt, int *comp, int *csize, unsigned long long *stat,
                                       const KernelTimer &tm, hipStream_t st);
 hipError_t launch_mark(const int *P, const int *nP, int nP_max, int *mark, int markid, hipStream_t st);
@@ -283,7 +284,7 @@ hipError_t launch_flags_all(const int *spx, const uint64_t *spm, int mw, const i
                             const uint8_t *cellC, int n, int first, int cells, int *flags, hipStream_t st);
 hipError_t launch_partition(const int *P, const int *nP, int nP_max, const int *spx, const uint64_t *spm, int mw,
                             const int *nsp, int nsp_max, const uint8_t *cellC_m, const int *gid, int G,
-                            const int *weakcnt, int thr, int m, int *cnt, int *outP, int *outW, int *outx,
+                            const int *weakcnt, int thr, int wbit, int m, int *cnt, int *outP, int *outW, int *outx,
                             uint64_t *outm, hipStream_t st);
 hipError_t launch_path_prune(const DevData &D, dmask_t smask, const int *W, const int *nW, int nW_max,
                              const int *spx, const uint64_t *spm, int mw, const int *nsp, int nsp_max, int *stamp,
@@ -360,6 +361,13 @@ hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, in
                           const BatchIndex &X, float s_star, float eps, bool phi, bool phi_all_exact,
                           const DevData &D, const int *rank_order, unsigned long long *stat, const KernelTimer &tm,
                           hipStream_t st);
+// phi decision bits (stored as k_b_phi_exact stores them) of the points of one
+// subspace (sub: one device BatchSub with S, matchmask, phi_w, phi_d and cnt)
+// that have no value yet and, when mark is given, are stamped markid; each
+// sums its phi in the neighbourhood tree's leaf order (rank_order).  fb: n
+// pairs of scratch, nfb one word
+hipError_t launch_phi_points(const DevData &D, const BatchSub *sub, const int *cnt, const int *mark, int markid,
+                             int2 *fb, int *nfb, const int *rank_order, float eps, float s_star, hipStream_t st);
 // dense components of nsub subspaces whose counts are complete
 hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, int kfast, const BatchIndex &X,
                                float s_star, hipStream_t st);

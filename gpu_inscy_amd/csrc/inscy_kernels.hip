@@ -102,9 +102,9 @@ struct PartArg {
     const int *nsp;
     const uint8_t *cellC_m;
     const int *gid;      // [cells] group of each cell of m
-    const int *weakcnt;  // neighbour counts of the subspace S ∪ {m}
+    const int *weakcnt;  // neighbour counts (or phi decision bits) of the subspace S ∪ {m}
     int thr, m, G;
-    int nP_blocks, mw;
+    int nP_blocks, mw, wbit;
 };
 
 __global__ void k_part_count(PartArg A, int *__restrict__ cnt) {
@@ -117,7 +117,7 @@ __global__ void k_part_count(PartArg A, int *__restrict__ cnt) {
             int p = A.P[i];
             int g = A.gid[A.cellC_m[p]];
             atomicAdd(&h[g], 1);
-            if (A.weakcnt[p] >= A.thr) atomicAdd(&h[A.G + g], 1);
+            if (meets(A.weakcnt[p], A.thr, A.wbit)) atomicAdd(&h[A.G + g], 1);
         }
     } else {
         int i = (blockIdx.x - A.nP_blocks) * blockDim.x + threadIdx.x;
@@ -148,7 +148,7 @@ __global__ void k_part_scatter(PartArg A, int *__restrict__ cnt, int *__restrict
         bool in = i < *A.nP;
         int p = in ? A.P[i] : 0;
         int g = in ? A.gid[A.cellC_m[p]] : 0;
-        bool weak = in && A.weakcnt[p] >= A.thr;
+        bool weak = in && meets(A.weakcnt[p], A.thr, A.wbit);
         int sp = wave_add_by_key(cur, g, in);
         int sw = wave_add_by_key(cur + A.G, g, weak);
         if (in) outP[base[g] + sp] = p;
@@ -281,12 +281,12 @@ __global__ __launch_bounds__(kBlock) void k_grid_count(int n, SubspaceArg S, Gri
 
 // dense flag per sorted position, union-find roots, zeroed component sizes
 __global__ void k_dense_init(const int *__restrict__ order, int n, const int *__restrict__ cnt, int thr_dense,
-                             uint8_t *__restrict__ dsorted, int *__restrict__ par, int *__restrict__ csize) {
+                             int dbit, uint8_t *__restrict__ dsorted, int *__restrict__ par, int *__restrict__ csize) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     // i: a sorted position for the dense flag, a point id for the rest
-    dsorted[i] = cnt[order[i]] >= thr_dense ? 1 : 0;
-    par[i] = cnt[i] >= thr_dense ? i : -1;
+    dsorted[i] = meets(cnt[order[i]], thr_dense, dbit) ? 1 : 0;
+    par[i] = meets(cnt[i], thr_dense, dbit) ? i : -1;
     csize[i] = 0;
 }
 
@@ -773,7 +773,7 @@ hipError_t launch_subspace_counts(const DevData &D, const SubspaceArg &S, const 
 }
 
 hipError_t launch_subspace_components(const DevData &D, const SubspaceArg &S, const GridArg &G, float s_star,
-                                      int cap, int lo, int thr_dense, const GridBuffers &B, const int *bound,
+                                      int cap, int lo, int thr_dense, int dbit, const GridBuffers &B, const int *bound,
                                       int *cnt, int *comp, int *csize, unsigned long long *stat,
                                       const KernelTimer &tm, hipStream_t st) {
     hipError_t e = build_grid(D, S, G, B, st);
@@ -783,7 +783,8 @@ hipError_t launch_subspace_components(const DevData &D, const SubspaceArg &S, co
     LAUNCH(k_grid_count, dim3(query_blocks(n)), dim3(kBlock), query_lds(S), st, n, S, G, B.off, B.order, B.Ys, B.Es,
            s_star, cap, (const int *) nullptr, 0, bound, lo, cnt, stat);
     if (tm.stop) (void) hipEventRecord(tm.stop, st);
-    LAUNCH(k_dense_init, dim3(blocks_for(n)), dim3(kBlock), 0, st, B.order, n, cnt, thr_dense, B.dsorted, B.par, csize);
+    LAUNCH(k_dense_init, dim3(blocks_for(n)), dim3(kBlock), 0, st, B.order, n, cnt, thr_dense, dbit, B.dsorted, B.par,
+           csize);
     LAUNCH(k_link_sample, dim3(query_blocks(n)), dim3(kBlock), query_lds(S), st, n, S, G, B.off, B.order, B.Ys, B.Es,
            B.dsorted, s_star, B.par);
     LAUNCH(k_uf_compress, dim3(blocks_for(n)), dim3(kBlock), 0, st, B.par, n);
@@ -810,12 +811,12 @@ hipError_t launch_flags_all(const int *spx, const uint64_t *spm, int mw, const i
 
 hipError_t launch_partition(const int *P, const int *nP, int nP_max, const int *spx, const uint64_t *spm, int mw,
                             const int *nsp, int nsp_max, const uint8_t *cellC_m, const int *gid, int G,
-                            const int *weakcnt, int thr, int m, int *cnt, int *outP, int *outW, int *outx,
+                            const int *weakcnt, int thr, int wbit, int m, int *cnt, int *outP, int *outW, int *outx,
                             uint64_t *outm, hipStream_t st) {
     int bp = (int) blocks_for(nP_max), bs = (int) blocks_for(nsp_max);
     LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) 6 * G, sizeof(int))), dim3(kBlock), 0, st, cnt, 0, (size_t) 6 * G);
     if (bp + bs == 0) return hipGetLastError();
-    PartArg A{P, nP, spx, spm, nsp, cellC_m, gid, weakcnt, thr, m, G, bp, mw};
+    PartArg A{P, nP, spx, spm, nsp, cellC_m, gid, weakcnt, thr, m, G, bp, mw, wbit};
     LAUNCH(k_part_count, dim3(bp + bs), dim3(kBlock), 0, st, A, cnt);
     LAUNCH(k_part_scatter, dim3(bp + bs), dim3(kBlock), 0, st, A, cnt, outP, outW, outx, outm);
     return hipGetLastError();

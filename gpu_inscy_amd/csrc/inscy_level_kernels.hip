@@ -335,9 +335,6 @@ __device__ __forceinline__ double phi_term(float acc, float eps) {
 // it are recomputed in leaf order (k_b_phi_exact).
 constexpr double kPhiRelDefault = 1.0 / 4194304.0;
 
-// density tests on a stored value: counts (bit == 0) or phi decision bits
-__device__ __forceinline__ bool meets(int v, int thr, int bit) { return bit ? (v >= 0 && (v & bit) != 0) : v >= thr; }
-
 // Stored value of phi mode: (q << 2) | bits, bits the decisions (1
 // weak-dense, 2 dense) and q = ceil(4 U) for an upper bound U on the
 // reference's phi of the point in S and in every superset of S (q =
@@ -1645,6 +1642,16 @@ __global__ void k_b_fill(T *p, T v, size_t count) {
     fill_span(p, v, count);
 }
 
+// the points of one subspace still without a value (cnt < 0) and, when mark
+// is given, stamped markid: (0, p) pairs for k_b_phi_exact
+__global__ void k_phi_collect(const int *__restrict__ cnt, const int *__restrict__ mark, int markid, int n,
+                              int2 *__restrict__ fb, int *__restrict__ nfb) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool want = p < n && cnt[p] < 0 && (!mark || mark[p] == markid);
+    const int at = wave_slot(want, nfb);
+    if (want) fb[at] = make_int2(0, p);
+}
+
 }  // namespace
 
 #define LAUNCH(kernel, grid, block, lds, st, ...) hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__)
@@ -1832,6 +1839,15 @@ static void launch_count_k(const BatchSub *subs, int nsub, const int *blk_off, i
                    X.off, X.order, X.Ys, X.Es, X.sig, X.qlist, X.qstart, s_star, eps, phirel, X.nb, X.fb, X.nfb,
                    X.fbcap, stat);
     }
+}
+
+hipError_t launch_phi_points(const DevData &D, const BatchSub *sub, const int *cnt, const int *mark, int markid,
+                             int2 *fb, int *nfb, const int *rank_order, float eps, float s_star, hipStream_t st) {
+    const int n = D.n;
+    LAUNCH(k_b_fill<int>, dim3(fill_blocks((size_t) 1, sizeof(int))), dim3(kBlock), 0, st, nfb, 0, (size_t) 1);
+    LAUNCH(k_phi_collect, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cnt, mark, markid, n, fb, nfb);
+    LAUNCH(k_b_phi_exact, dim3(1024), dim3(kBlock), 0, st, D, sub, fb, nfb, n, rank_order, eps, s_star);
+    return hipGetLastError();
 }
 
 hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, int nblk, int n, int kmax, int kfast,

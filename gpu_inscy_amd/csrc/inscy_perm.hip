@@ -309,8 +309,8 @@ void Engine::recurse_perm(const TreeRef &T, int first) {
         int *PW = arena.take<int>(std::max(T.nP_max, 1));
         const Subspace &sub = subs[smask];
         HIP_OK(launch_partition(T.P, T.nP, T.nP_max, nullptr, nullptr, mw, nullptr, 0, cellC + (size_t) m * n,
-                                d_gid + (size_t) (m - first) * cells, G, sub.cnt, weak_thr, m, cnt, PP, PW, nullptr,
-                                nullptr, st));
+                                d_gid + (size_t) (m - first) * cells, G, sub.cnt, weak_thr, phi_mode ? 1 : 0, m, cnt, PP,
+                                PW, nullptr, nullptr, st));
         HIP_OK(hipMemcpyAsync(pin, cnt, (size_t) 2 * G * sizeof(int), hipMemcpyDeviceToHost, st));
         sync();
         std::vector<int> c2(pin_i(), pin_i() + 2 * G);

@@ -42,11 +42,11 @@ def case(seed):
                          sigma=float(rng.choice([0.02, 0.04, 0.07, 0.12])), sub_dims=(1, min(d, 4)))
 
 
-def check(w, order, mode):
-    ref = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, order, trace=True)
+def check(w, order, mode, rect=True):
+    ref = O.run_ref(w.X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, rect, order, trace=True)
     _lib.debug_trace(True)
     try:
-        res = INSCY.run(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, order,
+        res = INSCY.run(torch.from_numpy(w.X).cuda(), w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, rect, order,
                         mode=mode)
         rows = _lib.debug_trace_rows()
     finally:
@@ -66,6 +66,24 @@ def test_permuted_level_order_matches_oracle(seed):
         check(w, order, _lib.MODE_EXACT)
 
 
+@pytest.mark.parametrize("seed", range(16))
+def test_permuted_level_order_with_phi_matches_oracle(seed):
+    # rectangular = false: phi summed in the neighbourhood tree's leaf order,
+    # whose levels follow the same permuted order
+    w = case(seed)
+    for order in (1, -1):
+        say("entropy phi", seed, order, w.n, w.d, w.cells)
+        check(w, order, _lib.MODE_EXACT, rect=False)
+
+
+@pytest.mark.parametrize("order", [1, -1])
+def test_config0_with_entropy_order_and_phi(order):
+    w = datagen.config(0)
+    say("config0 entropy phi", order)
+    check(w, order, _lib.MODE_EXACT, rect=False)
+    check(w, order, _lib.MODE_BATCHED, rect=False)
+
+
 @pytest.mark.parametrize("order", [1, -1])
 def test_config0_with_entropy_order(order):
     w = datagen.config(0)
@@ -79,9 +97,6 @@ def test_config0_with_entropy_order(order):
 def test_entropy_order_limits():
     w = datagen.config(0)
     X = torch.from_numpy(w.X).cuda()
-    with pytest.raises(_lib.InscyError) as e:
-        INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, False, 1)
-    assert e.value.code == _lib.INSCY_ERR_UNSUPPORTED
     with pytest.raises(_lib.InscyError) as e:
         INSCY.run(X, w.eps, w.F, w.num_obj, w.min_size, w.r, w.cells, True, 1, mode=_lib.MODE_COLLECT)
     assert e.value.code == _lib.INSCY_ERR_UNSUPPORTED

@@ -52,9 +52,9 @@ def test_argument_errors_are_codes_not_crashes():
     assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(bad), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
     cells = _lib.make_params(0.1, 1.0, 2, 1, 1.0, 0, True, 0)
     assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(cells), 1, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
-    # a permuted level order runs with neighbour-count density only
+    # a permuted level order runs in the exact and batched modes only
     ent = _lib.make_params(0.1, 1.0, 2, 1, 1.0, 3, False, 1)
-    assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(ent), 1, ctypes.byref(h)) == _lib.INSCY_ERR_UNSUPPORTED
+    assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(ent), 4, ctypes.byref(h)) == _lib.INSCY_ERR_UNSUPPORTED
     ent = _lib.make_params(0.1, 1.0, 2, 1, 1.0, 3, True, -1)
     assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(ent), 3, ctypes.byref(h)) == _lib.INSCY_ERR_UNSUPPORTED
     assert L.inscy_run(X.ctypes.data, 4, 2, ctypes.byref(p), 7, ctypes.byref(h)) == _lib.INSCY_ERR_ARGUMENT
@@ -86,8 +86,9 @@ def test_run_many_rejects_bad_arguments_before_any_device_call():
     d0 = (ctypes.c_int32 * 1)(129)
     assert L.inscy_run_many(ptrs, n, d0, ps, 1, 2, out) == _lib.INSCY_ERR_ARGUMENT
     assert b"data set 0" in L.inscy_last_error()
-    ent = (_lib.Params * 1)(_lib.make_params(0.1, 1.0, 2, 1, 1.0, 3, False, 1))
-    assert L.inscy_run_many(ptrs, n, d, ent, 1, 2, out) == _lib.INSCY_ERR_UNSUPPORTED
+    # the learned pruner belongs to the identity level order
+    ent = (_lib.Params * 1)(_lib.make_params(0.1, 1.0, 2, 1, 1.0, 3, True, 1))
+    assert L.inscy_run_many(ptrs, n, d, ent, 1, _lib.MODE_LEARNED, out) == _lib.INSCY_ERR_UNSUPPORTED
 
 
 def test_noalloc_counts_refuse_what_the_scratch_cannot_hold_before_any_device_call():
