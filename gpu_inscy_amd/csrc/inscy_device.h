@@ -205,6 +205,38 @@ __device__ __forceinline__ void wave_or_mask(dmask_t m, dmask_t *dst) {
 // (bit 1 weak-dense, bit 2 dense)
 __device__ __forceinline__ bool meets(int v, int thr, int bit) { return bit ? (v >= 0 && (v & bit) != 0) : v >= thr; }
 
+// term of phi (ClusteringCpu.cu:116-124) for a neighbour at squared distance
+// acc: d = sqrtf(acc) / eps in float, then 1 - d*d in double
+__device__ __forceinline__ double phi_term(float acc, float eps) {
+    const float dist = __fsqrt_rn(acc);
+    const float r = __fdiv_rn(dist, eps);
+    const float sq = __fmul_rn(r, r);
+    return 1.0 - (double) sq;
+}
+
+// The reference sums phi in float in the order of the neighbourhood tree's
+// leaves.  The passes here sum the same terms in double in scan order: with N
+// terms the float sum in leaf order lies within a relative N * 2^-22 of it
+// (each float rounding moves the running sum by at most 2^-24 of the final
+// value), so a threshold outside that band is decided exactly; points inside
+// it are recomputed in leaf order (k_b_phi_exact).
+constexpr double kPhiRelDefault = 1.0 / 4194304.0;
+
+// Stored value of phi mode: (q << 2) | bits, bits the decisions (1
+// weak-dense, 2 dense) and q = ceil(4 U) for an upper bound U on the
+// reference's phi of the point in S and in every superset of S (q =
+// kPhiNoBound: none known).  phi is monotone like the counts: a superset of S
+// has fewer neighbours, each term no larger (the float distance sums over
+// more dimensions, every rounding monotone), so T, the exact sum of the
+// float terms, only falls; the float leaf-order sum of N terms lies within a
+// relative N 2^-22 of T (the band of the passes), hence U = T_upper (1 + N
+// 2^-22) with N the neighbours in S bounds phi in every superset.
+constexpr int kPhiNoBound = (1 << 29) - 1;
+__device__ __forceinline__ int phi_code(int bits, double ub) {
+    const int q = ub >= 0.0 && ub < (double) (kPhiNoBound - 1) * 0.25 ? (int) ceil(ub * 4.0) : kPhiNoBound;
+    return (q << 2) | bits;
+}
+
 constexpr uint64_t kEmpty = ~0ull;
 
 __device__ __forceinline__ uint64_t mix64(uint64_t h) {

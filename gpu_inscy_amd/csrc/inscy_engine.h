@@ -602,8 +602,11 @@ private:
     }
 
     // phi mode (rectangular = false): the decision bits of the points of S
-    // without a value, stamped markid when mark is given, each summed in the
-    // neighbourhood tree's leaf order (the permuted-order driver's density)
+    // without a value, stamped markid when mark is given (the permuted-order
+    // driver's density): the grid pass decides a point whose thresholds lie
+    // outside the rounding band of its double scan-order sum (2^-22 per term,
+    // as in the level-synchronous passes); the others are summed again in the
+    // neighbourhood tree's leaf order
     void phi_points(dmask_t smask, Subspace &s, const int *mark, int markid) {
         BatchSub x;
         std::memset(&x, 0, sizeof(x));
@@ -613,13 +616,18 @@ private:
         x.phi_d = phi_dense[x.S.k];
         x.dbit = 2;
         x.cnt = s.cnt;
+        const GridArg G = grid_of(x.S);
         size_t mk = arena.mark();
         BatchSub *dx = arena.take<BatchSub>(1);
         put(dx, &x, sizeof(x));
         int2 *fb = arena.take<int2>(n);
         int *nfb = arena.take<int>(1);
+        GridBuffers B = grid_buffers(x.S, G);
         flush_uploads();
-        HIP_OK(launch_phi_points(D, dx, s.cnt, mark, markid, fb, nfb, rank_perm, prm.neighborhood_size, s_star, st));
+        const double phirel = phi_all_exact ? 1e30 : 1.0 / 4194304.0;
+        HIP_OK(launch_subspace_phi(D, x.S, G, B, s_star, prm.neighborhood_size, phirel, x.phi_w, x.phi_d, mark, markid,
+                                   s.cnt, fb, nfb, n, st));
+        HIP_OK(launch_phi_exact(D, dx, fb, nfb, n, rank_perm, prm.neighborhood_size, s_star, st));
         arena.release(mk);  // stream order keeps the buffers for the queued kernels
     }
 

@@ -361,13 +361,18 @@ hipError_t launch_b_count(const BatchSub *subs, int nsub, const int *blk_off, in
                           const BatchIndex &X, float s_star, float eps, bool phi, bool phi_all_exact,
                           const DevData &D, const int *rank_order, unsigned long long *stat, const KernelTimer &tm,
                           hipStream_t st);
-// phi decision bits (stored as k_b_phi_exact stores them) of the points of one
-// subspace (sub: one device BatchSub with S, matchmask, phi_w, phi_d and cnt)
-// that have no value yet and, when mark is given, are stamped markid; each
-// sums its phi in the neighbourhood tree's leaf order (rank_order).  fb: n
-// pairs of scratch, nfb one word
-hipError_t launch_phi_points(const DevData &D, const BatchSub *sub, const int *cnt, const int *mark, int markid,
-                             int2 *fb, int *nfb, const int *rank_order, float eps, float s_star, hipStream_t st);
+// phi decision bits of the points of one subspace without a value (cnt ==
+// -1; stamped markid when mark is given), from the grid pass of S: decided
+// where a threshold lies outside the rounding band (phirel per term), the
+// others listed in fb (capacity fbcap, count in nfb) as (0, point)
+hipError_t launch_subspace_phi(const DevData &D, const SubspaceArg &S, const GridArg &G, const GridBuffers &B,
+                              float s_star, float eps, double phirel, float phi_w, float phi_d, const int *mark,
+                              int markid, int *cnt, int2 *fb, int *nfb, int fbcap, hipStream_t st);
+// the listed points' phi summed in the neighbourhood tree's leaf order
+// (rank_order), stored in sub's cnt (sub: one device BatchSub with S,
+// matchmask, phi_w, phi_d and cnt)
+hipError_t launch_phi_exact(const DevData &D, const BatchSub *sub, const int2 *fb, const int *nfb, int fbcap,
+                            const int *rank_order, float eps, float s_star, hipStream_t st);
 // dense components of nsub subspaces whose counts are complete
 hipError_t launch_b_components(const BatchSub *subs, int nsub, int n, int kmax, int kfast, const BatchIndex &X,
                                float s_star, hipStream_t st);

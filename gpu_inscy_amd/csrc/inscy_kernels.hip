@@ -279,6 +279,73 @@ __global__ __launch_bounds__(kBlock) void k_grid_count(int n, SubspaceArg S, Gri
     }
 }
 
+// phi mode (rectangular = false): the decision bits of the points without a
+// value (cnt == -1; stamped markid when mark is given).  The lanes of a query
+// sum its phi terms in double in scan order; a threshold outside the rounding
+// band of the reference's float leaf-order sum (phirel per term) is decided
+// here, and a point with a threshold inside it goes to fb for the leaf-order
+// recomputation.  Stored like the level-synchronous passes: phi_code(bits, U).
+__global__ __launch_bounds__(kBlock) void k_grid_phi(int n, SubspaceArg S, GridArg G, const int *__restrict__ off,
+                                                     const int *__restrict__ order, const float *__restrict__ Ys,
+                                                     const uint16_t *__restrict__ Es, float s_star, float eps,
+                                                     double phirel, float phi_w, float phi_d,
+                                                     const int *__restrict__ mark, int markid, int *__restrict__ cnt,
+                                                     int2 *__restrict__ fb, int *__restrict__ nfb, int fbcap) {
+    constexpr int LPQ = kLanesPerQuery, QPB = kBlock / LPQ;
+    extern __shared__ char smem[];
+    float *sy = reinterpret_cast<float *>(smem);
+    int *se = reinterpret_cast<int *>(smem + sizeof(float) * QPB * S.k);
+    const int lq = threadIdx.x / LPQ, sub = threadIdx.x % LPQ;
+    const int i = blockIdx.x * QPB + lq;
+    const int p = i < n ? order[i] : 0;
+    const bool active = i < n && cnt[p] < 0 && (!mark || mark[p] == markid);
+    if (sub == 0 && active) stage_query(S, n, Ys, Es, i, sy + lq, se + lq, QPB);
+    __syncthreads();
+    if (!active) return;  // the walk's ballots below span only active query groups
+    const float *qy = sy + lq;
+    const int *qe = se + lq;
+    const int gshift = (__lane_id() / LPQ) * LPQ;
+    int found = 0;
+    double dsum = 0.0;
+    for_ranges(G, off, qe, QPB, [&](int a, int b) {
+        for (int base = a; base < b; base += LPQ) {
+            const int jj = base + sub;
+            float acc = 0.0f;
+            bool hit = jj < b && jj != i;
+            for (int t = 0; t < S.k && hit; ++t) {
+                if ((G.checkmask >> t) & 1ull) {
+                    const int de = (int) Es[(size_t) t * n + jj] - qe[t * QPB];
+                    if (de < -1 || de > 1) hit = false;
+                }
+                if (hit) {
+                    const float diff = qy[t * QPB] - Ys[(size_t) t * n + jj];
+                    const float sq = diff * diff;
+                    acc = acc + sq;
+                    if (acc > s_star) hit = false;
+                }
+            }
+            if (hit) dsum += phi_term(acc, eps);
+            found += __popcll((__ballot(hit) >> gshift) & ((1ull << LPQ) - 1ull));
+        }
+    });
+    for (int o = LPQ / 2; o > 0; o >>= 1) dsum += __shfl_xor(dsum, o);
+    if (sub != 0) return;
+    const double lo = dsum * (1.0 - (double) found * phirel);
+    const double hi = dsum * (1.0 + (double) found * phirel);
+    const double tw = (double) phi_w, td = (double) phi_d;
+    int code = 0;
+    bool unsure = false;
+    if (lo >= tw) code |= 1;
+    else if (!(hi < tw)) unsure = true;
+    if (lo >= td) code |= 2;
+    else if (!(hi < td)) unsure = true;
+    cnt[p] = phi_code(code, hi * (1.0 + (double) found * phirel));
+    if (unsure) {
+        const int at = atomicAdd(nfb, 1);
+        if (at < fbcap) fb[at] = make_int2(0, p);
+    }
+}
+
 // dense flag per sorted position, union-find roots, zeroed component sizes
 __global__ void k_dense_init(const int *__restrict__ order, int n, const int *__restrict__ cnt, int thr_dense,
                              int dbit, uint8_t *__restrict__ dsorted, int *__restrict__ par, int *__restrict__ csize) {
@@ -769,6 +836,17 @@ hipError_t launch_subspace_counts(const DevData &D, const SubspaceArg &S, const 
     LAUNCH(k_grid_count, dim3(query_blocks(D.n)), dim3(kBlock), query_lds(S), st, D.n, S, G, B.off, B.order, B.Ys,
            B.Es, s_star, cap, mark, markid, (const int *) nullptr, 0, cnt, stat);
     if (tm.stop) (void) hipEventRecord(tm.stop, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_subspace_phi(const DevData &D, const SubspaceArg &S, const GridArg &G, const GridBuffers &B,
+                              float s_star, float eps, double phirel, float phi_w, float phi_d, const int *mark,
+                              int markid, int *cnt, int2 *fb, int *nfb, int fbcap, hipStream_t st) {
+    hipError_t e = build_grid(D, S, G, B, st);
+    if (e != hipSuccess) return e;
+    LAUNCH(k_fill<int>, dim3(fill_blocks((size_t) 1, sizeof(int))), dim3(kBlock), 0, st, nfb, 0, (size_t) 1);
+    LAUNCH(k_grid_phi, dim3(query_blocks(D.n)), dim3(kBlock), query_lds(S), st, D.n, S, G, B.off, B.order, B.Ys, B.Es,
+           s_star, eps, phirel, phi_w, phi_d, mark, markid, cnt, fb, nfb, fbcap);
     return hipGetLastError();
 }
 

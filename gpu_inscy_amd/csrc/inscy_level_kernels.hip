@@ -318,38 +318,6 @@ __device__ __forceinline__ bool cand_test(const BatchSub &B, int n, const float 
     return true;
 }
 
-// term of phi (ClusteringCpu.cu:116-124) for a neighbour at squared distance
-// acc: d = sqrtf(acc) / eps in float, then 1 - d*d in double
-__device__ __forceinline__ double phi_term(float acc, float eps) {
-    const float dist = __fsqrt_rn(acc);
-    const float r = __fdiv_rn(dist, eps);
-    const float sq = __fmul_rn(r, r);
-    return 1.0 - (double) sq;
-}
-
-// The reference sums phi in float in the order of the neighbourhood tree's
-// leaves.  The passes here sum the same terms in double in scan order: with N
-// terms the float sum in leaf order lies within a relative N * 2^-22 of it
-// (each float rounding moves the running sum by at most 2^-24 of the final
-// value), so a threshold outside that band is decided exactly; points inside
-// it are recomputed in leaf order (k_b_phi_exact).
-constexpr double kPhiRelDefault = 1.0 / 4194304.0;
-
-// Stored value of phi mode: (q << 2) | bits, bits the decisions (1
-// weak-dense, 2 dense) and q = ceil(4 U) for an upper bound U on the
-// reference's phi of the point in S and in every superset of S (q =
-// kPhiNoBound: none known).  phi is monotone like the counts: a superset of S
-// has fewer neighbours, each term no larger (the float distance sums over
-// more dimensions, every rounding monotone), so T, the exact sum of the
-// float terms, only falls; the float leaf-order sum of N terms lies within a
-// relative N 2^-22 of T (the band of the passes), hence U = T_upper (1 + N
-// 2^-22) with N the neighbours in S bounds phi in every superset.
-constexpr int kPhiNoBound = (1 << 29) - 1;
-__device__ __forceinline__ int phi_code(int bits, double ub) {
-    const int q = ub >= 0.0 && ub < (double) (kPhiNoBound - 1) * 0.25 ? (int) ceil(ub * 4.0) : kPhiNoBound;
-    return (q << 2) | bits;
-}
-
 // LDS of the neighbour passes: per query k coordinates, k cells, its packed
 // cells (2 words) and its runs
 __device__ __forceinline__ void lds_layout(char *smem, int k, float *&sy, int *&se, uint64_t *&qs, int *&qr) {
@@ -1642,16 +1610,6 @@ __global__ void k_b_fill(T *p, T v, size_t count) {
     fill_span(p, v, count);
 }
 
-// the points of one subspace still without a value (cnt < 0) and, when mark
-// is given, stamped markid: (0, p) pairs for k_b_phi_exact
-__global__ void k_phi_collect(const int *__restrict__ cnt, const int *__restrict__ mark, int markid, int n,
-                              int2 *__restrict__ fb, int *__restrict__ nfb) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool want = p < n && cnt[p] < 0 && (!mark || mark[p] == markid);
-    const int at = wave_slot(want, nfb);
-    if (want) fb[at] = make_int2(0, p);
-}
-
 }  // namespace
 
 #define LAUNCH(kernel, grid, block, lds, st, ...) hipLaunchKernelGGL(kernel, grid, block, lds, st, __VA_ARGS__)
@@ -1841,12 +1799,9 @@ static void launch_count_k(const BatchSub *subs, int nsub, const int *blk_off, i
     }
 }
 
-hipError_t launch_phi_points(const DevData &D, const BatchSub *sub, const int *cnt, const int *mark, int markid,
-                             int2 *fb, int *nfb, const int *rank_order, float eps, float s_star, hipStream_t st) {
-    const int n = D.n;
-    LAUNCH(k_b_fill<int>, dim3(fill_blocks((size_t) 1, sizeof(int))), dim3(kBlock), 0, st, nfb, 0, (size_t) 1);
-    LAUNCH(k_phi_collect, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, cnt, mark, markid, n, fb, nfb);
-    LAUNCH(k_b_phi_exact, dim3(1024), dim3(kBlock), 0, st, D, sub, fb, nfb, n, rank_order, eps, s_star);
+hipError_t launch_phi_exact(const DevData &D, const BatchSub *sub, const int2 *fb, const int *nfb, int fbcap,
+                            const int *rank_order, float eps, float s_star, hipStream_t st) {
+    LAUNCH(k_b_phi_exact, dim3(1024), dim3(kBlock), 0, st, D, sub, fb, nfb, fbcap, rank_order, eps, s_star);
     return hipGetLastError();
 }
 
