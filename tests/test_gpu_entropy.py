@@ -82,6 +82,13 @@ def test_config0_with_entropy_order_and_phi(order):
     say("config0 entropy phi", order)
     check(w, order, _lib.MODE_EXACT, rect=False)
     check(w, order, _lib.MODE_BATCHED, rect=False)
+    # every phi by the leaf-order recomputation: the same result as the
+    # grid pass with its rounding band
+    _lib.debug_phi_exact(True)
+    try:
+        check(w, order, _lib.MODE_EXACT, rect=False)
+    finally:
+        _lib.debug_phi_exact(False)
 
 
 @pytest.mark.parametrize("order", [1, -1])
