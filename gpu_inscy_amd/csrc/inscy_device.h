@@ -201,6 +201,16 @@ __device__ __forceinline__ void wave_or_mask(dmask_t m, dmask_t *dst) {
     }
 }
 
+// bucket of coordinate x in an index's last dimension with F buckets per eps
+// cell (C cells): floor(x C F) within x's own cell's F buckets, so that the
+// order of buckets follows the order of cells and of coordinates (monotone in x)
+__device__ __forceinline__ int fine_key(float x, int C, int F) {
+    x = fminf(fmaxf(x, 0.0f), 1.0f);
+    const int c = ecell_of(x, ecell_width(C), C);
+    const int f = (int) floor((double) x * (double) C * (double) F);
+    return min(max(f, c * F), c * F + F - 1);
+}
+
 // density tests on a stored value: counts (bit == 0) or phi decision bits
 // (bit 1 weak-dense, bit 2 dense)
 __device__ __forceinline__ bool meets(int v, int thr, int bit) { return bit ? (v >= 0 && (v & bit) != 0) : v >= thr; }

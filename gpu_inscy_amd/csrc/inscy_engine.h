@@ -546,8 +546,11 @@ private:
     }
 
     // grid index of S: the (up to 3) most selective matched dimensions, not
-    // counting position `exclude` (the child's own dimension of a shared index)
-    GridArg grid_of(const SubspaceArg &S, int exclude = -1) const {
+    // counting position `exclude` (the child's own dimension of a shared index);
+    // fine (the level-synchronous driver): an index of at most two dimensions
+    // sorts its last one in kFineBuckets buckets per cell, so that a query's
+    // runs can leave out the buckets farther than eps (its cells stay few)
+    GridArg grid_of(const SubspaceArg &S, int exclude = -1, bool fine = false) const {
         GridArg G{};
         const uint64_t matched = match_of(S);
         std::vector<int> ts;
@@ -561,6 +564,7 @@ private:
             cellsj *= ecells;
             G.j++;
         }
+        G.F = fine && G.j <= 2 && cellsj * kFineBuckets <= grid_cap ? kFineBuckets : 1;
         G.checkmask = matched;
         for (int a = 0; a < G.j; ++a) {
             G.tI[a] = ts[a];

@@ -78,8 +78,17 @@ struct GridArg {
     int j;               // number of index dimensions
     int C;               // cells per index dimension (ceil(1/eps))
     int tI[3];           // positions within the subspace of the index dimensions
+    int F;               // buckets per cell of the last index dimension (0 or 1: the cells themselves)
     uint64_t checkmask;  // bit t: dimension S.dims[t] needs the per-candidate cell test
 };
+
+// grid cells of an index: C^j, the last index dimension in F buckets per cell
+inline long long grid_cells(const GridArg &G) {
+    long long c = G.F > 1 ? G.F : 1;
+    for (int a = 0; a < G.j; ++a) c *= G.C;
+    return c;
+}
+constexpr int kFineBuckets = 4;  // buckets per cell of a batched index's last dimension (indexes of j <= 2)
 
 // Scratch of one subspace grid pass (device pointers, sized for n points,
 // k dims and C^j + 1 grid cells).

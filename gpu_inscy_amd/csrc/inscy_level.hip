@@ -273,9 +273,8 @@ void Engine::expand_level(const std::vector<int> &frontier, const int *counts) {
         while (j < cs.size() && j - i < max_batch) {
             const ChildSub &c = cs[j];
             SubspaceArg S = arg_of(c.s);
-            GridArg G = grid_of(S);
-            long long ncell = 1;
-            for (int q = 0; q < G.j; ++q) ncell *= G.C;
+            GridArg G = grid_of(S, -1, true);
+            const long long ncell = grid_cells(G);
             // process_batch's takes per subspace: counts, keys, rank, order,
             // query flags and list, two sample edges, packed cells (own and
             // shared), dense flags and parents (50 bytes a point), k coordinate
@@ -330,8 +329,8 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
         bool shareable = kS >= 3 && __builtin_popcountll(match_of(x.S) & ((1ull << kS) - 1)) >= 3;
         GridArg gbest{};
         if (shareable) {
-            gbest = grid_of(x.S, x.S.k - 1);
-            const GridArg gopt = grid_of(x.S);
+            gbest = grid_of(x.S, x.S.k - 1, true);
+            const GridArg gopt = grid_of(x.S, -1, true);
             double cs_ = 1.0, co_ = 1.0;
             for (int a = 0; a < gbest.j; ++a) cs_ *= std::max(score[x.S.dims[gbest.tI[a]]], 1.0);
             for (int a = 0; a < gopt.j; ++a) co_ *= std::max(score[x.S.dims[gopt.tI[a]]], 1.0);
@@ -351,10 +350,10 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
             x.mcoord = coord;
             x.rep = 0;
             coord += n;
-            for (int q = 0; q < x.G.j; ++q) ncell *= x.G.C;
+            ncell = grid_cells(x.G);
         } else {
-            x.G = shareable ? grid_of(x.S, x.S.k - 1) : grid_of(x.S);
-            for (int q = 0; q < x.G.j; ++q) ncell *= x.G.C;
+            x.G = shareable ? grid_of(x.S, x.S.k - 1, true) : grid_of(x.S, -1, true);
+            ncell = grid_cells(x.G);
             x.obase = (long long) reps.size() * n;
             x.hoff = hoff;
             hoff += ncell;
@@ -406,11 +405,7 @@ void Engine::process_batch(const std::vector<ChildSub> &cs, size_t i0, size_t i1
     std::vector<int> miss, hit_reps;
     std::vector<KeptSort> hit_src, store_src;
     int max_cell = 0;
-    auto cells_of = [&](const BatchSub &x) {
-        int c = 1;
-        for (int q = 0; q < x.G.j; ++q) c *= x.G.C;
-        return c;
-    };
+    auto cells_of = [&](const BatchSub &x) { return (int) grid_cells(x.G); };
     for (int r : reps) {
         const BatchSub &x = hs[r];
         const int ncell = cells_of(x);

@@ -62,7 +62,14 @@ __global__ void k_b_keys(DevData D, const BatchSub *__restrict__ subs, const int
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= D.n) return;
     int key = 0;
-    for (int a = 0; a < B.G.j; ++a) key = key * B.G.C + D.cellE[(size_t) B.S.dims[B.G.tI[a]] * D.n + p];
+    const int jl = B.G.j - 1;
+    for (int a = 0; a < jl; ++a) key = key * B.G.C + D.cellE[(size_t) B.S.dims[B.G.tI[a]] * D.n + p];
+    // the last index dimension: its cell, or its bucket within the cell
+    const int dl = B.S.dims[B.G.tI[jl]];
+    if (B.G.F > 1)
+        key = key * (B.G.C * B.G.F) + fine_key(D.Xt[(size_t) dl * D.n + p], B.G.C, B.G.F);
+    else
+        key = key * B.G.C + D.cellE[(size_t) dl * D.n + p];
     keys[B.obase + p] = key;
     rank[B.obase + p] = atomicAdd(&hist[B.hoff + key], 1);
 }
@@ -231,8 +238,8 @@ __global__ void k_b_qstart(const int *__restrict__ qlist, const int *__restrict_
 // holds the query, so it is not empty).  q[21..30]: prefix sums of the runs'
 // lengths in groups of four (ceil(length / 4)), for the walks that take four
 // consecutive candidates of one run per lane.
-__device__ __forceinline__ void build_runs(const GridArg &G, const int *__restrict__ off, const int *qe, int stride,
-                                           int base0, int *q) {
+__device__ __forceinline__ void build_runs(const GridArg &G, const int *__restrict__ off, const int *qe,
+                                           const float *qy, int stride, float s_star, int base0, int *q) {
     // rows r = 3 xi + yi over the cells within one of the query's in the first
     // (and second) index dimension, each a run of the last one's cells within
     // one; every row's two offsets are loaded before any is used, so the
@@ -241,7 +248,18 @@ __device__ __forceinline__ void build_runs(const GridArg &G, const int *__restri
     const int c0 = qe[G.tI[0] * stride];
     const int c1 = G.j == 3 ? qe[G.tI[1] * stride] : 0;
     const int cl = qe[G.tI[G.j - 1] * stride];
-    const int lo = max(cl - 1, 0), hi = min(cl + 1, C - 1);
+    const int F = G.F > 1 ? G.F : 1, CF = C * F;
+    // a run: the buckets of the last dimension's cells within one of the
+    // query's, less those farther than eps from the query's coordinate (a
+    // neighbour's squared difference there is at most s_star; the bounds are
+    // rounded outwards and the bucket map is monotone, so no neighbour is lost)
+    int lo = max(cl - 1, 0) * F, hi = min(cl + 1, C - 1) * F + F - 1;
+    if (F > 1) {
+        const double xq = (double) qy[G.tI[G.j - 1] * stride];
+        const double em = sqrt((double) s_star) * 1.001 + 1e-6;
+        lo = max(lo, fine_key(__double2float_rd(xq - em), C, F));
+        hi = min(hi, fine_key(__double2float_ru(xq + em), C, F));
+    }
     const int x0 = max(c0 - 1, 0), nx = G.j == 1 ? 1 : min(c0 + 1, C - 1) - x0 + 1;
     const int y0 = max(c1 - 1, 0), ny = G.j == 3 ? min(c1 + 1, C - 1) - y0 + 1 : 1;
     const int own = G.j == 1 ? 0 : 3 * (c0 - x0) + (G.j == 3 ? c1 - y0 : 0);
@@ -251,7 +269,7 @@ __device__ __forceinline__ void build_runs(const GridArg &G, const int *__restri
         // rows past the edge of the grid load a row inside it (no branch
         // around the loads) and are dropped below
         const int xi = min(r / 3, nx - 1), yi = min(r % 3, ny - 1);
-        const int row = G.j == 1 ? 0 : G.j == 2 ? (x0 + xi) * C : ((x0 + xi) * C + y0 + yi) * C;
+        const int row = G.j == 1 ? 0 : G.j == 2 ? (x0 + xi) * CF : ((x0 + xi) * C + y0 + yi) * CF;
         a[r] = off[row + lo];
         e[r] = off[row + hi + 1];
     }
@@ -719,7 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_b_count(const BatchSub *__restrict__
         const uint64_t qsg = sigb[i];
         stage_query_b(B, n, Ys, p, i, sy + lq, se + lq, QPB);
         qs[lq] = qsg;
-        build_runs(B.G, off + B.hoff, se + lq, QPB, base0, q);
+        build_runs(B.G, off + B.hoff, se + lq, sy + lq, QPB, s_star, base0, q);
     }
     __syncthreads();
     long long seen = 0;
@@ -908,7 +926,7 @@ __global__ __launch_bounds__(kBlock) void k_b_link_full(const BatchSub *__restri
         const uint64_t qsg = sigb[i];
         stage_query_b(B, n, Ys, p, i, sy + lq, se + lq, QPB);
         qs[lq] = qsg;
-        build_runs(B.G, off + B.hoff, se + lq, QPB, (int) B.obase, q);
+        build_runs(B.G, off + B.hoff, se + lq, sy + lq, QPB, s_star, (int) B.obase, q);
     }
     __syncthreads();
     if (!active) return;
